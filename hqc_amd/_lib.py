@@ -1,0 +1,222 @@
+"""ctypes binding of libhqc_amd.so (the C-ABI in include/hqc_amd.h).
+
+Every wrapper takes torch tensors that already live on the GPU, checks shape,
+dtype, device and contiguity on the host, passes raw device pointers and the
+current HIP stream, and raises on a non-zero status.  There is no CPU
+fallback: if the library is missing or no GPU is present the calls raise.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhqc_amd.so")
+_ERRORS = {1: "bad argument", 2: "HIP error", 3: "size not supported"}
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+_SIGS = {
+    "hqc_version": [],
+    "hqc_gto_tables": [_P, _I, _I, _I, _D, _I, _P, _I, _I, _P, _P],
+    "hqc_ao_values_f64": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
+    "hqc_ao_values_f32": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
+    "hqc_ao_density_f64": [_P, _P, _I, _I, _I, _P, _P, _P],
+    "hqc_ao_density_f32": [_P, _P, _I, _I, _I, _P, _P],
+    "hqc_ao_vmat_workspace": [_I, _I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
+    "hqc_ao_vmat_f64": [_P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
+    "hqc_ao_vmat_f32": [_P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
+    "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
+    "hqc_vloc_G": [_P, _I, _I, _I, _D, _P, _P, _P],
+    "hqc_real_to_complex": [_P, _I, _I, _I, _D, _P, _P],
+    "hqc_complex_real": [_P, _I, _I, _I, _P, _P],
+    "hqc_poisson_mul": [_P, _P, _I, _I, _P, _P],
+    "hqc_xc_blocks": [_I],
+    "hqc_xc_eval": [_P, _P, _I, _I, _I, _I, _P, _P, _P],
+    "hqc_occupations": [_P, _I, _I, _I, _D, _I, _I, _I, _D, _D, _P, _P, _P],
+    "hqc_overlap_kinetic": [_P, _I, _I, _P, _I, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P],
+    "hqc_ewald_pairs": [_P, _I, _I, _P, _P, _I, _D, _D, _P, _P],
+    "hqc_force_blocks": [_I],
+    "hqc_grid_forces": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _D, _P, _P],
+    "hqc_vloc_force": [_P, _I, _I, _I, _D, _P, _P, _P, _P],
+}
+
+
+def lib():
+    """Load (once) and return the ctypes handle; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m hqc_amd.build`")
+        h = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(h, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        _lib = h
+    return _lib
+
+
+def symbols():
+    """Names of every entry point bound here (all declared in include/hqc_amd.h)."""
+    return sorted(_SIGS)
+
+
+def _call(name, *args):
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        raise RuntimeError(f"{name} failed: {_ERRORS.get(rc, rc)}")
+
+
+def _ptr(t, dtype=None, name="tensor"):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a GPU tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+F64 = torch.float64
+F32 = torch.float32
+I32 = torch.int32
+
+
+def gto_tables(xp, g, L, tmax, alpha, nl, out):
+    B, n, _ = xp.shape
+    na = alpha.numel()
+    assert out.shape == (B, n, 3, g, na, nl)
+    _call("hqc_gto_tables", _ptr(xp, F64, "xp"), B, n, g, float(L), int(tmax),
+          _ptr(alpha, F64), na, nl, _ptr(out, F64, "P"), _stream())
+
+
+def ao_values(P, coef, comp, nao_pad, ld, out):
+    B, n, _, g, na, nl = P.shape
+    nloc = coef.shape[1]
+    assert out.shape == (B, nao_pad, ld) and coef.shape[0] == na
+    name = "hqc_ao_values_f64" if out.dtype == F64 else "hqc_ao_values_f32"
+    _call(name, _ptr(P, F64, "P"), B, n, g, na, nl, _ptr(coef, F64), _ptr(comp, I32), nloc,
+          nao_pad, ld, _ptr(out, None, "phi"), _stream())
+
+
+def ao_density(phi, D, rho, X=None):
+    B, nao_pad, ld = phi.shape
+    assert D.shape == (B, nao_pad, nao_pad) and rho.shape == (B, ld)
+    if phi.dtype == F64:
+        if X is not None:
+            assert X.shape == phi.shape
+        _call("hqc_ao_density_f64", _ptr(phi, F64, "phi"), _ptr(D, F64, "D"), B, nao_pad, ld,
+              _ptr(rho, F64, "rho"), _ptr(X, F64, "X"), _stream())
+    else:
+        assert X is None
+        _call("hqc_ao_density_f32", _ptr(phi, F32, "phi"), _ptr(D, F64, "D"), B, nao_pad, ld,
+              _ptr(rho, F64, "rho"), _stream())
+
+
+def vmat_workspace(B, nao_pad, ld, prec):
+    n = ctypes.c_size_t(0)
+    _call("hqc_ao_vmat_workspace", B, nao_pad, ld, prec, ctypes.byref(n))
+    return n.value
+
+
+def ao_vmat(phi, v, dV, V, work):
+    B, nao_pad, ld = phi.shape
+    assert v.shape == (B, ld) and V.shape == (B, nao_pad, nao_pad)
+    name = "hqc_ao_vmat_f64" if phi.dtype == F64 else "hqc_ao_vmat_f32"
+    _call(name, _ptr(phi, None, "phi"), _ptr(v, F64, "v"), B, nao_pad, ld, float(dV),
+          _ptr(V, F64, "V"), _ptr(work, None, "work"), work.numel() * work.element_size(),
+          _stream())
+
+
+def fft3d(c, sign):
+    """In-place 3-D FFT of a complex128 tensor (B, g, g, g)."""
+    B, g = c.shape[0], c.shape[1]
+    assert c.dtype == torch.complex128 and c.shape[1:] == (g, g, g)
+    _call("hqc_fft3d_c2c_f64", _ptr(c, None, "c"), B, g, int(sign), _stream())
+
+
+def vloc_G(xp, g, L, VG, out):
+    B, n, _ = xp.shape
+    assert out.dtype == torch.complex128 and out.shape == (B, g, g, g)
+    _call("hqc_vloc_G", _ptr(xp, F64, "xp"), B, n, g, float(L), _ptr(VG, F64, "VG"),
+          _ptr(out, None, "out"), _stream())
+
+
+def real_to_complex(x, g, s, out):
+    B, ld = x.shape
+    _call("hqc_real_to_complex", _ptr(x, F64, "x"), B, g, ld, float(s), _ptr(out, None, "out"),
+          _stream())
+
+
+def complex_real(c, ld, out):
+    B, g = c.shape[0], c.shape[1]
+    assert out.shape == (B, ld)
+    _call("hqc_complex_real", _ptr(c, None, "c"), B, g, ld, _ptr(out, F64, "out"), _stream())
+
+
+def poisson_mul(c, VG, saved=None):
+    B, g = c.shape[0], c.shape[1]
+    _call("hqc_poisson_mul", _ptr(c, None, "c"), _ptr(VG, F64, "VG"), B, g,
+          _ptr(saved, None, "saved"), _stream())
+
+
+def xc_blocks(g):
+    return lib().hqc_xc_blocks(g)
+
+
+def xc_eval(rho, vh, g, xc_id, vout, part):
+    B, ld = rho.shape
+    _call("hqc_xc_eval", _ptr(rho, F64, "rho"), _ptr(vh, None, "vh"), B, g, ld, int(xc_id),
+          _ptr(vout, F64, "vout"), _ptr(part, F64, "part"), _stream())
+
+
+def occupations(w, nelec, sigma, method, search, max_cycle, tol, mu_shift, occ, mu):
+    B, nmo = w.shape
+    _call("hqc_occupations", _ptr(w, F64, "w"), B, nmo, int(nelec), float(sigma), int(method),
+          int(search), int(max_cycle), float(tol), float(mu_shift), _ptr(occ, F64, "occ"),
+          _ptr(mu, F64, "mu"), _stream())
+
+
+def overlap_kinetic(xp, lattice, alpha, coef, comp, S, T, dS=None, dT=None):
+    B, n, _ = xp.shape
+    na, nloc = coef.shape
+    _call("hqc_overlap_kinetic", _ptr(xp, F64, "xp"), B, n, _ptr(lattice, F64, "lattice"),
+          lattice.shape[0], _ptr(alpha, F64), na, _ptr(coef, F64), _ptr(comp, I32), nloc,
+          _ptr(S, F64, "S"), _ptr(T, F64, "T"), _ptr(dS, F64, "dS"), _ptr(dT, F64, "dT"),
+          _stream())
+
+
+def ewald_pairs(rij, Gv, gk, kappa, g0, out):
+    B, npair, _ = rij.shape
+    _call("hqc_ewald_pairs", _ptr(rij, F64, "rij"), B, npair, _ptr(Gv, F64, "Gv"),
+          _ptr(gk, F64, "gk"), Gv.shape[0], float(kappa), float(g0), _ptr(out, F64, "out"),
+          _stream())
+
+
+def force_blocks(g):
+    return lib().hqc_force_blocks(g)
+
+
+def grid_forces(P, alpha, coef, comp, nao_pad, ld, X, v, Zx, dV, part):
+    B, n, _, g, na, nl = P.shape
+    nloc = coef.shape[1]
+    _call("hqc_grid_forces", _ptr(P, F64, "P"), B, n, g, na, nl, _ptr(alpha, F64),
+          _ptr(coef, F64), _ptr(comp, I32), nloc, nao_pad, ld, _ptr(X, F64, "X"),
+          _ptr(v, F64, "v"), _ptr(Zx, F64, "Zx"), float(dV), _ptr(part, F64, "part"), _stream())
+
+
+def vloc_force(xp, g, L, VG, rhoG, dE):
+    B, n, _ = xp.shape
+    _call("hqc_vloc_force", _ptr(xp, F64, "xp"), B, n, g, float(L), _ptr(VG, F64, "VG"),
+          _ptr(rhoG, None, "rhoG"), _ptr(dE, F64, "dE"), _stream())

@@ -1,0 +1,185 @@
+// Batched 3-D complex FFT for the odd (and even) grids of the GPW solver.
+//
+// One launch per axis.  A block stages `LPB` lines of length g in LDS
+// (position-major, line-minor so that strided axes load coalesced), runs the
+// mixed-radix Stockham passes between two LDS buffers, and writes the lines
+// back in place.  Radix butterflies are specialised for 2,3,4,5,7,11,13; any
+// other prime factor uses a direct per-output DFT pass.  Twiddles come from a
+// length-g table exp(sign 2 pi i m/g) built in LDS with sincospi.
+#include "common.h"
+
+#define FFT_MAX_G 128
+#define FFT_MAX_STAGES 12
+#define FFT_THREADS 256
+
+struct FftPlan {
+  int nst;
+  int rad[FFT_MAX_STAGES];
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Stockham pass with compile-time radix R (see the header comment).
+template <int R>
+__device__ void pass_r(const double2* __restrict__ in, double2* __restrict__ out,
+                       const double2* __restrict__ tw, int N, int Ns, int lpb) {
+  const int NR = N / R;
+  const int step = N / (Ns * R);
+  for (int it = threadIdx.x; it < NR * lpb; it += blockDim.x) {
+    const int line = it % lpb, j = it / lpb;
+    const int k = j % Ns;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = in[(j + r * NR) * lpb + line];
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[k * r * step]);
+    const int idx = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int kk = 0; kk < R; ++kk) {
+      double2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const double2 wr = tw[((r * kk) % R) * NR];
+        acc.x += v[r].x * wr.x - v[r].y * wr.y;
+        acc.y += v[r].x * wr.y + v[r].y * wr.x;
+      }
+      out[(idx + kk * Ns) * lpb + line] = acc;
+    }
+  }
+}
+
+// Generic radix: one work item per output element.
+__device__ void pass_generic(const double2* __restrict__ in, double2* __restrict__ out,
+                             const double2* __restrict__ tw, int N, int Ns, int R, int lpb) {
+  const int NR = N / R;
+  const int step = N / (Ns * R);
+  for (int it = threadIdx.x; it < N * lpb; it += blockDim.x) {
+    const int line = it % lpb;
+    const int q = it / lpb;  // q = j*R + kk
+    const int j = q / R, kk = q % R;
+    const int k = j % Ns;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int r = 0; r < R; ++r) {
+      double2 x = in[(j + r * NR) * lpb + line];
+      if (r) x = cmul(x, tw[k * r * step]);
+      const double2 w = tw[((r * kk) % R) * NR];
+      acc.x += x.x * w.x - x.y * w.y;
+      acc.y += x.x * w.y + x.y * w.x;
+    }
+    const int idx = (j / Ns) * Ns * R + k;
+    out[(idx + kk * Ns) * lpb + line] = acc;
+  }
+}
+
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(double2* __restrict__ data, int g,
+                                                          int axis, int lpb, FftPlan plan,
+                                                          int sign) {
+  extern __shared__ double2 sm[];
+  double2* bufA = sm;
+  double2* bufB = sm + g * lpb;
+  double2* tw = bufB + g * lpb;
+  const long gg = (long)g * g;
+  double2* base = data + (long)blockIdx.y * gg * g;
+  const int line0 = blockIdx.x * lpb;
+  for (int m = threadIdx.x; m < g; m += blockDim.x) {
+    double s, c;
+    sincospi((double)(2 * m) / (double)g, &s, &c);
+    tw[m] = make_double2(c, sign < 0 ? -s : s);
+  }
+  const long stride = axis == 2 ? 1 : (axis == 1 ? g : gg);
+  for (int e = threadIdx.x; e < g * lpb; e += blockDim.x) {
+    int p, line;
+    if (axis == 2) {
+      line = e / g;
+      p = e - line * g;
+    } else {
+      line = e % lpb;
+      p = e / lpb;
+    }
+    const long l = line0 + line;
+    double2 val = make_double2(0.0, 0.0);
+    if (l < gg) {
+      const long lb = axis == 2 ? l * g : (axis == 1 ? (l / g) * gg + (l % g) : l);
+      val = base[lb + p * stride];
+    }
+    bufA[p * lpb + line] = val;
+  }
+  __syncthreads();
+  double2* in = bufA;
+  double2* out = bufB;
+  int Ns = 1;
+  for (int s = 0; s < plan.nst; ++s) {
+    const int R = plan.rad[s];
+    switch (R) {
+      case 2: pass_r<2>(in, out, tw, g, Ns, lpb); break;
+      case 3: pass_r<3>(in, out, tw, g, Ns, lpb); break;
+      case 4: pass_r<4>(in, out, tw, g, Ns, lpb); break;
+      case 5: pass_r<5>(in, out, tw, g, Ns, lpb); break;
+      case 7: pass_r<7>(in, out, tw, g, Ns, lpb); break;
+      case 11: pass_r<11>(in, out, tw, g, Ns, lpb); break;
+      case 13: pass_r<13>(in, out, tw, g, Ns, lpb); break;
+      default: pass_generic(in, out, tw, g, Ns, R, lpb); break;
+    }
+    __syncthreads();
+    double2* t = in;
+    in = out;
+    out = t;
+    Ns *= R;
+  }
+  for (int e = threadIdx.x; e < g * lpb; e += blockDim.x) {
+    int p, line;
+    if (axis == 2) {
+      line = e / g;
+      p = e - line * g;
+    } else {
+      line = e % lpb;
+      p = e / lpb;
+    }
+    const long l = line0 + line;
+    if (l < gg) {
+      const long lb = axis == 2 ? l * g : (axis == 1 ? (l / g) * gg + (l % g) : l);
+      base[lb + p * stride] = in[p * lpb + line];
+    }
+  }
+}
+
+static int make_plan(int g, FftPlan& plan) {
+  plan.nst = 0;
+  int N = g;
+  const int pref[] = {4, 2, 3, 5, 7, 11, 13};
+  for (int r : pref) {
+    while (N % r == 0) {
+      if (plan.nst >= FFT_MAX_STAGES) return HQC_E_SIZE;
+      plan.rad[plan.nst++] = r;
+      N /= r;
+    }
+  }
+  for (int r = 17; N > 1; r += 2) {
+    while (N % r == 0) {
+      if (plan.nst >= FFT_MAX_STAGES) return HQC_E_SIZE;
+      plan.rad[plan.nst++] = r;
+      N /= r;
+    }
+  }
+  return HQC_OK;
+}
+
+extern "C" int hqc_fft3d_c2c_f64(double* data, int B, int g, int sign, void* stream) {
+  if (B < 0 || g <= 0 || (sign != 1 && sign != -1)) return HQC_E_ARG;
+  if (g > FFT_MAX_G) return HQC_E_SIZE;
+  if (B == 0 || g == 1) return HQC_OK;
+  FftPlan plan;
+  if (make_plan(g, plan) != HQC_OK) return HQC_E_SIZE;
+  const int lpb = 16;
+  const long lines = (long)g * g;
+  const size_t lds = (size_t)(2 * g * lpb + g) * sizeof(double2);
+  dim3 grid(hqc_div_up(lines, lpb), B);
+  for (int axis = 2; axis >= 0; --axis) {
+    hipLaunchKernelGGL(k_fft_axis, grid, dim3(FFT_THREADS), lds, (hipStream_t)stream,
+                       (double2*)data, g, axis, lpb, plan, sign);
+    HQC_CHECK_LAUNCH();
+  }
+  return HQC_OK;
+}

@@ -1,0 +1,412 @@
+// Pointwise grid kernels: local potential in G space, Poisson multiply,
+// exchange-correlation, structure-factor force, plus the small per-geometry
+// kernels (occupations / chemical potential, Ewald pair sums).
+#include "common.h"
+
+// G vector component for FFT index k (FFT order), formed as in
+// hqc/pbc/solver.py:195: (k*2)*pi/L.
+__device__ __forceinline__ double gcomp(int k, int g, double L) {
+  const int kf = (k <= g / 2) ? k : k - g;
+  return (double)kf * 2.0 * M_PI / L;
+}
+
+// ---------------------------------------------------------------------------
+// v_loc(G) = -VG sum_A exp(-i G.R_A)     (hqc/pbc/solver.py:406-407)
+// ---------------------------------------------------------------------------
+__global__ void k_vloc_G(const double* __restrict__ xp, int n, int g, double L,
+                         const double* __restrict__ VG, double2* __restrict__ out) {
+  const int N = g * g * g;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (r >= N) return;
+  const int kz = r % g, ky = (r / g) % g, kx = r / (g * g);
+  const double Gx = gcomp(kx, g, L), Gy = gcomp(ky, g, L), Gz = gcomp(kz, g, L);
+  const double* x = xp + (long)b * n * 3;
+  double sr = 0.0, si = 0.0;
+  for (int A = 0; A < n; ++A) {
+    const double th = Gx * x[3 * A] + Gy * x[3 * A + 1] + Gz * x[3 * A + 2];
+    double s, c;
+    sincos(th, &s, &c);
+    sr += c;
+    si -= s;
+  }
+  const double vg = VG[r];
+  out[(long)b * N + r] = make_double2(-sr * vg, -si * vg);
+}
+
+extern "C" int hqc_vloc_G(const double* xp, int B, int n, int g, double L, const double* VG,
+                          double* out, void* stream) {
+  if (B < 0 || n <= 0 || g <= 0) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  const int N = g * g * g;
+  hipLaunchKernelGGL(k_vloc_G, dim3(hqc_div_up(N, 256), B), dim3(256), 0, (hipStream_t)stream,
+                     xp, n, g, L, VG, (double2*)out);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversions and the Poisson multiply.
+// ---------------------------------------------------------------------------
+__global__ void k_r2c(const double* __restrict__ in, int N, int ld, double s,
+                      double2* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (r >= N) return;
+  out[(long)b * N + r] = make_double2(s * in[(long)b * ld + r], 0.0);
+}
+
+__global__ void k_c2r(const double2* __restrict__ in, int N, int ld, double* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (r >= N) return;
+  out[(long)b * ld + r] = in[(long)b * N + r].x;
+}
+
+__global__ void k_poisson_mul(double2* __restrict__ c, const double* __restrict__ VG, int N,
+                              double2* __restrict__ saved) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (r >= N) return;
+  const long i = (long)b * N + r;
+  double2 v = c[i];
+  if (saved) saved[i] = v;
+  const double w = VG[r];
+  c[i] = make_double2(v.x * w, v.y * w);
+}
+
+extern "C" int hqc_real_to_complex(const double* in, int B, int g, int ld, double s, double* out,
+                                   void* stream) {
+  if (B < 0 || g <= 0 || ld < g * g * g) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  const int N = g * g * g;
+  hipLaunchKernelGGL(k_r2c, dim3(hqc_div_up(N, 256), B), dim3(256), 0, (hipStream_t)stream, in, N,
+                     ld, s, (double2*)out);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+extern "C" int hqc_complex_real(const double* in, int B, int g, int ld, double* out,
+                                void* stream) {
+  if (B < 0 || g <= 0 || ld < g * g * g) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  const int N = g * g * g;
+  hipLaunchKernelGGL(k_c2r, dim3(hqc_div_up(N, 256), B), dim3(256), 0, (hipStream_t)stream,
+                     (const double2*)in, N, ld, out);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+extern "C" int hqc_poisson_mul(double* c, const double* VG, int B, int g, double* saved,
+                               void* stream) {
+  if (B < 0 || g <= 0) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  const int N = g * g * g;
+  hipLaunchKernelGGL(k_poisson_mul, dim3(hqc_div_up(N, 256), B), dim3(256), 0,
+                     (hipStream_t)stream, (double2*)c, VG, N, (double2*)saved);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Exchange-correlation (hqc/tools/excor.py) with closed-form derivatives:
+//   v_xc = eps + rho d eps/d rho   (the reference takes this with jax.grad,
+//   hqc/pbc/solver.py:673).
+// ---------------------------------------------------------------------------
+#define XC_BLOCKS 256
+
+__device__ __forceinline__ void xc_point(double rho, int xc_id, double& eps, double& v) {
+  const double r13 = cbrt(rho);
+  const double ex = -0.75 * cbrt(3.0 / M_PI) * r13;   // -3/4 (3/pi)^(1/3) rho^(1/3)
+  eps = ex;
+  v = 4.0 / 3.0 * ex;
+  if (xc_id == 2) return;
+  const double ws = cbrt(3.0 / 4.0 / M_PI);
+  const double rs = ws / r13;
+  if (xc_id == 1) {
+    // VWN parameters of hqc/tools/excor.py:31-43
+    const double A = 0.0621814, y0 = -0.10498, bb = 3.72744, cc = 12.93532;
+    const double Yy0 = y0 * y0 + bb * y0 + cc;
+    const double Q = sqrt(4.0 * cc - bb * bb);
+    const double c0 = A / 2.0;
+    const double c1 = c0 * 2.0 * bb / Q;
+    const double c2 = c0 * bb * y0 / Yy0;
+    const double c3 = c2 * 2.0 * (bb + 2.0 * y0) / Q;
+    const double c4 = c1 - c3;
+    const double y = sqrt(rs);
+    const double Y = y * y + bb * y + cc;
+    const double ec = c0 * log(y * y / Y) + c4 * atan(Q / (2.0 * y + bb)) -
+                      c2 * log((y - y0) * (y - y0) / Y);
+    const double dY = 2.0 * y + bb;
+    const double dec = c0 * (2.0 / y - dY / Y) - c4 * Q / (2.0 * Y) - c2 * (2.0 / (y - y0) - dY / Y);
+    eps += ec;
+    v += ec - y / 6.0 * dec;
+    return;
+  }
+  double Ac, Cc;
+  if (xc_id == 3) {
+    Ac = 21.0; Cc = 0.045;        // HL71
+  } else if (xc_id == 4) {
+    Ac = 30.0; Cc = 0.0504;       // BH72
+  } else {
+    Ac = 11.4; Cc = 0.0666;       // GL77
+  }
+  const double x = rs / Ac;
+  const double lg = log(1.0 + 1.0 / x);
+  const double ec = -0.5 * Cc * ((1.0 + x * x * x) * lg + x / 2.0 - x * x - 1.0 / 3.0);
+  const double dec = -0.5 * Cc * (3.0 * x * x * lg - 1.0 / x + 1.5 - 3.0 * x);
+  eps += ec;
+  v += ec - x / 3.0 * dec;
+}
+
+__global__ __launch_bounds__(256) void k_xc_eval(const double* __restrict__ rho,
+                                                 const double2* __restrict__ vh, int N, int ld,
+                                                 int xc_id, double* __restrict__ vout,
+                                                 double* __restrict__ part) {
+  __shared__ double red[4];
+  const int b = blockIdx.y;
+  double se = 0.0, sh = 0.0;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
+    const double p = rho[(long)b * ld + r];
+    const double h = vh[(long)b * N + r].x;
+    double eps = 0.0, v = 0.0;
+    if (xc_id != 0) xc_point(p, xc_id, eps, v);
+    vout[(long)b * ld + r] = h + v;
+    se += p * eps;
+    sh += p * h;
+  }
+  se = block_sum(se, red);
+  sh = block_sum(sh, red);
+  if (threadIdx.x == 0) {
+    part[((long)b * gridDim.x + blockIdx.x) * 2] = se;
+    part[((long)b * gridDim.x + blockIdx.x) * 2 + 1] = sh;
+  }
+}
+
+extern "C" int hqc_xc_blocks(int g) {
+  (void)g;
+  return XC_BLOCKS;
+}
+
+extern "C" int hqc_xc_eval(const double* rho, const double* vh, int B, int g, int ld, int xc_id,
+                           double* vout, double* part, void* stream) {
+  if (B < 0 || g <= 0 || ld < g * g * g || xc_id < 0 || xc_id > 5) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  hipLaunchKernelGGL(k_xc_eval, dim3(XC_BLOCKS, B), dim3(256), 0, (hipStream_t)stream, rho,
+                     (const double2*)vh, g * g * g, ld, xc_id, vout, part);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// dE/dR_A of the local-potential energy through the structure factor:
+//   dE[b,A,c] = Re sum_G i G_c VG e^{-iG.R_A} conj(rho~(G))
+//             = sum_G G_c VG (a sin th + b cos th),  rho~ = a + i b, th = G.R_A
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_vloc_force(const double* __restrict__ xp, int n, int g,
+                                                    double L, const double* __restrict__ VG,
+                                                    const double2* __restrict__ rg,
+                                                    double* __restrict__ dE) {
+  __shared__ double red[4];
+  const int A = blockIdx.x, b = blockIdx.y;
+  const int N = g * g * g;
+  const double* x = xp + ((long)b * n + A) * 3;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  for (int r = threadIdx.x; r < N; r += blockDim.x) {
+    const int kz = r % g, ky = (r / g) % g, kx = r / (g * g);
+    const double Gx = gcomp(kx, g, L), Gy = gcomp(ky, g, L), Gz = gcomp(kz, g, L);
+    const double th = Gx * x[0] + Gy * x[1] + Gz * x[2];
+    double s, c;
+    sincos(th, &s, &c);
+    const double2 z = rg[(long)b * N + r];
+    const double t = VG[r] * (z.x * s + z.y * c);
+    f0 += Gx * t;
+    f1 += Gy * t;
+    f2 += Gz * t;
+  }
+  f0 = block_sum(f0, red);
+  f1 = block_sum(f1, red);
+  f2 = block_sum(f2, red);
+  if (threadIdx.x == 0) {
+    double* o = dE + ((long)b * n + A) * 3;
+    o[0] = f0;
+    o[1] = f1;
+    o[2] = f2;
+  }
+}
+
+extern "C" int hqc_vloc_force(const double* xp, int B, int n, int g, double L, const double* VG,
+                              const double* rhoG, double* dE, void* stream) {
+  if (B < 0 || n <= 0 || g <= 0) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  hipLaunchKernelGGL(k_vloc_force, dim3(n, B), dim3(256), 0, (hipStream_t)stream, xp, n, g, L,
+                     VG, (const double2*)rhoG, dE);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Occupations (hqc/tools/smearing.py).  One 64-lane wavefront per geometry;
+// every lane runs the same scalar control flow on wave-wide sums.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double occ_f(double mu, double w, double sigma, int method) {
+  if (method == 0) {
+    double x = (w - mu) / sigma;
+    x = fmin(fmax(x, -50.0), 50.0);
+    return 2.0 / (exp(x) + 1.0);
+  }
+  return 0.5 * (1.0 + erf((mu - w) / sigma));
+}
+
+__device__ double occ_N(const double* w, int nmo, double mu, double sigma, int method, int nelec) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int i = lane; i < nmo; i += 64) s += occ_f(mu, w[i], sigma, method);
+  return wave_sum(s) - (double)nelec;
+}
+
+__device__ void occ_dN(const double* w, int nmo, double mu, double sigma, int method, double& d1,
+                       double& d2) {
+  const int lane = threadIdx.x & 63;
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = lane; i < nmo; i += 64) {
+    if (method == 0) {
+      const double x = (w[i] - mu) / sigma;
+      if (fabs(x) < 50.0) {
+        const double ex = exp(x);
+        const double q = ex + 1.0;
+        s1 += 2.0 * ex / (q * q) / sigma;
+        s2 += -2.0 * ex * (1.0 - ex) / (q * q * q) / (sigma * sigma);
+      }
+    } else {
+      const double t = (mu - w[i]) / sigma;
+      const double e = exp(-t * t) / sqrt(M_PI);
+      s1 += e / sigma;
+      s2 += -2.0 * t * e / (sigma * sigma);
+    }
+  }
+  d1 = wave_sum(s1);
+  d2 = wave_sum(s2);
+}
+
+__global__ void k_occupations(const double* __restrict__ W, int nmo, int nelec, double sigma,
+                              int method, int search, int max_cycle, double tol, double mu_shift,
+                              double* __restrict__ occ, double* __restrict__ mu_out) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const double* w = W + (long)b * nmo;
+  double mu;
+  if (search == 0) {
+    double mn = 1e308, mx = -1e308;
+    for (int i = lane; i < nmo; i += 64) {
+      mn = fmin(mn, w[i]);
+      mx = fmax(mx, w[i]);
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      mn = fmin(mn, __shfl_xor(mn, off, 64));
+      mx = fmax(mx, __shfl_xor(mx, off, 64));
+    }
+    double lo = mn + mu_shift, hi = mx + mu_shift;
+    double ylo = -1.0, yhi = 1.0;
+    int loop = 0;
+    while (ylo < -tol && yhi > tol && loop < max_cycle) {
+      const double mid = (lo + hi) / 2.0;
+      const double ym = occ_N(w, nmo, mid, sigma, method, nelec);
+      // smearing.py:88-89, written with the same 0/1 weights
+      const double nlo = (double)(ym < 0.0) * mid + (double)(ym >= 0.0) * lo;
+      const double nhi = (double)(ym >= 0.0) * mid + (double)(ym < 0.0) * hi;
+      lo = nlo;
+      hi = nhi;
+      ylo = occ_N(w, nmo, lo, sigma, method, nelec);
+      yhi = occ_N(w, nmo, hi, sigma, method, nelec);
+      ++loop;
+    }
+    // smearing.py:102 return rule, kept verbatim
+    mu = (double)(lo > -tol) * lo + (double)(hi < tol) * hi +
+         (lo + hi) / 2.0 * (double)(loop == max_cycle);
+  } else {
+    double mu_old = 1.0;
+    mu = w[nelec / 2];
+    int loop = 0;
+    while (fabs(mu - mu_old) > tol && loop < max_cycle) {
+      const double y = occ_N(w, nmo, mu, sigma, method, nelec);
+      double d1, d2;
+      occ_dN(w, nmo, mu, sigma, method, d1, d2);
+      const double g1 = 2.0 * y * d1;
+      const double g2 = 2.0 * (d1 * d1 + y * d2);
+      mu_old = mu;
+      mu = mu - g1 / fabs(g2);
+      ++loop;
+    }
+  }
+  for (int i = lane; i < nmo; i += 64) occ[(long)b * nmo + i] = occ_f(mu, w[i], sigma, method);
+  if (lane == 0 && mu_out) mu_out[b] = mu;
+}
+
+extern "C" int hqc_occupations(const double* w, int B, int nmo, int nelec, double sigma,
+                               int method, int search, int max_cycle, double tol, double mu_shift,
+                               double* occ, double* mu, void* stream) {
+  if (B < 0 || nmo <= 0 || nelec < 0 || !(sigma > 0.0) || method < 0 || method > 1 ||
+      search < 0 || search > 1 || max_cycle < 0)
+    return HQC_E_ARG;
+  if (search == 1 && nelec / 2 >= nmo) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  hipLaunchKernelGGL(k_occupations, dim3(B), dim3(64), 0, (hipStream_t)stream, w, nmo, nelec,
+                     sigma, method, search, max_cycle, tol, mu_shift, occ, mu);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ewald pair potential psi(r) and its gradient (hqc/pbc/potential.py:35-67).
+// One block per (pair, geometry); the G sum is a block reduction.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ewald_pairs(const double* __restrict__ rij, int npair,
+                                                     const double* __restrict__ Gv,
+                                                     const double* __restrict__ gk, int nG,
+                                                     double kappa, double g0,
+                                                     double* __restrict__ out) {
+  __shared__ double red[4];
+  const int p = blockIdx.x, b = blockIdx.y;
+  const double* r = rij + ((long)b * npair + p) * 3;
+  const double rx = r[0], ry = r[1], rz = r[2];
+  double sc = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int i = threadIdx.x; i < nG; i += blockDim.x) {
+    const double G0 = Gv[3 * i], G1 = Gv[3 * i + 1], G2 = Gv[3 * i + 2];
+    double s, c;
+    sincos(2.0 * M_PI * (G0 * rx + G1 * ry + G2 * rz), &s, &c);
+    const double w = gk[i];
+    sc += w * c;
+    const double t = -2.0 * M_PI * w * s;
+    gx += t * G0;
+    gy += t * G1;
+    gz += t * G2;
+  }
+  sc = block_sum(sc, red);
+  gx = block_sum(gx, red);
+  gy = block_sum(gy, red);
+  gz = block_sum(gz, red);
+  if (threadIdx.x == 0) {
+    const double d = sqrt(rx * rx + ry * ry + rz * rz);
+    const double er = erfc(kappa * d);
+    const double dsr = -(er / (d * d) + 2.0 * kappa / sqrt(M_PI) * exp(-kappa * kappa * d * d) / d);
+    double* o = out + ((long)b * npair + p) * 4;
+    o[0] = er / d + sc + g0;
+    o[1] = dsr * rx / d + gx;
+    o[2] = dsr * ry / d + gy;
+    o[3] = dsr * rz / d + gz;
+  }
+}
+
+extern "C" int hqc_ewald_pairs(const double* rij, int B, int npair, const double* Gv,
+                               const double* gk, int nG, double kappa, double g0, double* out,
+                               void* stream) {
+  if (B < 0 || npair < 0 || nG <= 0) return HQC_E_ARG;
+  if (B == 0 || npair == 0) return HQC_OK;
+  hipLaunchKernelGGL(k_ewald_pairs, dim3(npair, B), dim3(256), 0, (hipStream_t)stream, rij, npair,
+                     Gv, gk, nG, kappa, g0, out);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+extern "C" int hqc_version(void) { return 100; }

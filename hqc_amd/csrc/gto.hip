@@ -1,0 +1,211 @@
+// Periodic GTO evaluation on the real-space grid: 1-D image-summed tables,
+// separable AO values, and the grid adjoint used for forces.
+//
+// Reference: hqc/pbc/gto.py:326-369 (tables and AO values),
+// hqc/pbc/solver.py:209-219 (how the solver applies them to the grid).
+#include "common.h"
+
+#define HQC_MAX_NL 4      // powers 0..3 (l_max = 1 plus two extra for gradients)
+#define HQC_MAX_NA 16     // primitives per atom
+#define HQC_MAX_NLOC 16   // AOs per atom
+
+// ---------------------------------------------------------------------------
+// 1-D tables: one thread per (b, A, d, j, a).
+// ---------------------------------------------------------------------------
+__global__ void k_gto_tables(const double* __restrict__ xp, long total, int n, int g,
+                             double L, int tmax, const double* __restrict__ alpha, int na,
+                             int nl, double* __restrict__ P) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int a = (int)(idx % na);
+  long t1 = idx / na;
+  const int j = (int)(t1 % g);
+  t1 /= g;
+  const int d = (int)(t1 % 3);
+  const long bA = t1 / 3;  // b*n + A
+  const double x = (double)j * L / (double)g - xp[bA * 3 + d];
+  const double al = alpha[a];
+  double acc[HQC_MAX_NL];
+#pragma unroll
+  for (int l = 0; l < HQC_MAX_NL; ++l) acc[l] = 0.0;
+  for (int t = -tmax; t <= tmax; ++t) {
+    const double xt = x - (double)t * L;
+    double p = exp(-al * xt * xt);
+#pragma unroll
+    for (int l = 0; l < HQC_MAX_NL; ++l) {
+      if (l < nl) {
+        acc[l] += p;
+        p *= xt;
+      }
+    }
+  }
+  double* out = P + idx * nl;
+  for (int l = 0; l < nl; ++l) out[l] = acc[l];
+}
+
+extern "C" int hqc_gto_tables(const double* xp, int B, int n, int g, double L, int tmax,
+                              const double* alpha, int na, int nl, double* P, void* stream) {
+  if (B < 0 || n <= 0 || g <= 0 || na <= 0 || nl <= 0 || tmax < 0) return HQC_E_ARG;
+  if (nl > HQC_MAX_NL || na > HQC_MAX_NA) return HQC_E_SIZE;
+  if (B == 0) return HQC_OK;
+  const long total = (long)B * n * 3 * g * na;
+  hipLaunchKernelGGL(k_gto_tables, dim3(hqc_div_up(total, 256)), dim3(256), 0,
+                     (hipStream_t)stream, xp, total, n, g, L, tmax, alpha, na, nl, P);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// AO values: one thread per (grid point r) for one atom A of geometry b.
+// Coefficients and component codes are staged in LDS.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_ao_values(const double* __restrict__ P, int n, int g,
+                                                   int na, int nl,
+                                                   const double* __restrict__ coef,
+                                                   const int* __restrict__ comp, int nloc,
+                                                   int nao_pad, int ld, T* __restrict__ phi) {
+  __shared__ double s_coef[HQC_MAX_NA * HQC_MAX_NLOC];
+  __shared__ int s_comp[HQC_MAX_NLOC];
+  for (int i = threadIdx.x; i < na * nloc; i += blockDim.x) s_coef[i] = coef[i];
+  for (int i = threadIdx.x; i < nloc; i += blockDim.x) s_comp[i] = comp[i];
+  __syncthreads();
+  const int A = blockIdx.y, b = blockIdx.z;
+  const int N = g * g * g;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  const int iz = r % g, iy = (r / g) % g, ix = r / (g * g);
+  const long tab = (long)(b * n + A) * 3 * g * na * nl;
+  const double* Px = P + tab + (long)(0 * g + ix) * na * nl;
+  const double* Py = P + tab + (long)(1 * g + iy) * na * nl;
+  const double* Pz = P + tab + (long)(2 * g + iz) * na * nl;
+  T* out = phi + ((long)b * nao_pad + (long)A * nloc) * ld + r;
+  for (int o = 0; o < nloc; ++o) {
+    const int c = s_comp[o];
+    const int px = (c == 0), py = (c == 1), pz = (c == 2);
+    double acc = 0.0;
+    for (int a = 0; a < na; ++a) {
+      const double w = s_coef[a * nloc + o];
+      if (w != 0.0) acc += w * Px[a * nl + px] * Py[a * nl + py] * Pz[a * nl + pz];
+    }
+    out[(long)o * ld] = (T)acc;
+  }
+}
+
+template <typename T>
+static int launch_ao_values(const double* P, int B, int n, int g, int na, int nl,
+                            const double* coef, const int* comp, int nloc, int nao_pad, int ld,
+                            T* phi, void* stream) {
+  if (B < 0 || n <= 0 || g <= 0 || na <= 0 || nl <= 0 || nloc <= 0) return HQC_E_ARG;
+  if (na > HQC_MAX_NA || nloc > HQC_MAX_NLOC) return HQC_E_SIZE;
+  if (nao_pad < n * nloc || ld < g * g * g) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  const int N = g * g * g;
+  hipLaunchKernelGGL(k_ao_values<T>, dim3(hqc_div_up(N, 256), n, B), dim3(256), 0,
+                     (hipStream_t)stream, P, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+extern "C" int hqc_ao_values_f64(const double* P, int B, int n, int g, int na, int nl,
+                                 const double* coef, const int* comp, int nloc, int nao_pad,
+                                 int ld, double* phi, void* stream) {
+  return launch_ao_values<double>(P, B, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi,
+                                  stream);
+}
+
+extern "C" int hqc_ao_values_f32(const double* P, int B, int n, int g, int na, int nl,
+                                 const double* coef, const int* comp, int nloc, int nao_pad,
+                                 int ld, float* phi, void* stream) {
+  return launch_ao_values<float>(P, B, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi,
+                                 stream);
+}
+
+// ---------------------------------------------------------------------------
+// Grid adjoint for forces.  For atom A of geometry b, each thread takes grid
+// points r (grid-stride), recomputes grad phi_p(r) for the AOs of A from the
+// tables (d/dx [x^l e^{-a x^2}] = l x^{l-1} e - 2a x^{l+1} e, image-summed),
+// and accumulates Z_p(r) grad phi_p(r) with Z = 2 v X (+ Zx).  Each block
+// writes its three partial sums; the host adds them in a fixed order.
+// ---------------------------------------------------------------------------
+#define HQC_FORCE_BLOCKS 64
+
+__global__ __launch_bounds__(256) void k_grid_forces(
+    const double* __restrict__ P, int n, int g, int na, int nl,
+    const double* __restrict__ alpha, const double* __restrict__ coef,
+    const int* __restrict__ comp, int nloc, int nao_pad, int ld,
+    const double* __restrict__ X, const double* __restrict__ v,
+    const double* __restrict__ Zx, double dV, double* __restrict__ part) {
+  __shared__ double s_coef[HQC_MAX_NA * HQC_MAX_NLOC];
+  __shared__ double s_alpha[HQC_MAX_NA];
+  __shared__ int s_comp[HQC_MAX_NLOC];
+  __shared__ double red[4];
+  for (int i = threadIdx.x; i < na * nloc; i += blockDim.x) s_coef[i] = coef[i];
+  for (int i = threadIdx.x; i < na; i += blockDim.x) s_alpha[i] = alpha[i];
+  for (int i = threadIdx.x; i < nloc; i += blockDim.x) s_comp[i] = comp[i];
+  __syncthreads();
+  const int A = blockIdx.y, b = blockIdx.z;
+  const int N = g * g * g;
+  const long tab = (long)(b * n + A) * 3 * g * na * nl;
+  double f[3] = {0.0, 0.0, 0.0};
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
+    const int iz = r % g, iy = (r / g) % g, ix = r / (g * g);
+    const double* Pd[3] = {P + tab + (long)(0 * g + ix) * na * nl,
+                           P + tab + (long)(1 * g + iy) * na * nl,
+                           P + tab + (long)(2 * g + iz) * na * nl};
+    const double vr = v ? 2.0 * v[(long)b * ld + r] : 0.0;
+    for (int o = 0; o < nloc; ++o) {
+      const long row = (long)b * nao_pad + (long)A * nloc + o;
+      double z = 0.0;
+      if (v) z = vr * X[row * ld + r];
+      if (Zx) z += Zx[row * ld + r];
+      const int c = s_comp[o];
+      double gr[3] = {0.0, 0.0, 0.0};
+      for (int a = 0; a < na; ++a) {
+        const double w = s_coef[a * nloc + o];
+        if (w == 0.0) continue;
+        const double m2a = -2.0 * s_alpha[a];
+        double val[3], der[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int pw = (c == d) ? 1 : 0;
+          const double* t = Pd[d] + a * nl;
+          val[d] = t[pw];
+          der[d] = m2a * t[pw + 1] + (pw ? t[0] : 0.0);
+        }
+        gr[0] += w * der[0] * val[1] * val[2];
+        gr[1] += w * val[0] * der[1] * val[2];
+        gr[2] += w * val[0] * val[1] * der[2];
+      }
+      f[0] += z * gr[0];
+      f[1] += z * gr[1];
+      f[2] += z * gr[2];
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    const double t = block_sum(f[c], red);
+    if (threadIdx.x == 0)
+      part[(((long)b * n + A) * gridDim.x + blockIdx.x) * 3 + c] = t * dV;
+  }
+}
+
+extern "C" int hqc_force_blocks(int g) {
+  (void)g;
+  return HQC_FORCE_BLOCKS;
+}
+
+extern "C" int hqc_grid_forces(const double* P, int B, int n, int g, int na, int nl,
+                               const double* alpha, const double* coef, const int* comp,
+                               int nloc, int nao_pad, int ld, const double* X, const double* v,
+                               const double* Zx, double dV, double* part, void* stream) {
+  if (B < 0 || n <= 0 || g <= 0 || na <= 0 || nloc <= 0) return HQC_E_ARG;
+  if (na > HQC_MAX_NA || nloc > HQC_MAX_NLOC) return HQC_E_SIZE;
+  if (nl < 3) return HQC_E_ARG;   // gradients need l_max + 2 powers (>= 2 for s)
+  if (v && !X) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  hipLaunchKernelGGL(k_grid_forces, dim3(HQC_FORCE_BLOCKS, n, B), dim3(256), 0,
+                     (hipStream_t)stream, P, n, g, na, nl, alpha, coef, comp, nloc, nao_pad,
+                     ld, X, v, Zx, dV, part);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}

@@ -1,0 +1,126 @@
+// Analytic periodic overlap and kinetic matrices (and their first-argument
+// derivatives for forces) by lattice sums.
+//
+// Reference: hqc/pbc/solver.py:233-389.  For primitive exponents a (on atom
+// A) and b (on atom B) with mu = ab/(a+b) and R = R_A - R_B + T, the s-s
+// overlap is pi^1.5 (a+b)^-1.5 exp(-mu R^2) and the kinetic integral is
+// -1/2 Laplacian_R of it (= mu(3 - 2 mu R^2) S, solver.py:252-254).  A p_i
+// function on A is (1/2a) d/dA_i of the s function and a p_j on B is
+// (1/2b) d/dB_j (solver.py:307-341); since d/dA = d/dR and d/dB = -d/dR,
+// every needed integral is a mixed R-derivative of exp(-mu R^2), i.e. a
+// product over x,y,z of Hermite factors e_k(R_d):
+//   d^k/dR^k exp(-mu R^2) = e_k(R) exp(-mu R^2),
+//   e_0 = 1, e_1 = -2 mu R, e_{k+1} = -2 mu R e_k - 2 k mu e_{k-1}.
+#include "common.h"
+
+#define INT_MAX_ORDER 5
+#define INT_MAX_NA 16
+#define INT_MAX_NLOC 16
+
+__device__ __forceinline__ void hermite(double mu, double R, double* e) {
+  e[0] = 1.0;
+  e[1] = -2.0 * mu * R;
+#pragma unroll
+  for (int k = 1; k < INT_MAX_ORDER; ++k) e[k + 1] = -2.0 * mu * R * e[k] - 2.0 * k * mu * e[k - 1];
+}
+
+// One thread per (b, A, Bat, o, q).
+__global__ __launch_bounds__(256) void k_overlap_kinetic(
+    const double* __restrict__ xp, long total, int n, const double* __restrict__ lat, int nT,
+    const double* __restrict__ alpha, int na, const double* __restrict__ coef,
+    const int* __restrict__ comp, int nloc, double* __restrict__ S, double* __restrict__ Tm,
+    double* __restrict__ dS, double* __restrict__ dT) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int q = (int)(idx % nloc);
+  long t1 = idx / nloc;
+  const int o = (int)(t1 % nloc);
+  t1 /= nloc;
+  const int Bat = (int)(t1 % n);
+  t1 /= n;
+  const int A = (int)(t1 % n);
+  const int b = (int)(t1 / n);
+  const double* xA = xp + ((long)b * n + A) * 3;
+  const double* xB = xp + ((long)b * n + Bat) * 3;
+  const double d0 = xA[0] - xB[0], d1 = xA[1] - xB[1], d2 = xA[2] - xB[2];
+  const int co = comp[o], cq = comp[q];
+  int base[3] = {0, 0, 0};
+  if (co >= 0) base[co] += 1;
+  if (cq >= 0) base[cq] += 1;
+  const bool deriv = dS != nullptr;
+  double s = 0.0, t = 0.0;
+  double ds[3] = {0.0, 0.0, 0.0}, dt[3] = {0.0, 0.0, 0.0};
+  for (int ia = 0; ia < na; ++ia) {
+    const double wa = coef[ia * nloc + o];
+    if (wa == 0.0) continue;
+    for (int ib = 0; ib < na; ++ib) {
+      const double wb = coef[ib * nloc + q];
+      if (wb == 0.0) continue;
+      const double a = alpha[ia], bb = alpha[ib];
+      const double mu = a * bb / (a + bb);
+      double fac = wa * wb * M_PI * sqrt(M_PI) * pow(a + bb, -1.5);
+      if (co >= 0) fac *= 1.0 / (2.0 * a);
+      if (cq >= 0) fac *= -1.0 / (2.0 * bb);
+      double ss = 0.0, tt = 0.0;
+      double dss[3] = {0.0, 0.0, 0.0}, dtt[3] = {0.0, 0.0, 0.0};
+      for (int it = 0; it < nT; ++it) {
+        const double R[3] = {d0 + lat[3 * it], d1 + lat[3 * it + 1], d2 + lat[3 * it + 2]};
+        const double E = exp(-mu * (R[0] * R[0] + R[1] * R[1] + R[2] * R[2]));
+        double h[3][INT_MAX_ORDER + 1];
+        hermite(mu, R[0], h[0]);
+        hermite(mu, R[1], h[1]);
+        hermite(mu, R[2], h[2]);
+        const double hx = h[0][base[0]], hy = h[1][base[1]], hz = h[2][base[2]];
+        ss += E * hx * hy * hz;
+        tt += E * (h[0][base[0] + 2] * hy * hz + hx * h[1][base[1] + 2] * hz +
+                   hx * hy * h[2][base[2] + 2]);
+        if (deriv) {
+          const double gx = h[0][base[0] + 1], gy = h[1][base[1] + 1], gz = h[2][base[2] + 1];
+          dss[0] += E * gx * hy * hz;
+          dss[1] += E * hx * gy * hz;
+          dss[2] += E * hx * hy * gz;
+          dtt[0] += E * (h[0][base[0] + 3] * hy * hz + gx * h[1][base[1] + 2] * hz +
+                         gx * hy * h[2][base[2] + 2]);
+          dtt[1] += E * (h[0][base[0] + 2] * gy * hz + hx * h[1][base[1] + 3] * hz +
+                         hx * gy * h[2][base[2] + 2]);
+          dtt[2] += E * (h[0][base[0] + 2] * hy * gz + hx * h[1][base[1] + 2] * gz +
+                         hx * hy * h[2][base[2] + 3]);
+        }
+      }
+      s += fac * ss;
+      t += -0.5 * fac * tt;
+      if (deriv) {
+        for (int c = 0; c < 3; ++c) {
+          ds[c] += fac * dss[c];
+          dt[c] += -0.5 * fac * dtt[c];
+        }
+      }
+    }
+  }
+  const long nao = (long)n * nloc;
+  const long e = ((long)b * nao + (long)A * nloc + o) * nao + (long)Bat * nloc + q;
+  S[e] = s;
+  Tm[e] = t;
+  if (deriv) {
+    for (int c = 0; c < 3; ++c) {
+      dS[e * 3 + c] = ds[c];
+      dT[e * 3 + c] = dt[c];
+    }
+  }
+}
+
+extern "C" int hqc_overlap_kinetic(const double* xp, int B, int n, const double* lattice, int nT,
+                                   const double* alpha, int na, const double* coef,
+                                   const int* comp, int nloc, double* S, double* T, double* dS,
+                                   double* dT, void* stream) {
+  if (B < 0 || n <= 0 || nT <= 0 || na <= 0 || nloc <= 0) return HQC_E_ARG;
+  if ((dS == nullptr) != (dT == nullptr)) return HQC_E_ARG;
+  if (na > INT_MAX_NA || nloc > INT_MAX_NLOC) return HQC_E_SIZE;
+  if (B == 0) return HQC_OK;
+  const long total = (long)B * n * n * nloc * nloc;
+  hipLaunchKernelGGL(k_overlap_kinetic, dim3(hqc_div_up(total, 256)), dim3(256), 0,
+                     (hipStream_t)stream, xp, total, n, lattice, nT, alpha, na, coef, comp, nloc,
+                     S, T, dS, dT);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}

@@ -1,0 +1,111 @@
+"""Forces by a hand-written adjoint of the grid contractions (new: the
+reference has none, doc/solver_algorithm.md:1210).
+
+The differentiated functional is the Mermin free energy
+    A = E_elec - sigma * S_e + E_pp            (Hartree; sigma = 0 without smearing)
+which is stationary in the orbitals and occupations at SCF convergence, so
+    -dA/dR_A = -Tr[dT/dR_A D] + Tr[W dS/dR_A]                  kinetic + Pulay
+               + dV sum_r sum_{p in A} Z_p(r) grad phi_p(r)      grid adjoint
+               - d/dR_A (structure factor of v_loc) . rho~       Hellmann-Feynman
+               - dE_pp/dR_A                                      Ewald
+with W = C diag(f eps) C^T and Z = dE/dphi / dV: 2 v_eff D phi for the local
+terms, minus the exchange pair terms for HF.  grad phi is recomputed from the
+1-D tables inside the kernel (never stored).
+"""
+import math
+
+import torch
+
+from .. import _lib as K
+from .engine import EXCHANGE_G0_CONST
+from .potential import ewald_pp
+
+
+def _exchange_adjoint(engine, st):
+    """Zx = -Y - c0 (D S~ D) phi for HF, (B, nao_pad, ld)."""
+    c = engine.c
+    B = st["B"]
+    phi = st["phi"][:, :c.nao, :c.N]
+    mo, occ = st["mo"], st["occ"]
+    Zx = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float64, device=c.device)
+    g = c.g
+    VG = c.VG.reshape(1, g, g, g)
+    c0 = 4 * math.pi / c.L * EXCHANGE_G0_CONST
+    for b in range(B):
+        keep = torch.nonzero(occ[b] > 1e-14).flatten()
+        C = mo[b][:, keep]
+        f = occ[b][keep]
+        psi = C.transpose(0, 1) @ phi[b]                       # (nocc, N)
+        Y = torch.zeros(c.nao, c.N, dtype=torch.float64, device=c.device)
+        for ii in range(len(f)):
+            pair = (psi * psi[ii]).to(torch.complex128).reshape(-1, g, g, g).contiguous()
+            pair *= c.dV
+            K.fft3d(pair, -1)
+            pair *= VG
+            K.fft3d(pair, +1)
+            u = pair.real.reshape(len(f), c.N)
+            Gi = (f[:, None] * psi * u).sum(0)                 # sum_j f_j psi_j u_ij
+            Y += f[ii] * torch.outer(C[:, ii], Gi)
+        D = st["D"][b]
+        M = D @ st["Sgrid"][b] @ D
+        Zx[b, :c.nao, :c.N] = -Y - c0 * (M @ phi[b])
+    return Zx
+
+
+def compute_forces(engine, st, xp_rs, kappa=10, Gmax=15):
+    """-dA/dR (B, n, 3) in Ha/Bohr after engine.solve; xp_rs (B, n, 3) in rs units."""
+    c = engine.c
+    B, dev = st["B"], c.device
+    f64 = torch.float64
+    D, occ, w1, mo = st["D"], st["occ"], st["w1"], st["mo"]
+    S = torch.empty(B, c.nao, c.nao, dtype=f64, device=dev)
+    T = torch.empty_like(S)
+    dS = torch.empty(B, c.nao, c.nao, 3, dtype=f64, device=dev)
+    dT = torch.empty_like(dS)
+    K.overlap_kinetic(st["xpb"], c.lattice, c.alpha, c.coef, c.comp, S, T, dS, dT)
+    W = (mo * (occ * w1).unsqueeze(-2)) @ mo.transpose(-1, -2)
+    atom = c.atom_of_ao
+    other = (atom[:, None] != atom[None, :]).to(f64)
+    t = 2.0 * ((W * other).unsqueeze(-1) * dS - (D * other).unsqueeze(-1) * dT).sum(2)
+    F = torch.zeros(B, c.n, 3, dtype=f64, device=dev)
+    F.index_add_(1, atom, t)
+    # grid adjoint (FP64 phi regardless of the SCF precision)
+    X = torch.zeros(B, c.nao_pad, c.ld, dtype=f64, device=dev)
+    rho = torch.zeros(B, c.ld, dtype=f64, device=dev)
+    K.ao_density(st["phi"], engine._pad(D), rho, X)
+    vh, rg = engine.poisson(rho, save=True)
+    v = torch.zeros(B, c.ld, dtype=f64, device=dev)
+    part = torch.empty(B, K.xc_blocks(c.g), 2, dtype=f64, device=dev)
+    K.xc_eval(rho, vh, c.g, engine.xc_id, v, part)
+    veff = (v + st["vloc"]).contiguous()
+    P = torch.empty(B, c.n, 3, c.g, c.na, 3, dtype=f64, device=dev)
+    K.gto_tables(st["xpb"], c.g, c.L, c.tmax_img, c.alpha, 3, P)
+    Zx = None if engine.dft else _exchange_adjoint(engine, st)
+    fp = torch.empty(B, c.n, K.force_blocks(c.g), 3, dtype=f64, device=dev)
+    K.grid_forces(P, c.alpha, c.coef, c.comp, c.nao_pad, c.ld, X, veff, Zx, c.dV, fp)
+    F += fp.sum(2)
+    dE = torch.empty(B, c.n, 3, dtype=f64, device=dev)
+    K.vloc_force(st["xpb"], c.g, c.L, c.VG, rg, dE)
+    F -= dE
+    Epp, Fpp = ewald_pp(xp_rs, c.L / c.rs, c.rs, kappa, Gmax, forces=True)
+    st["Epp"] = Epp
+    return F + Fpp
+
+
+class FreeEnergy(torch.autograd.Function):
+    """A(xp) in Ry for xp (B, n, 3) in rs units, with the analytic gradient.
+
+    forward runs the SCF and the adjoint once; backward returns
+    dA_Ry/dxp_rs = -2 rs F."""
+
+    @staticmethod
+    def forward(ctx, xp, runner):
+        A, F = runner.free_energy_and_forces(xp)
+        ctx.save_for_backward(F)
+        ctx.rs = runner.rs
+        return A
+
+    @staticmethod
+    def backward(ctx, gA):
+        (F,) = ctx.saved_tensors
+        return gA[:, None, None] * (-2.0 * ctx.rs) * F, None

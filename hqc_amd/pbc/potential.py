@@ -1,0 +1,67 @@
+"""Ewald proton-proton energy on the GPU (hqc/pbc/potential.py:6-130).
+
+potential_energy_pp(xp, L, rs, kappa=10, Gmax=15) -> V_pp in Ry, same units
+and conventions as the reference (xp and L in rs units, minimum image in
+fractional coordinates).  `ewald_pp` additionally returns -dE/dR in Ha/Bohr.
+"""
+import math
+
+import numpy as np
+import torch
+
+from .. import _lib as K
+
+_GCACHE = {}
+
+
+def kpoints(dim, Gmax, device):
+    """Integer vectors with 0 < |G|^2 <= Gmax^2 (potential.py:6-16)."""
+    key = (dim, Gmax, str(device))
+    if key not in _GCACHE:
+        r = np.arange(-Gmax, Gmax + 1)
+        G = np.array(np.meshgrid(*([r] * dim), indexing="ij")).reshape(dim, -1).T
+        G2 = (G ** 2).sum(-1)
+        G = G[(G2 <= Gmax ** 2) & (G2 > 0)].astype(float)
+        _GCACHE[key] = torch.tensor(G, dtype=torch.float64, device=device).contiguous()
+    return _GCACHE[key]
+
+
+def _weights(G, kappa):
+    Gn2 = (G * G).sum(-1)
+    return torch.exp(-math.pi ** 2 * Gn2 / kappa ** 2) / (math.pi * Gn2)
+
+
+def madelung(kappa, G):
+    """potential.py:18-33 (dim 3)."""
+    return _weights(G, kappa).sum() - math.pi / kappa ** 2 - 2 * kappa / math.sqrt(math.pi)
+
+
+def ewald_pp(xp, L, rs, kappa=10, Gmax=15, forces=False):
+    """xp (B, n, 3) rs units on the GPU -> E (B) Ry [, F (B, n, 3) Ha/Bohr]."""
+    B, n, _ = xp.shape
+    dev = xp.device
+    x = xp - L * torch.floor(xp / L)
+    i, j = torch.triu_indices(n, n, offset=1, device=dev)
+    rij = (x[:, i] - x[:, j]) / L
+    rij = (rij - torch.round(rij)).contiguous()
+    G = kpoints(3, Gmax, dev)
+    gk = _weights(G, kappa).contiguous()
+    out = torch.empty(B, rij.shape[1], 4, dtype=torch.float64, device=dev)
+    K.ewald_pairs(rij, G, gk, float(kappa), -math.pi / kappa ** 2, out)
+    E = 2 * out[..., 0].sum(-1) / L / rs + n * madelung(kappa, G) / L / rs
+    if not forces:
+        return E
+    Lb = L * rs
+    grad = out[..., 1:] / Lb ** 2
+    F = torch.zeros(B, n, 3, dtype=torch.float64, device=dev)
+    F.index_add_(1, i, -grad)
+    F.index_add_(1, j, grad)
+    return E, F
+
+
+def potential_energy_pp(xp, L, rs, kappa=10, Gmax=15):
+    """Reference signature: one geometry (n, 3) -> scalar tensor (Ry)."""
+    xp = torch.as_tensor(xp, dtype=torch.float64, device="cuda")
+    single = xp.dim() == 2
+    E = ewald_pp(xp.unsqueeze(0) if single else xp, L, rs, kappa, Gmax)
+    return E[0] if single else E

@@ -1,0 +1,168 @@
+/*
+ * C-ABI of libhqc_amd.so: hand-written gfx950 kernels for the periodic GPW
+ * (Gaussian + plane-wave) SCF path of hqc (hqc/pbc/solver.py, scf.py, gto.py,
+ * hqc/tools/smearing.py, excor.py, hqc/pbc/potential.py).
+ *
+ * The reference is pure Python/JAX and exposes no FFI: its plugin surface is
+ * the factory API make_lcao / make_pes / make_solver (hqc/pbc/lcao.py:11,
+ * hqc/pbc/pes.py:10, hqc/pbc/solver.py:14).  These entry points are what a
+ * host binding of that API calls, one per device operation of the reference's
+ * jitted program; each comment names the reference operation it replaces.
+ *
+ * Conventions (all entry points):
+ *  - every pointer is a DEVICE pointer owned by the caller; nothing is
+ *    allocated or freed by the library unless stated;
+ *  - `stream` is a hipStream_t (passed as void*); work is enqueued, never
+ *    synchronised;
+ *  - return 0 (HQC_OK) or an HQC_E_* code; no exception or exit crosses the ABI;
+ *  - arrays carry a leading batch dimension B (independent geometries of
+ *    identical shape), stored contiguously one geometry after another;
+ *  - grids are g x g x g points, point index r = (ix*g + iy)*g + iz, the point
+ *    coordinate is (ix, iy, iz) * L/g; "ld" is the padded row length of grid
+ *    arrays (a multiple of 64, >= g^3; padding must hold zeros);
+ *  - AO matrices are nao_pad x nao_pad (nao_pad = n_ao rounded up to 16, the
+ *    padding rows/columns hold zeros), row-major, symmetric where stated;
+ *  - complex arrays are interleaved (re, im) doubles.
+ */
+#ifndef HQC_AMD_H
+#define HQC_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HQC_OK 0
+#define HQC_E_ARG 1     /* bad shape / argument */
+#define HQC_E_HIP 2     /* HIP launch or runtime error */
+#define HQC_E_SIZE 3    /* size beyond what the kernel supports */
+
+/* Library version (major*100 + minor). */
+int hqc_version(void);
+
+/* 1-D periodic Gaussian-power tables.
+ * Replaces eval_pbc_gaussian_power_x (hqc/pbc/gto.py:326-339) vmapped over
+ * atoms and the 1-D grid (hqc/pbc/solver.py:211-214).
+ *   P[b,A,d,j,a,l] = sum_{t=-tmax..tmax} (x - tL)^l exp(-alpha_a (x - tL)^2),
+ *   x = j*L/g - xp[b,A,d], l = 0..nl-1.
+ * xp (B,n,3) Bohr; alpha (na); P (B,n,3,g,na,nl). */
+int hqc_gto_tables(const double* xp, int B, int n, int g, double L, int tmax,
+                   const double* alpha, int na, int nl, double* P, void* stream);
+
+/* AO values on the grid, the separable product of the 1-D tables.
+ * Replaces eval_pbc_ao on Rmesh (hqc/pbc/gto.py:341-369, solver.py:217-219).
+ *   phi[b, A*nloc+o, r] = sum_a coef[a,o] prod_d P[b,A,d,r_d,a,pow_d(o)]
+ * comp[o] = -1 for s, 0/1/2 for p_x/p_y/p_z.  phi (B,nao_pad,ld) FP64 or FP32
+ * (the _f32 variant rounds the FP64 value once). */
+int hqc_ao_values_f64(const double* P, int B, int n, int g, int na, int nl,
+                      const double* coef, const int* comp, int nloc, int nao_pad,
+                      int ld, double* phi, void* stream);
+int hqc_ao_values_f32(const double* P, int B, int n, int g, int na, int nl,
+                      const double* coef, const int* comp, int nloc, int nao_pad,
+                      int ld, float* phi, void* stream);
+
+/* Electron density on the grid and (optionally) X = D phi.
+ * Replaces the 'pr,qr,pq->r' contraction of Exc_Vxc_integral
+ * (hqc/pbc/solver.py:690) and the density used by hartree_rhoG (:627).
+ *   rho[b,r] = sum_pq D[b,p,q] phi[b,p,r] phi[b,q,r]    (D symmetric)
+ * FP64 MFMA (v_mfma_f64_16x16x4f64).  rho (B,ld); X (B,nao_pad,ld) or NULL. */
+int hqc_ao_density_f64(const double* phi, const double* D, int B, int nao_pad,
+                       int ld, double* rho, double* X, void* stream);
+/* FP32 operands on v_mfma_f32_16x16x4f32 with FP32 accumulation over the AO
+ * index; D is rounded to FP32; rho is written in FP64. */
+int hqc_ao_density_f32(const float* phi, const double* D, int B, int nao_pad,
+                       int ld, double* rho, void* stream);
+
+/* Grid potential matrix V[b] = dV * phi diag(v[b]) phi^T (symmetric, full).
+ * Replaces the 'pr,qr,r->pq' contraction (hqc/pbc/solver.py:693) and, through
+ * the FFT-Poisson equivalence, the rhoG contractions for V_ep (:427,470) and
+ * J (:628).  FP64 MFMA; per-chunk partial tiles are reduced in a fixed order
+ * (deterministic).  work: device scratch of hqc_ao_vmat_workspace bytes. */
+int hqc_ao_vmat_workspace(int B, int nao_pad, int ld, int prec, size_t* bytes);
+int hqc_ao_vmat_f64(const double* phi, const double* v, int B, int nao_pad, int ld,
+                    double dV, double* V, void* work, size_t work_bytes, void* stream);
+/* FP32 phi and v on v_mfma_f32_16x16x4f32, FP32 accumulation inside a chunk,
+ * FP64 accumulation across chunks; V written in FP64. */
+int hqc_ao_vmat_f32(const float* phi, const double* v, int B, int nao_pad, int ld,
+                    double dV, double* V, void* work, size_t work_bytes, void* stream);
+
+/* Batched in-place 3-D complex FFT of B grids g^3 (any g <= 128; mixed-radix
+ * Stockham passes in LDS, one launch per axis).  sign = -1 forward,
+ * +1 inverse, unnormalised.  Replaces jnp.fft.fft (hqc/pbc/solver.py:412)
+ * in the FFT-Poisson restatement. */
+int hqc_fft3d_c2c_f64(double* data, int B, int g, int sign, void* stream);
+
+/* Local (electron-proton) potential in G space, FFT order:
+ *   out[b,G] = -VG[G] * sum_A exp(-i G.R_A)   (hqc/pbc/solver.py:406-407).
+ * xp (B,n,3) Bohr; VG (g^3) FFT order; out complex (B,g^3). */
+int hqc_vloc_G(const double* xp, int B, int n, int g, double L, const double* VG,
+               double* out, void* stream);
+
+/* Real grid (B,ld) -> complex (B,g^3) scaled by s: out = (s*in, 0). */
+int hqc_real_to_complex(const double* in, int B, int g, int ld, double s, double* out,
+                        void* stream);
+/* Complex (B,g^3) -> real part into (B,ld). */
+int hqc_complex_real(const double* in, int B, int g, int ld, double* out, void* stream);
+/* Poisson kernel in G space: c[b,G] *= VG[G]; if saved != NULL the incoming
+ * c is copied there first (rho~(G), kept for forces). */
+int hqc_poisson_mul(double* c, const double* VG, int B, int g, double* saved, void* stream);
+
+/* Pointwise exchange-correlation and energy densities.
+ * Replaces exc_functional / vxc_functional (hqc/pbc/solver.py:669-678,
+ * hqc/tools/excor.py) with closed-form derivatives.
+ *   xc_id: 0 none (HF), 1 lda+vwn, 2 lda only, 3 lda+hl71, 4 lda+bh72, 5 lda+gl77
+ * rho (B,ld); vh complex (B,g^3) from the inverse FFT (real part = v_H);
+ * vout (B,ld) = v_H + v_xc (padding left untouched);
+ * part (B, nblk, 2) per-block sums of rho*eps_xc and rho*v_H; nblk is
+ * returned by hqc_xc_blocks. */
+int hqc_xc_blocks(int g);
+int hqc_xc_eval(const double* rho, const double* vh, int B, int g, int ld, int xc_id,
+                double* vout, double* part, void* stream);
+
+/* Chemical potential and occupations (hqc/tools/smearing.py:4-173), one
+ * wavefront per geometry, reproducing the reference loop semantics including
+ * the bisection return rule of smearing.py:102.
+ * w (B,nmo) Hartree ascending; method 0 fermi / 1 gauss; search 0 bisect /
+ * 1 newton; occ (B,nmo); mu (B). */
+int hqc_occupations(const double* w, int B, int nmo, int nelec, double sigma, int method,
+                    int search, int max_cycle, double tol, double mu_shift, double* occ,
+                    double* mu, void* stream);
+
+/* Analytic periodic overlap and kinetic matrices by lattice sums
+ * (hqc/pbc/solver.py:233-389, the s/p derivative trick of :307-341).
+ * xp (B,n,3) Bohr; lattice (nT,3); alpha (na); coef (na,nloc); comp (nloc).
+ * S,T (B,n_ao,n_ao).  If dS/dT are non-NULL also the first-argument
+ * derivatives dS[b,p,q,c] = dS_pq/dR_{atom(p),c} (B,n_ao,n_ao,3). */
+int hqc_overlap_kinetic(const double* xp, int B, int n, const double* lattice, int nT,
+                        const double* alpha, int na, const double* coef, const int* comp,
+                        int nloc, double* S, double* T, double* dS, double* dT,
+                        void* stream);
+
+/* Ewald proton-proton sum (hqc/pbc/potential.py:35-67,103-130).
+ * xfrac (B,npair,3) minimum-image fractional separations; Gv (nG,3) integer
+ * vectors with weights gk (nG); out (B,npair,4) = (psi, dpsi/dr_x,y,z). */
+int hqc_ewald_pairs(const double* rij, int B, int npair, const double* Gv, const double* gk,
+                    int nG, double kappa, double g0, double* out, void* stream);
+
+/* Forces: grid adjoint of the AO contractions (new; the reference has none).
+ *   F[b,A,c] = dV * sum_r sum_{p in A} Z_p(r) d phi_p(r)/d r_c,
+ *   Z = 2 v(r) X_p(r) + Zx_p(r)  (Zx may be NULL)
+ * grad phi is recomputed from tables P built with nl = 3 (s) or 4 (p), i.e.
+ * l_max + 2 powers; alpha (na) are the primitive exponents.
+ * part (B, n, nblk, 3), nblk = hqc_force_blocks(g). */
+int hqc_force_blocks(int g);
+int hqc_grid_forces(const double* P, int B, int n, int g, int na, int nl, const double* alpha,
+                    const double* coef, const int* comp, int nloc, int nao_pad, int ld,
+                    const double* X, const double* v, const double* Zx, double dV,
+                    double* part, void* stream);
+
+/* Structure-factor derivative of the local potential energy:
+ *   dE[b,A,c] = Re sum_G i G_c VG(G) exp(-i G.R_A) conj(rho~[b,G]). */
+int hqc_vloc_force(const double* xp, int B, int n, int g, double L, const double* VG,
+                   const double* rhoG, double* dE, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HQC_AMD_H */

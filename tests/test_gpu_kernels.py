@@ -1,0 +1,207 @@
+"""Kernel-level parity on the GPU: each C-ABI entry point against the oracle
+(or a plain torch FP64 reference for the pure linear-algebra kernels)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu_ref as R
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+@pytest.fixture(scope="module")
+def K():
+    from hqc_amd import _lib
+    _lib.lib()
+    return _lib
+
+
+def _sys(n=4, rs=1.5, basis="gth-dzvp", grid_length=0.3, seed=0):
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    xp = np.random.default_rng(seed).uniform(0, L, (n, 3))
+    return R.System(n, L, rs, basis, grid_length=grid_length), xp * rs, L
+
+
+@pytest.mark.parametrize("nao_pad,g", [(16, 7), (48, 13), (112, 21), (176, 9)])
+def test_density_and_vmat_f64(K, nao_pad, g):
+    print(f"[density/vmat f64] nao_pad={nao_pad} g={g}", flush=True)
+    torch.manual_seed(0)
+    N = g ** 3
+    ld = (N + 63) // 64 * 64
+    phi = torch.zeros(2, nao_pad, ld, dtype=torch.float64, device=DEV)
+    phi[:, :, :N] = torch.randn(2, nao_pad, N, dtype=torch.float64, device=DEV)
+    A = torch.randn(2, nao_pad, nao_pad, dtype=torch.float64, device=DEV)
+    D = (A + A.transpose(1, 2)).contiguous()
+    rho = torch.zeros(2, ld, dtype=torch.float64, device=DEV)
+    X = torch.zeros_like(phi)
+    K.ao_density(phi, D, rho, X)
+    Xref = D @ phi
+    rref = (phi * Xref).sum(1)
+    torch.cuda.synchronize()
+    assert torch.allclose(X, Xref, rtol=1e-12, atol=1e-10)
+    assert torch.allclose(rho, rref, rtol=1e-12, atol=1e-9)
+    v = torch.zeros(2, ld, dtype=torch.float64, device=DEV)
+    v[:, :N] = torch.randn(2, N, dtype=torch.float64, device=DEV)
+    V = torch.empty(2, nao_pad, nao_pad, dtype=torch.float64, device=DEV)
+    work = torch.empty(K.vmat_workspace(2, nao_pad, ld, 64) // 8 + 1, dtype=torch.float64,
+                       device=DEV)
+    K.ao_vmat(phi, v, 0.5, V, work)
+    Vref = 0.5 * (phi * v[:, None, :]) @ phi.transpose(1, 2)
+    torch.cuda.synchronize()
+    assert torch.allclose(V, Vref, rtol=1e-11, atol=1e-9 * math.sqrt(N))
+    assert torch.equal(V, V.transpose(1, 2))
+
+
+@pytest.mark.parametrize("nao_pad,g", [(48, 13), (112, 21)])
+def test_density_and_vmat_f32(K, nao_pad, g):
+    print(f"[density/vmat f32] nao_pad={nao_pad} g={g}", flush=True)
+    torch.manual_seed(1)
+    N = g ** 3
+    ld = (N + 63) // 64 * 64
+    phi = torch.zeros(1, nao_pad, ld, dtype=torch.float64, device=DEV)
+    phi[:, :, :N] = torch.rand(1, nao_pad, N, dtype=torch.float64, device=DEV)
+    A = torch.rand(1, nao_pad, nao_pad, dtype=torch.float64, device=DEV)
+    D = (A + A.transpose(1, 2)).contiguous()
+    phi32 = phi.float().contiguous()
+    rho = torch.zeros(1, ld, dtype=torch.float64, device=DEV)
+    K.ao_density(phi32, D, rho)
+    rref = (phi * (D @ phi)).sum(1)
+    assert torch.allclose(rho, rref, rtol=2e-5)
+    v = torch.zeros(1, ld, dtype=torch.float64, device=DEV)
+    v[:, :N] = torch.rand(1, N, dtype=torch.float64, device=DEV)
+    V = torch.empty(1, nao_pad, nao_pad, dtype=torch.float64, device=DEV)
+    work = torch.empty(K.vmat_workspace(1, nao_pad, ld, 32) // 8 + 1, dtype=torch.float64,
+                       device=DEV)
+    K.ao_vmat(phi32, v, 1.0, V, work)
+    Vref = (phi * v[:, None, :]) @ phi.transpose(1, 2)
+    assert torch.allclose(V, Vref, rtol=2e-5)
+
+
+@pytest.mark.parametrize("g", [1, 2, 5, 7, 13, 24, 25, 27, 33, 48, 49, 55, 61, 64, 65])
+def test_fft3d(K, g):
+    print(f"[fft3d] g={g}", flush=True)
+    torch.manual_seed(g)
+    x = torch.randn(2, g, g, g, dtype=torch.complex128, device=DEV)
+    y = x.clone()
+    K.fft3d(y, -1)
+    ref = torch.fft.fftn(x, dim=(1, 2, 3))
+    assert torch.allclose(y, ref, rtol=1e-12, atol=1e-11 * g ** 1.5)
+    K.fft3d(y, +1)
+    assert torch.allclose(y / g ** 3, x, rtol=1e-12, atol=1e-12)
+
+
+def test_gto_tables_and_values(K):
+    s, xpb, L = _sys()
+    print(f"[gto] g={s.g}", flush=True)
+    b = s.basis
+    P_ref = R.gto_tables(s, xpb, deriv=True)          # nl = l_max + 3
+    nl = P_ref.shape[-1]
+    xp_t = torch.tensor(xpb[None], device=DEV)
+    alpha = torch.tensor(b.alpha, device=DEV)
+    P = torch.empty(1, s.n, 3, s.g, len(b.alpha), nl, dtype=torch.float64, device=DEV)
+    K.gto_tables(xp_t, s.g, s.L, int(24 // s.L), alpha, nl, P)
+    assert np.allclose(P[0].cpu().numpy(), P_ref, rtol=1e-13, atol=1e-14)
+    nao_pad = (s.n_ao + 15) // 16 * 16
+    ld = (s.N + 63) // 64 * 64
+    phi = torch.zeros(1, nao_pad, ld, dtype=torch.float64, device=DEV)
+    coef = torch.tensor(b.coef, device=DEV).contiguous()
+    comp = torch.tensor(b.comp, dtype=torch.int32, device=DEV)
+    K.ao_values(P[..., :b.l_max + 1].contiguous(), coef, comp, nao_pad, ld, phi)
+    ref = R.ao_values(s, xpb)
+    assert np.allclose(phi[0, :s.n_ao, :s.N].cpu().numpy(), ref, rtol=1e-13, atol=1e-14)
+
+
+def test_overlap_kinetic(K):
+    for basis in ("gth-szv", "gth-dzv", "gth-dzvp"):
+        s, xpb, L = _sys(n=3, basis=basis)
+        print(f"[S/T] {basis}", flush=True)
+        b = s.basis
+        S_ref, T_ref, dS_ref, dT_ref = R.overlap_kinetic(s, xpb, deriv=True)
+        nao = s.n_ao
+        S = torch.empty(1, nao, nao, dtype=torch.float64, device=DEV)
+        T = torch.empty_like(S)
+        dS = torch.empty(1, nao, nao, 3, dtype=torch.float64, device=DEV)
+        dT = torch.empty_like(dS)
+        K.overlap_kinetic(torch.tensor(xpb[None], device=DEV),
+                          torch.tensor(s.lattice, device=DEV).contiguous(),
+                          torch.tensor(b.alpha, device=DEV), torch.tensor(b.coef, device=DEV),
+                          torch.tensor(b.comp, dtype=torch.int32, device=DEV), S, T, dS, dT)
+        for a, r in ((S, S_ref), (T, T_ref), (dS, dS_ref), (dT, dT_ref)):
+            assert np.allclose(a[0].cpu().numpy(), r, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("method,search", [("fermi", "bisect"), ("fermi", "newton"),
+                                           ("gauss", "bisect"), ("gauss", "newton")])
+def test_occupations(K, method, search):
+    rng = np.random.default_rng(3)
+    n, nmo = 16, 40
+    w = np.sort(rng.normal(0.0, 0.5, (5, nmo)), axis=1)
+    occ_fn = R.make_occupation(n, nmo, True, method, 0.05, search, 100, 1e-7)
+    wt = torch.tensor(w, device=DEV)
+    occ = torch.empty_like(wt)
+    mu = torch.empty(5, dtype=torch.float64, device=DEV)
+    if method == "fermi":
+        shift = -0.05 * math.log(2 * nmo / n - 1)
+    else:
+        import scipy.special
+        shift = -0.05 * scipy.special.erfinv(1 - n / nmo)
+    K.occupations(wt, n, 0.05, 0 if method == "fermi" else 1, 0 if search == "bisect" else 1,
+                  100, 1e-7, shift, occ, mu)
+    for i in range(5):
+        assert np.allclose(occ[i].cpu().numpy(), occ_fn(w[i]), rtol=1e-11, atol=1e-13)
+        assert abs(mu[i].item() - occ_fn.mu(w[i])) < 1e-11
+
+
+@pytest.mark.parametrize("xc", ["lda,vwn", "lda,", "lda,hl", "lda,bh", "lda,gl"])
+def test_xc_eval(K, xc):
+    g = 9
+    N = g ** 3
+    ld = (N + 63) // 64 * 64
+    rng = np.random.default_rng(4)
+    rho_np = rng.uniform(1e-4, 2.0, N)
+    vh_np = rng.normal(size=N)
+    rho = torch.zeros(1, ld, dtype=torch.float64, device=DEV)
+    rho[0, :N] = torch.tensor(rho_np, device=DEV)
+    vh = torch.zeros(1, g, g, g, dtype=torch.complex128, device=DEV)
+    vh.view(1, -1).real[0] = torch.tensor(vh_np, device=DEV)
+    xc_id = {"lda,vwn": 1, "lda,": 2, "lda,hl": 3, "lda,bh": 4, "lda,gl": 5}[xc]
+    v = torch.zeros(1, ld, dtype=torch.float64, device=DEV)
+    part = torch.empty(1, K.xc_blocks(g), 2, dtype=torch.float64, device=DEV)
+    K.xc_eval(rho, vh, g, xc_id, v, part)
+    eps, vxc = R.xc_eps_v(rho_np, xc)
+    assert np.allclose(v[0, :N].cpu().numpy(), vh_np + vxc, rtol=1e-13, atol=1e-13)
+    sums = part.sum(1)[0].cpu().numpy()
+    assert np.isclose(sums[0], np.sum(rho_np * eps), rtol=1e-12)
+    assert np.isclose(sums[1], np.sum(rho_np * vh_np), rtol=1e-11, atol=1e-11)
+
+
+def test_vloc_and_poisson(K):
+    s, xpb, L = _sys(n=5, basis="gth-szv", grid_length=0.25)
+    print(f"[vloc] g={s.g}", flush=True)
+    g = s.g
+    VG = torch.tensor(s.VG.ravel(), device=DEV)
+    out = torch.empty(1, g, g, g, dtype=torch.complex128, device=DEV)
+    K.vloc_G(torch.tensor(xpb[None], device=DEV), g, s.L, VG, out)
+    ref = R.vloc_G(s, xpb)
+    assert np.allclose(out[0].cpu().numpy(), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    K.fft3d(out, +1)
+    ld = (s.N + 63) // 64 * 64
+    vl = torch.zeros(1, ld, dtype=torch.float64, device=DEV)
+    K.complex_real(out, ld, vl)
+    assert np.allclose(vl[0, :s.N].cpu().numpy(), R.vloc_real(s, xpb), rtol=1e-11, atol=1e-11)
+
+
+def test_ewald(K):
+    from hqc_amd.pbc.potential import ewald_pp
+    n, rs = 14, 1.31
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    xp = np.random.default_rng(5).uniform(0, L, (2, n, 3))
+    E, F = ewald_pp(torch.tensor(xp, device=DEV), L, rs, forces=True)
+    for b in range(2):
+        Er, Fr = R.ewald_pp(xp[b], L, rs, forces=True)
+        assert abs(E[b].item() - Er) < 1e-11
+        assert np.allclose(F[b].cpu().numpy(), Fr, atol=1e-11)

@@ -1,0 +1,56 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every symbol
+that include/hqc_amd.h declares, and the ctypes binding covers the same set.
+No compute call is made (there is no GPU here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "hqc_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\bint\s+(hqc_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def libpath():
+    from hqc_amd import _lib, build
+    if not os.path.isfile(_lib.LIB_PATH):
+        build.build()
+    return _lib.LIB_PATH
+
+
+def test_library_exports_every_declared_symbol(libpath):
+    h = ctypes.CDLL(libpath)
+    missing = [s for s in _declared() if not hasattr(h, s)]
+    assert not missing, missing
+
+
+def test_binding_matches_header():
+    from hqc_amd import _lib
+    assert _lib.symbols() == _declared()
+
+
+def test_version_and_size_queries(libpath):
+    from hqc_amd import _lib
+    assert _lib.lib().hqc_version() >= 100
+    # pure host-side queries (no device work)
+    assert _lib.xc_blocks(49) > 0 and _lib.force_blocks(49) > 0
+    n = _lib.vmat_workspace(2, 112, 117696, 64)
+    assert n > 0 and n % (256 * 8) == 0
+    with pytest.raises(RuntimeError):
+        _lib.vmat_workspace(2, 100, 117696, 64)     # nao_pad must be a multiple of 16
+
+
+def test_no_cpu_fallback_without_gpu():
+    """Calling a compute entry point with CPU tensors raises; nothing silently
+    falls back to a host implementation."""
+    import torch
+    from hqc_amd import _lib
+    x = torch.zeros(1, 5, 5, 5, dtype=torch.complex128)
+    with pytest.raises(ValueError):
+        _lib.fft3d(x, -1)
