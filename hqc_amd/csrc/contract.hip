@@ -4,8 +4,10 @@
 //   vmat:     V_pq   = dV sum_r phi_p(r) v(r) phi_q(r)        (solver.py:693)
 //
 // MFMA 16x16x4 operand layout (64 lanes, one value per lane for A and B,
-// four per lane for C):  A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
-// C[i][j] in lane j + 16*(i/4), element i%4.
+// four per lane for C):  A[i][k] in lane i + 16k, B[k][j] in lane j + 16k.
+// The accumulator layout differs between the two instructions (measured on
+// gfx950):  f32: C[i][j] in lane j + 16*(i/4), element i%4;
+//           f64: C[i][j] in lane j + 16*(i%4), element i/4.
 #include "common.h"
 
 __device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
@@ -13,6 +15,19 @@ __device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
 }
 __device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Row (0..15) inside a 16x16 accumulator tile held by lane group kq = lane/16,
+// element e.
+template <typename T>
+__device__ __forceinline__ int acc_row(int kq, int e);
+template <>
+__device__ __forceinline__ int acc_row<double>(int kq, int e) {
+  return e * 4 + kq;
+}
+template <>
+__device__ __forceinline__ int acc_row<float>(int kq, int e) {
+  return kq * 4 + e;
 }
 
 template <typename T>
@@ -67,7 +82,7 @@ __global__ __launch_bounds__(256) void k_density(const T* __restrict__ phi,
       if (rt0 + i < NT) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const long row = (long)(rt0 + i) * 16 + kq * 4 + e;
+          const long row = (long)(rt0 + i) * 16 + acc_row<T>(kq, e);
           const double xv = (double)acc[i][e];
           rsum += xv * (double)P[row * ld + col];
           if (X) X[((long)b * nao_pad + row) * ld + col] = xv;
@@ -188,12 +203,13 @@ __global__ __launch_bounds__(256) void k_vmat(const T* __restrict__ phi,
   }
 }
 
+template <typename T>
 __global__ void k_vmat_reduce(const double* __restrict__ part, int nao_pad, int ntiles, int nwg,
                               double dV, double* __restrict__ V) {
   const int tile = blockIdx.x, b = blockIdx.y;
   const int idx = threadIdx.x;  // lane*4 + e
   const int lane = idx >> 2, e = idx & 3;
-  const int i = (lane >> 4) * 4 + e, j = lane & 15;
+  const int i = acc_row<T>(lane >> 4, e), j = lane & 15;
   const double* p = part + ((long)b * ntiles + tile) * nwg * 256 + idx;
   double s = 0.0;
   for (int k = 0; k < nwg; ++k) s += p[(long)k * 256];
@@ -258,8 +274,8 @@ static int launch_vmat(const T* phi, const double* v, int B, int nao_pad, int ld
     hipLaunchKernelGGL((k_vmat<T, 16>), grid, dim3(256), lds, s, phi, v, nao_pad, ld, ntiles,
                        nwg, part);
   HQC_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_vmat_reduce, dim3(ntiles, B), dim3(256), 0, s, part, nao_pad, ntiles, nwg,
-                     dV, V);
+  hipLaunchKernelGGL(k_vmat_reduce<T>, dim3(ntiles, B), dim3(256), 0, s, part, nao_pad, ntiles,
+                     nwg, dV, V);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }
