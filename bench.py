@@ -9,7 +9,7 @@ grid_length 0.16 (49^3), SCF tol 1e-8 Ha, bisection mu search (make_pes
 defaults otherwise); energy (E + V_pp) plus analytic forces for every
 geometry.  Inputs are already on the GPU when the timed region starts.
 
-    python bench.py [--steps K] [--warmup W] [--batch B] [--path fp64|fp32]
+    python bench.py [--steps K] [--warmup W] [--batch B] [--path fp64|mixed]
 
 Prints ONE JSON line.  Only one GPU is supported (--gpus N>1 exits).
 """
@@ -99,7 +99,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--path", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--path", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--compare", action="store_true",
                     help="also run the other precision path once and report its error")
@@ -134,7 +134,7 @@ def main():
         "metric": METRIC, "value": B / dt, "unit": "geometries/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.path == "fp64" else "f32(grid contractions)+f64",
+        "dtype": "f64" if a.path == "fp64" else "f64+f32(increments)",
         "data": "synthetic: uniform-random protons, numpy default_rng(seed), seeds 0..B-1",
         "config": {"workload": f"54 H rs=1.25 gth-dzv LDA(vwn) Fermi T=1e4K, grid 49^3 "
                                f"(grid_length 0.16), tol 1e-8, energy+forces, batch {B}",
@@ -144,12 +144,17 @@ def main():
                      "frac": achieved / peak, "traffic": None,
                      "ms_per_launch": ms, "flops_per_launch": flops},
     }
+    out["config"]["contractions"] = dict(pes.solver.engine.stats)
     if a.compare:
-        other = make("fp32" if a.path == "fp64" else "fp64")
+        other = make("mixed" if a.path == "fp64" else "fp64")
         E2, F2 = other.energy_and_forces(x)
+        w_a, w_b = st["w1"], other.solver.last["w1"]
         out["compare"] = {
+            "against": other.solver.engine.precision,
             "max_dE_per_atom_Ha": (E - E2).abs().max().item() / RY / N_ATOMS,
-            "max_dF_Ha_per_Bohr": (F - F2).abs().max().item()}
+            "max_dF_Ha_per_Bohr": (F - F2).abs().max().item(),
+            "max_dbands_Ha": (w_a - w_b).abs().max().item(),
+            "n_iter_mean_other": other.solver.last["nit"].double().mean().item()}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(1)
     print(json.dumps(out), flush=True)

@@ -27,6 +27,8 @@ _SIGS = {
     "hqc_ao_values_f32": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
     "hqc_ao_density_f64": [_P, _P, _I, _I, _I, _P, _P, _P],
     "hqc_ao_density_f32": [_P, _P, _I, _I, _I, _P, _P],
+    "hqc_orb_density_f64": [_P, _P, _I, _I, _I, _I, _P, _P],
+    "hqc_orb_density_f32": [_P, _P, _I, _I, _I, _I, _P, _P],
     "hqc_ao_vmat_workspace": [_I, _I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
     "hqc_ao_vmat_f64": [_P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_ao_vmat_f32": [_P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
@@ -122,6 +124,15 @@ def ao_density(phi, D, rho, X=None):
         assert X is None
         _call("hqc_ao_density_f32", _ptr(phi, F32, "phi"), _ptr(D, F64, "D"), B, nao_pad, ld,
               _ptr(rho, F64, "rho"), _stream())
+
+
+def orb_density(phi, At, rho):
+    B, nao_pad, ld = phi.shape
+    m_pad = At.shape[2]
+    assert At.shape == (B, nao_pad, m_pad) and rho.shape == (B, ld)
+    name = "hqc_orb_density_f64" if phi.dtype == F64 else "hqc_orb_density_f32"
+    _call(name, _ptr(phi, None, "phi"), _ptr(At, F64, "At"), B, nao_pad, m_pad, ld,
+          _ptr(rho, F64, "rho"), _stream())
 
 
 def vmat_workspace(B, nao_pad, ld, prec):

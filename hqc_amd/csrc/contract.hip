@@ -69,13 +69,11 @@ __global__ __launch_bounds__(256) void k_density(const T* __restrict__ phi,
       const int k = ks * 4 + kq;
       const T bv = P[(long)k * ld + col];
       const double* Drow = Db + (long)k * nao_pad + l16;
+      T av[NTG];
 #pragma unroll
-      for (int i = 0; i < NTG; ++i) {
-        if (rt0 + i < NT) {
-          const T av = (T)Drow[(rt0 + i) * 16];
-          acc[i] = mfma16(av, bv, acc[i]);
-        }
-      }
+      for (int i = 0; i < NTG; ++i) av[i] = (T)Drow[min(rt0 + i, NT - 1) * 16];
+#pragma unroll
+      for (int i = 0; i < NTG; ++i) acc[i] = mfma16(av[i], bv, acc[i]);
     }
 #pragma unroll
     for (int i = 0; i < NTG; ++i) {
@@ -100,8 +98,16 @@ static int launch_density(const T* phi, const double* D, int B, int nao_pad, int
                           double* rho, double* X, void* stream) {
   if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63)) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
-  hipLaunchKernelGGL((k_density<T, 8>), dim3(ld / 64, B), dim3(256), 0, (hipStream_t)stream,
-                     phi, D, nao_pad, ld, rho, X);
+#define DENS_CASE(G)                                                                       \
+  case G:                                                                                  \
+    hipLaunchKernelGGL((k_density<T, G>), dim3(ld / 64, B), dim3(256), 0, (hipStream_t)stream, \
+                       phi, D, nao_pad, ld, rho, X);                                       \
+    break;
+  switch (min(nao_pad >> 4, 8)) {
+    DENS_CASE(1) DENS_CASE(2) DENS_CASE(3) DENS_CASE(4) DENS_CASE(5) DENS_CASE(6) DENS_CASE(7)
+    default: DENS_CASE(8)
+  }
+#undef DENS_CASE
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }
@@ -114,6 +120,88 @@ extern "C" int hqc_ao_density_f64(const double* phi, const double* D, int B, int
 extern "C" int hqc_ao_density_f32(const float* phi, const double* D, int B, int nao_pad,
                                   int ld, double* rho, void* stream) {
   return launch_density<float>(phi, D, B, nao_pad, ld, rho, nullptr, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Orbital-form density rho(r) = sum_i (sum_k At[k][i] phi_k(r))^2 with
+// At = C sqrt(f) restricted to the orbitals whose occupation is not
+// negligible (m_pad columns, zero-padded).  Same tiling as k_density; the
+// epilogue squares the accumulator instead of re-reading phi.
+// ---------------------------------------------------------------------------
+template <typename T, int NTG>
+__global__ __launch_bounds__(256) void k_orb_density(const T* __restrict__ phi,
+                                                     const double* __restrict__ At, int nao_pad,
+                                                     int m_pad, int ld,
+                                                     double* __restrict__ rho) {
+  typedef typename vec4_of<T>::type V4;
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const long col = (long)blockIdx.x * 64 + w * 16 + l16;
+  const T* P = phi + (long)b * nao_pad * ld;
+  const double* Ab = At + (long)b * nao_pad * m_pad;
+  const int NT = m_pad >> 4, KS = nao_pad >> 2;
+  double rsum = 0.0;
+  for (int rt0 = 0; rt0 < NT; rt0 += NTG) {
+    V4 acc[NTG];
+#pragma unroll
+    for (int i = 0; i < NTG; ++i) acc[i] = V4{0, 0, 0, 0};
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = ks * 4 + kq;
+      const T bv = P[(long)k * ld + col];
+      const double* Arow = Ab + (long)k * m_pad + l16;
+      T av[NTG];
+#pragma unroll
+      for (int i = 0; i < NTG; ++i) av[i] = (T)Arow[min(rt0 + i, NT - 1) * 16];
+#pragma unroll
+      for (int i = 0; i < NTG; ++i) acc[i] = mfma16(av[i], bv, acc[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < NTG; ++i) {
+      if (rt0 + i < NT) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const double y = (double)acc[i][e];
+          rsum += y * y;
+        }
+      }
+    }
+  }
+  rsum += __shfl_xor(rsum, 16, 64);
+  rsum += __shfl_xor(rsum, 32, 64);
+  if (kq == 0) rho[(long)b * ld + col] = rsum;
+}
+
+template <typename T>
+static int launch_orb_density(const T* phi, const double* At, int B, int nao_pad, int m_pad,
+                              int ld, double* rho, void* stream) {
+  if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || m_pad <= 0 || (m_pad & 15) || ld <= 0 ||
+      (ld & 63))
+    return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+#define ODENS_CASE(G)                                                                      \
+  case G:                                                                                  \
+    hipLaunchKernelGGL((k_orb_density<T, G>), dim3(ld / 64, B), dim3(256), 0,               \
+                       (hipStream_t)stream, phi, At, nao_pad, m_pad, ld, rho);             \
+    break;
+  switch (min(m_pad >> 4, 8)) {
+    ODENS_CASE(1) ODENS_CASE(2) ODENS_CASE(3) ODENS_CASE(4) ODENS_CASE(5) ODENS_CASE(6)
+    ODENS_CASE(7)
+    default: ODENS_CASE(8)
+  }
+#undef ODENS_CASE
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+extern "C" int hqc_orb_density_f64(const double* phi, const double* At, int B, int nao_pad,
+                                   int m_pad, int ld, double* rho, void* stream) {
+  return launch_orb_density<double>(phi, At, B, nao_pad, m_pad, ld, rho, stream);
+}
+
+extern "C" int hqc_orb_density_f32(const float* phi, const double* At, int B, int nao_pad,
+                                   int m_pad, int ld, double* rho, void* stream) {
+  return launch_orb_density<float>(phi, At, B, nao_pad, m_pad, ld, rho, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -160,9 +248,11 @@ __global__ __launch_bounds__(256) void k_vmat(const T* __restrict__ phi,
   }
   const T* P = phi + (long)b * nao_pad * ld;
   const double* vb = v + (long)b * ld;
-  f64x4 acc[VMAT_TPW];
+  // accumulate in T over every chunk of this block (FP32 for the FP32 variant,
+  // whose callers use it on potential increments); blocks are summed in FP64
+  V4 acc[VMAT_TPW];
 #pragma unroll
-  for (int t = 0; t < VMAT_TPW; ++t) acc[t] = f64x4{0, 0, 0, 0};
+  for (int t = 0; t < VMAT_TPW; ++t) acc[t] = V4{0, 0, 0, 0};
   for (long c0 = (long)wg * KC; c0 < ld; c0 += (long)nwg * KC) {
     __syncthreads();
     for (int e = threadIdx.x; e < nao_pad * KC; e += 256) {
@@ -171,10 +261,7 @@ __global__ __launch_bounds__(256) void k_vmat(const T* __restrict__ phi,
     }
     for (int e = threadIdx.x; e < KC; e += 256) vs[e] = (T)vb[c0 + e];
     __syncthreads();
-    V4 cacc[VMAT_TPW];
-#pragma unroll
-    for (int t = 0; t < VMAT_TPW; ++t) cacc[t] = V4{0, 0, 0, 0};
-#pragma unroll 4
+#pragma unroll 2
     for (int ks = 0; ks < KC / 4; ++ks) {
       const int kk = ks * 4 + kq;
       const T vk = vs[kk];
@@ -183,14 +270,10 @@ __global__ __launch_bounds__(256) void k_vmat(const T* __restrict__ phi,
         if (tbase + t < ntiles) {
           const T a = Ps[rto[t] + kk];
           const T bb = vk * Ps[cto[t] + kk];
-          cacc[t] = mfma16(a, bb, cacc[t]);
+          acc[t] = mfma16(a, bb, acc[t]);
         }
       }
     }
-#pragma unroll
-    for (int t = 0; t < VMAT_TPW; ++t)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) acc[t][e] += (double)cacc[t][e];
   }
 #pragma unroll
   for (int t = 0; t < VMAT_TPW; ++t) {
@@ -198,7 +281,116 @@ __global__ __launch_bounds__(256) void k_vmat(const T* __restrict__ phi,
     if (tile < ntiles) {
       double* o = part + (((long)b * ntiles + tile) * nwg + wg) * 256 + lane * 4;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) o[e] = acc[t][e];
+      for (int e = 0; e < 4; ++e) o[e] = (double)acc[t][e];
+    }
+  }
+}
+
+// Pipelined variant for nao_pad <= 128: two LDS buffers; while the waves run
+// MFMAs on chunk i, every thread already holds its share of chunk i+1 in
+// registers (16-byte global loads issued before the compute), and stores it to
+// the other buffer afterwards, so one barrier per chunk separates the phases.
+#define VMAT_PIPE_MAXV 8
+
+template <typename T, int KC, int TPW>
+__global__ __launch_bounds__(256) void k_vmat_pipe(const T* __restrict__ phi,
+                                                   const double* __restrict__ v, int nao_pad,
+                                                   int ld, int ntiles, int nwg,
+                                                   double* __restrict__ part) {
+  typedef typename vec4_of<T>::type V4;
+  constexpr int VW = 16 / sizeof(T);            // elements per 16-byte vector
+  constexpr int VPR = KC / VW;                  // vectors per row of a chunk (16)
+  // LDS row stride chosen so that the 16 rows x 4 k of one MFMA fragment land
+  // on distinct 4-byte banks (KC+4 for FP32, KC+2 for FP64)
+  constexpr int STR = KC + (sizeof(T) == 4 ? 4 : 2);
+  typedef T vec_t __attribute__((ext_vector_type(VW)));
+  extern __shared__ unsigned char smem_raw[];
+  T* Ps = reinterpret_cast<T*>(smem_raw);                    // [2][nao_pad*STR]
+  T* vs = Ps + 2 * (long)nao_pad * STR;                       // [2][KC]
+  const int wg = blockIdx.x, grp = blockIdx.y, b = blockIdx.z;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int NT = nao_pad >> 4;
+  const int NV = nao_pad * VPR / 256;           // vectors per thread per chunk
+  const int tbase = grp * 4 * TPW + w * TPW;
+  // slots past the last tile recompute tile ntiles-1 (branch-free MFMA loop);
+  // their results are not stored
+  int rto[TPW], cto[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    int rt, ct;
+    tile_rc(min(tbase + t, ntiles - 1), NT, rt, ct);
+    rto[t] = (rt * 16 + l16) * STR;
+    cto[t] = (ct * 16 + l16) * STR;
+  }
+  const T* P = phi + (long)b * nao_pad * ld;
+  const double* vb = v + (long)b * ld;
+  vec_t stage[VMAT_PIPE_MAXV];
+  double vstage = 0.0;
+  auto gload = [&](long c0) {
+#pragma unroll
+    for (int j = 0; j < VMAT_PIPE_MAXV; ++j) {
+      if (j < NV) {
+        const int q = threadIdx.x + 256 * j;
+        const int row = q / VPR, cv = q - row * VPR;
+        stage[j] = *reinterpret_cast<const vec_t*>(P + (long)row * ld + c0 + cv * VW);
+      }
+    }
+    if (threadIdx.x < KC) vstage = vb[c0 + threadIdx.x];
+  };
+  auto lstore = [&](int buf) {
+    T* dst = Ps + (long)buf * nao_pad * STR;
+#pragma unroll
+    for (int j = 0; j < VMAT_PIPE_MAXV; ++j) {
+      if (j < NV) {
+        const int q = threadIdx.x + 256 * j;
+        const int row = q / VPR, cv = q - row * VPR;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) dst[row * STR + cv * VW + e] = stage[j][e];
+      }
+    }
+    if (threadIdx.x < KC) vs[buf * KC + threadIdx.x] = (T)vstage;
+  };
+  V4 acc[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) acc[t] = V4{0, 0, 0, 0};
+  long c0 = (long)wg * KC;
+  const long cstep = (long)nwg * KC;
+  if (c0 < ld) {
+    gload(c0);
+    lstore(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (; c0 < ld; c0 += cstep) {
+    const bool more = c0 + cstep < ld;
+    if (more) gload(c0 + cstep);
+    const T* Pb = Ps + (long)buf * nao_pad * STR;
+    const T* vsb = vs + buf * KC;
+#pragma unroll 4
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int kk = ks * 4 + kq;
+      const T vk = vsb[kk];
+      T av[TPW], bv[TPW];
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+        av[t] = Pb[rto[t] + kk];
+        bv[t] = Pb[cto[t] + kk];
+      }
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) acc[t] = mfma16(av[t], vk * bv[t], acc[t]);
+    }
+    if (more) lstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int tile = tbase + t;
+    if (tile < ntiles) {
+      double* o = part + (((long)b * ntiles + tile) * nwg + wg) * 256 + lane * 4;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) o[e] = (double)acc[t][e];
     }
   }
 }
@@ -227,15 +419,29 @@ __global__ void k_vmat_reduce(const double* __restrict__ part, int nao_pad, int 
   }
 }
 
+static bool vmat_pipelined(int nao_pad) { return nao_pad <= 16 * VMAT_PIPE_MAXV; }
+
+// Tiles per wave of the pipelined kernel: the smallest count that covers all
+// tiles with 4 waves (capped at VMAT_TPW; more tiles use several groups).
+static int vmat_pipe_tpw(int nao_pad) {
+  const int NT = nao_pad >> 4;
+  const int nt = NT * (NT + 1) / 2;
+  const int t = (nt + 3) / 4;
+  return t < VMAT_TPW ? t : VMAT_TPW;
+}
+
 static void vmat_shape(int B, int nao_pad, int ld, int prec, int& KC, int& ntiles, int& ngroups,
                        int& nwg) {
   KC = nao_pad <= 160 ? 64 : (nao_pad <= 320 ? 32 : 16);
   if (prec == 32 && nao_pad <= 320) KC = 64;
+  if (vmat_pipelined(nao_pad)) KC = prec == 32 ? 64 : 32;   // 16 vectors of 16 B per row
   const int NT = nao_pad >> 4;
   ntiles = NT * (NT + 1) / 2;
-  ngroups = (ntiles + 4 * VMAT_TPW - 1) / (4 * VMAT_TPW);
+  const int tpw = vmat_pipelined(nao_pad) ? vmat_pipe_tpw(nao_pad) : VMAT_TPW;
+  ngroups = (ntiles + 4 * tpw - 1) / (4 * tpw);
   const int nchunks = ld / KC;
-  long target = VMAT_TARGET_BLOCKS / ((long)(B > 0 ? B : 1) * ngroups);
+  const long target_blocks = vmat_pipelined(nao_pad) ? 2048 : VMAT_TARGET_BLOCKS;
+  long target = target_blocks / ((long)(B > 0 ? B : 1) * ngroups);
   if (target < 1) target = 1;
   if (target > nchunks) target = nchunks;
   nwg = (int)target;
@@ -259,11 +465,37 @@ static int launch_vmat(const T* phi, const double* v, int B, int nao_pad, int ld
   vmat_shape(B, nao_pad, ld, prec, KC, ntiles, ngroups, nwg);
   const size_t need = (size_t)B * ntiles * nwg * 256 * sizeof(double);
   if (!work || work_bytes < need) return HQC_E_ARG;
-  const size_t lds = ((size_t)nao_pad * (KC + 1) + KC) * sizeof(T);
-  if (lds > 160 * 1024) return HQC_E_SIZE;
   hipStream_t s = (hipStream_t)stream;
   double* part = (double*)work;
   dim3 grid(nwg, ngroups, B);
+  if (vmat_pipelined(nao_pad)) {
+    const size_t lds = 2 * ((size_t)nao_pad * (KC + (sizeof(T) == 4 ? 4 : 2)) + KC) * sizeof(T);
+    constexpr int K = sizeof(T) == 4 ? 64 : 32;
+#define VMAT_PIPE_CASE(TP)                                                                  \
+  case TP:                                                                                  \
+    hipLaunchKernelGGL((k_vmat_pipe<T, K, TP>), grid, dim3(256), lds, s, phi, v, nao_pad, ld, \
+                       ntiles, nwg, part);                                                  \
+    break;
+    switch (vmat_pipe_tpw(nao_pad)) {
+      VMAT_PIPE_CASE(1)
+      VMAT_PIPE_CASE(2)
+      VMAT_PIPE_CASE(3)
+      VMAT_PIPE_CASE(4)
+      VMAT_PIPE_CASE(5)
+      VMAT_PIPE_CASE(6)
+      VMAT_PIPE_CASE(7)
+      default:
+      VMAT_PIPE_CASE(8)
+    }
+#undef VMAT_PIPE_CASE
+    HQC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_vmat_reduce<T>, dim3(ntiles, B), dim3(256), 0, s, part, nao_pad, ntiles,
+                       nwg, dV, V);
+    HQC_CHECK_LAUNCH();
+    return HQC_OK;
+  }
+  const size_t lds = ((size_t)nao_pad * (KC + 1) + KC) * sizeof(T);
+  if (lds > 160 * 1024) return HQC_E_SIZE;
   if (KC == 64)
     hipLaunchKernelGGL((k_vmat<T, 64>), grid, dim3(256), lds, s, phi, v, nao_pad, ld, ntiles,
                        nwg, part);

@@ -74,8 +74,20 @@ class Cell:
 
 
 class Engine:
-    """Batched Gamma-point SCF on the GPU.  precision: 'fp64' or 'fp32' (the
-    AO x grid contractions run on FP32 MFMA; everything else stays FP64)."""
+    """Batched Gamma-point SCF on the GPU.
+
+    precision='fp64': every contraction in FP64 (MFMA f64).
+    precision='mixed': defect correction.  rho(D) and V(v) are linear, so the
+    engine keeps FP64 anchors (D_a, rho_a) and (v_a, V_a) and evaluates only the
+    increments rho(D - D_a) and V(v - v_a) with FP32 AO values on FP32 MFMA.
+    The increment error is proportional to the increment, which shrinks as the
+    SCF converges; an FP64 re-anchor is forced whenever the estimated increment
+    error exceeds MIXED_SAFETY times the current DIIS residual per matrix
+    element, so the iteration converges to the FP64 fixed point.  Eigensolves,
+    DIIS, occupations, FFTs, xc and every energy sum stay FP64."""
+
+    MIXED_EPS = 2e-6        # effective relative error of an FP32-accumulated increment
+    MIXED_SAFETY = 1e-2
 
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
@@ -113,9 +125,12 @@ class Engine:
         self.tol, self.max_cycle = float(tol), int(max_cycle)
         self.diis, self.diis_space = diis, int(diis_space)
         self.diis_start_cycle, self.diis_damp = int(diis_start_cycle), float(diis_damp)
-        if precision not in ("fp64", "fp32"):
-            raise ValueError("precision must be 'fp64' or 'fp32'")
+        if precision == "fp32":
+            precision = "mixed"
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
+        self.stats = {"full": 0, "low": 0}
         self._work = {}
 
     # ------------------------------------------------------------------ utils
@@ -150,6 +165,21 @@ class Engine:
         K.ao_density(phi, self._pad(D), rho, X)
         return rho
 
+    OCC_CUT = 1e-15   # orbitals below this occupation contribute < 1e-15 relative to rho
+
+    def orb_density(self, phi, mo, occ):
+        """rho = sum_i f_i psi_i^2 over the orbitals with f_i > OCC_CUT (equal to the
+        D form with D = C f C^T up to that cut)."""
+        c = self.c
+        B = mo.shape[0]
+        keep = max(int((occ > self.OCC_CUT).sum(-1).max().item()), 1)
+        m_pad = _round_up(keep, 16)
+        At = torch.zeros(B, c.nao_pad, m_pad, dtype=torch.float64, device=c.device)
+        At[:, :c.nao, :keep] = mo[:, :, :keep] * occ[:, None, :keep].clamp(min=0).sqrt()
+        rho = torch.zeros(B, c.ld, dtype=torch.float64, device=c.device)
+        K.orb_density(phi, At, rho)
+        return rho
+
     def poisson(self, rho, save=False):
         """v_H on the grid (complex buffer, real part = v_H) and optionally rho~(G)."""
         c = self.c
@@ -164,7 +194,9 @@ class Engine:
 
     # ----------------------------------------------------------------- set-up
     def setup(self, xpb):
-        """Per-geometry constants for xp (B, n, 3) in Bohr on the device."""
+        """Per-geometry constants for xp (B, n, 3) in Bohr on the device.  Every
+        tensor whose leading axis is the geometry is listed in st['per'] so the
+        SCF can keep the active geometries as a contiguous prefix."""
         c = self.c
         B = xpb.shape[0]
         dev, f64 = c.device, torch.float64
@@ -180,24 +212,35 @@ class Engine:
         phi = torch.zeros(B, c.nao_pad, c.ld, dtype=f64, device=dev)
         K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, phi)
         st["phi"] = phi
-        if self.precision == "fp32":
+        per = ["xpb", "S", "T", "v", "phi", "vloc", "V", "Hcore"]
+        if self.precision == "mixed":
             phi32 = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float32, device=dev)
             K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, phi32)
-            st["phi_c"] = phi32
-        else:
-            st["phi_c"] = phi
+            st["phi_lp"] = phi32
+            st["phisq"] = (phi[:, :c.nao, :c.N] ** 2).sum(1)             # sum_p phi_p(r)^2
+            st["Da"] = torch.zeros(B, c.nao, c.nao, dtype=f64, device=dev)
+            st["rhoa"] = torch.zeros(B, c.ld, dtype=f64, device=dev)
+            st["kappa"] = torch.zeros(B, dtype=f64, device=dev)          # min rho_a/phisq
+            st["va"] = torch.zeros(B, c.ld, dtype=f64, device=dev)
+            st["Va"] = torch.zeros(B, c.nao, c.nao, dtype=f64, device=dev)
+            st["vmax"] = torch.zeros(B, dtype=f64, device=dev)
+            st["va_set"] = torch.zeros(B, dtype=torch.bool, device=dev)
+            st["err"] = torch.full((B,), float("inf"), dtype=f64, device=dev)
+            per += ["phi_lp", "phisq", "Da", "rhoa", "kappa", "va", "Va", "vmax", "va_set", "err"]
         cb = torch.empty(B, c.g, c.g, c.g, dtype=torch.complex128, device=dev)
         K.vloc_G(st["xpb"], c.g, c.L, c.VG, cb)
         K.fft3d(cb, +1)
         vl = torch.zeros(B, c.ld, dtype=f64, device=dev)
         K.complex_real(cb, c.ld, vl)
         st["vloc"] = vl
-        st["V"] = self.vmat(st["phi_c"], vl)
+        st["V"] = self.vmat(phi, vl)
         st["Hcore"] = T + st["V"]
         if not self.dft:
             ones = torch.zeros(B, c.ld, dtype=f64, device=dev)
             ones[:, :c.N] = 1.0
             st["Sgrid"] = self.vmat(phi, ones)           # dV phi phi^T (exchange G=0 term)
+            per.append("Sgrid")
+        st["per"] = per
         return st
 
     # ------------------------------------------------------------- occupations
@@ -228,10 +271,11 @@ class Engine:
         return -2 * (a.sum(-1) + b.sum(-1))
 
     # ----------------------------------------------------------- two-electron
-    def exchange(self, st, idx, mo, occ):
-        """K (B, nao, nao) from pair-density FFTs (equals sum_rs D_rs (eris+eris0)[p,q,s,r])."""
+    def exchange(self, st, na, mo, occ):
+        """K (B, nao, nao) from pair-density FFTs (equals sum_rs D_rs (eris+eris0)[p,q,s,r])
+        for the first na geometries."""
         c = self.c
-        phi = st["phi"][idx, :c.nao, :c.N]
+        phi = st["phi"][:na, :c.nao, :c.N]
         B = phi.shape[0]
         Kmat = torch.zeros(B, c.nao, c.nao, dtype=torch.float64, device=c.device)
         psi = mo.transpose(-1, -2) @ phi                                # (B, nmo, N)
@@ -248,29 +292,58 @@ class Engine:
                 u = cb.real.reshape(c.nao, c.N)
                 Kmat[b] += occ[b, i] * c.dV * (pair @ u.transpose(0, 1))
         D = (mo * occ.unsqueeze(-2)) @ mo.transpose(-1, -2)
-        Sg = st["Sgrid"][idx]
+        Sg = st["Sgrid"][:na]
         c0 = 4 * math.pi / c.L * EXCHANGE_G0_CONST
         return Kmat + c0 * (Sg @ D @ Sg)
 
-    def fock2e(self, st, idx, D, mo=None, occ=None):
-        """(J + V_xc or J - K/2, E_H + E_xc or E_H + E_x) for the geometries idx."""
+    def fock2e(self, st, na, D, mo, occ, full=False):
+        """(J + V_xc or J - K/2, E_H + E_xc or E_H + E_x) for geometries [0, na)."""
         c = self.c
-        if idx is not None and idx.numel() == st["B"]:
-            idx = None                      # whole batch: no gather of phi
-        phi_c = st["phi_c"][idx] if idx is not None else st["phi_c"]
-        rho = self.density(phi_c, D)
+        sel = lambda key: st[key][:na]
+        phi = sel("phi")
+        mixed = self.precision == "mixed" and not full
+        low_rho = low_v = False
+        if mixed:
+            tol_e = self.MIXED_SAFETY * sel("err") / c.nao     # per-element error budget
+            dD = D - sel("Da")
+            # bound on the relative density error of the FP32 increment
+            rel = self.MIXED_EPS * torch.linalg.norm(dD, dim=(-2, -1)) / sel("kappa")
+            low_rho = bool(((rel <= 1e-2) & (rel * sel("vmax") <= tol_e)).all().item())
+        if low_rho:
+            drho = torch.zeros(na, c.ld, dtype=torch.float64, device=c.device)
+            K.ao_density(sel("phi_lp"), self._pad(dD), drho)
+            rho = sel("rhoa") + drho
+        else:
+            rho = self.orb_density(phi, mo, occ)
+            if self.precision == "mixed":
+                st["Da"][:na] = D
+                st["rhoa"][:na] = rho
+                st["kappa"][:na] = (rho[:, :c.N] / sel("phisq")).amin(-1)
         vh, _ = self.poisson(rho)
-        B = D.shape[0]
         nblk = K.xc_blocks(c.g)
-        v = torch.zeros(B, c.ld, dtype=torch.float64, device=c.device)
-        part = torch.empty(B, nblk, 2, dtype=torch.float64, device=c.device)
+        v = torch.zeros(na, c.ld, dtype=torch.float64, device=c.device)
+        part = torch.empty(na, nblk, 2, dtype=torch.float64, device=c.device)
         K.xc_eval(rho, vh, c.g, self.xc_id, v, part)
         E = part.sum(1)
         Exc, EH = E[:, 0] * c.dV, 0.5 * E[:, 1] * c.dV
-        V2 = self.vmat(phi_c, v)
+        if mixed:
+            dv = v - sel("va")
+            ok = sel("va_set") & (self.MIXED_EPS * dv.abs().amax(-1) <= tol_e)
+            low_v = bool(ok.all().item())
+        if low_v:
+            V2 = sel("Va") + self.vmat(sel("phi_lp"), dv)
+            self.stats["low"] += 1
+        else:
+            V2 = self.vmat(phi, v)
+            self.stats["full"] += 1
+            if self.precision == "mixed":
+                st["va"][:na] = v
+                st["Va"][:na] = V2
+                st["vmax"][:na] = v.abs().amax(-1)
+                st["va_set"][:na] = True
         if self.dft:
             return V2, EH + Exc
-        Vx = -0.5 * self.exchange(st, idx if idx is not None else slice(None), mo, occ)
+        Vx = -0.5 * self.exchange(st, na, mo, occ)
         Ex = 0.5 * torch.einsum("bpq,bqp->b", Vx, D)
         return V2 + Vx, EH + Ex
 
@@ -280,109 +353,166 @@ class Engine:
         w, cc = torch.linalg.eigh(v.transpose(-1, -2) @ F @ v)
         return w, v @ cc
 
+    @staticmethod
+    def _compact(st, keys, loc, order, keep, na):
+        """Move the geometries flagged in keep (bool over [0, na)) to a prefix by
+        swapping them with the inactive ones inside it; returns the new prefix."""
+        new_na = int(keep.sum().item())
+        if new_na == na:
+            return na
+        slots = torch.arange(na, device=keep.device)
+        holes = slots[:new_na][~keep[:new_na]]
+        movers = slots[new_na:][keep[new_na:]]
+        if holes.numel():
+            ia = torch.cat([holes, movers])
+            ib = torch.cat([movers, holes])
+            for k in keys:
+                st[k][ia] = st[k][ib]
+            for k in loc:
+                loc[k][ia] = loc[k][ib]
+            order[ia] = order[ib]
+        return new_na
+
     def scf(self, st):
-        """hqc/pbc/scf.py: returns mo (B,nao,nmo), w1 (B,nmo), E (B) Hartree,
-        converged (B) bool, n_iter (B)."""
+        """hqc/pbc/scf.py on the batch.  Returns tensors in SLOT order (the SCF
+        permutes geometries so the active ones form a prefix) plus `order`:
+        slot i holds geometry order[i].  mo (B,nao,nmo), w1 (B,nmo), E (B)
+        Hartree, converged (B), n_iter (B)."""
         c = self.c
         B = st["B"]
-        v, H, S = st["v"], st["Hcore"], st["S"]
-        w1, mo = self._eigh_orth(H, v)                                   # solver.py:740-745
+        dev = c.device
+        w1, mo = self._eigh_orth(st["Hcore"], st["v"])                   # solver.py:740-745
         D, occ = self.density_matrix(mo, w1)
-        allidx = torch.arange(B, device=c.device)
         tol = self.tol
-
-        def step(Fin, idx):
-            w1, mo = self._eigh_orth(Fin, v[idx])
-            D, occ = self.density_matrix(mo, w1)
-            V2, E2 = self.fock2e(st, idx, D, mo, occ)
-            Enew = torch.einsum("bpq,bqp->b", H[idx], D) + E2
-            F = H[idx] + V2
-            e = S[idx] @ D @ F - F @ D @ S[idx]
-            return mo, w1, Enew, F, e
+        order = torch.arange(B, device=dev)
+        keys = st["per"]
 
         if not self.diis:                                                 # scf.py:21-88
-            E_old = torch.zeros(B, dtype=torch.float64, device=c.device)
-            E = torch.ones_like(E_old)
-            Dc, Oc = D.clone(), occ.clone()
-            Mo, W1 = mo.clone(), w1.clone()
-            nit = torch.zeros(B, dtype=torch.long, device=c.device)
-            act = allidx
-            loop = 0
-            while act.numel() > 0:
-                V2, E2 = self.fock2e(st, act, Dc[act], Mo[act], Oc[act])
-                Enew = torch.einsum("bpq,bqp->b", H[act], Dc[act]) + E2
-                w1n, mon = self._eigh_orth(H[act] + V2, v[act])
+            loc = {"E_old": torch.zeros(B, dtype=torch.float64, device=dev),
+                   "E": torch.ones(B, dtype=torch.float64, device=dev),
+                   "Dc": D.clone(), "Oc": occ.clone(), "Mo": mo.clone(), "W1": w1.clone(),
+                   "nit": torch.zeros(B, dtype=torch.long, device=dev)}
+            na, loop = B, 0
+            while na > 0:
+                H = st["Hcore"][:na]
+                Dc = loc["Dc"][:na]
+                V2, E2 = self.fock2e(st, na, Dc, loc["Mo"][:na], loc["Oc"][:na], full=True)
+                Enew = torch.einsum("bpq,bqp->b", H, Dc) + E2
+                w1n, mon = self._eigh_orth(H + V2, st["v"][:na])
                 Dn, On = self.density_matrix(mon, w1n)
-                E_old[act], E[act] = E[act], Enew
-                Dc[act], Oc[act], Mo[act], W1[act] = Dn, On, mon, w1n
+                loc["E_old"][:na] = loc["E"][:na]
+                loc["E"][:na] = Enew
+                loc["Dc"][:na], loc["Oc"][:na] = Dn, On
+                loc["Mo"][:na], loc["W1"][:na] = mon, w1n
                 loop += 1
-                nit[act] = loop
-                keep = ((E[act] - E_old[act]).abs() > tol) & (loop < self.max_cycle)
-                act = act[keep]
-            return Mo, W1, E, nit != self.max_cycle, nit
+                loc["nit"][:na] = loop
+                keep = ((loc["E"][:na] - loc["E_old"][:na]).abs() > tol) & (loop < self.max_cycle)
+                na = self._compact(st, keys, loc, order, keep, na)
+            return loc["Mo"], loc["W1"], loc["E"], loc["nit"] != self.max_cycle, loc["nit"], order
+
+        def step(Fin, na):
+            w1, mo = self._eigh_orth(Fin, st["v"][:na])
+            D, occ = self.density_matrix(mo, w1)
+            V2, E2 = self.fock2e(st, na, D, mo, occ)
+            H, S = st["Hcore"][:na], st["S"][:na]
+            Enew = torch.einsum("bpq,bqp->b", H, D) + E2
+            F = H + V2
+            e = S @ D @ F - F @ D @ S
+            if self.precision == "mixed":
+                st["err"][:na] = torch.linalg.norm(e, dim=(-2, -1))
+            return mo, w1, Enew, F, e
+
+        def push(loc, na, F, e):
+            """Shift the DIIS window and the Pulay matrix B_ij = <e_i, e_j>."""
+            Fk, ek, Bh = loc["Fk"][:na], loc["ek"][:na], loc["Bh"][:na]
+            Fk_new = torch.cat([Fk[:, 1:], F.unsqueeze(1)], 1)
+            ek_new = torch.cat([ek[:, 1:], e.unsqueeze(1)], 1)
+            d = (ek_new * e.unsqueeze(1)).sum((-1, -2))                  # (na, m)
+            B_new = torch.empty_like(Bh)
+            B_new[:, :-1, :-1] = Bh[:, 1:, 1:]
+            B_new[:, -1, :] = d
+            B_new[:, :, -1] = d
+            loc["Fk"][:na], loc["ek"][:na], loc["Bh"][:na] = Fk_new, ek_new, B_new
 
         m = self.diis_space
-        V2, E2 = self.fock2e(st, None, D, mo, occ)                        # scf.py:122-128
+        V2, E2 = self.fock2e(st, B, D, mo, occ)                           # scf.py:122-128
+        H, S = st["Hcore"], st["S"]
         F0 = H + V2
         e0 = S @ D @ F0 - F0 @ D @ S
-        Fk = F0.unsqueeze(1).repeat(1, m, 1, 1)
-        ek = e0.unsqueeze(1).repeat(1, m, 1, 1)
-        E = torch.ones(B, dtype=torch.float64, device=c.device)
-        Mo, W1 = mo, w1
+        loc = {"Fk": F0.unsqueeze(1).repeat(1, m, 1, 1).contiguous(),
+               "ek": e0.unsqueeze(1).repeat(1, m, 1, 1).contiguous(),
+               "Bh": (e0 * e0).sum((-1, -2))[:, None, None].repeat(1, m, m).contiguous(),
+               "E": torch.ones(B, dtype=torch.float64, device=dev),
+               "E_old": torch.zeros(B, dtype=torch.float64, device=dev),
+               "Mo": mo.clone(), "W1": w1.clone(),
+               "nit": torch.zeros(B, dtype=torch.long, device=dev)}
+        na = B
         for _ in range(self.diis_start_cycle + m):                        # scf.py:130-172
-            Mo, W1, Enew, F, e = step(Fk[:, -1], allidx)
-            Fk = torch.cat([Fk[:, 1:], F.unsqueeze(1)], 1)
-            ek = torch.cat([ek[:, 1:], e.unsqueeze(1)], 1)
-            E = Enew
+            mo_, w1_, Enew, F, e = step(loc["Fk"][:, -1], na)
+            push(loc, na, F, e)
+            loc["Mo"][:], loc["W1"][:] = mo_, w1_
+            loc["E_old"][:] = loc["E"]
+            loc["E"][:] = Enew
         loop = self.diis_start_cycle + m
-        E_old = E - 1.0
-        nit = torch.full((B,), loop, dtype=torch.long, device=c.device)
-        Mo, W1 = Mo.clone(), W1.clone()
+        loc["E_old"][:] = loc["E"] - 1.0
+        loc["nit"][:] = loop
 
-        def cont(idx):
-            err = torch.linalg.norm(ek[idx, -1], dim=(-2, -1))
-            return (((E[idx] - E_old[idx]).abs() > tol) | (err > 10 * tol)) & (loop < self.max_cycle)
+        def cont(na):
+            err = torch.linalg.norm(loc["ek"][:na, -1], dim=(-2, -1))
+            dE = (loc["E"][:na] - loc["E_old"][:na]).abs()
+            return ((dE > tol) | (err > 10 * tol)) & (loop < self.max_cycle)
 
-        act = allidx[cont(allidx)]
-        eye = torch.eye(m, dtype=torch.float64, device=c.device)
-        while act.numel() > 0:                                            # scf.py:174-239
-            e = ek[act]
-            Bm = torch.einsum("bimn,bjmn->bij", e, e)
+        na = self._compact(st, keys, loc, order, cont(na), na)
+        eye = torch.eye(m, dtype=torch.float64, device=dev)
+        while na > 0:                                                     # scf.py:174-239
+            Bm = loc["Bh"][:na]
             eta = 1e-12 * (Bm.diagonal(dim1=-2, dim2=-1).sum(-1) / m + 1.0)
-            Bm = Bm + eta[:, None, None] * eye
-            nb = act.numel()
-            h = torch.zeros(nb, m + 1, m + 1, dtype=torch.float64, device=c.device)
-            h[:, :m, :m] = Bm
+            h = torch.zeros(na, m + 1, m + 1, dtype=torch.float64, device=dev)
+            h[:, :m, :m] = Bm + eta[:, None, None] * eye
             h[:, :m, m] = -1.0
             h[:, m, :m] = -1.0
-            rhs = torch.zeros(nb, m + 1, 1, dtype=torch.float64, device=c.device)
+            rhs = torch.zeros(na, m + 1, 1, dtype=torch.float64, device=dev)
             rhs[:, m, 0] = -1.0
             ck = torch.linalg.solve(h, rhs)[:, :m, 0]
-            Fg = torch.einsum("bk,bkpq->bpq", ck, Fk[act])
-            Fg = (1 - self.diis_damp) * Fg + self.diis_damp * Fk[act, -1]
-            mo_a, w1_a, Enew, F, e_new = step(Fg, act)
-            Fk[act] = torch.cat([Fk[act, 1:], F.unsqueeze(1)], 1)
-            ek[act] = torch.cat([ek[act, 1:], e_new.unsqueeze(1)], 1)
-            E_old[act] = E[act]
-            E[act] = Enew
-            Mo[act], W1[act] = mo_a, w1_a
+            Fk = loc["Fk"][:na]
+            Fg = (ck[:, :, None, None] * Fk).sum(1)
+            Fg = (1 - self.diis_damp) * Fg + self.diis_damp * Fk[:, -1]
+            mo_a, w1_a, Enew, F, e_new = step(Fg, na)
+            push(loc, na, F, e_new)
+            loc["E_old"][:na] = loc["E"][:na]
+            loc["E"][:na] = Enew
+            loc["Mo"][:na], loc["W1"][:na] = mo_a, w1_a
             loop += 1
-            nit[act] = loop
-            act = act[cont(act)]
-        return Mo, W1, E, nit != self.max_cycle, nit
+            loc["nit"][:na] = loop
+            na = self._compact(st, keys, loc, order, cont(na), na)
+        return loc["Mo"], loc["W1"], loc["E"], loc["nit"] != self.max_cycle, loc["nit"], order
 
     # ---------------------------------------------------------------- results
     def solve(self, xpb):
-        """Full solver output for a batch (B, n, 3) Bohr: dict of tensors (Hartree)."""
+        """Full solver output for a batch (B, n, 3) Bohr.
+
+        Per-geometry device state in st stays in slot order (st['order'][i] is
+        the geometry held by slot i); the user-facing results mo, D, w1, E, conv,
+        nit, Ki, Vep, Vee, Se, occ are returned in input order."""
         st = self.setup(xpb)
-        mo, w1, E, conv, nit = self.scf(st)
+        mo, w1, E, conv, nit, order = self.scf(st)
+        B = st["B"]
         D, occ = self.density_matrix(mo, w1)                              # solver.py:758-765
-        idx = torch.arange(st["B"], device=self.c.device)
-        V2, E2 = self.fock2e(st, None, D, mo, occ)
+        V2, E2 = self.fock2e(st, B, D, mo, occ, full=True)
         Ki = torch.einsum("bpq,bqp->b", st["T"], D)
         Vep = torch.einsum("bpq,bqp->b", st["V"], D)
-        Vee = E2
         Se = self.entropy(occ)
-        st.update(mo=mo, w1=w1, D=D, occ=occ, E=E, conv=conv, nit=nit, Ki=Ki, Vep=Vep, Vee=Vee,
-                  Se=Se, idx=idx)
+        slot = dict(mo=mo, w1=w1, D=D, occ=occ, E=E)
+        st["slot"] = slot
+        st["order"] = order
+
+        def unperm(t):
+            out = torch.empty_like(t)
+            out[order] = t
+            return out
+
+        st.update({k: unperm(t) for k, t in dict(mo=mo, w1=w1, D=D, occ=occ, E=E, conv=conv,
+                                                 nit=nit, Ki=Ki, Vep=Vep, Vee=E2,
+                                                 Se=Se).items()})
+        st["unperm"] = unperm
         return st

@@ -26,7 +26,7 @@ def _exchange_adjoint(engine, st):
     c = engine.c
     B = st["B"]
     phi = st["phi"][:, :c.nao, :c.N]
-    mo, occ = st["mo"], st["occ"]
+    mo, occ = st["slot"]["mo"], st["slot"]["occ"]
     Zx = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float64, device=c.device)
     g = c.g
     VG = c.VG.reshape(1, g, g, g)
@@ -46,18 +46,20 @@ def _exchange_adjoint(engine, st):
             u = pair.real.reshape(len(f), c.N)
             Gi = (f[:, None] * psi * u).sum(0)                 # sum_j f_j psi_j u_ij
             Y += f[ii] * torch.outer(C[:, ii], Gi)
-        D = st["D"][b]
+        D = st["slot"]["D"][b]
         M = D @ st["Sgrid"][b] @ D
         Zx[b, :c.nao, :c.N] = -Y - c0 * (M @ phi[b])
     return Zx
 
 
-def compute_forces(engine, st, xp_rs, kappa=10, Gmax=15):
-    """-dA/dR (B, n, 3) in Ha/Bohr after engine.solve; xp_rs (B, n, 3) in rs units."""
+def compute_forces(engine, st, kappa=10, Gmax=15):
+    """-dA/dR (B, n, 3) in Ha/Bohr after engine.solve, in input order.  Works on
+    the slot-ordered device state of the SCF and un-permutes at the end."""
     c = engine.c
     B, dev = st["B"], c.device
     f64 = torch.float64
-    D, occ, w1, mo = st["D"], st["occ"], st["w1"], st["mo"]
+    sl = st["slot"]
+    D, occ, w1, mo = sl["D"], sl["occ"], sl["w1"], sl["mo"]
     S = torch.empty(B, c.nao, c.nao, dtype=f64, device=dev)
     T = torch.empty_like(S)
     dS = torch.empty(B, c.nao, c.nao, 3, dtype=f64, device=dev)
@@ -87,9 +89,9 @@ def compute_forces(engine, st, xp_rs, kappa=10, Gmax=15):
     dE = torch.empty(B, c.n, 3, dtype=f64, device=dev)
     K.vloc_force(st["xpb"], c.g, c.L, c.VG, rg, dE)
     F -= dE
-    Epp, Fpp = ewald_pp(xp_rs, c.L / c.rs, c.rs, kappa, Gmax, forces=True)
-    st["Epp"] = Epp
-    return F + Fpp
+    Epp, Fpp = ewald_pp(st["xpb"] / c.rs, c.L / c.rs, c.rs, kappa, Gmax, forces=True)
+    st["Epp"] = st["unperm"](Epp)
+    return st["unperm"](F + Fpp)
 
 
 class FreeEnergy(torch.autograd.Function):

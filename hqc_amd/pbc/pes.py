@@ -42,13 +42,13 @@ class PES:
 
     def energy_and_forces(self, xp):
         st, single, x = self.solver.run(xp)
-        F = compute_forces(self.solver.engine, st, x, self.kappa, self.Gmax)
+        F = compute_forces(self.solver.engine, st, self.kappa, self.Gmax)
         Etot = st["E"] * RY + st["Epp"]
         return (Etot[0], F[0]) if single else (Etot, F)
 
     def free_energy_and_forces(self, x):
         st, _, x = self.solver.run(x)
-        F = compute_forces(self.solver.engine, st, x, self.kappa, self.Gmax)
+        F = compute_forces(self.solver.engine, st, self.kappa, self.Gmax)
         A = st["E"] * RY - RY * self.solver.engine.sigma * st["Se"] + st["Epp"]
         return A, F
 

@@ -74,6 +74,15 @@ int hqc_ao_density_f64(const double* phi, const double* D, int B, int nao_pad,
 int hqc_ao_density_f32(const float* phi, const double* D, int B, int nao_pad,
                        int ld, double* rho, void* stream);
 
+/* Density in orbital form, rho[b,r] = sum_i (sum_k At[b,k,i] phi[b,k,r])^2,
+ * At = C sqrt(f) (nao_pad x m_pad, m_pad a multiple of 16, zero-padded) over
+ * the orbitals with non-negligible occupation; equal to the D form above for
+ * D = C f C^T.  Same reference operation (hqc/pbc/solver.py:690). */
+int hqc_orb_density_f64(const double* phi, const double* At, int B, int nao_pad, int m_pad,
+                        int ld, double* rho, void* stream);
+int hqc_orb_density_f32(const float* phi, const double* At, int B, int nao_pad, int m_pad,
+                        int ld, double* rho, void* stream);
+
 /* Grid potential matrix V[b] = dV * phi diag(v[b]) phi^T (symmetric, full).
  * Replaces the 'pr,qr,r->pq' contraction (hqc/pbc/solver.py:693) and, through
  * the FFT-Poisson equivalence, the rhoG contractions for V_ep (:427,470) and
@@ -82,8 +91,8 @@ int hqc_ao_density_f32(const float* phi, const double* D, int B, int nao_pad,
 int hqc_ao_vmat_workspace(int B, int nao_pad, int ld, int prec, size_t* bytes);
 int hqc_ao_vmat_f64(const double* phi, const double* v, int B, int nao_pad, int ld,
                     double dV, double* V, void* work, size_t work_bytes, void* stream);
-/* FP32 phi and v on v_mfma_f32_16x16x4f32, FP32 accumulation inside a chunk,
- * FP64 accumulation across chunks; V written in FP64. */
+/* FP32 phi and v on v_mfma_f32_16x16x4f32 with FP32 accumulation inside a
+ * block, FP64 accumulation across blocks; V written in FP64. */
 int hqc_ao_vmat_f32(const float* phi, const double* v, int B, int nao_pad, int ld,
                     double dV, double* V, void* work, size_t work_bytes, void* stream);
 

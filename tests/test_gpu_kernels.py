@@ -81,6 +81,29 @@ def test_density_and_vmat_f32(K, nao_pad, g):
     assert torch.allclose(V, Vref, rtol=2e-5)
 
 
+@pytest.mark.parametrize("nao_pad,m,dt", [(48, 20, torch.float64), (112, 60, torch.float64),
+                                           (112, 60, torch.float32)])
+def test_orb_density(K, nao_pad, m, dt):
+    """rho from At = C sqrt(f) equals the D-form density with D = C f C^T."""
+    torch.manual_seed(2)
+    g = 11
+    N = g ** 3
+    ld = (N + 63) // 64 * 64
+    phi = torch.zeros(2, nao_pad, ld, dtype=torch.float64, device=DEV)
+    phi[:, :, :N] = torch.randn(2, nao_pad, N, dtype=torch.float64, device=DEV)
+    C = torch.randn(2, nao_pad, m, dtype=torch.float64, device=DEV)
+    f = torch.rand(2, m, dtype=torch.float64, device=DEV) * 2
+    m_pad = (m + 15) // 16 * 16
+    At = torch.zeros(2, nao_pad, m_pad, dtype=torch.float64, device=DEV)
+    At[:, :, :m] = C * f.sqrt()[:, None, :]
+    rho = torch.zeros(2, ld, dtype=torch.float64, device=DEV)
+    K.orb_density(phi.to(dt).contiguous(), At, rho)
+    D = (C * f[:, None, :]) @ C.transpose(1, 2)
+    rref = (phi * (D @ phi)).sum(1)
+    tol = 1e-11 if dt == torch.float64 else 1e-4
+    assert torch.allclose(rho, rref, rtol=tol, atol=tol * rref.abs().max().item())
+
+
 @pytest.mark.parametrize("g", [1, 2, 5, 7, 13, 24, 25, 27, 33, 48, 49, 55, 61, 64, 65])
 def test_fft3d(K, g):
     print(f"[fft3d] g={g}", flush=True)

@@ -100,3 +100,22 @@ def test_batched_equals_single():
     Eb = pes(xps).cpu().numpy()
     for b in range(3):
         assert abs(pes(xps[b]).item() - Eb[b]) < 1e-10
+
+
+def test_mixed_precision_matches_fp64():
+    """Defect-corrected FP32 increments converge to the FP64 answer within the
+    north-star thresholds (dE <= 1 uHa/atom, dF <= 1e-5 Ha/Bohr, dbands <= 1e-6 Ha)."""
+    from hqc_amd.pbc import make_pes
+    n, rs = 16, 1.25
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    xps = np.random.default_rng(3).uniform(0, L, (2, n, 3))
+    kw = dict(grid_length=0.2, dft=True, smearing=True, smearing_sigma=0.0316681, tol=1e-8)
+    p64 = make_pes(n, L, rs, "gth-dzv", precision="fp64", **kw)
+    pmx = make_pes(n, L, rs, "gth-dzv", precision="mixed", **kw)
+    E64, F64 = p64.energy_and_forces(xps)
+    Emx, Fmx = pmx.energy_and_forces(xps)
+    assert bool(pmx.solver.last["conv"].all())
+    assert (E64 - Emx).abs().max().item() / 2 / n <= 1e-6
+    assert (F64 - Fmx).abs().max().item() <= 1e-5
+    assert (p64.solver.last["w1"] - pmx.solver.last["w1"]).abs().max().item() <= 1e-6
+    assert pmx.solver.engine.stats["low"] > 0
