@@ -144,7 +144,10 @@ def main():
                      "frac": achieved / peak, "traffic": None,
                      "ms_per_launch": ms, "flops_per_launch": flops},
     }
-    out["config"]["contractions"] = dict(pes.solver.engine.stats)
+    eng = pes.solver.engine
+    out["config"]["contractions"] = dict(eng.stats)
+    if eng.eigh_calls:
+        out["config"]["jacobi_sweeps_per_eigh"] = eng.sweeps.item() / eng.eigh_calls
     if a.compare:
         other = make("mixed" if a.path == "fp64" else "fp64")
         E2, F2 = other.energy_and_forces(x)

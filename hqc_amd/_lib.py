@@ -40,12 +40,17 @@ _SIGS = {
     "hqc_xc_blocks": [_I],
     "hqc_xc_eval": [_P, _P, _I, _I, _I, _I, _P, _P, _P],
     "hqc_occupations": [_P, _I, _I, _I, _D, _I, _I, _I, _D, _D, _P, _P, _P],
+    "hqc_eigh_lds_bytes": [_I],
+    "hqc_eigh_jacobi": [_P, _I, _I, _P, _I, _D, _P, _P, _P, _P],
     "hqc_overlap_kinetic": [_P, _I, _I, _P, _I, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P],
     "hqc_ewald_pairs": [_P, _I, _I, _P, _P, _I, _D, _D, _P, _P],
     "hqc_force_blocks": [_I],
     "hqc_grid_forces": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _D, _P, _P],
     "hqc_vloc_force": [_P, _I, _I, _I, _D, _P, _P, _P, _P],
 }
+
+
+_RESTYPES = {"hqc_eigh_lds_bytes": ctypes.c_size_t}
 
 
 def lib():
@@ -58,7 +63,7 @@ def lib():
         for name, args in _SIGS.items():
             f = getattr(h, name)
             f.argtypes = args
-            f.restype = ctypes.c_int
+            f.restype = _RESTYPES.get(name, ctypes.c_int)
         _lib = h
     return _lib
 
@@ -197,6 +202,36 @@ def occupations(w, nelec, sigma, method, search, max_cycle, tol, mu_shift, occ, 
     _call("hqc_occupations", _ptr(w, F64, "w"), B, nmo, int(nelec), float(sigma), int(method),
           int(search), int(max_cycle), float(tol), float(mu_shift), _ptr(occ, F64, "occ"),
           _ptr(mu, F64, "mu"), _stream())
+
+
+EIGH_JACOBI_MAXN = 110
+_PAIRS = {}
+
+
+def jacobi_pairs(n, device):
+    """Round-robin (circle method) pairing schedule for n (padded to even) indices."""
+    key = (n, str(device))
+    if key not in _PAIRS:
+        npad = n + (n & 1)
+        arr = list(range(npad))
+        rows = []
+        for _ in range(npad - 1):
+            rows.append([x for i in range(npad // 2) for x in (arr[i], arr[npad - 1 - i])])
+            arr = [arr[0]] + [arr[-1]] + arr[1:-1]
+        _PAIRS[key] = torch.tensor(rows, dtype=torch.int32, device=device).contiguous()
+    return _PAIRS[key]
+
+
+def eigh_jacobi(A, max_sweeps=30, tol=1e-14, sweeps=None):
+    """Ascending eigenvalues and column eigenvectors of symmetric A (B, n, n), n <= 110."""
+    B, n, _ = A.shape
+    A = A.contiguous()
+    w = torch.empty(B, n, dtype=F64, device=A.device)
+    V = torch.empty(B, n, n, dtype=F64, device=A.device)
+    _call("hqc_eigh_jacobi", _ptr(A, F64, "A"), B, n, _ptr(jacobi_pairs(n, A.device), I32),
+          int(max_sweeps), float(tol), _ptr(w, F64, "w"), _ptr(V, F64, "V"),
+          _ptr(sweeps, I32, "sweeps"), _stream())
+    return w, V
 
 
 def overlap_kinetic(xp, lattice, alpha, coef, comp, S, T, dS=None, dT=None):

@@ -31,6 +31,12 @@ def _round_up(x, m):
     return (x + m - 1) // m * m
 
 
+def _tr(A, B):
+    """Batched Tr[A B] = sum_pq A_pq B_qp (elementwise; a batched einsum here
+    lowers to a slow GEMM shape)."""
+    return (A * B.transpose(-1, -2)).sum((-1, -2))
+
+
 class Cell:
     """Static data of one (n, L, rs, basis, grid) family, host and device copies."""
 
@@ -87,7 +93,8 @@ class Engine:
     DIIS, occupations, FFTs, xc and every energy sum stay FP64."""
 
     MIXED_EPS = 2e-6        # effective relative error of an FP32-accumulated increment
-    MIXED_SAFETY = 1e-2
+    MIXED_SAFETY = 1e-2     # potential increment: error per element <= 1e-2 residual / n_ao
+    MIXED_RHO_SAFETY = 1.0  # density increment: |v| x relative rho error <= residual
 
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
@@ -130,7 +137,9 @@ class Engine:
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
-        self.stats = {"full": 0, "low": 0}
+        self.stats = {"full": 0, "low": 0, "rho_low": 0}
+        self.sweeps = torch.zeros((), dtype=torch.long, device=cell.device)
+        self.eigh_calls = 0
         self._work = {}
 
     # ------------------------------------------------------------------ utils
@@ -259,7 +268,7 @@ class Engine:
 
     def density_matrix(self, mo, w1):
         occ = self.occupations(w1)
-        return (mo * occ.unsqueeze(-2)) @ mo.transpose(-1, -2), occ
+        return (mo * occ.unsqueeze(-2)) @ mo.transpose(-1, -2).contiguous(), occ
 
     @staticmethod
     def entropy(occ, eps=1e-15):
@@ -308,8 +317,10 @@ class Engine:
             dD = D - sel("Da")
             # bound on the relative density error of the FP32 increment
             rel = self.MIXED_EPS * torch.linalg.norm(dD, dim=(-2, -1)) / sel("kappa")
-            low_rho = bool(((rel <= 1e-2) & (rel * sel("vmax") <= tol_e)).all().item())
+            budget = self.MIXED_RHO_SAFETY * sel("err")
+            low_rho = bool(((rel <= 1e-2) & (rel * sel("vmax") <= budget)).all().item())
         if low_rho:
+            self.stats["rho_low"] += 1
             drho = torch.zeros(na, c.ld, dtype=torch.float64, device=c.device)
             K.ao_density(sel("phi_lp"), self._pad(dD), drho)
             rho = sel("rhoa") + drho
@@ -344,14 +355,32 @@ class Engine:
         if self.dft:
             return V2, EH + Exc
         Vx = -0.5 * self.exchange(st, na, mo, occ)
-        Ex = 0.5 * torch.einsum("bpq,bqp->b", Vx, D)
+        Ex = 0.5 * _tr(Vx, D)
         return V2 + Vx, EH + Ex
 
     # --------------------------------------------------------------------- SCF
-    @staticmethod
-    def _eigh_orth(F, v):
-        w, cc = torch.linalg.eigh(v.transpose(-1, -2) @ F @ v)
-        return w, v @ cc
+    def _eigh_orth(self, F, v, Cw=None):
+        """Roothaan step (hqc/pbc/scf.py:68-72): F' = v^T F v, eigh, mo = v C'.
+
+        n_ao <= 110 uses the LDS Jacobi kernel, warm-started in the previous
+        eigenbasis Cw when given (A = Cw^T F' Cw is then nearly diagonal and a
+        few sweeps suffice); larger problems use torch.linalg.eigh.  Returns
+        (w, mo, C') with C' orthonormal eigenvectors of F'."""
+        # transposed operands are made contiguous: rocBLAS picks a far slower
+        # kernel for batched 108x108 products with a transposed A
+        Fp = v.transpose(-1, -2).contiguous() @ (F @ v)
+        if self.c.nao <= K.EIGH_JACOBI_MAXN:
+            sw = torch.zeros(F.shape[0], dtype=torch.int32, device=F.device)
+            if Cw is not None:
+                w, U = K.eigh_jacobi(Cw.transpose(-1, -2).contiguous() @ (Fp @ Cw), sweeps=sw)
+                C = Cw @ U
+            else:
+                w, C = K.eigh_jacobi(Fp, sweeps=sw)
+            self.sweeps += sw.sum()
+            self.eigh_calls += F.shape[0]
+        else:
+            w, C = torch.linalg.eigh(Fp)
+        return w, v @ C, C
 
     @staticmethod
     def _compact(st, keys, loc, order, keep, na):
@@ -381,7 +410,7 @@ class Engine:
         c = self.c
         B = st["B"]
         dev = c.device
-        w1, mo = self._eigh_orth(st["Hcore"], st["v"])                   # solver.py:740-745
+        w1, mo, Cw = self._eigh_orth(st["Hcore"], st["v"])               # solver.py:740-745
         D, occ = self.density_matrix(mo, w1)
         tol = self.tol
         order = torch.arange(B, device=dev)
@@ -391,14 +420,15 @@ class Engine:
             loc = {"E_old": torch.zeros(B, dtype=torch.float64, device=dev),
                    "E": torch.ones(B, dtype=torch.float64, device=dev),
                    "Dc": D.clone(), "Oc": occ.clone(), "Mo": mo.clone(), "W1": w1.clone(),
-                   "nit": torch.zeros(B, dtype=torch.long, device=dev)}
+                   "Cw": Cw.clone(), "nit": torch.zeros(B, dtype=torch.long, device=dev)}
             na, loop = B, 0
             while na > 0:
                 H = st["Hcore"][:na]
                 Dc = loc["Dc"][:na]
                 V2, E2 = self.fock2e(st, na, Dc, loc["Mo"][:na], loc["Oc"][:na], full=True)
-                Enew = torch.einsum("bpq,bqp->b", H, Dc) + E2
-                w1n, mon = self._eigh_orth(H + V2, st["v"][:na])
+                Enew = _tr(H, Dc) + E2
+                w1n, mon, cwn = self._eigh_orth(H + V2, st["v"][:na], loc["Cw"][:na])
+                loc["Cw"][:na] = cwn
                 Dn, On = self.density_matrix(mon, w1n)
                 loc["E_old"][:na] = loc["E"][:na]
                 loc["E"][:na] = Enew
@@ -411,11 +441,12 @@ class Engine:
             return loc["Mo"], loc["W1"], loc["E"], loc["nit"] != self.max_cycle, loc["nit"], order
 
         def step(Fin, na):
-            w1, mo = self._eigh_orth(Fin, st["v"][:na])
+            w1, mo, cw = self._eigh_orth(Fin, st["v"][:na], loc["Cw"][:na])
+            loc["Cw"][:na] = cw
             D, occ = self.density_matrix(mo, w1)
             V2, E2 = self.fock2e(st, na, D, mo, occ)
             H, S = st["Hcore"][:na], st["S"][:na]
-            Enew = torch.einsum("bpq,bqp->b", H, D) + E2
+            Enew = _tr(H, D) + E2
             F = H + V2
             e = S @ D @ F - F @ D @ S
             if self.precision == "mixed":
@@ -444,7 +475,7 @@ class Engine:
                "Bh": (e0 * e0).sum((-1, -2))[:, None, None].repeat(1, m, m).contiguous(),
                "E": torch.ones(B, dtype=torch.float64, device=dev),
                "E_old": torch.zeros(B, dtype=torch.float64, device=dev),
-               "Mo": mo.clone(), "W1": w1.clone(),
+               "Mo": mo.clone(), "W1": w1.clone(), "Cw": Cw.clone(),
                "nit": torch.zeros(B, dtype=torch.long, device=dev)}
         na = B
         for _ in range(self.diis_start_cycle + m):                        # scf.py:130-172
@@ -499,8 +530,8 @@ class Engine:
         B = st["B"]
         D, occ = self.density_matrix(mo, w1)                              # solver.py:758-765
         V2, E2 = self.fock2e(st, B, D, mo, occ, full=True)
-        Ki = torch.einsum("bpq,bqp->b", st["T"], D)
-        Vep = torch.einsum("bpq,bqp->b", st["V"], D)
+        Ki = _tr(st["T"], D)
+        Vep = _tr(st["V"], D)
         Se = self.entropy(occ)
         slot = dict(mo=mo, w1=w1, D=D, occ=occ, E=E)
         st["slot"] = slot

@@ -138,6 +138,19 @@ int hqc_occupations(const double* w, int B, int nmo, int nelec, double sigma, in
                     int search, int max_cycle, double tol, double mu_shift, double* occ,
                     double* mu, void* stream);
 
+/* Batched symmetric eigensolver, cyclic two-sided Jacobi in LDS, n <= 110.
+ * Replaces jnp.linalg.eigh of the orthogonalised Fock matrix
+ * (hqc/pbc/scf.py:69,138,198, hqc/pbc/solver.py:741).
+ * A (B,n,n) symmetric; pairs ((npad-1) x npad ints, npad = n rounded up to
+ * even) is a round-robin pairing schedule: every row lists npad/2 disjoint
+ * pairs and every pair occurs once per sweep.  Stops when the off-diagonal
+ * Frobenius norm <= tol * ||A||_F or after max_sweeps.  w (B,n) ascending;
+ * V (B,n,n) eigenvectors as columns; sweeps (B) or NULL.
+ * hqc_eigh_lds_bytes(n) is the LDS one block needs. */
+size_t hqc_eigh_lds_bytes(int n);
+int hqc_eigh_jacobi(const double* A, int B, int n, const int* pairs, int max_sweeps, double tol,
+                    double* w, double* V, int* sweeps, void* stream);
+
 /* Analytic periodic overlap and kinetic matrices by lattice sums
  * (hqc/pbc/solver.py:233-389, the s/p derivative trick of :307-341).
  * xp (B,n,3) Bohr; lattice (nT,3); alpha (na); coef (na,nloc); comp (nloc).

@@ -228,3 +228,27 @@ def test_ewald(K):
         Er, Fr = R.ewald_pp(xp[b], L, rs, forces=True)
         assert abs(E[b].item() - Er) < 1e-11
         assert np.allclose(F[b].cpu().numpy(), Fr, atol=1e-11)
+
+
+@pytest.mark.parametrize("n", [2, 7, 8, 33, 107, 108])
+def test_eigh_jacobi(K, n):
+    """Jacobi eigensolver vs torch.linalg.eigh: eigenvalues, residual, orthogonality;
+    also warm-started in an approximate eigenbasis (as the SCF uses it)."""
+    torch.manual_seed(n)
+    A = torch.randn(3, n, n, dtype=torch.float64, device=DEV)
+    A = A + A.transpose(1, 2)
+    A[0] = torch.diag(torch.arange(n, dtype=torch.float64, device=DEV) % 3)  # degenerate
+    sw = torch.zeros(3, dtype=torch.int32, device=DEV)
+    w, V = K.eigh_jacobi(A, sweeps=sw)
+    wr = torch.linalg.eigvalsh(A)
+    scale = A.abs().max().item()
+    assert torch.allclose(w, wr, atol=1e-12 * scale)
+    I = torch.eye(n, dtype=torch.float64, device=DEV)
+    assert torch.allclose(V.transpose(1, 2) @ V, I.expand(3, n, n), atol=1e-12)
+    assert torch.allclose(A @ V, V * w[:, None, :], atol=1e-11 * scale)
+    # warm start: rotate by a nearby orthogonal matrix, few sweeps needed
+    _, C = torch.linalg.eigh(A + 1e-3 * torch.randn_like(A).transpose(1, 2).add(A * 0))
+    Aw = C.transpose(1, 2) @ A @ C
+    w2, U = K.eigh_jacobi(Aw, sweeps=sw)
+    assert torch.allclose(w2, wr, atol=1e-12 * scale)
+    assert torch.allclose(A @ (C @ U), (C @ U) * w2[:, None, :], atol=1e-11 * scale)
