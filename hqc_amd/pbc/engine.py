@@ -144,12 +144,13 @@ class Engine:
 
     # ------------------------------------------------------------------ utils
     def _workspace(self, B, prec):
-        key = (B, prec)
-        if key not in self._work:
-            nbytes = K.vmat_workspace(B, self.c.nao_pad, self.c.ld, prec)
-            self._work[key] = torch.empty(max(nbytes // 8, 1), dtype=torch.float64,
-                                          device=self.c.device)
-        return self._work[key]
+        # one grow-only buffer per precision; calls of any batch size share it
+        nbytes = K.vmat_workspace(B, self.c.nao_pad, self.c.ld, prec)
+        buf = self._work.get(prec)
+        if buf is None or buf.numel() * 8 < nbytes:
+            buf = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.c.device)
+            self._work[prec] = buf
+        return buf
 
     def _pad(self, M):
         c = self.c
@@ -161,11 +162,14 @@ class Engine:
 
     def vmat(self, phi, v):
         """dV * phi diag(v) phi^T for a batch -> (B, nao, nao)."""
+        return self.vmat_dv(phi, v, self.c.dV)
+
+    def vmat_dv(self, phi, v, dV):
         c = self.c
         B = phi.shape[0]
         prec = 64 if phi.dtype == torch.float64 else 32
         V = torch.empty(B, c.nao_pad, c.nao_pad, dtype=torch.float64, device=c.device)
-        K.ao_vmat(phi, v, c.dV, V, self._workspace(B, prec))
+        K.ao_vmat(phi, v, dV, V, self._workspace(B, prec))
         return V[:, :c.nao, :c.nao]
 
     def density(self, phi, D, X=None):
@@ -280,27 +284,42 @@ class Engine:
         return -2 * (a.sum(-1) + b.sum(-1))
 
     # ----------------------------------------------------------- two-electron
+    EXCHANGE_CHUNK_BYTES = 2 << 30      # pair-density working set per batch of orbitals
+
     def exchange(self, st, na, mo, occ):
-        """K (B, nao, nao) from pair-density FFTs (equals sum_rs D_rs (eris+eris0)[p,q,s,r])
-        for the first na geometries."""
+        """K (B, nao, nao) for geometries [0, na): equals sum_rs D_rs (eris+eris0)[p,q,s,r]
+        (hqc/pbc/solver.py:430-431, 604-614).
+
+        With A_i = sqrt(f_i) dV FFT(phi * psi_i) (one 3-D FFT per AO and occupied
+        orbital), K = sum_i Re[A_i diag(VG) A_i^H] = sum_i (Re A_i) VG (Re A_i)^T +
+        (Im A_i) VG (Im A_i)^T: the same MFMA contraction as V_xc, batched over the
+        orbitals; plus the G=0 term c0 S~ D S~ with the literal 0.22578495."""
         c = self.c
-        phi = st["phi"][:na, :c.nao, :c.N]
-        B = phi.shape[0]
-        Kmat = torch.zeros(B, c.nao, c.nao, dtype=torch.float64, device=c.device)
-        psi = mo.transpose(-1, -2) @ phi                                # (B, nmo, N)
-        keep = occ > 1e-14
-        g = c.g
-        for b in range(B):
-            for i in torch.nonzero(keep[b]).flatten().tolist():
-                pair = (phi[b] * psi[b, i]).contiguous()                 # (nao, N)
-                cb = pair.to(torch.complex128).reshape(c.nao, g, g, g).contiguous()
-                cb *= c.dV
+        g, N = c.g, c.N
+        phi = st["phi"][:na]
+        psi = mo.transpose(-1, -2).contiguous() @ phi[:, :c.nao, :]        # (na, nmo, ld)
+        Kmat = torch.zeros(na, c.nao, c.nao, dtype=torch.float64, device=c.device)
+        per_orb = c.nao * (N * 16 + 2 * c.nao_pad // c.nao * c.ld * 8 + c.ld * 8)
+        chunk = max(1, int(self.EXCHANGE_CHUNK_BYTES // per_orb))
+        VGp = torch.zeros(c.ld, dtype=torch.float64, device=c.device)
+        VGp[:N] = c.VG
+        for b in range(na):
+            idx = torch.nonzero(occ[b] > 1e-14).flatten()
+            w = occ[b, idx].sqrt() * c.dV
+            for s0 in range(0, idx.numel(), chunk):
+                ii = idx[s0:s0 + chunk]
+                m = ii.numel()
+                pair = phi[b, None, :c.nao, :N] * (psi[b, ii, None, :N] * w[s0:s0 + m, None, None])
+                cb = pair.to(torch.complex128).reshape(m * c.nao, g, g, g)
                 K.fft3d(cb, -1)
-                cb *= c.VG.reshape(1, g, g, g)
-                K.fft3d(cb, +1)
-                u = cb.real.reshape(c.nao, c.N)
-                Kmat[b] += occ[b, i] * c.dV * (pair @ u.transpose(0, 1))
-        D = (mo * occ.unsqueeze(-2)) @ mo.transpose(-1, -2)
+                cb = cb.reshape(m, c.nao, N)
+                part = torch.zeros(m, c.nao_pad, c.ld, dtype=torch.float64, device=c.device)
+                v = VGp.expand(m, c.ld).contiguous()
+                part[:, :c.nao, :N] = cb.real
+                Kmat[b] += self.vmat_dv(part, v, 1.0).sum(0)
+                part[:, :c.nao, :N] = cb.imag
+                Kmat[b] += self.vmat_dv(part, v, 1.0).sum(0)
+        D = (mo * occ.unsqueeze(-2)) @ mo.transpose(-1, -2).contiguous()
         Sg = st["Sgrid"][:na]
         c0 = 4 * math.pi / c.L * EXCHANGE_G0_CONST
         return Kmat + c0 * (Sg @ D @ Sg)
