@@ -9,6 +9,14 @@ grid_length 0.16 (49^3), SCF tol 1e-8 Ha, bisection mu search (make_pes
 defaults otherwise); energy (E + V_pp) plus analytic forces for every
 geometry.  Inputs are already on the GPU when the timed region starts.
 
+`value` is the throughput of --path (default: the mixed-precision batched
+path, batch 64).  Beside it the run measures, untimed for `value`:
+  * the FP64 path on the same batch, whose results also give the accuracy of
+    the mixed path (max |dE| per atom, |dF|, |d bands| against FP64);
+  * the plain FP64 path, one geometry per call (seeds 0..3);
+  * the roofline of the dominant kernel, from HIP events on its stream;
+  * the oracle on the host for one geometry (cpu_baseline).
+
     python bench.py [--steps K] [--warmup W] [--batch B] [--path fp64|mixed]
 
 Prints ONE JSON line.  Only one GPU is supported (--gpus N>1 exits).
@@ -75,6 +83,81 @@ def time_vmat(pes, B, reps=20):
     return ms, flops
 
 
+def time_orb_density(pes, B, reps=20):
+    """Average duration of the orbital-form density kernel (rho = sum_i f_i psi_i^2,
+    FP64) at the workload shape with the occupations of the last run."""
+    import torch
+    from hqc_amd import _lib as K
+    eng = pes.solver.engine
+    c = eng.c
+    last = pes.solver.last
+    mo, occ = last["mo"][:B], last["occ"][:B]
+    keep = max(int((occ > eng.OCC_CUT).sum(-1).max().item()), 1)
+    m_pad = (keep + 15) // 16 * 16
+    At = torch.zeros(B, c.nao_pad, m_pad, dtype=torch.float64, device="cuda")
+    At[:, :c.nao, :keep] = mo[:, :, :keep] * occ[:, None, :keep].clamp(min=0).sqrt()
+    phi = torch.rand(B, c.nao_pad, c.ld, dtype=torch.float64, device="cuda")
+    rho = torch.zeros(B, c.ld, dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        K.orb_density(phi, At, rho)
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        K.orb_density(phi, At, rho)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    flops = 2.0 * (c.nao + 1) * keep * c.N * B       # psi = C^T phi, then sum f psi^2
+    return ms, flops, keep
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_54H_dzv_b64.txt")
+
+
+def measured_traffic(kernel_prefix, B):
+    """HBM bytes per launch of a kernel at this workload with batch 64, from the
+    calibrated rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE table kept with the profiles
+    (None for other batch sizes or when the table is absent)."""
+    if B != 64 or not os.path.isfile(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        parts = line.split()
+        if len(parts) >= 5 and not line.startswith("#") and line[13:].startswith(kernel_prefix):
+            return float(parts[-1])
+    return None
+
+
+def run_path(path, x, steps, warmup):
+    """(pes, seconds per call of energy_and_forces(x), E, F)."""
+    import torch
+    pes = make(path)
+    for _ in range(warmup):
+        pes.energy_and_forces(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        E, F = pes.energy_and_forces(x)
+    torch.cuda.synchronize()
+    return pes, (time.perf_counter() - t0) / steps, E, F
+
+
+def plain_fp64(ngeom=4):
+    """The plain FP64 path: one geometry per call, seeds 0..ngeom-1 (after one
+    untimed call)."""
+    import torch
+    xps, _ = workload_xp(ngeom)
+    pes = make("fp64")
+    pes.energy_and_forces(torch.tensor(xps[:1], dtype=torch.float64, device="cuda"))
+    xs = [torch.tensor(xps[b:b + 1], dtype=torch.float64, device="cuda") for b in range(ngeom)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for x in xs:
+        pes.energy_and_forces(x)
+    torch.cuda.synchronize()
+    return ngeom / (time.perf_counter() - t0)
+
+
 def cpu_baseline(sample_geoms=1):
     """Oracle (NumPy FP64, FFT-Poisson restatement) on the host: energy + analytic
     forces of `sample_geoms` config geometries."""
@@ -93,17 +176,100 @@ def cpu_baseline(sample_geoms=1):
                       f"{dt:.1f} s"}
 
 
+def kernel_probe(name, B, reps=5):
+    """Launch one kernel `reps` times at the workload shape (B geometries) and exit:
+    the program that rocprofv3 --pmc counts.  'copy' is the calibration kernel: a
+    device-to-device copy of exactly 1 GiB (read 1 GiB, write 1 GiB)."""
+    import torch
+    from hqc_amd import _lib as K
+    if name == "copy":
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(reps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        return
+    pes = make("mixed" if name == "vmat_f32" else "fp64")
+    c = pes.solver.engine.c
+    if name in ("vmat_f64", "vmat_f32"):
+        dt = torch.float64 if name == "vmat_f64" else torch.float32
+        phi = torch.rand(B, c.nao_pad, c.ld, dtype=torch.float64, device="cuda").to(dt)
+        v = torch.rand(B, c.ld, dtype=torch.float64, device="cuda")
+        for _ in range(reps):
+            pes.solver.engine.vmat(phi, v)
+    elif name == "orb_density":
+        keep = 64
+        At = torch.rand(B, c.nao_pad, keep, dtype=torch.float64, device="cuda")
+        phi = torch.rand(B, c.nao_pad, c.ld, dtype=torch.float64, device="cuda")
+        rho = torch.zeros(B, c.ld, dtype=torch.float64, device="cuda")
+        for _ in range(reps):
+            K.orb_density(phi, At, rho)
+    else:
+        raise SystemExit(f"unknown kernel {name}")
+    torch.cuda.synchronize()
+
+
+def summarize_trace(d, top=25):
+    """Text table (kernel, calls, avg us, total ms, %) from a rocprofv3 results db
+    (its top_kernels view holds durations in microseconds)."""
+    import glob
+    import re
+    import sqlite3
+    db = glob.glob(os.path.join(d, "**", "*.db"), recursive=True)[0]
+    cur = sqlite3.connect(db).cursor()
+    rows = cur.execute("select name,total_calls,total_duration,average,percentage "
+                       "from top_kernels").fetchall()
+
+    def short(n):
+        n = re.sub(r"\(.*", "", n)
+        return ("rocBLAS " + n[:40] + "...") if n.startswith("Cijk_") else n[:70]
+    tot = sum(r[2] for r in rows)
+    lines = [f"{'kernel':72s} {'calls':>6s} {'avg_us':>9s} {'total_ms':>9s} {'%':>6s}"]
+    for r in rows[:top]:
+        lines.append(f"{short(r[0]):72s} {r[1]:6d} {r[3]:9.1f} {r[2] / 1e3:9.2f} "
+                     f"{r[4]:6.2f}")
+    lines.append(f"total GPU kernel time: {tot / 1e3:.1f} ms over "
+                 f"{sum(r[1] for r in rows)} dispatches")
+    return "\n".join(lines)
+
+
+def summarize_pmc(d):
+    """Mean value per dispatch of each counter, per kernel, from rocprofv3 --pmc
+    csv output."""
+    import csv
+    import glob
+    acc = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for row in csv.DictReader(open(f)):
+            key = (row["Kernel_Name"][:60], row["Counter_Name"])
+            acc.setdefault(key, []).append(float(row["Counter_Value"]))
+    lines = [f"{'kernel':62s} {'counter':14s} {'dispatches':>10s} {'mean':>14s}"]
+    for (k, cn), vals in sorted(acc.items()):
+        lines.append(f"{k:62s} {cn:14s} {len(vals):10d} {sum(vals) / len(vals):14.1f}")
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--path", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--path", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--compare", action="store_true",
-                    help="also run the other precision path once and report its error")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the FP64 batched and plain runs (accuracy and speedups)")
+    ap.add_argument("--kernel-probe", default=None,
+                    help="vmat_f64|vmat_f32|orb_density|copy: launch that kernel only")
+    ap.add_argument("--summarize-trace", default=None, metavar="DIR")
+    ap.add_argument("--summarize-pmc", default=None, metavar="DIR")
     a = ap.parse_args()
+    if a.summarize_trace:
+        print(summarize_trace(a.summarize_trace))
+        return
+    if a.summarize_pmc:
+        print(summarize_pmc(a.summarize_pmc))
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (--gpus 1); "
               f"--gpus {a.gpus} is not supported.", file=sys.stderr)
@@ -112,24 +278,39 @@ def main():
     from hqc_amd.pbc.engine import RY
 
     B = a.batch
+    if a.kernel_probe:
+        kernel_probe(a.kernel_probe, B)
+        return
     xps, _ = workload_xp(B)
     x = torch.tensor(xps, dtype=torch.float64, device="cuda")
-    pes = make(a.path)
-    for _ in range(a.warmup):
-        pes.energy_and_forces(x)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        E, F = pes.energy_and_forces(x)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / a.steps
+    pes, dt, E, F = run_path(a.path, x, a.steps, a.warmup)
     st = pes.solver.last
+    eng = pes.solver.engine
     nit = st["nit"].double().mean().item()
     conv = bool(st["conv"].all().item())
-    ms, flops = time_vmat(pes, B)
-    prec = "f64" if a.path == "fp64" else "f32"
-    achieved = flops / (ms * 1e-3) / 1e12
-    peak = MI355X_PEAK[prec]
+    stats = dict(eng.stats)
+    sweeps = eng.sweeps.item() / eng.eigh_calls if eng.eigh_calls else None
+    # dominant kernels: potential matrix (FP64 or FP32) and FP64 orbital density
+    ms_v, fl_v = time_vmat(pes, B)
+    ms_d, fl_d, keep = time_orb_density(pes, B)
+    prec_v = "f64" if a.path == "fp64" else "f32"
+    n_fock = stats["full"] + stats["low"]
+    n_dens64 = n_fock - stats.get("rho_low", 0)
+    vname = "k_vmat_pipe<" + ("double" if prec_v == "f64" else "float")
+    cand = [
+        {"kernel": vname + "> (dV phi diag(v) phi^T)", "ms_per_launch": ms_v,
+         "flops_per_launch": fl_v, "peak": MI355X_PEAK[prec_v],
+         "calls_per_step": n_fock / max(a.steps + a.warmup, 1),
+         "traffic": measured_traffic(vname, B)},
+        {"kernel": "k_orb_density<double> (sum_i f_i (C^T phi)_i^2)", "ms_per_launch": ms_d,
+         "flops_per_launch": fl_d, "peak": MI355X_PEAK["f64"], "orbitals": keep,
+         "calls_per_step": n_dens64 / max(a.steps + a.warmup, 1),
+         "traffic": measured_traffic("k_orb_density<double", B)},
+    ]
+    for k in cand:
+        k["achieved"] = k["flops_per_launch"] / (k["ms_per_launch"] * 1e-3) / 1e12
+        k["frac"] = k["achieved"] / k["peak"]
+    dom = max(cand, key=lambda k: k["ms_per_launch"] * k["calls_per_step"])
     out = {
         "metric": METRIC, "value": B / dt, "unit": "geometries/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3,
@@ -138,26 +319,31 @@ def main():
         "data": "synthetic: uniform-random protons, numpy default_rng(seed), seeds 0..B-1",
         "config": {"workload": f"54 H rs=1.25 gth-dzv LDA(vwn) Fermi T=1e4K, grid 49^3 "
                                f"(grid_length 0.16), tol 1e-8, energy+forces, batch {B}",
-                   "path": a.path, "batch": B, "n_iter_mean": nit, "all_converged": conv},
-        "roofline": {"bound": "mfma", "kernel": "k_vmat (dV phi diag(v) phi^T)",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "ms_per_launch": ms, "flops_per_launch": flops},
+                   "path": a.path, "batch": B, "n_iter_mean": nit, "all_converged": conv,
+                   "contractions": stats, "jacobi_sweeps_per_eigh": sweeps},
+        "roofline": {"bound": "mfma", "kernel": dom["kernel"], "achieved": dom["achieved"],
+                     "peak": dom["peak"], "unit": "TFLOP/s", "frac": dom["frac"],
+                     "traffic": dom["traffic"], "ms_per_launch": dom["ms_per_launch"],
+                     "flops_per_launch": dom["flops_per_launch"]},
+        "kernels": cand,
     }
-    eng = pes.solver.engine
-    out["config"]["contractions"] = dict(eng.stats)
-    if eng.eigh_calls:
-        out["config"]["jacobi_sweeps_per_eigh"] = eng.sweeps.item() / eng.eigh_calls
-    if a.compare:
-        other = make("mixed" if a.path == "fp64" else "fp64")
-        E2, F2 = other.energy_and_forces(x)
-        w_a, w_b = st["w1"], other.solver.last["w1"]
+    if not a.no_compare:
+        other = "fp64" if a.path == "mixed" else "mixed"
+        pes2, dt2, E2, F2 = run_path(other, x, 1, 1)
+        w_a, w_b = st["w1"], pes2.solver.last["w1"]
         out["compare"] = {
-            "against": other.solver.engine.precision,
+            "against": other, "batched_value": B / dt2,
             "max_dE_per_atom_Ha": (E - E2).abs().max().item() / RY / N_ATOMS,
             "max_dF_Ha_per_Bohr": (F - F2).abs().max().item(),
             "max_dbands_Ha": (w_a - w_b).abs().max().item(),
-            "n_iter_mean_other": other.solver.last["nit"].double().mean().item()}
+            "n_iter_mean_other": pes2.solver.last["nit"].double().mean().item(),
+            "all_converged_other": bool(pes2.solver.last["conv"].all().item())}
+        del pes2
+        plain = plain_fp64()
+        out["compare"]["fp64_plain_value"] = plain
+        out["speedup_vs_fp64_plain"] = out["value"] / plain
+        fp64_batched = out["value"] if a.path == "fp64" else B / dt2
+        out["speedup_vs_fp64_batched"] = out["value"] / fp64_batched
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(1)
     print(json.dumps(out), flush=True)

@@ -33,6 +33,9 @@ _SIGS = {
     "hqc_ao_vmat_f64": [_P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_ao_vmat_f32": [_P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
+    "hqc_pair_pack": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
+    "hqc_pair_split_f64": [_P, _I, _I, _I, _I, _I, _P, _P],
+    "hqc_pair_split_f32": [_P, _I, _I, _I, _I, _I, _P, _P],
     "hqc_vloc_G": [_P, _I, _I, _I, _D, _P, _P, _P],
     "hqc_real_to_complex": [_P, _I, _I, _I, _D, _P, _P],
     "hqc_complex_real": [_P, _I, _I, _I, _P, _P],
@@ -160,6 +163,36 @@ def fft3d(c, sign):
     B, g = c.shape[0], c.shape[1]
     assert c.dtype == torch.complex128 and c.shape[1:] == (g, g, g)
     _call("hqc_fft3d_c2c_f64", _ptr(c, None, "c"), B, g, int(sign), _stream())
+
+
+def pair_pack(phi, psi, table, nao, g, z):
+    """z (mp, nao, g^3) complex128 <- phi[b,p] (psi[b,i] + i psi[b,j]) per row (b, i, j)
+    of the host int32 table (validated here, then copied to the device)."""
+    B, nao_pad, ld = phi.shape
+    nmo = psi.shape[1]
+    mp = table.shape[0]
+    assert not table.is_cuda and table.dtype == torch.int32 and table.shape == (mp, 3)
+    assert psi.shape == (B, nmo, ld)
+    assert z.dtype == torch.complex128 and z.shape == (mp, nao, g ** 3)
+    if mp == 0:
+        return
+    if not (0 <= int(table[:, 0].min()) and int(table[:, 0].max()) < B
+            and 0 <= int(table[:, 1].min()) and int(table[:, 1].max()) < nmo
+            and -1 <= int(table[:, 2].min()) and int(table[:, 2].max()) < nmo):
+        raise ValueError("pair table entry out of range")
+    td = table.to(phi.device)
+    _call("hqc_pair_pack", _ptr(phi, F64, "phi"), _ptr(psi, F64, "psi"),
+          _ptr(td, torch.int32, "table"), mp, nao, nao_pad, nmo, g, ld, _ptr(z, None, "z"),
+          _stream())
+
+
+def pair_split(z, nao_pad, ld, M):
+    """M (2 mp, nao_pad, ld) <- real half-spectrum blocks of the transformed pairs."""
+    mp, nao, N = z.shape
+    g = round(N ** (1 / 3))
+    assert g ** 3 == N and M.shape == (2 * mp, nao_pad, ld)
+    name = "hqc_pair_split_f64" if M.dtype == F64 else "hqc_pair_split_f32"
+    _call(name, _ptr(z, None, "z"), mp, nao, nao_pad, g, ld, _ptr(M, None, "M"), _stream())
 
 
 def vloc_G(xp, g, L, VG, out):

@@ -76,6 +76,17 @@ class Cell:
         self.comp = torch.tensor(b.comp, dtype=torch.int32, device=dev)
         self.lattice = torch.tensor(self.lattice_np, dtype=f64, device=dev).contiguous()
         self.VG = torch.tensor(VG.ravel(), dtype=f64, device=dev)
+        # exchange weights on the half G-set H (centred order, columns < (N-1)/2) for the
+        # [Re | Im] row blocks of the pair-density transforms: 2 VG on both halves
+        nh = (self.N - 1) // 2
+        cc = np.arange(nh)
+        h = g // 2
+        s1, s2, s3 = cc // (g * g) - h, (cc // g) % g - h, cc % g - h
+        kk = ((s1 % g) * g + s2 % g) * g + s3 % g
+        wh = np.zeros(self.ld)
+        wh[:nh] = 2 * VG.ravel()[kk]
+        wh[nh:2 * nh] = wh[:nh]
+        self.VGh = torch.tensor(wh, dtype=f64, device=dev)
         self.atom_of_ao = torch.arange(self.nao, device=dev) // self.nloc
 
 
@@ -284,41 +295,50 @@ class Engine:
         return -2 * (a.sum(-1) + b.sum(-1))
 
     # ----------------------------------------------------------- two-electron
-    EXCHANGE_CHUNK_BYTES = 2 << 30      # pair-density working set per batch of orbitals
+    EXCHANGE_CHUNK_BYTES = 4 << 30      # pair grids + row blocks per launch group
 
     def exchange(self, st, na, mo, occ):
         """K (B, nao, nao) for geometries [0, na): equals sum_rs D_rs (eris+eris0)[p,q,s,r]
         (hqc/pbc/solver.py:430-431, 604-614).
 
-        With A_i = sqrt(f_i) dV FFT(phi * psi_i) (one 3-D FFT per AO and occupied
-        orbital), K = sum_i Re[A_i diag(VG) A_i^H] = sum_i (Re A_i) VG (Re A_i)^T +
-        (Im A_i) VG (Im A_i)^T: the same MFMA contraction as V_xc, batched over the
-        orbitals; plus the G=0 term c0 S~ D S~ with the literal 0.22578495."""
+        With A_i = sqrt(f_i) dV FFT(phi * psi_i), K = sum_i sum_G VG Re[A_i A_i^H]
+        over G != 0, plus the G=0 term c0 S~ D S~ with the literal 0.22578495.
+        Two occupied orbitals share one complex FFT (hqc_pair_pack); the split kernel
+        returns the [Re | Im] blocks on half the G-set, and the sum over orbitals of
+        M diag(2 VG) M^T is the AO x grid MFMA contraction, batched over all pairs
+        of all geometries."""
         c = self.c
-        g, N = c.g, c.N
-        phi = st["phi"][:na]
-        psi = mo.transpose(-1, -2).contiguous() @ phi[:, :c.nao, :]        # (na, nmo, ld)
-        Kmat = torch.zeros(na, c.nao, c.nao, dtype=torch.float64, device=c.device)
-        per_orb = c.nao * (N * 16 + 2 * c.nao_pad // c.nao * c.ld * 8 + c.ld * 8)
-        chunk = max(1, int(self.EXCHANGE_CHUNK_BYTES // per_orb))
-        VGp = torch.zeros(c.ld, dtype=torch.float64, device=c.device)
-        VGp[:N] = c.VG
+        dev = c.device
+        keep = (occ > 1e-14).cpu()
+        rows, nmax = [], 0
         for b in range(na):
-            idx = torch.nonzero(occ[b] > 1e-14).flatten()
-            w = occ[b, idx].sqrt() * c.dV
-            for s0 in range(0, idx.numel(), chunk):
-                ii = idx[s0:s0 + chunk]
-                m = ii.numel()
-                pair = phi[b, None, :c.nao, :N] * (psi[b, ii, None, :N] * w[s0:s0 + m, None, None])
-                cb = pair.to(torch.complex128).reshape(m * c.nao, g, g, g)
-                K.fft3d(cb, -1)
-                cb = cb.reshape(m, c.nao, N)
-                part = torch.zeros(m, c.nao_pad, c.ld, dtype=torch.float64, device=c.device)
-                v = VGp.expand(m, c.ld).contiguous()
-                part[:, :c.nao, :N] = cb.real
-                Kmat[b] += self.vmat_dv(part, v, 1.0).sum(0)
-                part[:, :c.nao, :N] = cb.imag
-                Kmat[b] += self.vmat_dv(part, v, 1.0).sum(0)
+            idx = torch.nonzero(keep[b]).flatten().tolist()
+            if idx:
+                nmax = max(nmax, idx[-1] + 1)
+            for k in range(0, len(idx), 2):
+                rows.append((b, idx[k], idx[k + 1] if k + 1 < len(idx) else -1))
+        Kmat = torch.zeros(na, c.nao, c.nao, dtype=torch.float64, device=dev)
+        if rows:
+            wgt = occ[:, :nmax].clamp_min(0).sqrt() * c.dV * keep[:na, :nmax].to(dev)
+            psi = (mo[:, :, :nmax] * wgt.unsqueeze(-2)).transpose(-1, -2).contiguous() \
+                @ st["phi"][:na, :c.nao, :]                                  # (na, nmax, ld)
+            table = torch.tensor(rows, dtype=torch.int32)
+            per_pair = c.nao * c.N * 16 + 2 * c.nao_pad * (c.ld + c.nao_pad + 1) * 8
+            chunk = max(1, min(len(rows), int(self.EXCHANGE_CHUNK_BYTES // per_pair)))
+            v = c.VGh.expand(2 * chunk, c.ld).contiguous()
+            phi = st["phi"][:na]
+            for s0 in range(0, len(rows), chunk):
+                t = table[s0:s0 + chunk]
+                mp = t.shape[0]
+                z = torch.empty(mp, c.nao, c.N, dtype=torch.complex128, device=dev)
+                K.pair_pack(phi, psi, t, c.nao, c.g, z)
+                K.fft3d(z.view(mp * c.nao, c.g, c.g, c.g), -1)
+                M = torch.empty(2 * mp, c.nao_pad, c.ld, dtype=torch.float64, device=dev)
+                K.pair_split(z, c.nao_pad, c.ld, M)
+                del z
+                part = self.vmat_dv(M, v[:2 * mp], 1.0)
+                gid = t[:, 0].long().repeat_interleave(2).to(dev)
+                Kmat.index_add_(0, gid, part)
         D = (mo * occ.unsqueeze(-2)) @ mo.transpose(-1, -2).contiguous()
         Sg = st["Sgrid"][:na]
         c0 = 4 * math.pi / c.L * EXCHANGE_G0_CONST

@@ -102,6 +102,27 @@ int hqc_ao_vmat_f32(const float* phi, const double* v, int B, int nao_pad, int l
  * in the FFT-Poisson restatement. */
 int hqc_fft3d_c2c_f64(double* data, int B, int g, int sign, void* stream);
 
+/* Hartree-Fock exchange pair densities, two orbitals per complex grid:
+ *   z[m, p, r] = phi[b, p, r] * (psi[b, i, r] + i psi[b, j, r]),
+ * table[m] = (b, i, j) with j = -1 for a lone orbital; psi rows carry the
+ * weight sqrt(f_i) dV.  z is (mp, nao, g^3) complex128, ready for
+ * hqc_fft3d_c2c_f64(sign -1).  Replaces the AO-pair densities rhoG that feed
+ * eris (hqc/pbc/solver.py:412-431) on the exchange route K = sum_rs D_rs eris. */
+int hqc_pair_pack(const double* phi, const double* psi, const int* table, int mp, int nao,
+                  int nao_pad, int nmo, int g, int ld, double* z, void* stream);
+
+/* Split each transformed pair grid Z = FFT(a + i b) into the real row blocks
+ * M[2m] = [Re A_a(H) | Im A_a(H)] and M[2m+1] = [Re A_b(H) | Im A_b(H)] over the
+ * half G-set H (centred order, columns < (g^3-1)/2); M is (2 mp, nao_pad, ld),
+ * rows >= nao and columns >= g^3-1 written as zero.  Then
+ * K = sum_m M_m diag(w) M_m^T with w = 2 VG on both halves (hqc_ao_vmat_*,
+ * dV = 1) is the G != 0 part of the exchange matrix
+ * (hqc/pbc/solver.py:430-431, 604-614). */
+int hqc_pair_split_f64(const double* z, int mp, int nao, int nao_pad, int g, int ld, double* M,
+                       void* stream);
+int hqc_pair_split_f32(const double* z, int mp, int nao, int nao_pad, int g, int ld, float* M,
+                       void* stream);
+
 /* Local (electron-proton) potential in G space, FFT order:
  *   out[b,G] = -VG[G] * sum_A exp(-i G.R_A)   (hqc/pbc/solver.py:406-407).
  * xp (B,n,3) Bohr; VG (g^3) FFT order; out complex (B,g^3). */
