@@ -42,6 +42,11 @@ _SIGS = {
     "hqc_poisson_mul": [_P, _P, _I, _I, _P, _P],
     "hqc_xc_blocks": [_I],
     "hqc_xc_eval": [_P, _P, _I, _I, _I, _I, _P, _P, _P],
+    "hqc_grad_G": [_P, _I, _I, _D, _D, _P, _P],
+    "hqc_div_G": [_P, _I, _I, _D, _D, _P, _P],
+    "hqc_gga_blocks": [_I],
+    "hqc_xc_gga_eval": [_P, _P, _P, _I, _I, _I, _P, _P, _P, _P],
+    "hqc_pbe_points": [_P, _P, _I, _P, _P],
     "hqc_occupations": [_P, _I, _I, _I, _D, _I, _I, _I, _D, _D, _P, _P, _P],
     "hqc_eigh_lds_bytes": [_I],
     "hqc_eigh_jacobi": [_P, _I, _I, _P, _I, _D, _P, _P, _P, _P],
@@ -193,6 +198,43 @@ def pair_split(z, nao_pad, ld, M):
     assert g ** 3 == N and M.shape == (2 * mp, nao_pad, ld)
     name = "hqc_pair_split_f64" if M.dtype == F64 else "hqc_pair_split_f32"
     _call(name, _ptr(z, None, "z"), mp, nao, nao_pad, g, ld, _ptr(M, None, "M"), _stream())
+
+
+def grad_G(rhoG, g, L, scale, out):
+    B = rhoG.shape[0]
+    assert rhoG.dtype == torch.complex128 and rhoG.shape == (B, g, g, g)
+    assert out.dtype == torch.complex128 and out.shape == (B, 3, g, g, g)
+    _call("hqc_grad_G", _ptr(rhoG, None, "rhoG"), B, g, float(L), float(scale),
+          _ptr(out, None, "out"), _stream())
+
+
+def div_G(wG, g, L, scale, out):
+    B = wG.shape[0]
+    assert wG.dtype == torch.complex128 and wG.shape == (B, 3, g, g, g)
+    assert out.dtype == torch.complex128 and out.shape == (B, g, g, g)
+    _call("hqc_div_G", _ptr(wG, None, "wG"), B, g, float(L), float(scale),
+          _ptr(out, None, "out"), _stream())
+
+
+def gga_blocks(g):
+    return lib().hqc_gga_blocks(int(g))
+
+
+def xc_gga_eval(rho, grad, vh, g, vout, w, part):
+    B, ld = rho.shape
+    assert grad.shape == (B, 3, g, g, g) and w.shape == (B, 3, g, g, g)
+    assert vh.shape == (B, g, g, g) and vout.shape == (B, ld)
+    assert part.shape == (B, gga_blocks(g), 2)
+    _call("hqc_xc_gga_eval", _ptr(rho, F64, "rho"), _ptr(grad, None, "grad"),
+          _ptr(vh, None, "vh"), B, g, ld, _ptr(vout, F64, "vout"), _ptr(w, None, "w"),
+          _ptr(part, F64, "part"), _stream())
+
+
+def pbe_points(rho, sigma, out):
+    n = rho.numel()
+    assert sigma.numel() == n and out.shape == (n, 3)
+    _call("hqc_pbe_points", _ptr(rho, F64, "rho"), _ptr(sigma, F64, "sigma"), n,
+          _ptr(out, F64, "out"), _stream())
 
 
 def vloc_G(xp, g, L, VG, out):

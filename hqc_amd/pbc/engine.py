@@ -24,6 +24,7 @@ from ..basis import HBasis
 
 RY = 2.0
 EXCHANGE_G0_CONST = 0.22578495          # hqc/pbc/solver.py:404
+GGA_NAMES = {"pbe", "gga_x_pbe", "gga_c_pbe"}    # PBE exchange + correlation (new)
 XC_IDS = {"": 2, "vwn": 1, "hl71": 3, "hl": 3, "bh72": 4, "bh": 4, "gl77": 5, "gl": 5}
 
 
@@ -113,9 +114,15 @@ class Engine:
                  diis_damp=0.0, precision="fp64"):
         self.c = cell
         self.dft = dft
+        self.gga = False
         if dft:
             xname, cname = xc.split(",")
-            if xname != "lda":
+            if xname in GGA_NAMES or cname in GGA_NAMES:
+                if not (xname in GGA_NAMES and cname in GGA_NAMES):
+                    raise ValueError("GGA needs PBE exchange and PBE correlation ('pbe,pbe').")
+                self.gga = True
+                cname = ""
+            elif xname != "lda":
                 raise ValueError("exchange functional not supported yet.")
             if cname in ("pz81", "pz"):
                 raise NameError("name 'pw91' is not defined")   # hqc/tools/excor.py:116-117
@@ -215,6 +222,34 @@ class Engine:
         K.poisson_mul(cbuf, c.VG, saved)
         K.fft3d(cbuf, +1)
         return cbuf, saved
+
+    def xc_potential(self, rho, save_rg=False):
+        """(v = v_H + v_xc (B, ld), per-geometry sums (B, 2) of the xc energy density and
+        rho v_H, rho~(G) = dV FFT(rho) if save_rg or GGA).
+
+        GGA (PBE): grad rho = IFFT(i G rho~) / Omega on the odd grid, then
+        v_xc = de/drho - div(2 de/dsigma grad rho) with the same spectral
+        derivative, the exact derivative of the discrete functional."""
+        c = self.c
+        B, g = rho.shape[0], c.g
+        vh, rg = self.poisson(rho, save=self.gga or save_rg)
+        v = torch.zeros(B, c.ld, dtype=torch.float64, device=c.device)
+        if not self.gga:
+            part = torch.empty(B, K.xc_blocks(g), 2, dtype=torch.float64, device=c.device)
+            K.xc_eval(rho, vh, g, self.xc_id, v, part)
+            return v, part.sum(1), rg
+        grad = torch.empty(B, 3, g, g, g, dtype=torch.complex128, device=c.device)
+        K.grad_G(rg, g, c.L, 1.0 / c.omega, grad)
+        K.fft3d(grad.view(3 * B, g, g, g), +1)
+        w = torch.empty_like(grad)
+        part = torch.empty(B, K.gga_blocks(g), 2, dtype=torch.float64, device=c.device)
+        K.xc_gga_eval(rho, grad, vh, g, v, w, part)
+        del grad
+        K.fft3d(w.view(3 * B, g, g, g), -1)
+        K.div_G(w, g, c.L, 1.0 / c.N, vh)
+        K.fft3d(vh, +1)
+        v[:, :c.N] -= vh.real.reshape(B, c.N)
+        return v, part.sum(1), rg
 
     # ----------------------------------------------------------------- set-up
     def setup(self, xpb):
@@ -369,12 +404,7 @@ class Engine:
                 st["Da"][:na] = D
                 st["rhoa"][:na] = rho
                 st["kappa"][:na] = (rho[:, :c.N] / sel("phisq")).amin(-1)
-        vh, _ = self.poisson(rho)
-        nblk = K.xc_blocks(c.g)
-        v = torch.zeros(na, c.ld, dtype=torch.float64, device=c.device)
-        part = torch.empty(na, nblk, 2, dtype=torch.float64, device=c.device)
-        K.xc_eval(rho, vh, c.g, self.xc_id, v, part)
-        E = part.sum(1)
+        v, E, _ = self.xc_potential(rho)
         Exc, EH = E[:, 0] * c.dV, 0.5 * E[:, 1] * c.dV
         if mixed:
             dv = v - sel("va")

@@ -75,10 +75,7 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
     X = torch.zeros(B, c.nao_pad, c.ld, dtype=f64, device=dev)
     rho = torch.zeros(B, c.ld, dtype=f64, device=dev)
     K.ao_density(st["phi"], engine._pad(D), rho, X)
-    vh, rg = engine.poisson(rho, save=True)
-    v = torch.zeros(B, c.ld, dtype=f64, device=dev)
-    part = torch.empty(B, K.xc_blocks(c.g), 2, dtype=f64, device=dev)
-    K.xc_eval(rho, vh, c.g, engine.xc_id, v, part)
+    v, _, rg = engine.xc_potential(rho, save_rg=True)
     veff = (v + st["vloc"]).contiguous()
     P = torch.empty(B, c.n, 3, c.g, c.na, 3, dtype=f64, device=dev)
     K.gto_tables(st["xpb"], c.g, c.L, c.tmax_img, c.alpha, 3, P)

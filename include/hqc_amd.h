@@ -150,6 +150,25 @@ int hqc_xc_blocks(int g);
 int hqc_xc_eval(const double* rho, const double* vh, int B, int g, int ld, int xc_id,
                 double* vout, double* part, void* stream);
 
+/* GGA (PBE) on the grid; new in this build (the reference accepts only LDA
+ * exchange, hqc/tools/excor.py:14-17).  Spectral derivatives on the odd grid:
+ *   hqc_grad_G: out (B,3,g^3) complex = scale * i G_d rhoG (inverse FFT ->
+ *     grad rho with scale = 1/Omega when rhoG = dV FFT(rho));
+ *   hqc_xc_gga_eval: rho (B,ld), grad (B,3,g^3) complex (real part used), vh as
+ *     for hqc_xc_eval; vout (B,ld) = v_H + de/drho, w (B,3,g^3) complex =
+ *     2 de/dsigma grad rho, part (B, hqc_gga_blocks, 2) sums of e and rho v_H;
+ *   hqc_div_G: out (B,g^3) = scale * sum_d i G_d wG[b,d] (forward FFT of w in,
+ *     inverse FFT of out with scale 1/g^3 -> div w); v_xc = de/drho - div w;
+ *   hqc_pbe_points: out (n,3) = (e, de/drho, de/dsigma) at given points. */
+int hqc_grad_G(const double* rhoG, int B, int g, double L, double scale, double* out,
+               void* stream);
+int hqc_div_G(const double* wG, int B, int g, double L, double scale, double* out,
+              void* stream);
+int hqc_gga_blocks(int g);
+int hqc_xc_gga_eval(const double* rho, const double* grad, const double* vh, int B, int g,
+                    int ld, double* vout, double* w, double* part, void* stream);
+int hqc_pbe_points(const double* rho, const double* sigma, int n, double* out, void* stream);
+
 /* Chemical potential and occupations (hqc/tools/smearing.py:4-173), one
  * wavefront per geometry, reproducing the reference loop semantics including
  * the bisection return rule of smearing.py:102.

@@ -597,6 +597,114 @@ def xc_eps_v(rho, xc="lda,vwn"):
 
 
 # --------------------------------------------------------------------------
+# GGA (PBE) exchange-correlation on the grid.  New in this build: the reference
+# accepts only LDA exchange (hqc/tools/excor.py:14-17).
+# --------------------------------------------------------------------------
+PBE_KAPPA = 0.804
+PBE_BETA = 0.06672455060314922
+PBE_MU = PBE_BETA * np.pi ** 2 / 3                  # 0.2195149727645171
+PBE_GAMMA = (1 - np.log(2)) / np.pi ** 2
+PW92 = dict(A=0.0310907, a1=0.21370, b1=7.5957, b2=3.5876, b3=1.6382, b4=0.49294)
+RHO_CUT = 1e-12                                     # e = v = 0 below this density
+GGA_NAMES = {"pbe", "gga_x_pbe", "gga_c_pbe"}
+
+
+def is_gga(xc):
+    xname, cname = xc.split(",")
+    if xname in GGA_NAMES or cname in GGA_NAMES:
+        if not (xname in GGA_NAMES and cname in GGA_NAMES):
+            raise ValueError("GGA needs PBE exchange and PBE correlation ('pbe,pbe').")
+        return True
+    return False
+
+
+def pw92_eps_c(rs):
+    """Perdew-Wang 1992 correlation energy per electron, unpolarised
+    (PRB 45, 13244, eq. 10 with p = 1); complex-step safe."""
+    A, a1 = PW92["A"], PW92["a1"]
+    q = 2 * A * (PW92["b1"] * rs ** 0.5 + PW92["b2"] * rs + PW92["b3"] * rs ** 1.5
+                 + PW92["b4"] * rs ** 2)
+    return -2 * A * (1 + a1 * rs) * np.log(1 + 1 / q)
+
+
+def pbe_x(rho, sigma):
+    """PBE exchange energy per volume rho eps_x^LDA F_x(s), s^2 = sigma / (2 k_F rho)^2."""
+    kF = (3 * np.pi ** 2 * rho) ** (1 / 3)
+    ex_lda = -0.75 * (3 / np.pi) ** (1 / 3) * rho ** (1 / 3)
+    s2 = sigma / (4 * kF ** 2 * rho ** 2)
+    Fx = 1 + PBE_KAPPA - PBE_KAPPA / (1 + PBE_MU * s2 / PBE_KAPPA)
+    return rho * ex_lda * Fx
+
+
+def pbe_c(rho, sigma):
+    """PBE correlation energy per volume rho (eps_c^PW92 + H(t)),
+    t^2 = sigma / (2 k_s rho)^2, k_s = sqrt(4 k_F / pi) (unpolarised, phi = 1)."""
+    kF = (3 * np.pi ** 2 * rho) ** (1 / 3)
+    rs = (3 / (4 * np.pi * rho)) ** (1 / 3)
+    ec = pw92_eps_c(rs)
+    ks = (4 * kF / np.pi) ** 0.5
+    t2 = sigma / (4 * ks ** 2 * rho ** 2)
+    bg = PBE_BETA / PBE_GAMMA
+    Ac = bg / (np.exp(-ec / PBE_GAMMA) - 1)
+    H = PBE_GAMMA * np.log(1 + bg * t2 * (1 + Ac * t2) / (1 + Ac * t2 + Ac ** 2 * t2 ** 2))
+    return rho * (ec + H)
+
+
+def pbe_energy_density(rho, sigma):
+    """e(rho, sigma) = pbe_x + pbe_c per volume, PBE (Perdew, Burke, Ernzerhof,
+    PRL 77, 3865 (1996)), unpolarised, sigma = |grad rho|^2.  Written with **
+    powers so that it accepts complex arguments (complex-step derivatives).
+    Pinned by: the LDA limit sigma -> 0, the gradient expansions F_x = 1 + mu s^2
+    and H = beta t^2, the bounds F_x -> 1 + kappa and eps_c + H -> 0
+    (tests/test_oracle_gga.py)."""
+    return pbe_x(rho, sigma) + pbe_c(rho, sigma)
+
+
+def pbe_derivs(rho, sigma):
+    """(e, de/drho, de/dsigma) by complex-step differentiation (exact to rounding:
+    Im f(x + ih)/h with h = 1e-30 x); zero below RHO_CUT."""
+    rho = np.asarray(rho, float)
+    sigma = np.asarray(sigma, float)
+    ok = rho > RHO_CUT
+    r = np.where(ok, rho, 1.0)
+    sg = np.where(ok, sigma, 0.0)
+    h = 1e-30
+    e = pbe_energy_density(r, sg)
+    vr = pbe_energy_density(r + 1j * h * r, sg).imag / (h * r)
+    hs = h * np.maximum(sg, 1e-20)
+    vs = pbe_energy_density(r, sg + 1j * hs).imag / hs
+    z = np.zeros_like(rho)
+    return np.where(ok, e, z), np.where(ok, vr, z), np.where(ok, vs, z)
+
+
+def spectral_grad(sysm, f):
+    """(3, N) gradient of a real grid function by FFT: IFFT(iG F(G)), exact for
+    the band-limited interpolant on the odd grid (real, no Nyquist term)."""
+    g = sysm.g
+    F = np.fft.fftn(f.reshape(g, g, g))
+    return np.stack([np.fft.ifftn(1j * sysm.G[..., c] * F).real.ravel() for c in range(3)])
+
+
+def spectral_div(sysm, w):
+    g = sysm.g
+    acc = np.zeros((g, g, g), complex)
+    for c in range(3):
+        acc += 1j * sysm.G[..., c] * np.fft.fftn(w[c].reshape(g, g, g))
+    return np.fft.ifftn(acc).real.ravel()
+
+
+def xc_gga(sysm, rho):
+    """(E_xc, v_xc) of the discrete PBE functional E = dV sum_r e(rho_r, |D rho|_r^2)
+    with D the spectral derivative: v_xc = de/drho - D.(2 de/dsigma D rho), the exact
+    derivative dE/drho_r / dV (D^T = -D).  Pinned by: finite differences of E in one
+    grid value (tests/test_oracle_gga.py)."""
+    grad = spectral_grad(sysm, rho)
+    sigma = (grad ** 2).sum(0)
+    e, vr, vs = pbe_derivs(rho, sigma)
+    return e.sum() * sysm.dV, vr - spectral_div(sysm, 2 * vs * grad)
+
+
+# --------------------------------------------------------------------------
 # Occupations and chemical potential (hqc/tools/smearing.py)
 # --------------------------------------------------------------------------
 def make_occupation(n, n_mo, smearing=False, smearing_method="fermi", smearing_sigma=0.0,
@@ -770,7 +878,8 @@ class Solver:
         self.gamma, self.literal = gamma, literal
         self.smearing = smearing
         self.sigma = smearing_sigma if smearing else 0.0
-        if dft:
+        self.gga = dft and is_gga(xc)
+        if dft and not self.gga:
             xc_eps_v(np.array([0.1]), xc)     # raise early on unsupported names
         self.occupation = make_occupation(n, self.sys.n_ao, smearing, smearing_method,
                                           smearing_sigma, search_method, search_cycle,
@@ -817,6 +926,9 @@ class Solver:
         if self.dft:
             def xc_fn(dm):
                 rho = grid_density(phi, dm)
+                if self.gga:
+                    Exc, vxc = xc_gga(s, rho)
+                    return Exc, grid_matrix(s, phi, vxc)
                 eps, vxc = xc_eps_v(rho, self.xc)
                 return np.sum(rho * eps) * s.dV, grid_matrix(s, phi, vxc)
         else:
@@ -925,7 +1037,7 @@ def forces(solver, xp, kappa=10, Gmax=15):
     vh, rg = poisson(s, rho)
     veff = vloc_real(s, xpb) + vh
     if solver.dft:
-        veff = veff + xc_eps_v(rho, solver.xc)[1]
+        veff = veff + (xc_gga(s, rho)[1] if solver.gga else xc_eps_v(rho, solver.xc)[1])
     X = dm @ phi
     Z = 2 * veff[None, :] * X
     if not solver.dft:

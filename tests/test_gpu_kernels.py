@@ -202,6 +202,42 @@ def test_xc_eval(K, xc):
     assert np.isclose(sums[1], np.sum(rho_np * vh_np), rtol=1e-11, atol=1e-11)
 
 
+def test_pbe_points(K):
+    """Dual-number PBE (e, de/drho, de/dsigma) against the oracle's complex-step values."""
+    rng = np.random.default_rng(6)
+    rho = rng.uniform(1e-4, 3.0, 500)
+    sigma = rng.uniform(0, 5.0, 500) * rho ** (8 / 3)
+    rho[:3] = [0.0, 1e-13, 0.5]
+    sigma[2] = 0.0
+    out = torch.empty(500, 3, dtype=torch.float64, device=DEV)
+    K.pbe_points(torch.tensor(rho, device=DEV), torch.tensor(sigma, device=DEV), out)
+    ref = np.stack(R.pbe_derivs(rho, sigma), axis=1)
+    assert np.allclose(out.cpu().numpy(), ref, rtol=1e-12, atol=1e-15)
+
+
+def test_gga_potential(K):
+    """Engine GGA route (spectral gradient, PBE, divergence) against the oracle's
+    discrete functional: v = v_H + v_xc and E_xc."""
+    from hqc_amd.pbc.engine import Cell, Engine
+    n, rs, gl = 3, 1.4, 0.25
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    s = R.System(n, L, rs, "gth-dzv", grid_length=gl)
+    xp = np.random.default_rng(2).uniform(0, s.L, (n, 3))
+    phi = R.ao_values(s, xp)
+    rho_np = (phi ** 2).sum(0) * 0.5
+    eng = Engine(Cell(n, L, rs, "gth-dzv", grid_length=gl), dft=True, xc="pbe,pbe")
+    c = eng.c
+    print(f"[gga potential] g={c.g}", flush=True)
+    rho = torch.zeros(1, c.ld, dtype=torch.float64, device=DEV)
+    rho[0, :c.N] = torch.tensor(rho_np, device=DEV)
+    v, E, _ = eng.xc_potential(rho)
+    Exc, vxc = R.xc_gga(s, rho_np)
+    vh, _ = R.poisson(s, rho_np)
+    ref = vh + vxc
+    assert np.allclose(v[0, :c.N].cpu().numpy(), ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+    assert abs(E[0, 0].item() * c.dV - Exc) < 1e-11 * abs(Exc)
+
+
 def test_vloc_and_poisson(K):
     s, xpb, L = _sys(n=5, basis="gth-szv", grid_length=0.25)
     print(f"[vloc] g={s.g}", flush=True)

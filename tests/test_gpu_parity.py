@@ -30,6 +30,8 @@ CASES = [
     ("hf_dzvp_smear", 4, 1.5, "gth-dzvp", 0.3,
      dict(dft=False, smearing=True, smearing_sigma=0.1, search_method="newton")),
     ("dft_dzvp_fp", 4, 1.5, "gth-dzvp", 0.3, dict(dft=True, diis=False, max_cycle=200)),
+    ("gga_pbe_dzv_smear", 4, 1.5, "gth-dzv", 0.25,
+     dict(dft=True, xc="pbe,pbe", smearing=True, smearing_sigma=0.1)),
 ]
 
 
@@ -74,14 +76,17 @@ def test_lcao_and_pes_config1():
     assert abs(Etot.item() - (ref[3] + Vpp_ref)) < 1e-8
 
 
-@pytest.mark.parametrize("dft,basis", [(True, "gth-dzv"), (False, "gth-dzvp")])
-def test_forces_vs_oracle(dft, basis):
+@pytest.mark.parametrize("dft,basis,xc", [(True, "gth-dzv", "lda,vwn"), (False, "gth-dzvp", ""),
+                                           (True, "gth-dzvp", "pbe,pbe")])
+def test_forces_vs_oracle(dft, basis, xc):
     from hqc_amd.pbc import make_pes
     n, rs = 4, 1.5
     xp, L = _xp(n, 1)
-    print(f"[forces] dft={dft} {basis}", flush=True)
+    print(f"[forces] dft={dft} {basis} {xc}", flush=True)
     kw = dict(dft=dft, smearing=True, smearing_sigma=0.1, tol=1e-11, search_tol=1e-13,
               search_cycle=200, max_cycle=300)
+    if dft:
+        kw["xc"] = xc
     s = R.Solver(n, L, rs, basis, grid_length=0.3, **kw)
     s.solve(xp)
     Fref = R.forces(s, xp)
