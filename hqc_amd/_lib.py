@@ -53,6 +53,11 @@ _SIGS = {
     "hqc_overlap_kinetic": [_P, _I, _I, _P, _I, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P],
     "hqc_ewald_pairs": [_P, _I, _I, _P, _P, _I, _D, _D, _P, _P],
     "hqc_force_blocks": [_I],
+    "hqc_gto_tables_k": [_P, _P, _I, _I, _I, _D, _I, _P, _I, _I, _P, _P],
+    "hqc_ao_values_k": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
+    "hqc_overlap_kinetic_k": [_P, _P, _I, _I, _P, _I, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P],
+    "hqc_grid_forces_k": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _D, _P,
+                          _P],
     "hqc_grid_forces": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _D, _P, _P],
     "hqc_vloc_force": [_P, _I, _I, _I, _D, _P, _P, _P, _P],
 }
@@ -123,6 +128,24 @@ def ao_values(P, coef, comp, nao_pad, ld, out):
     name = "hqc_ao_values_f64" if out.dtype == F64 else "hqc_ao_values_f32"
     _call(name, _ptr(P, F64, "P"), B, n, g, na, nl, _ptr(coef, F64), _ptr(comp, I32), nloc,
           nao_pad, ld, _ptr(out, None, "phi"), _stream())
+
+
+def gto_tables_k(xp, kpt, g, L, tmax, alpha, nl, out):
+    B, n, _ = xp.shape
+    na = alpha.numel()
+    assert kpt.shape == (B, 3) and out.dtype == torch.complex128
+    assert out.shape == (B, n, 3, g, na, nl)
+    _call("hqc_gto_tables_k", _ptr(xp, F64, "xp"), _ptr(kpt, F64, "kpt"), B, n, g, float(L),
+          int(tmax), _ptr(alpha, F64), na, nl, _ptr(out, None, "P"), _stream())
+
+
+def ao_values_k(P, coef, comp, nao_pad, ld, out):
+    """Stacked complex AO values: out (B, 2 nao_pad, ld) FP64, Re rows then Im rows."""
+    B, n, _, g, na, nl = P.shape
+    nloc = coef.shape[1]
+    assert P.dtype == torch.complex128 and out.shape == (B, 2 * nao_pad, ld)
+    _call("hqc_ao_values_k", _ptr(P, None, "P"), B, n, g, na, nl, _ptr(coef, F64),
+          _ptr(comp, I32), nloc, nao_pad, ld, _ptr(out, F64, "phi"), _stream())
 
 
 def ao_density(phi, D, rho, X=None):
@@ -318,6 +341,18 @@ def overlap_kinetic(xp, lattice, alpha, coef, comp, S, T, dS=None, dT=None):
           _stream())
 
 
+def overlap_kinetic_k(xp, kpt, lattice, alpha, coef, comp, S, T, dS=None, dT=None):
+    B, n, _ = xp.shape
+    na, nloc = coef.shape
+    for t in (S, T, dS, dT):
+        assert t is None or t.dtype == torch.complex128
+    assert kpt.shape == (B, 3)
+    _call("hqc_overlap_kinetic_k", _ptr(xp, F64, "xp"), _ptr(kpt, F64, "kpt"), B, n,
+          _ptr(lattice, F64, "lattice"), lattice.shape[0], _ptr(alpha, F64), na, _ptr(coef, F64),
+          _ptr(comp, I32), nloc, _ptr(S, None, "S"), _ptr(T, None, "T"), _ptr(dS, None, "dS"),
+          _ptr(dT, None, "dT"), _stream())
+
+
 def ewald_pairs(rij, Gv, gk, kappa, g0, out):
     B, npair, _ = rij.shape
     _call("hqc_ewald_pairs", _ptr(rij, F64, "rij"), B, npair, _ptr(Gv, F64, "Gv"),
@@ -333,6 +368,15 @@ def grid_forces(P, alpha, coef, comp, nao_pad, ld, X, v, Zx, dV, part):
     B, n, _, g, na, nl = P.shape
     nloc = coef.shape[1]
     _call("hqc_grid_forces", _ptr(P, F64, "P"), B, n, g, na, nl, _ptr(alpha, F64),
+          _ptr(coef, F64), _ptr(comp, I32), nloc, nao_pad, ld, _ptr(X, F64, "X"),
+          _ptr(v, F64, "v"), _ptr(Zx, F64, "Zx"), float(dV), _ptr(part, F64, "part"), _stream())
+
+
+def grid_forces_k(P, alpha, coef, comp, nao_pad, ld, X, v, Zx, dV, part):
+    B, n, _, g, na, nl = P.shape
+    nloc = coef.shape[1]
+    assert P.dtype == torch.complex128 and X.shape == (B, 2 * nao_pad, ld)
+    _call("hqc_grid_forces_k", _ptr(P, None, "P"), B, n, g, na, nl, _ptr(alpha, F64),
           _ptr(coef, F64), _ptr(comp, I32), nloc, nao_pad, ld, _ptr(X, F64, "X"),
           _ptr(v, F64, "v"), _ptr(Zx, F64, "Zx"), float(dV), _ptr(part, F64, "part"), _stream())
 

@@ -122,6 +122,122 @@ extern "C" int hqc_ao_values_f32(const double* P, int B, int n, int g, int na, i
 }
 
 // ---------------------------------------------------------------------------
+// k-point tables and AO values (hqc/pbc/gto.py:371-420): the image sum carries
+// the Bloch phase, P_k[.., l] = sum_t (x - tL)^l e^{-a (x - tL)^2} e^{i k_d t L},
+// so the tables and the AO values are complex.  AO values are written
+// "stacked": rows [0, nao_pad) hold Re phi, rows [nao_pad, 2 nao_pad) Im phi,
+// which turns every complex contraction into a real one on 2 nao_pad rows.
+// ---------------------------------------------------------------------------
+__global__ void k_gto_tables_k(const double* __restrict__ xp, const double* __restrict__ kpt,
+                               long total, int n, int g, double L, int tmax,
+                               const double* __restrict__ alpha, int na, int nl,
+                               double2* __restrict__ P) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int a = (int)(idx % na);
+  long t1 = idx / na;
+  const int j = (int)(t1 % g);
+  t1 /= g;
+  const int d = (int)(t1 % 3);
+  const long bA = t1 / 3;  // b*n + A
+  const long b = bA / n;
+  const double kd = kpt[b * 3 + d];
+  const double x = (double)j * L / (double)g - xp[bA * 3 + d];
+  const double al = alpha[a];
+  double re[HQC_MAX_NL], im[HQC_MAX_NL];
+#pragma unroll
+  for (int l = 0; l < HQC_MAX_NL; ++l) re[l] = im[l] = 0.0;
+  for (int t = -tmax; t <= tmax; ++t) {
+    const double xt = x - (double)t * L;
+    double s, c;
+    sincos(kd * (double)t * L, &s, &c);
+    double p = exp(-al * xt * xt);
+#pragma unroll
+    for (int l = 0; l < HQC_MAX_NL; ++l) {
+      if (l < nl) {
+        re[l] += p * c;
+        im[l] += p * s;
+        p *= xt;
+      }
+    }
+  }
+  double2* out = P + idx * nl;
+  for (int l = 0; l < nl; ++l) out[l] = make_double2(re[l], im[l]);
+}
+
+extern "C" int hqc_gto_tables_k(const double* xp, const double* kpt, int B, int n, int g,
+                                double L, int tmax, const double* alpha, int na, int nl,
+                                double* P, void* stream) {
+  if (B < 0 || n <= 0 || g <= 0 || na <= 0 || nl <= 0 || tmax < 0 || !kpt) return HQC_E_ARG;
+  if (nl > HQC_MAX_NL || na > HQC_MAX_NA) return HQC_E_SIZE;
+  if (B == 0) return HQC_OK;
+  const long total = (long)B * n * 3 * g * na;
+  hipLaunchKernelGGL(k_gto_tables_k, dim3(hqc_div_up(total, 256)), dim3(256), 0,
+                     (hipStream_t)stream, xp, kpt, total, n, g, L, tmax, alpha, na, nl,
+                     (double2*)P);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ __launch_bounds__(256) void k_ao_values_k(const double2* __restrict__ P, int n,
+                                                     int g, int na, int nl,
+                                                     const double* __restrict__ coef,
+                                                     const int* __restrict__ comp, int nloc,
+                                                     int nao_pad, int ld,
+                                                     double* __restrict__ phi) {
+  __shared__ double s_coef[HQC_MAX_NA * HQC_MAX_NLOC];
+  __shared__ int s_comp[HQC_MAX_NLOC];
+  for (int i = threadIdx.x; i < na * nloc; i += blockDim.x) s_coef[i] = coef[i];
+  for (int i = threadIdx.x; i < nloc; i += blockDim.x) s_comp[i] = comp[i];
+  __syncthreads();
+  const int A = blockIdx.y, b = blockIdx.z;
+  const int N = g * g * g;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  const int iz = r % g, iy = (r / g) % g, ix = r / (g * g);
+  const long tab = (long)(b * n + A) * 3 * g * na * nl;
+  const double2* Px = P + tab + (long)(0 * g + ix) * na * nl;
+  const double2* Py = P + tab + (long)(1 * g + iy) * na * nl;
+  const double2* Pz = P + tab + (long)(2 * g + iz) * na * nl;
+  double* out_re = phi + ((long)b * 2 * nao_pad + (long)A * nloc) * ld + r;
+  double* out_im = out_re + (long)nao_pad * ld;
+  for (int o = 0; o < nloc; ++o) {
+    const int c = s_comp[o];
+    const int px = (c == 0), py = (c == 1), pz = (c == 2);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int a = 0; a < na; ++a) {
+      const double w = s_coef[a * nloc + o];
+      if (w != 0.0) {
+        const double2 t = cmul(cmul(Px[a * nl + px], Py[a * nl + py]), Pz[a * nl + pz]);
+        acc.x += w * t.x;
+        acc.y += w * t.y;
+      }
+    }
+    out_re[(long)o * ld] = acc.x;
+    out_im[(long)o * ld] = acc.y;
+  }
+}
+
+extern "C" int hqc_ao_values_k(const double* P, int B, int n, int g, int na, int nl,
+                               const double* coef, const int* comp, int nloc, int nao_pad,
+                               int ld, double* phi, void* stream) {
+  if (B < 0 || n <= 0 || g <= 0 || na <= 0 || nl <= 0 || nloc <= 0) return HQC_E_ARG;
+  if (na > HQC_MAX_NA || nloc > HQC_MAX_NLOC) return HQC_E_SIZE;
+  if (nao_pad < n * nloc || ld < g * g * g) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  const int N = g * g * g;
+  hipLaunchKernelGGL(k_ao_values_k, dim3(hqc_div_up(N, 256), n, B), dim3(256), 0,
+                     (hipStream_t)stream, (const double2*)P, n, g, na, nl, coef, comp, nloc,
+                     nao_pad, ld, phi);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Grid adjoint for forces.  For atom A of geometry b, each thread takes grid
 // points r (grid-stride), recomputes grad phi_p(r) for the AOs of A from the
 // tables (d/dx [x^l e^{-a x^2}] = l x^{l-1} e - 2a x^{l+1} e, image-summed),
@@ -206,6 +322,99 @@ extern "C" int hqc_grid_forces(const double* P, int B, int n, int g, int na, int
   hipLaunchKernelGGL(k_grid_forces, dim3(HQC_FORCE_BLOCKS, n, B), dim3(256), 0,
                      (hipStream_t)stream, P, n, g, na, nl, alpha, coef, comp, nloc, nao_pad,
                      ld, X, v, Zx, dV, part);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grid adjoint at a k-point.  With stacked rows (Re phi, Im phi) and the
+// stacked X = M phi of the density kernel (M = [[Re D, Im D], [-Im D, Re D]]),
+// rho = sum_rows phi X and the explicit derivative of E with respect to R_A is
+// the sum over both halves of Z grad(row), Z = 2 v X (+ Zx): each thread
+// forms the complex gradient of phi_p from the complex tables and adds
+// Z_re . grad Re phi_p + Z_im . grad Im phi_p.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grid_forces_k(
+    const double2* __restrict__ P, int n, int g, int na, int nl,
+    const double* __restrict__ alpha, const double* __restrict__ coef,
+    const int* __restrict__ comp, int nloc, int nao_pad, int ld,
+    const double* __restrict__ X, const double* __restrict__ v,
+    const double* __restrict__ Zx, double dV, double* __restrict__ part) {
+  __shared__ double s_coef[HQC_MAX_NA * HQC_MAX_NLOC];
+  __shared__ double s_alpha[HQC_MAX_NA];
+  __shared__ int s_comp[HQC_MAX_NLOC];
+  __shared__ double red[4];
+  for (int i = threadIdx.x; i < na * nloc; i += blockDim.x) s_coef[i] = coef[i];
+  for (int i = threadIdx.x; i < na; i += blockDim.x) s_alpha[i] = alpha[i];
+  for (int i = threadIdx.x; i < nloc; i += blockDim.x) s_comp[i] = comp[i];
+  __syncthreads();
+  const int A = blockIdx.y, b = blockIdx.z;
+  const int N = g * g * g;
+  const long tab = (long)(b * n + A) * 3 * g * na * nl;
+  double f[3] = {0.0, 0.0, 0.0};
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
+    const int iz = r % g, iy = (r / g) % g, ix = r / (g * g);
+    const double2* Pd[3] = {P + tab + (long)(0 * g + ix) * na * nl,
+                            P + tab + (long)(1 * g + iy) * na * nl,
+                            P + tab + (long)(2 * g + iz) * na * nl};
+    const double vr = v ? 2.0 * v[(long)b * ld + r] : 0.0;
+    for (int o = 0; o < nloc; ++o) {
+      const long row = (long)b * 2 * nao_pad + (long)A * nloc + o;
+      const long rowi = row + nao_pad;
+      double zr = 0.0, zi = 0.0;
+      if (v) {
+        zr = vr * X[row * ld + r];
+        zi = vr * X[rowi * ld + r];
+      }
+      if (Zx) {
+        zr += Zx[row * ld + r];
+        zi += Zx[rowi * ld + r];
+      }
+      const int c = s_comp[o];
+      double2 gr[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+      for (int a = 0; a < na; ++a) {
+        const double w = s_coef[a * nloc + o];
+        if (w == 0.0) continue;
+        const double m2a = -2.0 * s_alpha[a];
+        double2 val[3], der[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int pw = (c == d) ? 1 : 0;
+          const double2* t = Pd[d] + a * nl;
+          val[d] = t[pw];
+          der[d] = make_double2(m2a * t[pw + 1].x + (pw ? t[0].x : 0.0),
+                                m2a * t[pw + 1].y + (pw ? t[0].y : 0.0));
+        }
+        const double2 g0 = cmul(cmul(der[0], val[1]), val[2]);
+        const double2 g1 = cmul(cmul(val[0], der[1]), val[2]);
+        const double2 g2 = cmul(cmul(val[0], val[1]), der[2]);
+        gr[0].x += w * g0.x; gr[0].y += w * g0.y;
+        gr[1].x += w * g1.x; gr[1].y += w * g1.y;
+        gr[2].x += w * g2.x; gr[2].y += w * g2.y;
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) f[d] += zr * gr[d].x + zi * gr[d].y;
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    const double t = block_sum(f[c], red);
+    if (threadIdx.x == 0)
+      part[(((long)b * n + A) * gridDim.x + blockIdx.x) * 3 + c] = t * dV;
+  }
+}
+
+extern "C" int hqc_grid_forces_k(const double* P, int B, int n, int g, int na, int nl,
+                                 const double* alpha, const double* coef, const int* comp,
+                                 int nloc, int nao_pad, int ld, const double* X, const double* v,
+                                 const double* Zx, double dV, double* part, void* stream) {
+  if (B < 0 || n <= 0 || g <= 0 || na <= 0 || nloc <= 0) return HQC_E_ARG;
+  if (na > HQC_MAX_NA || nloc > HQC_MAX_NLOC) return HQC_E_SIZE;
+  if (nl < 3) return HQC_E_ARG;
+  if (v && !X) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  hipLaunchKernelGGL(k_grid_forces_k, dim3(HQC_FORCE_BLOCKS, n, B), dim3(256), 0,
+                     (hipStream_t)stream, (const double2*)P, n, g, na, nl, alpha, coef, comp,
+                     nloc, nao_pad, ld, X, v, Zx, dV, part);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }

@@ -28,12 +28,16 @@ __device__ __forceinline__ void hermite(double mu, double R, double* e, int nmax
 // unit-normalised integral and are skipped.
 #define INT_EXP_CUT 46.0
 
-// One thread per (b, A, Bat, o, q).
+// One thread per (b, A, Bat, o, q).  CPLX: k-point version, every lattice term
+// carries the Bloch phase e^{-i k.T} (hqc/pbc/solver.py:270-271) and the
+// outputs are complex (interleaved re, im).
+template <bool CPLX>
 __global__ __launch_bounds__(256) void k_overlap_kinetic(
-    const double* __restrict__ xp, long total, int n, const double* __restrict__ lat, int nT,
-    const double* __restrict__ alpha, int na, const double* __restrict__ coef,
-    const int* __restrict__ comp, int nloc, double* __restrict__ S, double* __restrict__ Tm,
-    double* __restrict__ dS, double* __restrict__ dT) {
+    const double* __restrict__ xp, const double* __restrict__ kpt, long total, int n,
+    const double* __restrict__ lat, int nT, const double* __restrict__ alpha, int na,
+    const double* __restrict__ coef, const int* __restrict__ comp, int nloc,
+    double* __restrict__ S, double* __restrict__ Tm, double* __restrict__ dS,
+    double* __restrict__ dT) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= total) return;
   const int q = (int)(idx % nloc);
@@ -47,6 +51,12 @@ __global__ __launch_bounds__(256) void k_overlap_kinetic(
   const double* xA = xp + ((long)b * n + A) * 3;
   const double* xB = xp + ((long)b * n + Bat) * 3;
   const double d0 = xA[0] - xB[0], d1 = xA[1] - xB[1], d2 = xA[2] - xB[2];
+  double k0 = 0.0, k1 = 0.0, k2 = 0.0;
+  if (CPLX) {
+    k0 = kpt[b * 3];
+    k1 = kpt[b * 3 + 1];
+    k2 = kpt[b * 3 + 2];
+  }
   const int co = comp[o], cq = comp[q];
   int base[3] = {0, 0, 0};
   if (co >= 0) base[co] += 1;
@@ -54,8 +64,9 @@ __global__ __launch_bounds__(256) void k_overlap_kinetic(
   const bool deriv = dS != nullptr;
   int nmax[3];
   for (int d = 0; d < 3; ++d) nmax[d] = base[d] + 2 + (deriv ? 1 : 0);
-  double s = 0.0, t = 0.0;
-  double ds[3] = {0.0, 0.0, 0.0}, dt[3] = {0.0, 0.0, 0.0};
+  // [0] real part, [1] imaginary part (CPLX only)
+  double s[2] = {0.0, 0.0}, t[2] = {0.0, 0.0};
+  double ds[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}}, dt[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
   for (int ia = 0; ia < na; ++ia) {
     const double wa = coef[ia * nloc + o];
     if (wa == 0.0) continue;
@@ -67,68 +78,124 @@ __global__ __launch_bounds__(256) void k_overlap_kinetic(
       double fac = wa * wb * M_PI * sqrt(M_PI) * pow(a + bb, -1.5);
       if (co >= 0) fac *= 1.0 / (2.0 * a);
       if (cq >= 0) fac *= -1.0 / (2.0 * bb);
-      double ss = 0.0, tt = 0.0;
-      double dss[3] = {0.0, 0.0, 0.0}, dtt[3] = {0.0, 0.0, 0.0};
+      double ss[2] = {0.0, 0.0}, tt[2] = {0.0, 0.0};
+      double dss[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+      double dtt[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
       for (int it = 0; it < nT; ++it) {
-        const double R[3] = {d0 + lat[3 * it], d1 + lat[3 * it + 1], d2 + lat[3 * it + 2]};
+        const double* T3 = lat + 3 * it;
+        const double R[3] = {d0 + T3[0], d1 + T3[1], d2 + T3[2]};
         const double mr2 = mu * (R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
         if (mr2 > INT_EXP_CUT) continue;
+        double ph[2] = {1.0, 0.0};
+        if (CPLX) {
+          double sn, cs;
+          sincos(k0 * T3[0] + k1 * T3[1] + k2 * T3[2], &sn, &cs);
+          ph[0] = cs;
+          ph[1] = -sn;
+        }
         const double E = exp(-mr2);
         double h[3][INT_MAX_ORDER + 1];
         hermite(mu, R[0], h[0], nmax[0]);
         hermite(mu, R[1], h[1], nmax[1]);
         hermite(mu, R[2], h[2], nmax[2]);
         const double hx = h[0][base[0]], hy = h[1][base[1]], hz = h[2][base[2]];
-        ss += E * hx * hy * hz;
-        tt += E * (h[0][base[0] + 2] * hy * hz + hx * h[1][base[1] + 2] * hz +
-                   hx * hy * h[2][base[2] + 2]);
+        const double vs = E * hx * hy * hz;
+        const double vt = E * (h[0][base[0] + 2] * hy * hz + hx * h[1][base[1] + 2] * hz +
+                               hx * hy * h[2][base[2] + 2]);
+        double vds[3], vdt[3];
         if (deriv) {
           const double gx = h[0][base[0] + 1], gy = h[1][base[1] + 1], gz = h[2][base[2] + 1];
-          dss[0] += E * gx * hy * hz;
-          dss[1] += E * hx * gy * hz;
-          dss[2] += E * hx * hy * gz;
-          dtt[0] += E * (h[0][base[0] + 3] * hy * hz + gx * h[1][base[1] + 2] * hz +
-                         gx * hy * h[2][base[2] + 2]);
-          dtt[1] += E * (h[0][base[0] + 2] * gy * hz + hx * h[1][base[1] + 3] * hz +
-                         hx * gy * h[2][base[2] + 2]);
-          dtt[2] += E * (h[0][base[0] + 2] * hy * gz + hx * h[1][base[1] + 2] * gz +
-                         hx * hy * h[2][base[2] + 3]);
+          vds[0] = E * gx * hy * hz;
+          vds[1] = E * hx * gy * hz;
+          vds[2] = E * hx * hy * gz;
+          vdt[0] = E * (h[0][base[0] + 3] * hy * hz + gx * h[1][base[1] + 2] * hz +
+                        gx * hy * h[2][base[2] + 2]);
+          vdt[1] = E * (h[0][base[0] + 2] * gy * hz + hx * h[1][base[1] + 3] * hz +
+                        hx * gy * h[2][base[2] + 2]);
+          vdt[2] = E * (h[0][base[0] + 2] * hy * gz + hx * h[1][base[1] + 2] * gz +
+                        hx * hy * h[2][base[2] + 3]);
+        }
+#pragma unroll
+        for (int part = 0; part < (CPLX ? 2 : 1); ++part) {
+          ss[part] += ph[part] * vs;
+          tt[part] += ph[part] * vt;
+          if (deriv) {
+            for (int c = 0; c < 3; ++c) {
+              dss[part][c] += ph[part] * vds[c];
+              dtt[part][c] += ph[part] * vdt[c];
+            }
+          }
         }
       }
-      s += fac * ss;
-      t += -0.5 * fac * tt;
-      if (deriv) {
-        for (int c = 0; c < 3; ++c) {
-          ds[c] += fac * dss[c];
-          dt[c] += -0.5 * fac * dtt[c];
+#pragma unroll
+      for (int part = 0; part < (CPLX ? 2 : 1); ++part) {
+        s[part] += fac * ss[part];
+        t[part] += -0.5 * fac * tt[part];
+        if (deriv) {
+          for (int c = 0; c < 3; ++c) {
+            ds[part][c] += fac * dss[part][c];
+            dt[part][c] += -0.5 * fac * dtt[part][c];
+          }
         }
       }
     }
   }
   const long nao = (long)n * nloc;
   const long e = ((long)b * nao + (long)A * nloc + o) * nao + (long)Bat * nloc + q;
-  S[e] = s;
-  Tm[e] = t;
-  if (deriv) {
-    for (int c = 0; c < 3; ++c) {
-      dS[e * 3 + c] = ds[c];
-      dT[e * 3 + c] = dt[c];
+  if (!CPLX) {
+    S[e] = s[0];
+    Tm[e] = t[0];
+    if (deriv) {
+      for (int c = 0; c < 3; ++c) {
+        dS[e * 3 + c] = ds[0][c];
+        dT[e * 3 + c] = dt[0][c];
+      }
+    }
+  } else {
+    S[2 * e] = s[0];
+    S[2 * e + 1] = s[1];
+    Tm[2 * e] = t[0];
+    Tm[2 * e + 1] = t[1];
+    if (deriv) {
+      for (int c = 0; c < 3; ++c) {
+        dS[2 * (e * 3 + c)] = ds[0][c];
+        dS[2 * (e * 3 + c) + 1] = ds[1][c];
+        dT[2 * (e * 3 + c)] = dt[0][c];
+        dT[2 * (e * 3 + c) + 1] = dt[1][c];
+      }
     }
   }
+}
+
+template <bool CPLX>
+static int launch_ovlp(const double* xp, const double* kpt, int B, int n, const double* lattice,
+                       int nT, const double* alpha, int na, const double* coef, const int* comp,
+                       int nloc, double* S, double* T, double* dS, double* dT, void* stream) {
+  if (B < 0 || n <= 0 || nT <= 0 || na <= 0 || nloc <= 0) return HQC_E_ARG;
+  if ((dS == nullptr) != (dT == nullptr)) return HQC_E_ARG;
+  if (CPLX && !kpt) return HQC_E_ARG;
+  if (na > INT_MAX_NA || nloc > INT_MAX_NLOC) return HQC_E_SIZE;
+  if (B == 0) return HQC_OK;
+  const long total = (long)B * n * n * nloc * nloc;
+  hipLaunchKernelGGL(k_overlap_kinetic<CPLX>, dim3(hqc_div_up(total, 256)), dim3(256), 0,
+                     (hipStream_t)stream, xp, kpt, total, n, lattice, nT, alpha, na, coef, comp,
+                     nloc, S, T, dS, dT);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
 }
 
 extern "C" int hqc_overlap_kinetic(const double* xp, int B, int n, const double* lattice, int nT,
                                    const double* alpha, int na, const double* coef,
                                    const int* comp, int nloc, double* S, double* T, double* dS,
                                    double* dT, void* stream) {
-  if (B < 0 || n <= 0 || nT <= 0 || na <= 0 || nloc <= 0) return HQC_E_ARG;
-  if ((dS == nullptr) != (dT == nullptr)) return HQC_E_ARG;
-  if (na > INT_MAX_NA || nloc > INT_MAX_NLOC) return HQC_E_SIZE;
-  if (B == 0) return HQC_OK;
-  const long total = (long)B * n * n * nloc * nloc;
-  hipLaunchKernelGGL(k_overlap_kinetic, dim3(hqc_div_up(total, 256)), dim3(256), 0,
-                     (hipStream_t)stream, xp, total, n, lattice, nT, alpha, na, coef, comp, nloc,
-                     S, T, dS, dT);
-  HQC_CHECK_LAUNCH();
-  return HQC_OK;
+  return launch_ovlp<false>(xp, nullptr, B, n, lattice, nT, alpha, na, coef, comp, nloc, S, T,
+                            dS, dT, stream);
+}
+
+extern "C" int hqc_overlap_kinetic_k(const double* xp, const double* kpt, int B, int n,
+                                     const double* lattice, int nT, const double* alpha, int na,
+                                     const double* coef, const int* comp, int nloc, double* S,
+                                     double* T, double* dS, double* dT, void* stream) {
+  return launch_ovlp<true>(xp, kpt, B, n, lattice, nT, alpha, na, coef, comp, nloc, S, T, dS,
+                           dT, stream);
 }

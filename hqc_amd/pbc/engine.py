@@ -252,7 +252,7 @@ class Engine:
         return v, part.sum(1), rg
 
     # ----------------------------------------------------------------- set-up
-    def setup(self, xpb):
+    def setup(self, xpb, kpt=None):
         """Per-geometry constants for xp (B, n, 3) in Bohr on the device.  Every
         tensor whose leading axis is the geometry is listed in st['per'] so the
         SCF can keep the active geometries as a contiguous prefix."""
@@ -424,7 +424,7 @@ class Engine:
         if self.dft:
             return V2, EH + Exc
         Vx = -0.5 * self.exchange(st, na, mo, occ)
-        Ex = 0.5 * _tr(Vx, D)
+        Ex = 0.5 * _tr(Vx, D).real
         return V2 + Vx, EH + Ex
 
     # --------------------------------------------------------------------- SCF
@@ -495,7 +495,7 @@ class Engine:
                 H = st["Hcore"][:na]
                 Dc = loc["Dc"][:na]
                 V2, E2 = self.fock2e(st, na, Dc, loc["Mo"][:na], loc["Oc"][:na], full=True)
-                Enew = _tr(H, Dc) + E2
+                Enew = _tr(H, Dc).real + E2
                 w1n, mon, cwn = self._eigh_orth(H + V2, st["v"][:na], loc["Cw"][:na])
                 loc["Cw"][:na] = cwn
                 Dn, On = self.density_matrix(mon, w1n)
@@ -515,7 +515,7 @@ class Engine:
             D, occ = self.density_matrix(mo, w1)
             V2, E2 = self.fock2e(st, na, D, mo, occ)
             H, S = st["Hcore"][:na], st["S"][:na]
-            Enew = _tr(H, D) + E2
+            Enew = _tr(H, D).real + E2
             F = H + V2
             e = S @ D @ F - F @ D @ S
             if self.precision == "mixed":
@@ -563,15 +563,17 @@ class Engine:
             return ((dE > tol) | (err > 10 * tol)) & (loop < self.max_cycle)
 
         na = self._compact(st, keys, loc, order, cont(na), na)
-        eye = torch.eye(m, dtype=torch.float64, device=dev)
+        # complex (k-point) error vectors give a complex symmetric B, as 'imn,jmn->ij'
+        # does in the reference (no conjugate)
+        eye = torch.eye(m, dtype=loc["Bh"].dtype, device=dev)
         while na > 0:                                                     # scf.py:174-239
             Bm = loc["Bh"][:na]
             eta = 1e-12 * (Bm.diagonal(dim1=-2, dim2=-1).sum(-1) / m + 1.0)
-            h = torch.zeros(na, m + 1, m + 1, dtype=torch.float64, device=dev)
+            h = torch.zeros(na, m + 1, m + 1, dtype=Bm.dtype, device=dev)
             h[:, :m, :m] = Bm + eta[:, None, None] * eye
             h[:, :m, m] = -1.0
             h[:, m, :m] = -1.0
-            rhs = torch.zeros(na, m + 1, 1, dtype=torch.float64, device=dev)
+            rhs = torch.zeros(na, m + 1, 1, dtype=Bm.dtype, device=dev)
             rhs[:, m, 0] = -1.0
             ck = torch.linalg.solve(h, rhs)[:, :m, 0]
             Fk = loc["Fk"][:na]
@@ -588,19 +590,19 @@ class Engine:
         return loc["Mo"], loc["W1"], loc["E"], loc["nit"] != self.max_cycle, loc["nit"], order
 
     # ---------------------------------------------------------------- results
-    def solve(self, xpb):
+    def solve(self, xpb, kpt=None):
         """Full solver output for a batch (B, n, 3) Bohr.
 
         Per-geometry device state in st stays in slot order (st['order'][i] is
         the geometry held by slot i); the user-facing results mo, D, w1, E, conv,
         nit, Ki, Vep, Vee, Se, occ are returned in input order."""
-        st = self.setup(xpb)
+        st = self.setup(xpb, kpt)
         mo, w1, E, conv, nit, order = self.scf(st)
         B = st["B"]
         D, occ = self.density_matrix(mo, w1)                              # solver.py:758-765
         V2, E2 = self.fock2e(st, B, D, mo, occ, full=True)
-        Ki = _tr(st["T"], D)
-        Vep = _tr(st["V"], D)
+        Ki = _tr(st["T"], D).real
+        Vep = _tr(st["V"], D).real
         Se = self.entropy(occ)
         slot = dict(mo=mo, w1=w1, D=D, occ=occ, E=E)
         st["slot"] = slot

@@ -55,6 +55,8 @@ def _exchange_adjoint(engine, st):
 def compute_forces(engine, st, kappa=10, Gmax=15):
     """-dA/dR (B, n, 3) in Ha/Bohr after engine.solve, in input order.  Works on
     the slot-ordered device state of the SCF and un-permutes at the end."""
+    if st.get("kpt") is not None:
+        return compute_forces_k(engine, st, kappa, Gmax)
     c = engine.c
     B, dev = st["B"], c.device
     f64 = torch.float64
@@ -91,6 +93,47 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
     return st["unperm"](F + Fpp)
 
 
+def compute_forces_k(engine, st, kappa=10, Gmax=15):
+    """-dA/dR at one k-point per batch entry (DFT).  Complex S, T, D, W enter as
+    2 Re sum_q [conj(W_pq) dS_pq - conj(D_pq) dT_pq]; the grid adjoint uses the
+    stacked rows (Re phi; Im phi) with X = M rows, M = [[Re D, Im D], [-Im D, Re D]]."""
+    if not engine.dft:
+        raise NotImplementedError("Hartree-Fock forces at a k-point are not implemented")
+    c = engine.c
+    B, dev = st["B"], c.device
+    f64, c128 = torch.float64, torch.complex128
+    sl = st["slot"]
+    D, occ, w1, mo = sl["D"], sl["occ"], sl["w1"], sl["mo"]
+    S = torch.empty(B, c.nao, c.nao, dtype=c128, device=dev)
+    T = torch.empty_like(S)
+    dS = torch.empty(B, c.nao, c.nao, 3, dtype=c128, device=dev)
+    dT = torch.empty_like(dS)
+    K.overlap_kinetic_k(st["xpb"], st["kpt"], c.lattice, c.alpha, c.coef, c.comp, S, T, dS, dT)
+    W = (mo * (occ * w1).unsqueeze(-2)) @ mo.mH
+    atom = c.atom_of_ao
+    other = (atom[:, None] != atom[None, :]).to(f64)
+    t = 2.0 * ((W * other).conj().unsqueeze(-1) * dS
+               - (D * other).conj().unsqueeze(-1) * dT).sum(2).real
+    F = torch.zeros(B, c.n, 3, dtype=f64, device=dev)
+    F.index_add_(1, atom, t)
+    X = torch.zeros(B, 2 * c.nao_pad, c.ld, dtype=f64, device=dev)
+    rho = torch.zeros(B, c.ld, dtype=f64, device=dev)
+    K.ao_density(st["phi"], engine.stacked_dm(D), rho, X)
+    v, _, rg = engine.xc_potential(rho, save_rg=True)
+    veff = (v + st["vloc"]).contiguous()
+    P = torch.empty(B, c.n, 3, c.g, c.na, 3, dtype=c128, device=dev)
+    K.gto_tables_k(st["xpb"], st["kpt"], c.g, c.L, c.tmax_img, c.alpha, 3, P)
+    fp = torch.empty(B, c.n, K.force_blocks(c.g), 3, dtype=f64, device=dev)
+    K.grid_forces_k(P, c.alpha, c.coef, c.comp, c.nao_pad, c.ld, X, veff, None, c.dV, fp)
+    F += fp.sum(2)
+    dE = torch.empty(B, c.n, 3, dtype=f64, device=dev)
+    K.vloc_force(st["xpb"], c.g, c.L, c.VG, rg, dE)
+    F -= dE
+    Epp, Fpp = ewald_pp(st["xpb"] / c.rs, c.L / c.rs, c.rs, kappa, Gmax, forces=True)
+    st["Epp"] = st["unperm"](Epp)
+    return st["unperm"](F + Fpp)
+
+
 class FreeEnergy(torch.autograd.Function):
     """A(xp) in Ry for xp (B, n, 3) in rs units, with the analytic gradient.
 
@@ -98,8 +141,8 @@ class FreeEnergy(torch.autograd.Function):
     dA_Ry/dxp_rs = -2 rs F."""
 
     @staticmethod
-    def forward(ctx, xp, runner):
-        A, F = runner.free_energy_and_forces(xp)
+    def forward(ctx, xp, runner, kpt=None):
+        A, F = runner.free_energy_and_forces(xp, kpt)
         ctx.save_for_backward(F)
         ctx.rs = runner.rs
         return A
@@ -107,4 +150,4 @@ class FreeEnergy(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gA):
         (F,) = ctx.saved_tensors
-        return gA[:, None, None] * (-2.0 * ctx.rs) * F, None
+        return gA[:, None, None] * (-2.0 * ctx.rs) * F, None, None

@@ -3,7 +3,8 @@
 lcao(xp) -> (mo_coeff (n_ao, n_mo), bands (n_mo,) [Ry], E [Ry, without V_pp]),
 the 3-tuple of hqc/pbc/lcao.py:77-78.  search_method defaults to 'newton' as
 in the reference.  xp may also be (B, n, 3): the B geometries run together
-and every output gains a leading batch axis.
+and every output gains a leading batch axis.  gamma=False gives lcao(xp, kpt)
+(hqc/pbc/lcao.py:80-94) with complex mo_coeff.
 """
 import torch
 
@@ -23,8 +24,8 @@ def make_lcao(n, L, rs, basis, rcut=24, tol=1e-7, max_cycle=100, grid_length=0.1
                          search_method=search_method, search_cycle=search_cycle,
                          search_tol=search_tol, gamma=gamma, precision=precision)
 
-    def lcao(xp):
-        out = solver.batch(xp)
+    def lcao(xp, kpt=None):
+        out = solver.batch(xp, kpt)
         mo, bands, E = out[0], out[2], out[3]
         if torch.as_tensor(xp).dim() == 2:
             return mo[0], bands[0], E[0]

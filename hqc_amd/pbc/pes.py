@@ -8,7 +8,8 @@ Additions (not in the reference): pes.energy_and_forces(xp) returns
 (E+V_pp [Ry], F [Ha/Bohr]) with F = -dA/dR of the free energy
 A = E - sigma*S_e + V_pp (equal to E+V_pp without smearing), and
 pes.free_energy(xp) is a torch.autograd-differentiable A [Ry] whose gradient
-comes from the hand-written adjoint.  xp may be (B, n, 3) everywhere.
+comes from the hand-written adjoint.  xp may be (B, n, 3) everywhere.  With
+gamma=False every call takes kpt as well (pes_kpt, hqc/pbc/pes.py:88-97).
 """
 import torch
 
@@ -28,8 +29,8 @@ class PES:
     def _vpp(self, x):
         return ewald_pp(x, self.L, self.rs, self.kappa, self.Gmax)
 
-    def __call__(self, xp):
-        st, single, x = self.solver.run(xp)
+    def __call__(self, xp, kpt=None):
+        st, single, x = self.solver.run(xp, kpt)
         Vpp = self._vpp(x)
         Etot = st["E"] * RY + Vpp
         if self.mode == "default":
@@ -40,24 +41,24 @@ class PES:
             out = tuple(o[0] for o in out)
         return out[0] if self.mode == "default" else out
 
-    def energy_and_forces(self, xp):
-        st, single, x = self.solver.run(xp)
+    def energy_and_forces(self, xp, kpt=None):
+        st, single, x = self.solver.run(xp, kpt)
         F = compute_forces(self.solver.engine, st, self.kappa, self.Gmax)
         Etot = st["E"] * RY + st["Epp"]
         return (Etot[0], F[0]) if single else (Etot, F)
 
-    def free_energy_and_forces(self, x):
-        st, _, x = self.solver.run(x)
+    def free_energy_and_forces(self, x, kpt=None):
+        st, _, x = self.solver.run(x, kpt)
         F = compute_forces(self.solver.engine, st, self.kappa, self.Gmax)
         A = st["E"] * RY - RY * self.solver.engine.sigma * st["Se"] + st["Epp"]
         return A, F
 
-    def free_energy(self, xp):
+    def free_energy(self, xp, kpt=None):
         x, single = _as_batch(xp.detach() if torch.is_tensor(xp) else xp, self.solver.n,
                               self.solver.cell.device)
         if torch.is_tensor(xp) and xp.requires_grad:
             x = xp if xp.dim() == 3 else xp.unsqueeze(0)
-        A = FreeEnergy.apply(x, self)
+        A = FreeEnergy.apply(x, self, kpt)
         return A[0] if single else A
 
 

@@ -7,14 +7,16 @@ converged) of hqc/pbc/solver.py:767.  Results are FP64 torch tensors on the
 GPU.  `use_jit` is accepted and ignored (there is no tracing step).
 
 Beyond the reference: solver.batch(xp) takes (B, n, 3) and runs the B SCFs
-together; `precision='fp32'` runs the AO x grid contractions on FP32 MFMA.
-Only the Gamma point is implemented on the GPU (gamma=False raises
-NotImplementedError).
+together; `precision='mixed'` runs the AO x grid contraction increments on
+FP32 MFMA (Gamma only).  gamma=False gives solver(xp, kpt) at one k-point
+(hqc/pbc/solver.py:769-837, 901-962); kpt (3,) or (B, 3) in 1/Bohr, so a
+Monkhorst-Pack mesh of one geometry is a batch of repeated positions.
 """
 import numpy as np
 import torch
 
 from .engine import RY, Cell, Engine
+from .kengine import KEngine
 
 
 def _as_batch(xp, n, device):
@@ -35,11 +37,10 @@ class GPWSolver:
                  dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100,
                  search_tol=1e-7, gamma=True, precision="fp64", device="cuda"):
-        if not gamma:
-            raise NotImplementedError("k-point solver is not implemented on the GPU yet")
         self.n, self.L, self.rs = n, L, rs
+        self.gamma = gamma
         self.cell = Cell(n, L, rs, basis, rcut=rcut, grid_length=grid_length, device=device)
-        self.engine = Engine(self.cell, dft=dft, xc=xc, smearing=smearing,
+        self.engine = (Engine if gamma else KEngine)(self.cell, dft=dft, xc=xc, smearing=smearing,
                              smearing_method=smearing_method, smearing_sigma=smearing_sigma,
                              search_method=search_method, search_cycle=search_cycle,
                              search_tol=search_tol, tol=tol, max_cycle=max_cycle, diis=diis,
@@ -47,20 +48,34 @@ class GPWSolver:
                              diis_damp=diis_damp, precision=precision)
         self.last = None
 
-    def run(self, xp):
+    def _kpts(self, kpt, B):
+        if self.gamma:
+            if kpt is not None:
+                raise TypeError("a Gamma-point solver takes no kpt (make it with gamma=False)")
+            return None
+        if kpt is None:
+            raise TypeError("gamma=False: call solver(xp, kpt)")
+        k = torch.as_tensor(np.asarray(kpt) if not torch.is_tensor(kpt) else kpt,
+                            dtype=torch.float64).to(self.cell.device)
+        if k.dim() == 1:
+            k = k.unsqueeze(0).expand(B, 3)
+        assert k.shape == (B, 3)
+        return k.contiguous()
+
+    def run(self, xp, kpt=None):
         """Solve a batch; returns (state dict, single flag, xp tensor in rs units)."""
         x, single = _as_batch(xp, self.n, self.cell.device)
-        st = self.engine.solve(x * self.rs)
+        st = self.engine.solve(x * self.rs, self._kpts(kpt, x.shape[0]))
         self.last = st
         return st, single, x
 
-    def batch(self, xp):
-        st, _, _ = self.run(xp)
+    def batch(self, xp, kpt=None):
+        st, _, _ = self.run(xp, kpt)
         return (st["mo"], st["D"], st["w1"] * RY, st["E"] * RY, st["Ki"] * RY,
                 st["Vep"] * RY, st["Vee"] * RY, st["Se"], st["conv"])
 
-    def __call__(self, xp):
-        out = self.batch(xp)
+    def __call__(self, xp, kpt=None):
+        out = self.batch(xp, kpt)
         if torch.as_tensor(xp).dim() == 2:
             return tuple(o[0] for o in out)
         return out

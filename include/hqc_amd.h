@@ -62,6 +62,21 @@ int hqc_ao_values_f32(const double* P, int B, int n, int g, int na, int nl,
                       const double* coef, const int* comp, int nloc, int nao_pad,
                       int ld, float* phi, void* stream);
 
+/* k-point tables and AO values (hqc/pbc/gto.py:371-420,
+ * eval_pbc_gaussian_power_x_kpt / eval_pbc_ao_kpt):
+ *   P[b,A,d,j,a,l] = sum_t (x - tL)^l exp(-alpha_a (x - tL)^2) exp(i k[b,d] t L)
+ * complex128 (interleaved), kpt (B,3) in 1/Bohr.  hqc_ao_values_k writes the
+ * complex AO values "stacked" into phi (B, 2*nao_pad, ld) FP64: rows
+ * [0,nao_pad) Re phi, rows [nao_pad, 2 nao_pad) Im phi.  With this layout
+ *   V = dV phi^H diag(v) phi: M = hqc_ao_vmat_f64 on the stacked rows,
+ *     Re V = M_rr + M_ii, Im V = M_ri - M_ir;
+ *   rho = Re sum_pq D_pq phi_p phi_q^*: hqc_ao_density_f64 with the real
+ *     symmetric M = [[Re D, Im D], [-Im D, Re D]]. */
+int hqc_gto_tables_k(const double* xp, const double* kpt, int B, int n, int g, double L,
+                     int tmax, const double* alpha, int na, int nl, double* P, void* stream);
+int hqc_ao_values_k(const double* P, int B, int n, int g, int na, int nl, const double* coef,
+                    const int* comp, int nloc, int nao_pad, int ld, double* phi, void* stream);
+
 /* Electron density on the grid and (optionally) X = D phi.
  * Replaces the 'pr,qr,pq->r' contraction of Exc_Vxc_integral
  * (hqc/pbc/solver.py:690) and the density used by hartree_rhoG (:627).
@@ -201,6 +216,15 @@ int hqc_overlap_kinetic(const double* xp, int B, int n, const double* lattice, i
                         int nloc, double* S, double* T, double* dS, double* dT,
                         void* stream);
 
+/* k-point overlap and kinetic matrices, eval_overlap_kinetic_kpt
+ * (hqc/pbc/solver.py:257-305): every lattice term carries exp(-i k.T)
+ * (:270-271).  kpt (B,3) 1/Bohr; S,T (B,n_ao,n_ao) complex128 interleaved;
+ * dS, dT (B,n_ao,n_ao,3) complex128 or NULL (first-argument derivatives). */
+int hqc_overlap_kinetic_k(const double* xp, const double* kpt, int B, int n,
+                          const double* lattice, int nT, const double* alpha, int na,
+                          const double* coef, const int* comp, int nloc, double* S, double* T,
+                          double* dS, double* dT, void* stream);
+
 /* Ewald proton-proton sum (hqc/pbc/potential.py:35-67,103-130).
  * xfrac (B,npair,3) minimum-image fractional separations; Gv (nG,3) integer
  * vectors with weights gk (nG); out (B,npair,4) = (psi, dpsi/dr_x,y,z). */
@@ -218,6 +242,16 @@ int hqc_grid_forces(const double* P, int B, int n, int g, int na, int nl, const 
                     const double* coef, const int* comp, int nloc, int nao_pad, int ld,
                     const double* X, const double* v, const double* Zx, double dV,
                     double* part, void* stream);
+
+/* Grid adjoint at a k-point (new): complex tables P (B,n,3,g,na,nl)
+ * complex128 from hqc_gto_tables_k with nl = l_max + 2; X, Zx stacked
+ * (B, 2*nao_pad, ld) as from hqc_ao_density_f64 on stacked rows;
+ *   F[b,A,c] = dV sum_r sum_{p in A} (Z_re d Re phi_p/dr_c + Z_im d Im phi_p/dr_c),
+ *   Z = 2 v X + Zx. */
+int hqc_grid_forces_k(const double* P, int B, int n, int g, int na, int nl, const double* alpha,
+                      const double* coef, const int* comp, int nloc, int nao_pad, int ld,
+                      const double* X, const double* v, const double* Zx, double dV,
+                      double* part, void* stream);
 
 /* Structure-factor derivative of the local potential energy:
  *   dE[b,A,c] = Re sum_G i G_c VG(G) exp(-i G.R_A) conj(rho~[b,G]). */

@@ -379,10 +379,10 @@ def _ao_from_tables(sysm, P, grad_dir=None):
     return np.array(out)
 
 
-def ao_gradients(sysm, xp_bohr):
-    """grad phi (3, n_ao, N_g) = d phi / d r on the grid (Gamma), from the same
-    1-D tables with one extra power.  New in this build (forces)."""
-    P = gto_tables(sysm, xp_bohr, deriv=True)
+def ao_gradients(sysm, xp_bohr, kpt=None):
+    """grad phi (3, n_ao, N_g) = d phi / d r on the grid (complex at a k-point),
+    from the same 1-D tables with one extra power.  New in this build (forces)."""
+    P = gto_tables(sysm, xp_bohr, kpt, deriv=True)
     return np.array([_ao_from_tables(sysm, P, grad_dir=d) for d in range(3)])
 
 
@@ -784,7 +784,11 @@ def entropy(occ, eps=1e-15):
 # SCF (hqc/pbc/scf.py)
 # --------------------------------------------------------------------------
 def _eigh_orth(F, v):
-    w, c = np.linalg.eigh(np.conj(v.T) @ F @ v)
+    """eigh of v^H F v as jnp.linalg.eigh does it (symmetrize_input=True: the Hermitian
+    part (A + A^H)/2 is decomposed); this matters at k-points, where the DIIS
+    extrapolation with complex coefficients leaves F slightly non-Hermitian."""
+    A = np.conj(v.T) @ F @ v
+    w, c = np.linalg.eigh(0.5 * (A + np.conj(A.T)))
     return w, v @ c
 
 
@@ -952,7 +956,7 @@ class Solver:
         occ = self.occupation(w1)
         Se = entropy(occ)
         self.last = dict(S=S, T=T, V=V, Hcore=Hcore, phi=phi, dm=dm, occ=occ, w1=w1, mo=mo,
-                         n_iter=nit, xpb=xpb, E=E)
+                         n_iter=nit, xpb=xpb, E=E, kpt=k)
         return mo, dm, w1 * RY, E * RY, Ki * RY, Vep * RY, Vee * RY, Se, conv
 
 
@@ -1010,7 +1014,8 @@ def ewald_pp(xp, L, rs, kappa=10, Gmax=15, forces=False):
 # Forces of the Mermin free energy A = E - sigma S_e + V_pp (new in this build)
 # --------------------------------------------------------------------------
 def forces(solver, xp, kappa=10, Gmax=15):
-    """Analytic -dA/dR (n, 3) in Ha/Bohr at Gamma after solver.solve(xp).
+    """Analytic -dA/dR (n, 3) in Ha/Bohr after solver.solve(xp[, kpt]) (Gamma, or a
+    k-point for DFT; complex matrices enter as 2 Re Tr[conj(W) dS] etc.).
 
     A = E_elec - sigma S_e + E_pp (Hartree).  Terms: kinetic -Tr[dT D], Pulay
     +Tr[W dS] (W = C f eps C^T), grid adjoint dV sum_r Z_p grad phi_p with
@@ -1023,14 +1028,17 @@ def forces(solver, xp, kappa=10, Gmax=15):
     xpb = st["xpb"]
     n, nloc = s.n, s.basis.n_ao_atom
     dm, occ, w1, mo, phi = st["dm"], st["occ"], st["w1"], st["mo"], st["phi"]
-    _, _, dS, dT = overlap_kinetic(s, xpb, deriv=True)
-    W = (mo * (occ * w1)) @ mo.T
+    k = st.get("kpt")
+    if k is not None and not solver.dft:
+        raise NotImplementedError("Hartree-Fock forces at a k-point")
+    _, _, dS, dT = overlap_kinetic(s, xpb, k, deriv=True)
+    W = (mo * (occ * w1)) @ np.conj(mo).T
     atom = np.repeat(np.arange(n), nloc)
     F = np.zeros((n, 3))
     other = atom[:, None] != atom[None, :]
     for c in range(3):
-        tk = 2 * np.einsum("pq,pq->p", dm * other, dT[:, :, c])
-        ts = 2 * np.einsum("pq,pq->p", W * other, dS[:, :, c])
+        tk = 2 * np.einsum("pq,pq->p", np.conj(dm * other), dT[:, :, c]).real
+        ts = 2 * np.einsum("pq,pq->p", np.conj(W * other), dS[:, :, c]).real
         np.add.at(F[:, c], atom, -tk + ts)
     # grid adjoint
     rho = grid_density(phi, dm)
@@ -1038,7 +1046,7 @@ def forces(solver, xp, kappa=10, Gmax=15):
     veff = vloc_real(s, xpb) + vh
     if solver.dft:
         veff = veff + (xc_gga(s, rho)[1] if solver.gga else xc_eps_v(rho, solver.xc)[1])
-    X = dm @ phi
+    X = dm @ np.conj(phi)
     Z = 2 * veff[None, :] * X
     if not solver.dft:
         g = s.g
@@ -1056,9 +1064,9 @@ def forces(solver, xp, kappa=10, Gmax=15):
         St = phi @ phi.T * s.dV
         M = dm @ St @ dm
         Z = Z - Y - c0 * (M @ phi)
-    gphi = ao_gradients(s, xpb)
+    gphi = ao_gradients(s, xpb, k)
     for c in range(3):
-        np.add.at(F[:, c], atom, s.dV * np.einsum("pr,pr->p", Z, gphi[c]))
+        np.add.at(F[:, c], atom, s.dV * np.einsum("pr,pr->p", Z, gphi[c]).real)
     # explicit v_loc structure-factor term: dE/dR_A = Re sum_G iG VG e^{-iG.R_A} conj(rho~)
     ph = np.exp(-1j * np.einsum("xyzd,nd->nxyz", s.G, xpb))
     for c in range(3):
@@ -1068,9 +1076,9 @@ def forces(solver, xp, kappa=10, Gmax=15):
     return F + Fpp
 
 
-def free_energy(solver, xp, kappa=10, Gmax=15):
+def free_energy(solver, xp, kappa=10, Gmax=15, kpt=None):
     """A = E_elec - sigma S_e + V_pp in Hartree (the functional `forces` differentiates)."""
-    out = solver.solve(xp)
+    out = solver.solve(xp, kpt)
     E = out[3] / RY
     return E - solver.sigma * out[7] + ewald_pp(xp, solver.sys.L / solver.rs, solver.rs,
                                                 kappa, Gmax) / RY

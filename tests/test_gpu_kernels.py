@@ -157,6 +157,42 @@ def test_overlap_kinetic(K):
             assert np.allclose(a[0].cpu().numpy(), r, rtol=1e-12, atol=1e-13)
 
 
+def test_kpoint_tables_values_and_integrals(K):
+    """Bloch-phase tables, stacked complex AO values and complex S/T (+ derivatives)
+    against the oracle at a general k-point."""
+    s, xpb, L = _sys(n=3, basis="gth-dzvp")
+    b = s.basis
+    kpt = np.array([0.31, -0.57, 0.12])
+    print(f"[k tables] g={s.g}", flush=True)
+    P_ref = R.gto_tables(s, xpb, kpt, deriv=True)
+    nl = P_ref.shape[-1]
+    xp_t = torch.tensor(xpb[None], device=DEV)
+    k_t = torch.tensor(kpt[None], device=DEV)
+    alpha = torch.tensor(b.alpha, device=DEV)
+    P = torch.empty(1, s.n, 3, s.g, len(b.alpha), nl, dtype=torch.complex128, device=DEV)
+    K.gto_tables_k(xp_t, k_t, s.g, s.L, int(24 // s.L), alpha, nl, P)
+    assert np.allclose(P[0].cpu().numpy(), P_ref, rtol=1e-13, atol=1e-14)
+    nao_pad = (s.n_ao + 15) // 16 * 16
+    ld = (s.N + 63) // 64 * 64
+    phi = torch.zeros(1, 2 * nao_pad, ld, dtype=torch.float64, device=DEV)
+    coef = torch.tensor(b.coef, device=DEV).contiguous()
+    comp = torch.tensor(b.comp, dtype=torch.int32, device=DEV)
+    K.ao_values_k(P[..., :b.l_max + 1].contiguous(), coef, comp, nao_pad, ld, phi)
+    ref = R.ao_values(s, xpb, kpt)
+    got = phi[0, :s.n_ao, :s.N].cpu().numpy() + 1j * phi[0, nao_pad:nao_pad + s.n_ao, :s.N].cpu().numpy()
+    assert np.allclose(got, ref, rtol=1e-13, atol=1e-14)
+    S_ref, T_ref, dS_ref, dT_ref = R.overlap_kinetic(s, xpb, kpt, deriv=True)
+    nao = s.n_ao
+    S = torch.empty(1, nao, nao, dtype=torch.complex128, device=DEV)
+    T = torch.empty_like(S)
+    dS = torch.empty(1, nao, nao, 3, dtype=torch.complex128, device=DEV)
+    dT = torch.empty_like(dS)
+    K.overlap_kinetic_k(xp_t, k_t, torch.tensor(s.lattice, device=DEV).contiguous(), alpha,
+                        torch.tensor(b.coef, device=DEV), comp, S, T, dS, dT)
+    for a, r in ((S, S_ref), (T, T_ref), (dS, dS_ref), (dT, dT_ref)):
+        assert np.allclose(a[0].cpu().numpy(), r, rtol=1e-12, atol=1e-13)
+
+
 @pytest.mark.parametrize("method,search", [("fermi", "bisect"), ("fermi", "newton"),
                                            ("gauss", "bisect"), ("gauss", "newton")])
 def test_occupations(K, method, search):

@@ -124,3 +124,56 @@ def test_mixed_precision_matches_fp64():
     assert (F64 - Fmx).abs().max().item() <= 1e-5
     assert (p64.solver.last["w1"] - pmx.solver.last["w1"]).abs().max().item() <= 1e-6
     assert pmx.solver.engine.stats["low"] > 0
+
+
+KCASES = [
+    # name, basis, kwargs
+    ("dft_lda_k", "gth-dzv", dict(dft=True, smearing=True, smearing_sigma=0.1)),
+    ("dft_pbe_k", "gth-dzvp", dict(dft=True, xc="pbe,pbe", smearing=True, smearing_sigma=0.05)),
+    ("hf_k", "gth-dzv", dict(dft=False)),
+]
+
+
+@pytest.mark.parametrize("case", KCASES, ids=[c[0] for c in KCASES])
+def test_kpoint_solver_parity(case):
+    """Single-k SCF (complex Bloch AOs) against the oracle, batched over a 2x2x2
+    Monkhorst-Pack mesh of one geometry plus a second geometry at a general k."""
+    from hqc_amd.pbc import make_solver
+    name, basis, kw = case
+    n, rs, gl = 4, 1.5, 0.3
+    xp, L = _xp(n, 4)
+    xp2, _ = _xp(n, 5)
+    Lb = L * rs
+    mp = np.array([[i, j, k] for i in (-0.25, 0.25) for j in (-0.25, 0.25) for k in (-0.25, 0.25)])
+    kpts = np.concatenate([mp * 2 * np.pi / Lb, [[0.21, -0.4, 0.33]]])
+    xps = np.concatenate([np.repeat(xp[None], 8, 0), xp2[None]])
+    print(f"[k parity] {name}", flush=True)
+    kw = dict(kw, tol=1e-10, search_tol=1e-12)
+    sol = make_solver(n, L, rs, basis, grid_length=gl, gamma=False, **kw)
+    out = sol.batch(xps, kpts)
+    for b in (0, 3, 8):
+        ref = R.Solver(n, L, rs, basis, grid_length=gl, gamma=False, **kw).solve(xps[b], kpts[b])
+        assert bool(out[8][b]) == bool(ref[8])
+        assert abs(out[3][b].item() - ref[3]) < 1e-8, (b, out[3][b].item(), ref[3])
+        assert np.allclose(out[2][b].cpu().numpy(), ref[2], atol=1e-7)
+        for k in (4, 5, 6):
+            assert abs(out[k][b].item() - ref[k]) < 1e-7
+        assert abs(out[7][b].item() - ref[7]) < 1e-7
+        assert np.allclose(out[1][b].cpu().numpy(), ref[1], atol=1e-6)
+
+
+@pytest.mark.parametrize("xc", ["lda,vwn", "pbe,pbe"])
+def test_kpoint_forces_vs_oracle(xc):
+    from hqc_amd.pbc import make_pes
+    n, rs = 4, 1.5
+    xp, L = _xp(n, 6)
+    kpt = np.array([0.3, -0.2, 0.45])
+    print(f"[k forces] {xc}", flush=True)
+    kw = dict(dft=True, xc=xc, smearing=True, smearing_sigma=0.1, tol=1e-11, search_tol=1e-13,
+              search_cycle=200, max_cycle=300)
+    s = R.Solver(n, L, rs, "gth-dzv", grid_length=0.3, gamma=False, **kw)
+    s.solve(xp, kpt)
+    Fref = R.forces(s, xp)
+    pes = make_pes(n, L, rs, "gth-dzv", grid_length=0.3, gamma=False, **kw)
+    E, F = pes.energy_and_forces(xp, kpt)
+    assert np.allclose(F.cpu().numpy(), Fref, atol=1e-8), np.abs(F.cpu().numpy() - Fref).max()

@@ -119,3 +119,26 @@ def test_gga_scf_forces_match_finite_differences():
         xm = xp.copy(); xm[1, c] -= h / rs
         fd = -(R.free_energy(sv, xq) - R.free_energy(sv, xm)) / (2 * h)
         assert abs(F[1, c] - fd) < 1e-7, (c, F[1, c], fd)
+
+
+@pytest.mark.parametrize("xc", ["lda,vwn", "pbe,pbe"])
+def test_kpoint_forces_match_finite_differences(xc):
+    """Forces of the free energy at a fixed k-point (complex Bloch AOs) against
+    central differences."""
+    n, rs = 3, 1.5
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    rng = np.random.default_rng(9)
+    xp = rng.uniform(0, L, (n, 3))
+    kpt = rng.uniform(-np.pi / L / rs, np.pi / L / rs, 3)
+    sv = R.Solver(n, L, rs, "gth-dzv", grid_length=0.35, dft=True, xc=xc, tol=1e-12,
+                  max_cycle=400, smearing=True, smearing_sigma=0.1, search_tol=1e-14,
+                  search_cycle=200, gamma=False)
+    out = sv.solve(xp, kpt)
+    assert out[8]
+    F = R.forces(sv, xp)
+    h = 1e-4
+    for c in range(3):
+        xq = xp.copy(); xq[2, c] += h / rs
+        xm = xp.copy(); xm[2, c] -= h / rs
+        fd = -(R.free_energy(sv, xq, kpt=kpt) - R.free_energy(sv, xm, kpt=kpt)) / (2 * h)
+        assert abs(F[2, c] - fd) < 1e-7, (c, F[2, c], fd)
