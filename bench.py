@@ -294,13 +294,13 @@ def main():
     ms_v, fl_v = time_vmat(pes, B)
     ms_d, fl_d, keep = time_orb_density(pes, B)
     prec_v = "f64" if a.path == "fp64" else "f32"
-    n_fock = stats["full"] + stats["low"]
-    n_dens64 = n_fock - stats.get("rho_low", 0)
+    n_vmat = stats["v64"] if a.path == "fp64" else stats["v32"]
+    n_dens64 = stats["rho64"]
     vname = "k_vmat_pipe<" + ("double" if prec_v == "f64" else "float")
     cand = [
         {"kernel": vname + "> (dV phi diag(v) phi^T)", "ms_per_launch": ms_v,
          "flops_per_launch": fl_v, "peak": MI355X_PEAK[prec_v],
-         "calls_per_step": n_fock / max(a.steps + a.warmup, 1),
+         "calls_per_step": n_vmat / max(a.steps + a.warmup, 1),
          "traffic": measured_traffic(vname, B)},
         {"kernel": "k_orb_density<double> (sum_i f_i (C^T phi)_i^2)", "ms_per_launch": ms_d,
          "flops_per_launch": fl_d, "peak": MI355X_PEAK["f64"], "orbitals": keep,

@@ -101,12 +101,18 @@ class Engine:
     The increment error is proportional to the increment, which shrinks as the
     SCF converges; an FP64 re-anchor is forced whenever the estimated increment
     error exceeds MIXED_SAFETY times the current DIIS residual per matrix
-    element, so the iteration converges to the FP64 fixed point.  Eigensolves,
-    DIIS, occupations, FFTs, xc and every energy sum stay FP64."""
+    element, so the iteration converges to the FP64 fixed point.  While the DIIS
+    residual is still large (>= MIXED_DIRECT_ERR, the fixed-point warm-up) rho
+    and V are taken in FP32 outright.  When only a few geometries (<= MIXED_SPLIT
+    of the batch) fail their bound, the batch takes the FP32 increment and just
+    those are recomputed in FP64.  Eigensolves, DIIS, occupations, FFTs, xc and
+    every energy sum stay FP64."""
 
     MIXED_EPS = 2e-6        # effective relative error of an FP32-accumulated increment
     MIXED_SAFETY = 1e-2     # potential increment: error per element <= 1e-2 residual / n_ao
     MIXED_RHO_SAFETY = 1.0  # density increment: |v| x relative rho error <= residual
+    MIXED_DIRECT_ERR = 1e-1  # residual above which rho and V are taken in FP32 outright
+    MIXED_SPLIT = 0.125      # largest failing share re-anchored alone (else all in FP64)
 
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
@@ -155,7 +161,8 @@ class Engine:
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
-        self.stats = {"full": 0, "low": 0, "rho_low": 0}
+        # kernel launches: FP64 / FP32 densities and potential matrices
+        self.stats = {"rho64": 0, "rho32": 0, "v64": 0, "v32": 0}
         self.sweeps = torch.zeros((), dtype=torch.long, device=cell.device)
         self.eigh_calls = 0
         self._work = {}
@@ -379,48 +386,107 @@ class Engine:
         c0 = 4 * math.pi / c.L * EXCHANGE_G0_CONST
         return Kmat + c0 * (Sg @ D @ Sg)
 
+    def _mixed_plan(self, direct, inc, na):
+        """Batch-level choice: 'direct' when every geometry is still in the large-residual
+        phase, otherwise FP32 increments for the batch with an FP64 re-anchor of the
+        geometries that fail their bound, when those are at most MIXED_SPLIT of the
+        batch; else FP64 for the whole batch.  Returns (mode, FP64 index or None)."""
+        if bool(direct.all().item()):
+            return "direct", None
+        fail = ~inc
+        nf = int(fail.sum().item())
+        if nf == 0:
+            return "inc", None
+        if nf <= max(1, int(self.MIXED_SPLIT * na)):
+            return "inc", torch.nonzero(fail).flatten()
+        return "fp64", None
+
+    def _mixed_density(self, st, na, D, mo, occ):
+        """rho for geometries [0, na) in the mixed mode: FP32 orbital form outright
+        while the DIIS residual is large (err >= MIXED_DIRECT_ERR), FP32 increments
+        rho_a + rho(D - D_a) where the increment bound holds, FP64 (and a new anchor)
+        elsewhere (see _mixed_plan)."""
+        c = self.c
+        sel = lambda key: st[key][:na]
+        err = sel("err")
+        direct = err >= self.MIXED_DIRECT_ERR
+        dD = D - sel("Da")
+        # bound on the relative density error of the FP32 increment
+        rel = self.MIXED_EPS * torch.linalg.norm(dD, dim=(-2, -1)) / sel("kappa")
+        inc = (rel <= 1e-2) & (rel * sel("vmax") <= self.MIXED_RHO_SAFETY * err)
+        mode, idx = self._mixed_plan(direct, inc, na)
+        if mode == "direct":
+            # orbital form sum_i f_i psi_i^2: non-negative in any precision, unlike the
+            # D form, whose FP32 rounding can turn a tiny density negative
+            self.stats["rho32"] += 1
+            return self.orb_density(sel("phi_lp"), mo, occ)
+        if mode == "fp64":
+            idx = torch.arange(na, device=c.device)
+            rho = r64 = self.orb_density(sel("phi"), mo, occ)
+        else:
+            rho = torch.zeros(na, c.ld, dtype=torch.float64, device=c.device)
+            K.ao_density(sel("phi_lp"), self._pad(dD), rho)
+            rho += sel("rhoa")
+            self.stats["rho32"] += 1
+            if idx is None:
+                return rho
+            r64 = self.orb_density(st["phi"][idx], mo[idx], occ[idx])
+            rho[idx] = r64
+        self.stats["rho64"] += 1
+        st["Da"][idx] = D[idx]
+        st["rhoa"][idx] = r64
+        st["kappa"][idx] = (r64[:, :c.N] / st["phisq"][idx]).amin(-1)
+        return rho
+
+    def _mixed_vmat(self, st, na, v):
+        """V(v) for geometries [0, na) in the mixed mode, planned like the density:
+        FP32 V(v) outright in the large-residual phase, V_a + V(v - v_a) in FP32 where
+        the increment bound holds, FP64 (and a new anchor) elsewhere."""
+        c = self.c
+        sel = lambda key: st[key][:na]
+        err = sel("err")
+        st["vmax"][:na] = v.abs().amax(-1)
+        direct = err >= self.MIXED_DIRECT_ERR
+        dv = v - sel("va")
+        tol_e = self.MIXED_SAFETY * err / c.nao                 # per-element error budget
+        inc = sel("va_set") & (self.MIXED_EPS * dv.abs().amax(-1) <= tol_e)
+        mode, idx = self._mixed_plan(direct, inc, na)
+        if mode == "direct":
+            self.stats["v32"] += 1
+            return self.vmat(sel("phi_lp"), v)
+        if mode == "fp64":
+            idx = torch.arange(na, device=c.device)
+            V2 = V64 = self.vmat(sel("phi"), v)
+        else:
+            V2 = sel("Va") + self.vmat(sel("phi_lp"), dv)
+            self.stats["v32"] += 1
+            if idx is None:
+                return V2
+            V64 = self.vmat(st["phi"][idx], v[idx].contiguous())
+            V2[idx] = V64
+        self.stats["v64"] += 1
+        st["va"][idx] = v[idx]
+        st["Va"][idx] = V64
+        st["va_set"][idx] = True
+        return V2
+
     def fock2e(self, st, na, D, mo, occ, full=False):
         """(J + V_xc or J - K/2, E_H + E_xc or E_H + E_x) for geometries [0, na)."""
         c = self.c
         sel = lambda key: st[key][:na]
         phi = sel("phi")
         mixed = self.precision == "mixed" and not full
-        low_rho = low_v = False
-        if mixed:
-            tol_e = self.MIXED_SAFETY * sel("err") / c.nao     # per-element error budget
-            dD = D - sel("Da")
-            # bound on the relative density error of the FP32 increment
-            rel = self.MIXED_EPS * torch.linalg.norm(dD, dim=(-2, -1)) / sel("kappa")
-            budget = self.MIXED_RHO_SAFETY * sel("err")
-            low_rho = bool(((rel <= 1e-2) & (rel * sel("vmax") <= budget)).all().item())
-        if low_rho:
-            self.stats["rho_low"] += 1
-            drho = torch.zeros(na, c.ld, dtype=torch.float64, device=c.device)
-            K.ao_density(sel("phi_lp"), self._pad(dD), drho)
-            rho = sel("rhoa") + drho
-        else:
+        if not mixed:
             rho = self.orb_density(phi, mo, occ)
-            if self.precision == "mixed":
-                st["Da"][:na] = D
-                st["rhoa"][:na] = rho
-                st["kappa"][:na] = (rho[:, :c.N] / sel("phisq")).amin(-1)
-        v, E, _ = self.xc_potential(rho)
-        Exc, EH = E[:, 0] * c.dV, 0.5 * E[:, 1] * c.dV
-        if mixed:
-            dv = v - sel("va")
-            ok = sel("va_set") & (self.MIXED_EPS * dv.abs().amax(-1) <= tol_e)
-            low_v = bool(ok.all().item())
-        if low_v:
-            V2 = sel("Va") + self.vmat(sel("phi_lp"), dv)
-            self.stats["low"] += 1
-        else:
+            self.stats["rho64"] += 1
+            v, E, _ = self.xc_potential(rho)
             V2 = self.vmat(phi, v)
-            self.stats["full"] += 1
-            if self.precision == "mixed":
-                st["va"][:na] = v
-                st["Va"][:na] = V2
-                st["vmax"][:na] = v.abs().amax(-1)
-                st["va_set"][:na] = True
+            self.stats["v64"] += 1
+        else:
+            rho = self._mixed_density(st, na, D, mo, occ)
+            v, E, _ = self.xc_potential(rho)
+            V2 = self._mixed_vmat(st, na, v)
+        Exc, EH = E[:, 0] * c.dV, 0.5 * E[:, 1] * c.dV
         if self.dft:
             return V2, EH + Exc
         Vx = -0.5 * self.exchange(st, na, mo, occ)

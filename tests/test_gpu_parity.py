@@ -123,7 +123,7 @@ def test_mixed_precision_matches_fp64():
     assert (E64 - Emx).abs().max().item() / 2 / n <= 1e-6
     assert (F64 - Fmx).abs().max().item() <= 1e-5
     assert (p64.solver.last["w1"] - pmx.solver.last["w1"]).abs().max().item() <= 1e-6
-    assert pmx.solver.engine.stats["low"] > 0
+    assert pmx.solver.engine.stats["v32"] > 0 and pmx.solver.engine.stats["rho32"] > 0
 
 
 KCASES = [
