@@ -39,11 +39,11 @@ _SIGS = {
     "hqc_vloc_G": [_P, _I, _I, _I, _D, _P, _P, _P],
     "hqc_real_to_complex": [_P, _I, _I, _I, _D, _P, _P],
     "hqc_complex_real": [_P, _I, _I, _I, _P, _P],
-    "hqc_poisson_mul": [_P, _P, _I, _I, _P, _P],
+    "hqc_poisson_mul": [_P, _P, _I, _I, _I, _P, _P],
     "hqc_xc_blocks": [_I],
     "hqc_xc_eval": [_P, _P, _I, _I, _I, _I, _P, _P, _P],
-    "hqc_grad_G": [_P, _I, _I, _D, _D, _P, _P],
-    "hqc_div_G": [_P, _I, _I, _D, _D, _P, _P],
+    "hqc_grad_G": [_P, _I, _I, _D, _P, _D, _P, _P],
+    "hqc_div_G": [_P, _I, _I, _D, _P, _D, _P, _P],
     "hqc_gga_blocks": [_I],
     "hqc_xc_gga_eval": [_P, _P, _P, _I, _I, _I, _P, _P, _P, _P],
     "hqc_pbe_points": [_P, _P, _I, _P, _P],
@@ -223,20 +223,31 @@ def pair_split(z, nao_pad, ld, M):
     _call(name, _ptr(z, None, "z"), mp, nao, nao_pad, g, ld, _ptr(M, None, "M"), _stream())
 
 
+def _lengths(L, B):
+    """(scalar L, per-geometry tensor or None) from a float or a (B,) tensor."""
+    if torch.is_tensor(L):
+        assert L.shape == (B,)
+        return 0.0, L
+    return float(L), None
+
+
 def grad_G(rhoG, g, L, scale, out):
+    """L: float, or (B,) FP64 tensor of per-geometry cell lengths."""
     B = rhoG.shape[0]
     assert rhoG.dtype == torch.complex128 and rhoG.shape == (B, g, g, g)
     assert out.dtype == torch.complex128 and out.shape == (B, 3, g, g, g)
-    _call("hqc_grad_G", _ptr(rhoG, None, "rhoG"), B, g, float(L), float(scale),
-          _ptr(out, None, "out"), _stream())
+    L0, Lb = _lengths(L, B)
+    _call("hqc_grad_G", _ptr(rhoG, None, "rhoG"), B, g, L0 if Lb is None else 1.0,
+          _ptr(Lb, F64, "Lb"), float(scale), _ptr(out, None, "out"), _stream())
 
 
 def div_G(wG, g, L, scale, out):
     B = wG.shape[0]
     assert wG.dtype == torch.complex128 and wG.shape == (B, 3, g, g, g)
     assert out.dtype == torch.complex128 and out.shape == (B, g, g, g)
-    _call("hqc_div_G", _ptr(wG, None, "wG"), B, g, float(L), float(scale),
-          _ptr(out, None, "out"), _stream())
+    L0, Lb = _lengths(L, B)
+    _call("hqc_div_G", _ptr(wG, None, "wG"), B, g, L0 if Lb is None else 1.0,
+          _ptr(Lb, F64, "Lb"), float(scale), _ptr(out, None, "out"), _stream())
 
 
 def gga_blocks(g):
@@ -280,8 +291,11 @@ def complex_real(c, ld, out):
 
 
 def poisson_mul(c, VG, saved=None):
+    """c *= VG in G space; VG (g^3,) shared or (B, g^3) per geometry."""
     B, g = c.shape[0], c.shape[1]
-    _call("hqc_poisson_mul", _ptr(c, None, "c"), _ptr(VG, F64, "VG"), B, g,
+    batched = VG.dim() == 2
+    assert VG.shape[-1] == g ** 3 and (not batched or VG.shape[0] == B)
+    _call("hqc_poisson_mul", _ptr(c, None, "c"), _ptr(VG, F64, "VG"), B, g, int(batched),
           _ptr(saved, None, "saved"), _stream())
 
 

@@ -93,12 +93,14 @@ __device__ __forceinline__ double gvec(int k, int g, double L) {
 }
 
 // out[b, d, k] = i G_d(k) rhoG[b, k] * scale
-__global__ __launch_bounds__(256) void k_grad_G(const double2* __restrict__ rg, int g, double L,
-                                                double scale, double2* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_grad_G(const double2* __restrict__ rg, int g, double L0,
+                                                const double* __restrict__ Lb, double scale,
+                                                double2* __restrict__ out) {
   const int N = g * g * g;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (k >= N) return;
+  const double L = Lb ? Lb[b] : L0;
   const int kz = k % g, ky = (k / g) % g, kx = k / (g * g);
   const double G[3] = {gvec(kx, g, L), gvec(ky, g, L), gvec(kz, g, L)};
   const double2 z = rg[(long)b * N + k];
@@ -108,12 +110,14 @@ __global__ __launch_bounds__(256) void k_grad_G(const double2* __restrict__ rg, 
 }
 
 // out[b, k] = scale * sum_d i G_d(k) wG[b, d, k]
-__global__ __launch_bounds__(256) void k_div_G(const double2* __restrict__ wg, int g, double L,
-                                               double scale, double2* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_div_G(const double2* __restrict__ wg, int g, double L0,
+                                               const double* __restrict__ Lb, double scale,
+                                               double2* __restrict__ out) {
   const int N = g * g * g;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (k >= N) return;
+  const double L = Lb ? Lb[b] : L0;
   const int kz = k % g, ky = (k / g) % g, kx = k / (g * g);
   const double G[3] = {gvec(kx, g, L), gvec(ky, g, L), gvec(kz, g, L)};
   double re = 0.0, im = 0.0;
@@ -184,24 +188,24 @@ __global__ void k_pbe_points(const double* __restrict__ rho, const double* __res
 
 }  // namespace
 
-extern "C" int hqc_grad_G(const double* rhoG, int B, int g, double L, double scale, double* out,
-                          void* stream) {
+extern "C" int hqc_grad_G(const double* rhoG, int B, int g, double L, const double* Lb,
+                          double scale, double* out, void* stream) {
   if (B < 0 || g <= 0 || !(L > 0.0)) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
   const int N = g * g * g;
   hipLaunchKernelGGL(k_grad_G, dim3(hqc_div_up(N, 256), B), dim3(256), 0, (hipStream_t)stream,
-                     (const double2*)rhoG, g, L, scale, (double2*)out);
+                     (const double2*)rhoG, g, L, Lb, scale, (double2*)out);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }
 
-extern "C" int hqc_div_G(const double* wG, int B, int g, double L, double scale, double* out,
-                         void* stream) {
+extern "C" int hqc_div_G(const double* wG, int B, int g, double L, const double* Lb,
+                         double scale, double* out, void* stream) {
   if (B < 0 || g <= 0 || !(L > 0.0)) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
   const int N = g * g * g;
   hipLaunchKernelGGL(k_div_G, dim3(hqc_div_up(N, 256), B), dim3(256), 0, (hipStream_t)stream,
-                     (const double2*)wG, g, L, scale, (double2*)out);
+                     (const double2*)wG, g, L, Lb, scale, (double2*)out);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }

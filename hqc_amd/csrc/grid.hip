@@ -64,14 +64,14 @@ __global__ void k_c2r(const double2* __restrict__ in, int N, int ld, double* __r
 }
 
 __global__ void k_poisson_mul(double2* __restrict__ c, const double* __restrict__ VG, int N,
-                              double2* __restrict__ saved) {
+                              long vg_stride, double2* __restrict__ saved) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (r >= N) return;
   const long i = (long)b * N + r;
   double2 v = c[i];
   if (saved) saved[i] = v;
-  const double w = VG[r];
+  const double w = VG[b * vg_stride + r];
   c[i] = make_double2(v.x * w, v.y * w);
 }
 
@@ -97,13 +97,14 @@ extern "C" int hqc_complex_real(const double* in, int B, int g, int ld, double* 
   return HQC_OK;
 }
 
-extern "C" int hqc_poisson_mul(double* c, const double* VG, int B, int g, double* saved,
-                               void* stream) {
+extern "C" int hqc_poisson_mul(double* c, const double* VG, int B, int g, int vg_batched,
+                               double* saved, void* stream) {
   if (B < 0 || g <= 0) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
   const int N = g * g * g;
   hipLaunchKernelGGL(k_poisson_mul, dim3(hqc_div_up(N, 256), B), dim3(256), 0,
-                     (hipStream_t)stream, (double2*)c, VG, N, (double2*)saved);
+                     (hipStream_t)stream, (double2*)c, VG, N, vg_batched ? (long)N : 0L,
+                     (double2*)saved);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }

@@ -1,7 +1,10 @@
-"""Periodic (PBC) Gamma-point GPW solver: make_solver, make_lcao, make_pes."""
+"""Periodic (PBC) GPW solver at Gamma or one k-point: make_solver, make_lcao,
+make_pes (hqc/pbc/solver.py, lcao.py, pes.py), plus make_pes_batch for lists
+of systems of different sizes."""
+from .batch import make_pes_batch
 from .lcao import make_lcao
 from .pes import make_pes
 from .potential import potential_energy_pp
 from .solver import make_solver
 
-__all__ = ["make_lcao", "make_pes", "make_solver", "potential_energy_pp"]
+__all__ = ["make_lcao", "make_pes", "make_pes_batch", "make_solver", "potential_energy_pp"]

@@ -1,0 +1,96 @@
+"""Batched PES over geometries of different atom counts and cell sizes.
+
+The reference evaluates one geometry per call (hqc/pbc/pes.py:78-86) and a
+sweep over systems of different sizes needs one factory, and one compile, per
+(n, L).  Here a list of geometries is split into buckets of equal atom count n
+and equal odd grid g (hqc/pbc/solver.py:159); each bucket runs as one batched
+SCF in which every geometry keeps its own cell length L (so its own dV, VG,
+lattice sums and image count), then the batched force adjoint.
+
+    pes = make_pes_batch("gth-dzv", grid_length=0.16, dft=False)
+    E, F, info = pes.energy_and_forces(xps, rs)   # lists, input order
+
+xps[i] (n_i, 3) in rs units, rs[i] Bohr, L[i] in rs units (default the
+uniform-gas cell (4 pi n_i / 3)^(1/3)); E[i] = E_elec + V_pp in Ry,
+F[i] (n_i, 3) in Ha/Bohr (forces of the free energy, as PES.energy_and_forces).
+"""
+import math
+
+import numpy as np
+import torch
+
+from .engine import RY, Cell, Engine
+from .forces import compute_forces
+
+
+def uniform_gas_length(n):
+    """Cell length in rs units holding n electrons at density 3/(4 pi rs^3)."""
+    return (4.0 / 3.0 * math.pi * n) ** (1.0 / 3.0)
+
+
+class BatchPES:
+    def __init__(self, basis, rcut=24, grid_length=0.12, max_batch=64, device="cuda",
+                 kappa=10, Gmax=15, **engine_kw):
+        self.basis, self.rcut, self.grid_length = basis, rcut, grid_length
+        self.max_batch, self.device = int(max_batch), device
+        self.kappa, self.Gmax = kappa, Gmax
+        self.engine_kw = engine_kw
+        self._engines = {}
+
+    def _engine(self, n, L, rs, g):
+        key = (n, g)
+        if key not in self._engines:
+            cell = Cell(n, L, rs, self.basis, rcut=self.rcut, grid_length=self.grid_length,
+                        device=self.device)
+            assert cell.g == g
+            self._engines[key] = Engine(cell, **self.engine_kw)
+        return self._engines[key]
+
+    def buckets(self, ns, Lb_bohr):
+        """{(n, g): [indices]} in input order."""
+        out = {}
+        for i, (n, Lb) in enumerate(zip(ns, Lb_bohr)):
+            g = round(Lb / self.grid_length / 2) * 2 + 1
+            out.setdefault((n, g), []).append(i)
+        return out
+
+    def energy_and_forces(self, xps, rs, L=None):
+        ng = len(xps)
+        rs = np.broadcast_to(np.asarray(rs, dtype=float), (ng,))
+        ns = [int(np.asarray(x).shape[0]) for x in xps]
+        Lrs = np.array([uniform_gas_length(n) for n in ns]) if L is None else \
+            np.broadcast_to(np.asarray(L, dtype=float), (ng,))
+        Lb = Lrs * rs
+        E, F = [None] * ng, [None] * ng
+        conv, nit = [None] * ng, [None] * ng
+        for (n, g), idx in self.buckets(ns, Lb).items():
+            for s0 in range(0, len(idx), self.max_batch):
+                part = idx[s0:s0 + self.max_batch]
+                eng = self._engine(n, Lrs[part[0]], rs[part[0]], g)
+                xpb = torch.tensor(np.stack([np.asarray(xps[i], dtype=float) * rs[i]
+                                             for i in part]), dtype=torch.float64,
+                                   device=eng.c.device)
+                Lt = torch.tensor(Lb[part], dtype=torch.float64, device=eng.c.device)
+                st = eng.solve(xpb, Lb=Lt)
+                Fb = compute_forces(eng, st, self.kappa, self.Gmax)
+                Eb = st["E"] * RY + st["Epp"]
+                for j, i in enumerate(part):
+                    E[i], F[i] = Eb[j], Fb[j]
+                    conv[i], nit[i] = bool(st["conv"][j].item()), int(st["nit"][j].item())
+        return E, F, {"converged": conv, "n_iter": nit,
+                      "buckets": {k: len(v) for k, v in self.buckets(ns, Lb).items()}}
+
+
+def make_pes_batch(basis, rcut=24, tol=1e-7, max_cycle=100, grid_length=0.12, diis=True,
+                   diis_space=8, diis_start_cycle=1, diis_damp=0.0, dft=False, xc="lda,vwn",
+                   smearing=False, smearing_method="fermi", smearing_sigma=0.0,
+                   search_method="bisect", search_cycle=100, search_tol=1e-7, Gmax=15,
+                   kappa=10, precision="fp64", max_batch=64):
+    """make_pes's options (hqc/pbc/pes.py:10-31) for a list of systems of any size."""
+    return BatchPES(basis, rcut=rcut, grid_length=grid_length, max_batch=max_batch,
+                    kappa=kappa, Gmax=Gmax, dft=dft, xc=xc, smearing=smearing,
+                    smearing_method=smearing_method, smearing_sigma=smearing_sigma,
+                    search_method=search_method, search_cycle=search_cycle,
+                    search_tol=search_tol, tol=tol, max_cycle=max_cycle, diis=diis,
+                    diis_space=diis_space, diis_start_cycle=diis_start_cycle,
+                    diis_damp=diis_damp, precision=precision)

@@ -38,57 +38,88 @@ def _tr(A, B):
     return (A * B.transpose(-1, -2)).sum((-1, -2))
 
 
-class Cell:
-    """Static data of one (n, L, rs, basis, grid) family, host and device copies."""
+class CellConsts:
+    """Cell-size dependent constants for one cell length L (Bohr) on a fixed grid."""
 
-    def __init__(self, n, L, rs, basis, rcut=24.0, grid_length=0.12, device="cuda"):
-        self.device = torch.device(device)
-        self.n, self.rs = n, rs
-        self.basis = HBasis(basis)
-        b = self.basis
-        self.L = L * rs                                    # Bohr (solver.py:154)
-        self.omega = self.L ** 3
-        self.g = round(self.L / grid_length / 2) * 2 + 1   # odd (solver.py:159)
-        self.N = self.g ** 3
-        self.ld = _round_up(self.N, 64)
-        self.dV = (self.L / self.g) ** 3
-        self.nloc, self.na = b.nloc, b.na
-        self.nao = n * b.nloc
-        self.nao_pad = _round_up(self.nao, 16)
-        self.nl = b.l_max + 1
-        self.tmax_img = int(24.0 // self.L)                # gto.py:48 (rcut fixed at 24 there)
-        tmax = rcut // self.L                              # solver.py:168
+    def __init__(self, L, g, N, ld, rcut, device):
+        f64 = torch.float64
+        self.L = L
+        self.omega = L ** 3
+        self.dV = (L / g) ** 3
+        self.tmax_img = int(24.0 // L)                   # gto.py:48 (rcut fixed at 24 there)
+        tmax = rcut // L                                 # solver.py:168
         nt = np.arange(-tmax, tmax + 1)
-        T = np.array(np.meshgrid(nt, nt, nt, indexing="ij")).reshape(3, -1).T * self.L
+        T = np.array(np.meshgrid(nt, nt, nt, indexing="ij")).reshape(3, -1).T * L
         self.lattice_np = T[(T ** 2).sum(1) <= rcut ** 2]
-        g = self.g
         k = np.arange(g)
         k = np.where(k <= g // 2, k, k - g).astype(float)
-        G1 = k * 2 * np.pi / self.L
+        G1 = k * 2 * np.pi / L
         G2 = G1[:, None, None] ** 2 + G1[None, :, None] ** 2 + G1[None, None, :] ** 2
         with np.errstate(divide="ignore"):
             VG = 4 * np.pi / self.omega / G2
         VG[0, 0, 0] = 0.0
         self.VG_np = VG
+        self.lattice = torch.tensor(self.lattice_np, dtype=f64, device=device).contiguous()
+        self.VG = torch.tensor(VG.ravel(), dtype=f64, device=device)
+        # exchange weights on the half G-set H (centred order, columns < (N-1)/2) for the
+        # [Re | Im] row blocks of the pair-density transforms: 2 VG on both halves
+        nh = (N - 1) // 2
+        cc = np.arange(nh)
+        h = g // 2
+        s1, s2, s3 = cc // (g * g) - h, (cc // g) % g - h, cc % g - h
+        kk = ((s1 % g) * g + s2 % g) * g + s3 % g
+        wh = np.zeros(ld)
+        wh[:nh] = 2 * VG.ravel()[kk]
+        wh[nh:2 * nh] = wh[:nh]
+        self.VGh = torch.tensor(wh, dtype=f64, device=device)
+        self.c0 = 4 * math.pi / L * EXCHANGE_G0_CONST   # eris0 factor (solver.py:431)
+
+
+class Cell:
+    """Static data of one (n, L, rs, basis, grid) family, host and device copies.
+
+    Geometries of other cell lengths that land on the same odd grid (same n, g)
+    can share the cell: consts(L) gives their L-dependent constants."""
+
+    def __init__(self, n, L, rs, basis, rcut=24.0, grid_length=0.12, device="cuda"):
+        self.device = torch.device(device)
+        self.n, self.rs = n, rs
+        self.rcut, self.grid_length = rcut, grid_length
+        self.basis = HBasis(basis)
+        b = self.basis
+        self.L = L * rs                                    # Bohr (solver.py:154)
+        self.g = self.grid_of(self.L)                      # odd (solver.py:159)
+        self.N = self.g ** 3
+        self.ld = _round_up(self.N, 64)
+        self.nloc, self.na = b.nloc, b.na
+        self.nao = n * b.nloc
+        self.nao_pad = _round_up(self.nao, 16)
+        self.nl = b.l_max + 1
+        self._consts = {}
+        cst = self.consts(self.L)
+        self.omega, self.dV, self.tmax_img = cst.omega, cst.dV, cst.tmax_img
+        self.lattice_np, self.lattice = cst.lattice_np, cst.lattice
+        self.VG_np, self.VG, self.VGh = cst.VG_np, cst.VG, cst.VGh
         dev = self.device
         f64 = torch.float64
         self.alpha = torch.tensor(b.alpha, dtype=f64, device=dev)
         self.coef = torch.tensor(b.coef, dtype=f64, device=dev).contiguous()
         self.comp = torch.tensor(b.comp, dtype=torch.int32, device=dev)
-        self.lattice = torch.tensor(self.lattice_np, dtype=f64, device=dev).contiguous()
-        self.VG = torch.tensor(VG.ravel(), dtype=f64, device=dev)
-        # exchange weights on the half G-set H (centred order, columns < (N-1)/2) for the
-        # [Re | Im] row blocks of the pair-density transforms: 2 VG on both halves
-        nh = (self.N - 1) // 2
-        cc = np.arange(nh)
-        h = g // 2
-        s1, s2, s3 = cc // (g * g) - h, (cc // g) % g - h, cc % g - h
-        kk = ((s1 % g) * g + s2 % g) * g + s3 % g
-        wh = np.zeros(self.ld)
-        wh[:nh] = 2 * VG.ravel()[kk]
-        wh[nh:2 * nh] = wh[:nh]
-        self.VGh = torch.tensor(wh, dtype=f64, device=dev)
         self.atom_of_ao = torch.arange(self.nao, device=dev) // self.nloc
+
+    def grid_of(self, L_bohr):
+        """Odd grid size of hqc/pbc/solver.py:159 (Python round, half to even)."""
+        return round(L_bohr / self.grid_length / 2) * 2 + 1
+
+    def consts(self, L_bohr):
+        key = float(L_bohr)
+        if key not in self._consts:
+            if self.grid_of(key) != getattr(self, "g", self.grid_of(key)):
+                raise ValueError(f"cell length {key} Bohr gives another grid than {self.g}^3")
+            self._consts[key] = CellConsts(key, self.grid_of(key), self.grid_of(key) ** 3,
+                                           _round_up(self.grid_of(key) ** 3, 64), self.rcut,
+                                           self.device)
+        return self._consts[key]
 
 
 class Engine:
@@ -185,9 +216,23 @@ class Engine:
         out[:, :c.nao, :c.nao] = M
         return out
 
-    def vmat(self, phi, v):
-        """dV * phi diag(v) phi^T for a batch -> (B, nao, nao)."""
-        return self.vmat_dv(phi, v, self.c.dV)
+    def vmat(self, phi, v, dV=None):
+        """dV * phi diag(v) phi^T for a batch -> (B, nao, nao); dV a float (None: the
+        cell's) or a (B,) tensor of per-geometry volume elements."""
+        if dV is None:
+            return self.vmat_dv(phi, v, self.c.dV)
+        if torch.is_tensor(dV):
+            return self.vmat_dv(phi, v * dV[:, None], 1.0)
+        return self.vmat_dv(phi, v, dV)
+
+    @staticmethod
+    def _dv(st, na):
+        """Volume element(s) of geometries [0, na): the cell's float, or (na,) per geometry."""
+        return st["dVb"][:na] if st.get("hetero") else st["dV"]
+
+    @staticmethod
+    def _geo(st, na):
+        return {k: st[k][:na] for k in ("Lb", "dVb", "VGb")} if st.get("hetero") else None
 
     def vmat_dv(self, phi, v, dV):
         c = self.c
@@ -218,19 +263,27 @@ class Engine:
         K.orb_density(phi, At, rho)
         return rho
 
-    def poisson(self, rho, save=False):
-        """v_H on the grid (complex buffer, real part = v_H) and optionally rho~(G)."""
+    def poisson(self, rho, save=False, geo=None):
+        """v_H on the grid (complex buffer, real part = v_H) and optionally
+        rho~(G) = dV FFT(rho).  geo: per-geometry (Lb, dVb, VGb) of a batch of
+        different cell sizes (None: the cell's constants)."""
         c = self.c
         B = rho.shape[0]
         cbuf = torch.empty(B, c.g, c.g, c.g, dtype=torch.complex128, device=c.device)
-        K.real_to_complex(rho, c.g, c.dV, cbuf)
+        if geo is None:
+            K.real_to_complex(rho, c.g, c.dV, cbuf)
+        else:
+            K.real_to_complex((rho * geo["dVb"][:, None]).contiguous(), c.g, 1.0, cbuf)
         K.fft3d(cbuf, -1)
         saved = torch.empty_like(cbuf) if save else None
-        K.poisson_mul(cbuf, c.VG, saved)
+        if geo is None:
+            K.poisson_mul(cbuf, c.VG, saved)
+        else:
+            K.poisson_mul(cbuf, geo["VGb"], saved)
         K.fft3d(cbuf, +1)
         return cbuf, saved
 
-    def xc_potential(self, rho, save_rg=False):
+    def xc_potential(self, rho, save_rg=False, geo=None):
         """(v = v_H + v_xc (B, ld), per-geometry sums (B, 2) of the xc energy density and
         rho v_H, rho~(G) = dV FFT(rho) if save_rg or GGA).
 
@@ -239,49 +292,113 @@ class Engine:
         derivative, the exact derivative of the discrete functional."""
         c = self.c
         B, g = rho.shape[0], c.g
-        vh, rg = self.poisson(rho, save=self.gga or save_rg)
+        vh, rg = self.poisson(rho, save=self.gga or save_rg, geo=geo)
         v = torch.zeros(B, c.ld, dtype=torch.float64, device=c.device)
         if not self.gga:
             part = torch.empty(B, K.xc_blocks(g), 2, dtype=torch.float64, device=c.device)
             K.xc_eval(rho, vh, g, self.xc_id, v, part)
             return v, part.sum(1), rg
         grad = torch.empty(B, 3, g, g, g, dtype=torch.complex128, device=c.device)
-        K.grad_G(rg, g, c.L, 1.0 / c.omega, grad)
+        if geo is None:
+            K.grad_G(rg, g, c.L, 1.0 / c.omega, grad)
+        else:                                   # rho~ / Omega_b = FFT(rho) / N
+            K.grad_G((rg / geo["dVb"][:, None, None, None]).contiguous(), g, geo["Lb"],
+                     1.0 / c.N, grad)
         K.fft3d(grad.view(3 * B, g, g, g), +1)
         w = torch.empty_like(grad)
         part = torch.empty(B, K.gga_blocks(g), 2, dtype=torch.float64, device=c.device)
         K.xc_gga_eval(rho, grad, vh, g, v, w, part)
         del grad
         K.fft3d(w.view(3 * B, g, g, g), -1)
-        K.div_G(w, g, c.L, 1.0 / c.N, vh)
+        K.div_G(w, g, c.L if geo is None else geo["Lb"], 1.0 / c.N, vh)
         K.fft3d(vh, +1)
         v[:, :c.N] -= vh.real.reshape(B, c.N)
         return v, part.sum(1), rg
 
     # ----------------------------------------------------------------- set-up
-    def setup(self, xpb, kpt=None):
+    def _cell_groups(self, B, Lb):
+        """[(CellConsts, index tensor or None for the whole batch)] per distinct cell length."""
+        c = self.c
+        if Lb is None:
+            return [(c.consts(c.L), None)]
+        vals = Lb.cpu().numpy()
+        groups = []
+        for L in np.unique(vals):
+            idx = torch.tensor(np.nonzero(vals == L)[0], device=c.device)
+            groups.append((c.consts(float(L)), None if len(idx) == B else idx))
+        return groups
+
+    @staticmethod
+    def _sub(t, idx):
+        return t if idx is None else t[idx].contiguous()
+
+    @staticmethod
+    def _put(dst, idx, val):
+        if idx is None:
+            if val is not dst:
+                dst.copy_(val)
+        else:
+            dst[idx] = val
+
+    def setup(self, xpb, kpt=None, Lb=None):
         """Per-geometry constants for xp (B, n, 3) in Bohr on the device.  Every
         tensor whose leading axis is the geometry is listed in st['per'] so the
-        SCF can keep the active geometries as a contiguous prefix."""
+        SCF can keep the active geometries as a contiguous prefix.
+
+        Lb (B,) Bohr gives each geometry its own cell length (all on this cell's
+        grid); the L-dependent set-up kernels then run once per distinct length."""
         c = self.c
         B = xpb.shape[0]
         dev, f64 = c.device, torch.float64
-        st = {"xpb": xpb.contiguous(), "B": B}
+        hetero = Lb is not None
+        if hetero:
+            Lb = torch.as_tensor(Lb, dtype=f64).to(dev).reshape(B).contiguous()
+        groups = self._cell_groups(B, Lb)
+        st = {"xpb": xpb.contiguous(), "B": B, "hetero": hetero, "dV": c.dV}
+        st["Lb"] = Lb if hetero else torch.full((B,), c.L, dtype=f64, device=dev)
+        st["dVb"] = (st["Lb"] / c.g) ** 3
+        per = ["xpb", "Lb", "dVb", "S", "T", "v", "phi", "vloc", "V", "Hcore"]
+        if hetero:
+            st["VGb"] = torch.empty(B, c.N, dtype=f64, device=dev)
+            for cst, idx in groups:
+                self._put(st["VGb"], idx, cst.VG.expand(B if idx is None else len(idx), c.N))
+            per.append("VGb")
         S = torch.empty(B, c.nao, c.nao, dtype=f64, device=dev)
         T = torch.empty_like(S)
-        K.overlap_kinetic(st["xpb"], c.lattice, c.alpha, c.coef, c.comp, S, T)
+        phi = torch.zeros(B, c.nao_pad, c.ld, dtype=f64, device=dev)
+        mixed = self.precision == "mixed"
+        phi32 = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float32, device=dev) if mixed else None
+        vl = torch.zeros(B, c.ld, dtype=f64, device=dev)
+        for cst, idx in groups:
+            x = self._sub(st["xpb"], idx)
+            nb = x.shape[0]
+            Sg = S if idx is None else torch.empty(nb, c.nao, c.nao, dtype=f64, device=dev)
+            Tg = T if idx is None else torch.empty_like(Sg)
+            K.overlap_kinetic(x, cst.lattice, c.alpha, c.coef, c.comp, Sg, Tg)
+            self._put(S, idx, Sg)
+            self._put(T, idx, Tg)
+            P = torch.empty(nb, c.n, 3, c.g, c.na, c.nl, dtype=f64, device=dev)
+            K.gto_tables(x, c.g, cst.L, cst.tmax_img, c.alpha, c.nl, P)
+            pg = phi if idx is None else torch.zeros(nb, c.nao_pad, c.ld, dtype=f64, device=dev)
+            K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, pg)
+            self._put(phi, idx, pg)
+            if mixed:
+                pg32 = phi32 if idx is None else torch.zeros(nb, c.nao_pad, c.ld,
+                                                              dtype=torch.float32, device=dev)
+                K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, pg32)
+                self._put(phi32, idx, pg32)
+            cb = torch.empty(nb, c.g, c.g, c.g, dtype=torch.complex128, device=dev)
+            K.vloc_G(x, c.g, cst.L, cst.VG, cb)
+            K.fft3d(cb, +1)
+            vg = vl if idx is None else torch.zeros(nb, c.ld, dtype=f64, device=dev)
+            K.complex_real(cb, c.ld, vg)
+            self._put(vl, idx, vg)
         w, u = torch.linalg.eigh(S)                                     # solver.py:729-730
         st["S"], st["T"] = S, T
         st["v"] = u * w.rsqrt().unsqueeze(-2)
-        P = torch.empty(B, c.n, 3, c.g, c.na, c.nl, dtype=f64, device=dev)
-        K.gto_tables(st["xpb"], c.g, c.L, c.tmax_img, c.alpha, c.nl, P)
-        phi = torch.zeros(B, c.nao_pad, c.ld, dtype=f64, device=dev)
-        K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, phi)
         st["phi"] = phi
-        per = ["xpb", "S", "T", "v", "phi", "vloc", "V", "Hcore"]
-        if self.precision == "mixed":
-            phi32 = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float32, device=dev)
-            K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, phi32)
+        dVs = self._dv(st, B)
+        if mixed:
             st["phi_lp"] = phi32
             st["phisq"] = (phi[:, :c.nao, :c.N] ** 2).sum(1)             # sum_p phi_p(r)^2
             st["Da"] = torch.zeros(B, c.nao, c.nao, dtype=f64, device=dev)
@@ -293,19 +410,20 @@ class Engine:
             st["va_set"] = torch.zeros(B, dtype=torch.bool, device=dev)
             st["err"] = torch.full((B,), float("inf"), dtype=f64, device=dev)
             per += ["phi_lp", "phisq", "Da", "rhoa", "kappa", "va", "Va", "vmax", "va_set", "err"]
-        cb = torch.empty(B, c.g, c.g, c.g, dtype=torch.complex128, device=dev)
-        K.vloc_G(st["xpb"], c.g, c.L, c.VG, cb)
-        K.fft3d(cb, +1)
-        vl = torch.zeros(B, c.ld, dtype=f64, device=dev)
-        K.complex_real(cb, c.ld, vl)
         st["vloc"] = vl
-        st["V"] = self.vmat(phi, vl)
+        st["V"] = self.vmat(phi, vl, dVs)
         st["Hcore"] = T + st["V"]
         if not self.dft:
             ones = torch.zeros(B, c.ld, dtype=f64, device=dev)
             ones[:, :c.N] = 1.0
-            st["Sgrid"] = self.vmat(phi, ones)           # dV phi phi^T (exchange G=0 term)
-            per.append("Sgrid")
+            st["Sgrid"] = self.vmat(phi, ones, dVs)      # dV phi phi^T (exchange G=0 term)
+            st["VGh"] = torch.empty(B, c.ld, dtype=f64, device=dev)
+            st["c0b"] = torch.empty(B, dtype=f64, device=dev)
+            for cst, idx in groups:
+                nb = B if idx is None else len(idx)
+                self._put(st["VGh"], idx, cst.VGh.expand(nb, c.ld))
+                self._put(st["c0b"], idx, torch.full((nb,), cst.c0, dtype=f64, device=dev))
+            per += ["Sgrid", "VGh", "c0b"]
         st["per"] = per
         return st
 
@@ -361,13 +479,15 @@ class Engine:
                 rows.append((b, idx[k], idx[k + 1] if k + 1 < len(idx) else -1))
         Kmat = torch.zeros(na, c.nao, c.nao, dtype=torch.float64, device=dev)
         if rows:
-            wgt = occ[:, :nmax].clamp_min(0).sqrt() * c.dV * keep[:na, :nmax].to(dev)
+            dVs = self._dv(st, na)
+            dVs = dVs[:, None] if torch.is_tensor(dVs) else dVs
+            wgt = occ[:, :nmax].clamp_min(0).sqrt() * dVs * keep[:na, :nmax].to(dev)
             psi = (mo[:, :, :nmax] * wgt.unsqueeze(-2)).transpose(-1, -2).contiguous() \
                 @ st["phi"][:na, :c.nao, :]                                  # (na, nmax, ld)
             table = torch.tensor(rows, dtype=torch.int32)
             per_pair = c.nao * c.N * 16 + 2 * c.nao_pad * (c.ld + c.nao_pad + 1) * 8
             chunk = max(1, min(len(rows), int(self.EXCHANGE_CHUNK_BYTES // per_pair)))
-            v = c.VGh.expand(2 * chunk, c.ld).contiguous()
+            VGh = st["VGh"]
             phi = st["phi"][:na]
             for s0 in range(0, len(rows), chunk):
                 t = table[s0:s0 + chunk]
@@ -378,13 +498,12 @@ class Engine:
                 M = torch.empty(2 * mp, c.nao_pad, c.ld, dtype=torch.float64, device=dev)
                 K.pair_split(z, c.nao_pad, c.ld, M)
                 del z
-                part = self.vmat_dv(M, v[:2 * mp], 1.0)
                 gid = t[:, 0].long().repeat_interleave(2).to(dev)
+                part = self.vmat_dv(M, VGh[gid], 1.0)
                 Kmat.index_add_(0, gid, part)
         D = (mo * occ.unsqueeze(-2)) @ mo.transpose(-1, -2).contiguous()
         Sg = st["Sgrid"][:na]
-        c0 = 4 * math.pi / c.L * EXCHANGE_G0_CONST
-        return Kmat + c0 * (Sg @ D @ Sg)
+        return Kmat + st["c0b"][:na, None, None] * (Sg @ D @ Sg)
 
     def _mixed_plan(self, direct, inc, na):
         """Batch-level choice: 'direct' when every geometry is still in the large-residual
@@ -451,18 +570,20 @@ class Engine:
         tol_e = self.MIXED_SAFETY * err / c.nao                 # per-element error budget
         inc = sel("va_set") & (self.MIXED_EPS * dv.abs().amax(-1) <= tol_e)
         mode, idx = self._mixed_plan(direct, inc, na)
+        dVs = self._dv(st, na)
         if mode == "direct":
             self.stats["v32"] += 1
-            return self.vmat(sel("phi_lp"), v)
+            return self.vmat(sel("phi_lp"), v, dVs)
         if mode == "fp64":
             idx = torch.arange(na, device=c.device)
-            V2 = V64 = self.vmat(sel("phi"), v)
+            V2 = V64 = self.vmat(sel("phi"), v, dVs)
         else:
-            V2 = sel("Va") + self.vmat(sel("phi_lp"), dv)
+            V2 = sel("Va") + self.vmat(sel("phi_lp"), dv, dVs)
             self.stats["v32"] += 1
             if idx is None:
                 return V2
-            V64 = self.vmat(st["phi"][idx], v[idx].contiguous())
+            V64 = self.vmat(st["phi"][idx], v[idx].contiguous(),
+                            dVs[idx] if torch.is_tensor(dVs) else dVs)
             V2[idx] = V64
         self.stats["v64"] += 1
         st["va"][idx] = v[idx]
@@ -476,17 +597,19 @@ class Engine:
         sel = lambda key: st[key][:na]
         phi = sel("phi")
         mixed = self.precision == "mixed" and not full
+        dVs = self._dv(st, na)
+        geo = self._geo(st, na)
         if not mixed:
             rho = self.orb_density(phi, mo, occ)
             self.stats["rho64"] += 1
-            v, E, _ = self.xc_potential(rho)
-            V2 = self.vmat(phi, v)
+            v, E, _ = self.xc_potential(rho, geo=geo)
+            V2 = self.vmat(phi, v, dVs)
             self.stats["v64"] += 1
         else:
             rho = self._mixed_density(st, na, D, mo, occ)
-            v, E, _ = self.xc_potential(rho)
+            v, E, _ = self.xc_potential(rho, geo=geo)
             V2 = self._mixed_vmat(st, na, v)
-        Exc, EH = E[:, 0] * c.dV, 0.5 * E[:, 1] * c.dV
+        Exc, EH = E[:, 0] * dVs, 0.5 * E[:, 1] * dVs
         if self.dft:
             return V2, EH + Exc
         Vx = -0.5 * self.exchange(st, na, mo, occ)
@@ -656,13 +779,13 @@ class Engine:
         return loc["Mo"], loc["W1"], loc["E"], loc["nit"] != self.max_cycle, loc["nit"], order
 
     # ---------------------------------------------------------------- results
-    def solve(self, xpb, kpt=None):
+    def solve(self, xpb, kpt=None, Lb=None):
         """Full solver output for a batch (B, n, 3) Bohr.
 
         Per-geometry device state in st stays in slot order (st['order'][i] is
         the geometry held by slot i); the user-facing results mo, D, w1, E, conv,
         nit, Ki, Vep, Vee, Se, occ are returned in input order."""
-        st = self.setup(xpb, kpt)
+        st = self.setup(xpb, kpt, Lb)
         mo, w1, E, conv, nit, order = self.scf(st)
         B = st["B"]
         D, occ = self.density_matrix(mo, w1)                              # solver.py:758-765

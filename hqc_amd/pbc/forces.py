@@ -22,16 +22,18 @@ from .potential import ewald_pp
 
 
 def _exchange_adjoint(engine, st):
-    """Zx = -Y - c0 (D S~ D) phi for HF, (B, nao_pad, ld)."""
+    """Zx = -Y - c0 (D S~ D) phi for HF, (B, nao_pad, ld), each geometry with its own
+    cell constants (VG, dV, c0)."""
     c = engine.c
     B = st["B"]
     phi = st["phi"][:, :c.nao, :c.N]
     mo, occ = st["slot"]["mo"], st["slot"]["occ"]
     Zx = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float64, device=c.device)
     g = c.g
-    VG = c.VG.reshape(1, g, g, g)
-    c0 = 4 * math.pi / c.L * EXCHANGE_G0_CONST
+    Lh = st["Lb"].cpu().tolist()
     for b in range(B):
+        cst = c.consts(Lh[b])
+        VG = cst.VG.reshape(1, g, g, g)
         keep = torch.nonzero(occ[b] > 1e-14).flatten()
         C = mo[b][:, keep]
         f = occ[b][keep]
@@ -39,7 +41,7 @@ def _exchange_adjoint(engine, st):
         Y = torch.zeros(c.nao, c.N, dtype=torch.float64, device=c.device)
         for ii in range(len(f)):
             pair = (psi * psi[ii]).to(torch.complex128).reshape(-1, g, g, g).contiguous()
-            pair *= c.dV
+            pair *= cst.dV
             K.fft3d(pair, -1)
             pair *= VG
             K.fft3d(pair, +1)
@@ -48,13 +50,14 @@ def _exchange_adjoint(engine, st):
             Y += f[ii] * torch.outer(C[:, ii], Gi)
         D = st["slot"]["D"][b]
         M = D @ st["Sgrid"][b] @ D
-        Zx[b, :c.nao, :c.N] = -Y - c0 * (M @ phi[b])
+        Zx[b, :c.nao, :c.N] = -Y - cst.c0 * (M @ phi[b])
     return Zx
 
 
 def compute_forces(engine, st, kappa=10, Gmax=15):
     """-dA/dR (B, n, 3) in Ha/Bohr after engine.solve, in input order.  Works on
-    the slot-ordered device state of the SCF and un-permutes at the end."""
+    the slot-ordered device state of the SCF and un-permutes at the end.  The
+    cell-length dependent kernels run once per distinct cell length."""
     if st.get("kpt") is not None:
         return compute_forces_k(engine, st, kappa, Gmax)
     c = engine.c
@@ -62,11 +65,20 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
     f64 = torch.float64
     sl = st["slot"]
     D, occ, w1, mo = sl["D"], sl["occ"], sl["w1"], sl["mo"]
-    S = torch.empty(B, c.nao, c.nao, dtype=f64, device=dev)
-    T = torch.empty_like(S)
+    groups = engine._cell_groups(B, st["Lb"] if st.get("hetero") else None)
+    sub, put = engine._sub, engine._put
     dS = torch.empty(B, c.nao, c.nao, 3, dtype=f64, device=dev)
     dT = torch.empty_like(dS)
-    K.overlap_kinetic(st["xpb"], c.lattice, c.alpha, c.coef, c.comp, S, T, dS, dT)
+    for cst, idx in groups:
+        x = sub(st["xpb"], idx)
+        nb = x.shape[0]
+        S = torch.empty(nb, c.nao, c.nao, dtype=f64, device=dev)
+        T = torch.empty_like(S)
+        dSg = dS if idx is None else torch.empty(nb, c.nao, c.nao, 3, dtype=f64, device=dev)
+        dTg = dT if idx is None else torch.empty_like(dSg)
+        K.overlap_kinetic(x, cst.lattice, c.alpha, c.coef, c.comp, S, T, dSg, dTg)
+        put(dS, idx, dSg)
+        put(dT, idx, dTg)
     W = (mo * (occ * w1).unsqueeze(-2)) @ mo.transpose(-1, -2)
     atom = c.atom_of_ao
     other = (atom[:, None] != atom[None, :]).to(f64)
@@ -77,20 +89,32 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
     X = torch.zeros(B, c.nao_pad, c.ld, dtype=f64, device=dev)
     rho = torch.zeros(B, c.ld, dtype=f64, device=dev)
     K.ao_density(st["phi"], engine._pad(D), rho, X)
-    v, _, rg = engine.xc_potential(rho, save_rg=True)
+    v, _, rg = engine.xc_potential(rho, save_rg=True, geo=engine._geo(st, B))
     veff = (v + st["vloc"]).contiguous()
-    P = torch.empty(B, c.n, 3, c.g, c.na, 3, dtype=f64, device=dev)
-    K.gto_tables(st["xpb"], c.g, c.L, c.tmax_img, c.alpha, 3, P)
     Zx = None if engine.dft else _exchange_adjoint(engine, st)
-    fp = torch.empty(B, c.n, K.force_blocks(c.g), 3, dtype=f64, device=dev)
-    K.grid_forces(P, c.alpha, c.coef, c.comp, c.nao_pad, c.ld, X, veff, Zx, c.dV, fp)
-    F += fp.sum(2)
-    dE = torch.empty(B, c.n, 3, dtype=f64, device=dev)
-    K.vloc_force(st["xpb"], c.g, c.L, c.VG, rg, dE)
-    F -= dE
-    Epp, Fpp = ewald_pp(st["xpb"] / c.rs, c.L / c.rs, c.rs, kappa, Gmax, forces=True)
+    Epp = torch.empty(B, dtype=f64, device=dev)
+    nblk = K.force_blocks(c.g)
+    for cst, idx in groups:
+        x = sub(st["xpb"], idx)
+        nb = x.shape[0]
+        P = torch.empty(nb, c.n, 3, c.g, c.na, 3, dtype=f64, device=dev)
+        K.gto_tables(x, c.g, cst.L, cst.tmax_img, c.alpha, 3, P)
+        fp = torch.empty(nb, c.n, nblk, 3, dtype=f64, device=dev)
+        K.grid_forces(P, c.alpha, c.coef, c.comp, c.nao_pad, c.ld, sub(X, idx), sub(veff, idx),
+                      None if Zx is None else sub(Zx, idx), cst.dV, fp)
+        dE = torch.empty(nb, c.n, 3, dtype=f64, device=dev)
+        K.vloc_force(x, c.g, cst.L, cst.VG, sub(rg, idx), dE)
+        # Ewald in Bohr units (rs = 1): the same sums as at (xp/rs, L/rs, rs)
+        Eg, Fpp = ewald_pp(x, cst.L, 1.0, kappa, Gmax, forces=True)
+        Fg = fp.sum(2) - dE + Fpp
+        if idx is None:
+            F += Fg
+            Epp.copy_(Eg)
+        else:
+            F[idx] += Fg
+            Epp[idx] = Eg
     st["Epp"] = st["unperm"](Epp)
-    return st["unperm"](F + Fpp)
+    return st["unperm"](F)
 
 
 def compute_forces_k(engine, st, kappa=10, Gmax=15):

@@ -83,13 +83,18 @@ class KEngine(Engine):
         return M
 
     # ------------------------------------------------------------------ set-up
-    def setup(self, xpb, kpt=None):
+    def setup(self, xpb, kpt=None, Lb=None):
+        if Lb is not None:
+            raise NotImplementedError("k-point batches share one cell length")
         if kpt is None:
             raise ValueError("the k-point engine needs kpt (B, 3) in 1/Bohr")
         c = self.c
         B = xpb.shape[0]
         dev, f64, c128 = c.device, torch.float64, torch.complex128
-        st = {"xpb": xpb.contiguous(), "kpt": kpt.contiguous(), "B": B}
+        st = {"xpb": xpb.contiguous(), "kpt": kpt.contiguous(), "B": B, "hetero": False,
+              "dV": c.dV}
+        st["Lb"] = torch.full((B,), c.L, dtype=f64, device=dev)
+        st["dVb"] = torch.full((B,), c.dV, dtype=f64, device=dev)
         S = torch.empty(B, c.nao, c.nao, dtype=c128, device=dev)
         T = torch.empty_like(S)
         K.overlap_kinetic_k(st["xpb"], st["kpt"], c.lattice, c.alpha, c.coef, c.comp, S, T)
@@ -101,7 +106,7 @@ class KEngine(Engine):
         phi = torch.zeros(B, 2 * c.nao_pad, c.ld, dtype=f64, device=dev)
         K.ao_values_k(P, c.coef, c.comp, c.nao_pad, c.ld, phi)
         st["phi"] = phi
-        per = ["xpb", "kpt", "S", "T", "v", "phi", "vloc", "V", "Hcore"]
+        per = ["xpb", "kpt", "Lb", "dVb", "S", "T", "v", "phi", "vloc", "V", "Hcore"]
         cb = torch.empty(B, c.g, c.g, c.g, dtype=c128, device=dev)
         K.vloc_G(st["xpb"], c.g, c.L, c.VG, cb)
         K.fft3d(cb, +1)

@@ -149,9 +149,11 @@ int hqc_real_to_complex(const double* in, int B, int g, int ld, double s, double
                         void* stream);
 /* Complex (B,g^3) -> real part into (B,ld). */
 int hqc_complex_real(const double* in, int B, int g, int ld, double* out, void* stream);
-/* Poisson kernel in G space: c[b,G] *= VG[G]; if saved != NULL the incoming
- * c is copied there first (rho~(G), kept for forces). */
-int hqc_poisson_mul(double* c, const double* VG, int B, int g, double* saved, void* stream);
+/* Poisson kernel in G space: c[b,G] *= VG[G] (VG (g^3) shared, or (B,g^3) one
+ * row per geometry when vg_batched != 0, for batches of different cell sizes);
+ * if saved != NULL the incoming c is copied there first (kept for forces). */
+int hqc_poisson_mul(double* c, const double* VG, int B, int g, int vg_batched, double* saved,
+                    void* stream);
 
 /* Pointwise exchange-correlation and energy densities.
  * Replaces exc_functional / vxc_functional (hqc/pbc/solver.py:669-678,
@@ -174,11 +176,12 @@ int hqc_xc_eval(const double* rho, const double* vh, int B, int g, int ld, int x
  *     2 de/dsigma grad rho, part (B, hqc_gga_blocks, 2) sums of e and rho v_H;
  *   hqc_div_G: out (B,g^3) = scale * sum_d i G_d wG[b,d] (forward FFT of w in,
  *     inverse FFT of out with scale 1/g^3 -> div w); v_xc = de/drho - div w;
- *   hqc_pbe_points: out (n,3) = (e, de/drho, de/dsigma) at given points. */
-int hqc_grad_G(const double* rhoG, int B, int g, double L, double scale, double* out,
-               void* stream);
-int hqc_div_G(const double* wG, int B, int g, double L, double scale, double* out,
-              void* stream);
+ *   hqc_pbe_points: out (n,3) = (e, de/drho, de/dsigma) at given points.
+ * Lb (B) gives each geometry its own cell length (NULL: L for all). */
+int hqc_grad_G(const double* rhoG, int B, int g, double L, const double* Lb, double scale,
+               double* out, void* stream);
+int hqc_div_G(const double* wG, int B, int g, double L, const double* Lb, double scale,
+              double* out, void* stream);
 int hqc_gga_blocks(int g);
 int hqc_xc_gga_eval(const double* rho, const double* grad, const double* vh, int B, int g,
                     int ld, double* vout, double* w, double* part, void* stream);

@@ -177,3 +177,31 @@ def test_kpoint_forces_vs_oracle(xc):
     pes = make_pes(n, L, rs, "gth-dzv", grid_length=0.3, gamma=False, **kw)
     E, F = pes.energy_and_forces(xp, kpt)
     assert np.allclose(F.cpu().numpy(), Fref, atol=1e-8), np.abs(F.cpu().numpy() - Fref).max()
+
+
+def test_batch_of_different_cells():
+    """make_pes_batch over systems of different atom counts and cell lengths (some
+    sharing a grid, so one batch holds several cell lengths) against the oracle
+    geometry by geometry: E + V_pp and forces."""
+    from hqc_amd.pbc import make_pes_batch
+    rng = np.random.default_rng(12)
+    specs = [(3, 1.30), (3, 1.35), (2, 1.30), (3, 1.50), (3, 1.32)]
+    xps, rss = [], []
+    for n, rs in specs:
+        L = (4 / 3 * np.pi * n) ** (1 / 3)
+        xps.append(rng.uniform(0, L, (n, 3)))
+        rss.append(rs)
+    for kw in (dict(dft=True, smearing=True, smearing_sigma=0.1),
+               dict(dft=False, smearing=True, smearing_sigma=0.1)):
+        print(f"[batch cells] {kw}", flush=True)
+        kw = dict(kw, tol=1e-11, search_tol=1e-13, search_cycle=200, max_cycle=300)
+        pes = make_pes_batch("gth-dzv", grid_length=0.3, **kw)
+        E, F, info = pes.energy_and_forces(xps, rss)
+        assert max(info["buckets"].values()) >= 3              # several lengths in one batch
+        for i, ((n, rs), xp) in enumerate(zip(specs, xps)):
+            L = (4 / 3 * np.pi * n) ** (1 / 3)
+            s = R.Solver(n, L, rs, "gth-dzv", grid_length=0.3, **kw)
+            ref = s.solve(xp)
+            Eref = ref[3] + R.ewald_pp(xp, L, rs)
+            assert abs(E[i].item() - Eref) < 1e-8, (i, E[i].item(), Eref)
+            assert np.allclose(F[i].cpu().numpy(), R.forces(s, xp), atol=1e-8)

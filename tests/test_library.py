@@ -54,3 +54,14 @@ def test_no_cpu_fallback_without_gpu():
     x = torch.zeros(1, 5, 5, 5, dtype=torch.complex128)
     with pytest.raises(ValueError):
         _lib.fft3d(x, -1)
+
+
+def test_batch_buckets_by_atom_count_and_grid():
+    """Systems are grouped by (n, odd grid); lengths on the same grid share a batch."""
+    from hqc_amd.pbc.batch import BatchPES, uniform_gas_length
+    pes = BatchPES("gth-dzv", grid_length=0.3)
+    ns = [3, 3, 2, 3, 3]
+    rs = [1.30, 1.35, 1.30, 1.50, 1.32]
+    Lb = [uniform_gas_length(n) * r for n, r in zip(ns, rs)]
+    b = pes.buckets(ns, Lb)
+    assert b == {(3, 11): [0, 1, 4], (2, 9): [2], (3, 13): [3]}
