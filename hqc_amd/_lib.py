@@ -58,6 +58,7 @@ _SIGS = {
     "hqc_overlap_kinetic_k": [_P, _P, _I, _I, _P, _I, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P],
     "hqc_grid_forces_k": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _D, _P,
                           _P],
+    "hqc_ao_points": [_P, _P, _P, _I, _I, _I, _D, _I, _P, _I, _I, _P, _P, _I, _P, _P],
     "hqc_grid_forces": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _D, _P, _P],
     "hqc_vloc_force": [_P, _I, _I, _I, _D, _P, _P, _P, _P],
 }
@@ -146,6 +147,20 @@ def ao_values_k(P, coef, comp, nao_pad, ld, out):
     assert P.dtype == torch.complex128 and out.shape == (B, 2 * nao_pad, ld)
     _call("hqc_ao_values_k", _ptr(P, None, "P"), B, n, g, na, nl, _ptr(coef, F64),
           _ptr(comp, I32), nloc, nao_pad, ld, _ptr(out, F64, "phi"), _stream())
+
+
+def ao_points(xp, xe, kpt, L, tmax, alpha, nl, coef, comp, out):
+    """AO values at points: xp (B,n,3), xe (B,ne,3) Bohr, kpt (B,3) or None;
+    out (B, ne, n*nloc) FP64 (complex128 with kpt)."""
+    B, n, _ = xp.shape
+    ne = xe.shape[1]
+    nloc = coef.shape[1]
+    assert xe.shape == (B, ne, 3) and out.shape == (B, ne, n * nloc)
+    assert out.dtype == (F64 if kpt is None else torch.complex128)
+    assert kpt is None or kpt.shape == (B, 3)
+    _call("hqc_ao_points", _ptr(xp, F64, "xp"), _ptr(xe, F64, "xe"), _ptr(kpt, F64, "kpt"), B,
+          n, ne, float(L), int(tmax), _ptr(alpha, F64), alpha.numel(), int(nl), _ptr(coef, F64),
+          _ptr(comp, I32), nloc, _ptr(out, None, "out"), _stream())
 
 
 def ao_density(phi, D, rho, X=None):

@@ -238,6 +238,93 @@ extern "C" int hqc_ao_values_k(const double* P, int B, int n, int g, int na, int
 }
 
 // ---------------------------------------------------------------------------
+// AO values at arbitrary points (electron positions), make_pbc_gto's
+// eval_pbc_ao / eval_pbc_ao_kpt (hqc/pbc/gto.py:341-369, 388-420): for r =
+// x_e - R_A the 1-D image sums over t = -tmax..tmax, times the Bloch phase
+// e^{i k_d t L} at a k-point.  One thread per (b, e, A); out (B, ne, nao)
+// real, or complex interleaved when kpt != NULL.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ao_points(
+    const double* __restrict__ xp, const double* __restrict__ xe, const double* __restrict__ kpt,
+    long total, int n, int ne, double L, int tmax, const double* __restrict__ alpha, int na,
+    int nl, const double* __restrict__ coef, const int* __restrict__ comp, int nloc,
+    double* __restrict__ out) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int A = (int)(idx % n);
+  const long be = idx / n;             // b * ne + e
+  const long b = be / ne;
+  const double* x = xe + be * 3;
+  const double* R = xp + (b * n + A) * 3;
+  const bool cplx = kpt != nullptr;
+  // 1-D tables: P[d][a][l] (re, im)
+  double Pr[3][HQC_MAX_NA][2], Pi[3][HQC_MAX_NA][2];
+  for (int d = 0; d < 3; ++d) {
+    const double r = x[d] - R[d];
+    const double kd = cplx ? kpt[b * 3 + d] : 0.0;
+    for (int a = 0; a < na; ++a) {
+      double sr0 = 0.0, sr1 = 0.0, si0 = 0.0, si1 = 0.0;
+      for (int t = -tmax; t <= tmax; ++t) {
+        const double xt = r - (double)t * L;
+        const double e = exp(-alpha[a] * xt * xt);
+        double c = 1.0, s = 0.0;
+        if (cplx) sincos(kd * (double)t * L, &s, &c);
+        sr0 += e * c;
+        si0 += e * s;
+        sr1 += e * xt * c;
+        si1 += e * xt * s;
+      }
+      Pr[d][a][0] = sr0;
+      Pr[d][a][1] = sr1;
+      Pi[d][a][0] = si0;
+      Pi[d][a][1] = si1;
+    }
+  }
+  const long nao = (long)n * nloc;
+  for (int o = 0; o < nloc; ++o) {
+    const int c = comp[o];
+    double ar = 0.0, ai = 0.0;
+    for (int a = 0; a < na; ++a) {
+      const double w = coef[a * nloc + o];
+      if (w == 0.0) continue;
+      double vr = 1.0, vi = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const int pw = (c == d) ? 1 : 0;
+        const double qr = Pr[d][a][pw], qi = Pi[d][a][pw];
+        const double tr = vr * qr - vi * qi;
+        vi = vr * qi + vi * qr;
+        vr = tr;
+      }
+      ar += w * vr;
+      ai += w * vi;
+    }
+    const long oi = be * nao + (long)A * nloc + o;
+    if (cplx) {
+      out[2 * oi] = ar;
+      out[2 * oi + 1] = ai;
+    } else {
+      out[oi] = ar;
+    }
+  }
+}
+
+extern "C" int hqc_ao_points(const double* xp, const double* xe, const double* kpt, int B, int n,
+                             int ne, double L, int tmax, const double* alpha, int na, int nl,
+                             const double* coef, const int* comp, int nloc, double* out,
+                             void* stream) {
+  if (B < 0 || n <= 0 || ne < 0 || na <= 0 || nloc <= 0 || tmax < 0 || !(L > 0.0))
+    return HQC_E_ARG;
+  if (na > HQC_MAX_NA || nloc > HQC_MAX_NLOC || nl > 2) return HQC_E_SIZE;
+  if (B == 0 || ne == 0) return HQC_OK;
+  const long total = (long)B * ne * n;
+  hipLaunchKernelGGL(k_ao_points, dim3(hqc_div_up(total, 256)), dim3(256), 0,
+                     (hipStream_t)stream, xp, xe, kpt, total, n, ne, L, tmax, alpha, na, nl, coef,
+                     comp, nloc, out);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Grid adjoint for forces.  For atom A of geometry b, each thread takes grid
 // points r (grid-stride), recomputes grad phi_p(r) for the AOs of A from the
 // tables (d/dx [x^l e^{-a x^2}] = l x^{l-1} e - 2a x^{l+1} e, image-summed),

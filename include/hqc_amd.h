@@ -77,6 +77,17 @@ int hqc_gto_tables_k(const double* xp, const double* kpt, int B, int n, int g, d
 int hqc_ao_values_k(const double* P, int B, int n, int g, int na, int nl, const double* coef,
                     const int* comp, int nloc, int nao_pad, int ld, double* phi, void* stream);
 
+/* AO values at arbitrary points, make_pbc_gto's eval_pbc_ao / eval_pbc_ao_kpt
+ * (hqc/pbc/gto.py:341-369, 388-420) as used by make_slater
+ * (hqc/pbc/slater.py:59-99): for each geometry b, point e and atom A, the 1-D
+ * image sums of x_e - R_A over t = -tmax..tmax (tmax = rcut // L, times
+ * e^{i k_d t L} when kpt != NULL).  xp (B,n,3), xe (B,ne,3) Bohr, kpt (B,3) or
+ * NULL; out (B,ne,n*nloc) FP64, complex128 interleaved with kpt.  l_max <= 1
+ * (nl <= 2). */
+int hqc_ao_points(const double* xp, const double* xe, const double* kpt, int B, int n, int ne,
+                  double L, int tmax, const double* alpha, int na, int nl, const double* coef,
+                  const int* comp, int nloc, double* out, void* stream);
+
 /* Electron density on the grid and (optionally) X = D phi.
  * Replaces the 'pr,qr,pq->r' contraction of Exc_Vxc_integral
  * (hqc/pbc/solver.py:690) and the density used by hartree_rhoG (:627).

@@ -386,6 +386,42 @@ def ao_gradients(sysm, xp_bohr, kpt=None):
     return np.array([_ao_from_tables(sysm, P, grad_dir=d) for d in range(3)])
 
 
+def ao_points(basis, L_bohr, xp_bohr, xe_bohr, rcut=24.0, kpt=None):
+    """phi (n_e, n_ao) at electron positions: make_pbc_gto's eval_pbc_ao /
+    eval_pbc_ao_kpt (hqc/pbc/gto.py:326-369, 371-420) with the 1-D image set
+    (-tmax..tmax) L, tmax = rcut // L (gto.py:48-49), and the phase
+    e^{i k_d t L} at a k-point.  basis: an AtomBasis.
+    Pinned by: equality with ao_values at grid points (tests/test_oracle_slater.py)."""
+    b = basis
+    tmax = rcut // L_bohr
+    lat = np.arange(-tmax, tmax + 1) * L_bohr
+    r = np.asarray(xe_bohr, float)[:, None, :] - np.asarray(xp_bohr, float)[None, :, :]
+    xt = r[..., None] - lat                                          # (ne, n, 3, t)
+    gauss = np.exp(-b.alpha[:, None] * xt[..., None, :] ** 2)        # (ne, n, 3, a, t)
+    if kpt is not None:
+        gauss = gauss * np.exp(1j * np.asarray(kpt)[None, None, :, None, None] * lat)
+    P = np.stack([(gauss * xt[..., None, :] ** l).sum(-1) for l in range(b.l_max + 1)], -1)
+    ne, n = r.shape[0], r.shape[1]
+    out = np.zeros((ne, n, b.n_ao_atom), dtype=P.dtype)
+    for o in range(b.n_ao_atom):
+        pw = [1 if b.comp[o] == d else 0 for d in range(3)]
+        f = P[:, :, 0, :, pw[0]] * P[:, :, 1, :, pw[1]] * P[:, :, 2, :, pw[2]]   # (ne, n, a)
+        out[:, :, o] = f @ b.coef[:, o]
+    return out.reshape(ne, n * b.n_ao_atom)
+
+
+def slater(n, L, rs, basis, xp, xe, mo_coeff, kpt=None, state_idx=None, rcut=24.0):
+    """(slater_up, slater_dn) of make_slater (hqc/pbc/slater.py:7-104): AO values at
+    the electrons times the occupied columns; state_idx defaults to the ground
+    state [0..n/2-1, 0..n/2-1]."""
+    assert n % 2 == 0
+    if state_idx is None:
+        state_idx = np.concatenate([np.arange(n // 2), np.arange(n // 2)])
+    ao = ao_points(AtomBasis(basis), L * rs, np.asarray(xp) * rs, np.asarray(xe) * rs, rcut, kpt)
+    h = n // 2
+    return ao[:h] @ mo_coeff[:, state_idx[:h]], ao[h:] @ mo_coeff[:, state_idx[h:]]
+
+
 # --------------------------------------------------------------------------
 # Literal pair-density route (hqc/pbc/solver.py:391-629) -- small systems only
 # --------------------------------------------------------------------------

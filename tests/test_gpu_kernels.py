@@ -324,3 +324,37 @@ def test_eigh_jacobi(K, n):
     w2, U = K.eigh_jacobi(Aw, sweeps=sw)
     assert torch.allclose(w2, wr, atol=1e-12 * scale)
     assert torch.allclose(A @ (C @ U), (C @ U) * w2[:, None, :], atol=1e-11 * scale)
+
+
+@pytest.mark.parametrize("gamma", [True, False])
+def test_slater_matrices(K, gamma):
+    """make_slater / make_pbc_gto against the oracle, single and batched walkers."""
+    from hqc_amd.pbc import make_pbc_gto, make_slater
+    n, rs, basis = 6, 1.3, "gth-dzvp"
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    rng = np.random.default_rng(7)
+    xp = rng.uniform(0, L, (n, 3))
+    xe = rng.uniform(0, L, (5, n, 3))
+    nao = n * 5
+    kpt = None if gamma else np.array([0.25, -0.1, 0.4])
+    mo = rng.normal(size=(nao, nao)) if gamma else \
+        rng.normal(size=(nao, nao)) + 1j * rng.normal(size=(nao, nao))
+    print(f"[slater] gamma={gamma}", flush=True)
+    sl = make_slater(n, L, rs, basis, gamma=gamma)
+    args = (xp, xe, mo) if gamma else (xp, xe, mo, kpt)
+    up, dn = sl(*args)
+    assert up.shape == (5, n // 2, n // 2)
+    for w in range(5):
+        ru, rd = R.slater(n, L, rs, basis, xp, xe[w], mo, kpt=kpt)
+        assert np.allclose(up[w].cpu().numpy(), ru, rtol=1e-12, atol=1e-13)
+        assert np.allclose(dn[w].cpu().numpy(), rd, rtol=1e-12, atol=1e-13)
+    # excited-state form with explicit state indices
+    idx = np.array([0, 2, 4, 1, 3, 5])
+    sl2 = make_slater(n, L, rs, basis, groundstate=False, gamma=gamma)
+    up2, _ = sl2(xp, xe[0], mo, idx) if gamma else sl2(xp, xe[0], mo, kpt, idx)
+    ru2, _ = R.slater(n, L, rs, basis, xp, xe[0], mo, kpt=kpt, state_idx=idx)
+    assert np.allclose(up2.cpu().numpy(), ru2, rtol=1e-12, atol=1e-13)
+    gto = make_pbc_gto(basis, L * rs, gamma=gamma)
+    one = gto(xp * rs, xe[0, 0] * rs) if gamma else gto(xp * rs, xe[0, 0] * rs, kpt)
+    ref = R.ao_points(R.AtomBasis(basis), L * rs, xp * rs, xe[0, :1] * rs, kpt=kpt)[0]
+    assert one.shape == (nao,) and np.allclose(one.cpu().numpy(), ref, rtol=1e-12, atol=1e-14)
