@@ -5,9 +5,9 @@ different sizes."""
 from .batch import make_pes_batch
 from .lcao import make_lcao
 from .pes import make_pes
-from .potential import potential_energy_pp
+from .potential import potential_energy, potential_energy_pp
 from .slater import make_pbc_gto, make_slater
 from .solver import make_solver
 
 __all__ = ["make_lcao", "make_pbc_gto", "make_pes", "make_pes_batch", "make_slater",
-           "make_solver", "potential_energy_pp"]
+           "make_solver", "potential_energy", "potential_energy_pp"]

@@ -1,8 +1,10 @@
-"""Ewald proton-proton energy on the GPU (hqc/pbc/potential.py:6-130).
+"""Ewald sums on the GPU (hqc/pbc/potential.py:6-130).
 
 potential_energy_pp(xp, L, rs, kappa=10, Gmax=15) -> V_pp in Ry, same units
 and conventions as the reference (xp and L in rs units, minimum image in
 fractional coordinates).  `ewald_pp` additionally returns -dE/dR in Ha/Bohr.
+potential_energy(x, kappa, G, L, rs) -> (v_pp, v_ep, v_ee) for batches of
+proton + electron configurations (the VMC observables path).
 """
 import math
 
@@ -65,3 +67,36 @@ def potential_energy_pp(xp, L, rs, kappa=10, Gmax=15):
     single = xp.dim() == 2
     E = ewald_pp(xp.unsqueeze(0) if single else xp, L, rs, kappa, Gmax)
     return E[0] if single else E
+
+
+def potential_energy(x, kappa, G, L, rs):
+    """hqc/pbc/potential.py:70-101, batched like its jax.vmap: x (W, n, 3) (or (n, 3))
+    in rs units, first n/2 rows protons (Z = +1), last n/2 electrons (Z = -1).
+    Returns (v_pp, v_ep, v_ee), each 2 rs / L times the psi pair sums of like
+    protons, unlike pairs (negated) and like electrons ("Ry/rs^2" as there)."""
+    x = torch.as_tensor(np.asarray(x) if not torch.is_tensor(x) else x,
+                        dtype=torch.float64).to("cuda")
+    single = x.dim() == 2
+    if single:
+        x = x.unsqueeze(0)
+    W, n, _ = x.shape
+    dev = x.device
+    x = x - L * torch.floor(x / L)
+    i, j = torch.triu_indices(n, n, offset=1, device=dev)
+    rij = (x[:, i] - x[:, j]) / L
+    rij = (rij - torch.round(rij)).contiguous()
+    Gd = torch.as_tensor(np.asarray(G) if not torch.is_tensor(G) else G,
+                         dtype=torch.float64).to(dev).contiguous()
+    gk = _weights(Gd, kappa).contiguous()
+    out = torch.empty(W, rij.shape[1], 4, dtype=torch.float64, device=dev)
+    K.ewald_pairs(rij, Gd, gk, float(kappa), -math.pi / kappa ** 2, out)
+    v = out[..., 0]
+    Z = torch.cat([torch.ones(n // 2), -torch.ones(n // 2)]).to(dev)
+    tc = Z[i] + Z[j]
+    f = 2 * rs / L
+    vpp = f * (v * (tc == 2)).sum(-1)
+    vep = -f * (v * (tc == 0)).sum(-1)
+    vee = f * (v * (tc == -2)).sum(-1)
+    if single:
+        return vpp[0], vep[0], vee[0]
+    return vpp, vep, vee

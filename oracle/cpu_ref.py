@@ -1046,6 +1046,28 @@ def ewald_pp(xp, L, rs, kappa=10, Gmax=15, forces=False):
     return E, F
 
 
+def potential_energy(x, kappa, G, L, rs):
+    """(v_pp, v_ep, v_ee) of hqc/pbc/potential.py:70-101 for one configuration x (n, 3)
+    in rs units whose first n/2 rows are protons (Z = +1) and last n/2 electrons
+    (Z = -1): 2 rs / L times the pair sums of psi by total pair charge, v_ep with
+    a minus sign.  Pinned by: kappa independence of the neutral total
+    v_pp + v_ep + v_ee + n rs/L Madelung (tests/test_oracle_slater.py)."""
+    x = np.asarray(x, float)
+    n = x.shape[0]
+    x = x - L * np.floor(x / L)
+    i, j = np.triu_indices(n, k=1)
+    rij = (x[i] - x[j]) / L
+    rij = rij - np.rint(rij)
+    Gn = np.linalg.norm(G, axis=-1)
+    gk = np.exp(-np.pi ** 2 * Gn ** 2 / kappa ** 2) / (np.pi * Gn ** 2)
+    d = np.linalg.norm(rij, axis=-1)
+    v = scipy.special.erfc(kappa * d) / d + np.cos(2 * np.pi * rij @ G.T) @ gk - np.pi / kappa ** 2
+    Z = np.concatenate([np.ones(n // 2), -np.ones(n // 2)])
+    tc = Z[i] + Z[j]
+    f = 2 * rs / L
+    return f * v[tc == 2].sum(), -f * v[tc == 0].sum(), f * v[tc == -2].sum()
+
+
 # --------------------------------------------------------------------------
 # Forces of the Mermin free energy A = E - sigma S_e + V_pp (new in this build)
 # --------------------------------------------------------------------------

@@ -358,3 +358,17 @@ def test_slater_matrices(K, gamma):
     one = gto(xp * rs, xe[0, 0] * rs) if gamma else gto(xp * rs, xe[0, 0] * rs, kpt)
     ref = R.ao_points(R.AtomBasis(basis), L * rs, xp * rs, xe[0, :1] * rs, kpt=kpt)[0]
     assert one.shape == (nao,) and np.allclose(one.cpu().numpy(), ref, rtol=1e-12, atol=1e-14)
+
+
+def test_potential_energy_protons_electrons(K):
+    from hqc_amd.pbc import potential_energy
+    rng = np.random.default_rng(9)
+    n, rs = 10, 1.4
+    L = (4 / 3 * np.pi * (n // 2)) ** (1 / 3)
+    x = rng.uniform(-L, 2 * L, (3, n, 3))
+    G = R.ewald_kpoints(15)
+    vpp, vep, vee = potential_energy(x, 10.0, G, L, rs)
+    for w in range(3):
+        ref = R.potential_energy(x[w], 10.0, G, L, rs)
+        for a, r in zip((vpp[w], vep[w], vee[w]), ref):
+            assert abs(a.item() - r) < 1e-11 * max(1.0, abs(r))

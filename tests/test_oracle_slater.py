@@ -53,3 +53,22 @@ def test_slater_groundstate_shapes_and_orthogonality():
     # exchanging two same-spin electrons swaps rows
     up2, _ = R.slater(n, L, rs, "gth-dzv", xp, xe[[1, 0, 2, 3]], mo)
     assert np.allclose(up2, up[[1, 0]])
+
+
+def test_potential_energy_neutral_total_kappa_independent():
+    """Protons + electrons: sum_ij q_i q_j psi_ij + (n/2) Madelung (each charge's
+    self term) is the Ewald energy of the neutral cell and cannot depend on kappa;
+    the like-charge parts reproduce the proton Ewald sum."""
+    rng = np.random.default_rng(5)
+    n, rs = 8, 1.3
+    L = (4 / 3 * np.pi * (n // 2)) ** (1 / 3)
+    x = rng.uniform(0, L, (n, 3))
+    G = R.ewald_kpoints(15)
+    tot = []
+    for kappa in (9.0, 9.5, 10.0):      # window where both truncations are < 1e-9
+        vpp, vep, vee = R.potential_energy(x, kappa, G, L, rs)
+        tot.append(vpp + vep + vee + n * rs / L * R.madelung(kappa, G))
+    assert np.allclose(tot, tot[0], rtol=1e-9, atol=1e-12)
+    vpp, _, _ = R.potential_energy(x, 10.0, G, L, rs)
+    Epp = R.ewald_pp(x[:n // 2], L, rs)                  # Ry, with the Madelung term
+    assert np.isclose(vpp / rs ** 2 + (n // 2) * R.madelung(10.0, G) / L / rs, Epp, rtol=1e-12)
