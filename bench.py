@@ -79,7 +79,9 @@ def time_vmat(pes, B, reps=20):
     e1.record(s)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
-    flops = 2.0 * c.nao ** 2 * c.N * B             # algorithmic, no symmetry credit
+    # algorithmic count of a symmetric rank-N update (V is symmetric): n(n+1) N
+    # multiply-adds... as flops: 2 * n(n+1)/2 * N per geometry
+    flops = 1.0 * c.nao * (c.nao + 1) * c.N * B
     return ms, flops
 
 
@@ -298,7 +300,8 @@ def main():
     n_dens64 = stats["rho64"]
     vname = "k_vmat_pipe<" + ("double" if prec_v == "f64" else "float")
     cand = [
-        {"kernel": vname + "> (dV phi diag(v) phi^T)", "ms_per_launch": ms_v,
+        {"kernel": vname + "> (dV phi diag(v) phi^T, symmetric: n(n+1)N flops)",
+         "ms_per_launch": ms_v,
          "flops_per_launch": fl_v, "peak": MI355X_PEAK[prec_v],
          "calls_per_step": n_vmat / max(a.steps + a.warmup, 1),
          "traffic": measured_traffic(vname, B)},

@@ -213,7 +213,7 @@ extern "C" int hqc_orb_density_f32(const float* phi, const double* At, int B, in
 // a fixed order and writes both triangles scaled by dV.
 // ---------------------------------------------------------------------------
 #define VMAT_TPW 8
-#define VMAT_TARGET_BLOCKS 512
+#define VMAT_TARGET_BLOCKS 1024
 
 __device__ __forceinline__ void tile_rc(int t, int NT, int& rt, int& ct) {
   int r = 0, rem = t;
@@ -225,63 +225,118 @@ __device__ __forceinline__ void tile_rc(int t, int NT, int& rt, int& ct) {
   ct = r + rem;
 }
 
+// Large n_ao (nao_pad > 128, e.g. 128 H dzvp or the stacked complex rows of a
+// k-point): 2-D output blocking.  Workgroup = one 128x128 block pair (I <= J) of
+// V; wave (wr, wc) owns a 64x64 quadrant = 4x4 MFMA tiles (the quadrant below
+// the diagonal of a diagonal block is skipped).  Per grid chunk of KC points the
+// 128 rows of block I and of block J are staged in LDS (double-buffered, the next
+// chunk held in registers during the MFMAs), so every phi element read from HBM
+// feeds 128 output columns instead of 32.
+#define VBIG 128
+
 template <typename T, int KC>
-__global__ __launch_bounds__(256) void k_vmat(const T* __restrict__ phi,
-                                              const double* __restrict__ v, int nao_pad, int ld,
-                                              int ntiles, int nwg, double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_vmat_big(const T* __restrict__ phi,
+                                                  const double* __restrict__ v, int nao_pad,
+                                                  int ld, int ntiles, int nwg,
+                                                  double* __restrict__ part) {
   typedef typename vec4_of<T>::type V4;
+  constexpr int VW = 16 / sizeof(T);            // elements per 16-byte vector
+  constexpr int VPR = KC / VW;                  // vectors per row per chunk
+  constexpr int NV = 2 * VBIG * VPR / 256;      // vectors per thread per chunk
+  constexpr int STR = KC + (sizeof(T) == 4 ? 4 : 2);
+  typedef T vec_t __attribute__((ext_vector_type(VW)));
   extern __shared__ unsigned char smem_raw[];
-  T* Ps = reinterpret_cast<T*>(smem_raw);
-  T* vs = Ps + (long)nao_pad * (KC + 1);
-  const int wg = blockIdx.x, grp = blockIdx.y, b = blockIdx.z;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int l16 = lane & 15, kq = lane >> 4;
-  const int NT = nao_pad >> 4;
-  const int tbase = grp * 4 * VMAT_TPW + w * VMAT_TPW;
-  int rto[VMAT_TPW], cto[VMAT_TPW];
-#pragma unroll
-  for (int t = 0; t < VMAT_TPW; ++t) {
-    int rt = 0, ct = 0;
-    if (tbase + t < ntiles) tile_rc(tbase + t, NT, rt, ct);
-    rto[t] = (rt * 16 + l16) * (KC + 1);
-    cto[t] = (ct * 16 + l16) * (KC + 1);
+  T* Ls = reinterpret_cast<T*>(smem_raw);       // [2 buffers][2 operands][VBIG*STR]
+  T* vs = Ls + 4 * VBIG * STR;                  // [2][KC]
+  const int wg = blockIdx.x, bp = blockIdx.y, b = blockIdx.z;
+  const int NT = nao_pad >> 4, NB = (nao_pad + VBIG - 1) / VBIG;
+  int I = 0, rem = bp;
+  while (rem >= NB - I) {
+    rem -= NB - I;
+    ++I;
   }
+  const int J = I + rem;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const bool active = !(I == J && wr > wc);
+  const int l16 = lane & 15, kq = lane >> 4;
   const T* P = phi + (long)b * nao_pad * ld;
   const double* vb = v + (long)b * ld;
-  // accumulate in T over every chunk of this block (FP32 for the FP32 variant,
-  // whose callers use it on potential increments); blocks are summed in FP64
-  V4 acc[VMAT_TPW];
+  vec_t stage[NV];
+  double vstage = 0.0;
+  auto gload = [&](long c0) {
 #pragma unroll
-  for (int t = 0; t < VMAT_TPW; ++t) acc[t] = V4{0, 0, 0, 0};
-  for (long c0 = (long)wg * KC; c0 < ld; c0 += (long)nwg * KC) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < nao_pad * KC; e += 256) {
-      const int row = e / KC, kk = e - row * KC;
-      Ps[row * (KC + 1) + kk] = P[(long)row * ld + c0 + kk];
+    for (int j = 0; j < NV; ++j) {
+      const int q = threadIdx.x + 256 * j;
+      const int o = q / (VBIG * VPR), row = (q / VPR) % VBIG, cv = q % VPR;
+      const int grow = min((o == 0 ? I : J) * VBIG + row, nao_pad - 1);
+      stage[j] = *reinterpret_cast<const vec_t*>(P + (long)grow * ld + c0 + cv * VW);
     }
-    for (int e = threadIdx.x; e < KC; e += 256) vs[e] = (T)vb[c0 + e];
-    __syncthreads();
-#pragma unroll 2
-    for (int ks = 0; ks < KC / 4; ++ks) {
-      const int kk = ks * 4 + kq;
-      const T vk = vs[kk];
+    if (threadIdx.x < KC) vstage = vb[c0 + threadIdx.x];
+  };
+  auto lstore = [&](int buf) {
 #pragma unroll
-      for (int t = 0; t < VMAT_TPW; ++t) {
-        if (tbase + t < ntiles) {
-          const T a = Ps[rto[t] + kk];
-          const T bb = vk * Ps[cto[t] + kk];
-          acc[t] = mfma16(a, bb, acc[t]);
+    for (int j = 0; j < NV; ++j) {
+      const int q = threadIdx.x + 256 * j;
+      const int o = q / (VBIG * VPR), row = (q / VPR) % VBIG, cv = q % VPR;
+      T* dst = Ls + (long)(buf * 2 + o) * VBIG * STR + row * STR + cv * VW;
+#pragma unroll
+      for (int e = 0; e < VW; ++e) dst[e] = stage[j][e];
+    }
+    if (threadIdx.x < KC) vs[buf * KC + threadIdx.x] = (T)vstage;
+  };
+  V4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = V4{0, 0, 0, 0};
+  long c0 = (long)wg * KC;
+  const long cstep = (long)nwg * KC;
+  if (c0 < ld) {
+    gload(c0);
+    lstore(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (; c0 < ld; c0 += cstep) {
+    const bool more = c0 + cstep < ld;
+    if (more) gload(c0 + cstep);
+    if (active) {
+      const T* Ab = Ls + (long)(buf * 2) * VBIG * STR + (wr * 64 + l16) * STR;
+      const T* Bb = Ls + (long)(buf * 2 + 1) * VBIG * STR + (wc * 64 + l16) * STR;
+      const T* vsb = vs + buf * KC;
+#pragma unroll 2
+      for (int ks = 0; ks < KC / 4; ++ks) {
+        const int kk = ks * 4 + kq;
+        const T vk = vsb[kk];
+        T a[4], bb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          a[t] = Ab[t * 16 * STR + kk];
+          bb[t] = vk * Bb[t * 16 * STR + kk];
         }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(a[i], bb[j], acc[i][j]);
       }
     }
+    if (more) lstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
+  if (!active) return;
 #pragma unroll
-  for (int t = 0; t < VMAT_TPW; ++t) {
-    const int tile = tbase + t;
-    if (tile < ntiles) {
-      double* o = part + (((long)b * ntiles + tile) * nwg + wg) * 256 + lane * 4;
+  for (int i = 0; i < 4; ++i) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) o[e] = (double)acc[t][e];
+    for (int j = 0; j < 4; ++j) {
+      const int R = I * (VBIG / 16) + wr * 4 + i, C = J * (VBIG / 16) + wc * 4 + j;
+      if (R < NT && C < NT && R <= C) {
+        const int tile = R * NT - (R * (R - 1)) / 2 + (C - R);
+        double* o = part + (((long)b * ntiles + tile) * nwg + wg) * 256 + lane * 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o[e] = (double)acc[i][j][e];
+      }
     }
   }
 }
@@ -432,15 +487,21 @@ static int vmat_pipe_tpw(int nao_pad) {
 
 static void vmat_shape(int B, int nao_pad, int ld, int prec, int& KC, int& ntiles, int& ngroups,
                        int& nwg) {
-  KC = nao_pad <= 160 ? 64 : (nao_pad <= 320 ? 32 : 16);
-  if (prec == 32 && nao_pad <= 320) KC = 64;
-  if (vmat_pipelined(nao_pad)) KC = prec == 32 ? 64 : 32;   // 16 vectors of 16 B per row
   const int NT = nao_pad >> 4;
   ntiles = NT * (NT + 1) / 2;
-  const int tpw = vmat_pipelined(nao_pad) ? vmat_pipe_tpw(nao_pad) : VMAT_TPW;
-  ngroups = (ntiles + 4 * tpw - 1) / (4 * tpw);
+  long target_blocks;
+  if (vmat_pipelined(nao_pad)) {
+    KC = prec == 32 ? 64 : 32;                 // 16 vectors of 16 B per row
+    const int tpw = vmat_pipe_tpw(nao_pad);
+    ngroups = (ntiles + 4 * tpw - 1) / (4 * tpw);
+    target_blocks = 2048;
+  } else {
+    KC = prec == 32 ? 32 : 16;                 // 8 vectors of 16 B per row
+    const int NB = (nao_pad + VBIG - 1) / VBIG;
+    ngroups = NB * (NB + 1) / 2;
+    target_blocks = VMAT_TARGET_BLOCKS;
+  }
   const int nchunks = ld / KC;
-  const long target_blocks = vmat_pipelined(nao_pad) ? 2048 : VMAT_TARGET_BLOCKS;
   long target = target_blocks / ((long)(B > 0 ? B : 1) * ngroups);
   if (target < 1) target = 1;
   if (target > nchunks) target = nchunks;
@@ -494,17 +555,13 @@ static int launch_vmat(const T* phi, const double* v, int B, int nao_pad, int ld
     HQC_CHECK_LAUNCH();
     return HQC_OK;
   }
-  const size_t lds = ((size_t)nao_pad * (KC + 1) + KC) * sizeof(T);
-  if (lds > 160 * 1024) return HQC_E_SIZE;
-  if (KC == 64)
-    hipLaunchKernelGGL((k_vmat<T, 64>), grid, dim3(256), lds, s, phi, v, nao_pad, ld, ntiles,
-                       nwg, part);
-  else if (KC == 32)
-    hipLaunchKernelGGL((k_vmat<T, 32>), grid, dim3(256), lds, s, phi, v, nao_pad, ld, ntiles,
-                       nwg, part);
-  else
-    hipLaunchKernelGGL((k_vmat<T, 16>), grid, dim3(256), lds, s, phi, v, nao_pad, ld, ntiles,
-                       nwg, part);
+  constexpr int KB = sizeof(T) == 4 ? 32 : 16;
+  const size_t lds = ((size_t)4 * VBIG * (KB + (sizeof(T) == 4 ? 4 : 2)) + 2 * KB) * sizeof(T);
+  if (hipFuncSetAttribute((const void*)k_vmat_big<T, KB>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    (void)hipGetLastError();
+  hipLaunchKernelGGL((k_vmat_big<T, KB>), grid, dim3(256), lds, s, phi, v, nao_pad, ld, ntiles,
+                     nwg, part);
   HQC_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_vmat_reduce<T>, dim3(ntiles, B), dim3(256), 0, s, part, nao_pad, ntiles,
                      nwg, dV, V);

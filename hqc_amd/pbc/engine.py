@@ -148,8 +148,9 @@ class Engine:
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
                  tol=1e-7, max_cycle=100, diis=True, diis_space=8, diis_start_cycle=1,
-                 diis_damp=0.0, precision="fp64"):
+                 diis_damp=0.0, precision="fp64", ovlp_cut=0.0):
         self.c = cell
+        self.ovlp_cut = float(ovlp_cut)
         self.dft = dft
         self.gga = False
         if dft:
@@ -340,6 +341,22 @@ class Engine:
         else:
             dst[idx] = val
 
+    LINDEP_SHIFT = 1e4     # Hartree: energy given to overlap directions removed by ovlp_cut
+
+    def _orthogonalizer(self, st, S):
+        """v = U s^-1/2 (hqc/pbc/solver.py:729-730, no eigenvalue cut).  With
+        ovlp_cut > 0 (an option beyond the reference, for nearly linearly dependent
+        bases such as dense dzvp) overlap eigenvalues <= ovlp_cut get a zero column
+        in v and the eigenvalue LINDEP_SHIFT in F' = v^H F v, so they become empty
+        high-lying bands and every shape stays n_ao."""
+        w, u = torch.linalg.eigh(S)
+        if self.ovlp_cut > 0.0:
+            keep = w > self.ovlp_cut
+            st["v"] = u * torch.where(keep, w.clamp(min=self.ovlp_cut).rsqrt(), 0.0).unsqueeze(-2)
+            st["lindep"] = (~keep).to(torch.float64) * self.LINDEP_SHIFT
+        else:
+            st["v"] = u * w.rsqrt().unsqueeze(-2)
+
     def setup(self, xpb, kpt=None, Lb=None):
         """Per-geometry constants for xp (B, n, 3) in Bohr on the device.  Every
         tensor whose leading axis is the geometry is listed in st['per'] so the
@@ -393,9 +410,10 @@ class Engine:
             vg = vl if idx is None else torch.zeros(nb, c.ld, dtype=f64, device=dev)
             K.complex_real(cb, c.ld, vg)
             self._put(vl, idx, vg)
-        w, u = torch.linalg.eigh(S)                                     # solver.py:729-730
         st["S"], st["T"] = S, T
-        st["v"] = u * w.rsqrt().unsqueeze(-2)
+        self._orthogonalizer(st, S)
+        if "lindep" in st:
+            per.append("lindep")
         st["phi"] = phi
         dVs = self._dv(st, B)
         if mixed:
@@ -617,7 +635,7 @@ class Engine:
         return V2 + Vx, EH + Ex
 
     # --------------------------------------------------------------------- SCF
-    def _eigh_orth(self, F, v, Cw=None):
+    def _eigh_orth(self, F, v, Cw=None, shift=None):
         """Roothaan step (hqc/pbc/scf.py:68-72): F' = v^T F v, eigh, mo = v C'.
 
         n_ao <= 110 uses the LDS Jacobi kernel, warm-started in the previous
@@ -627,6 +645,8 @@ class Engine:
         # transposed operands are made contiguous: rocBLAS picks a far slower
         # kernel for batched 108x108 products with a transposed A
         Fp = v.transpose(-1, -2).contiguous() @ (F @ v)
+        if shift is not None:
+            Fp = Fp + torch.diag_embed(shift)
         if self.c.nao <= K.EIGH_JACOBI_MAXN:
             sw = torch.zeros(F.shape[0], dtype=torch.int32, device=F.device)
             if Cw is not None:
@@ -639,6 +659,17 @@ class Engine:
         else:
             w, C = torch.linalg.eigh(Fp)
         return w, v @ C, C
+
+    @staticmethod
+    def _errvec(st, na, S, D, F):
+        """DIIS error SDF - FDS (hqc/pbc/solver.py:696-700).  With an overlap cut it is
+        taken in the orthogonalised basis, v^H (SDF - FDS) v, whose zero is the
+        stationarity condition of the truncated problem."""
+        e = S @ D @ F - F @ D @ S
+        if "lindep" in st:
+            v = st["v"][:na]
+            e = v.mH @ e @ v
+        return e
 
     @staticmethod
     def _compact(st, keys, loc, order, keep, na):
@@ -668,7 +699,9 @@ class Engine:
         c = self.c
         B = st["B"]
         dev = c.device
-        w1, mo, Cw = self._eigh_orth(st["Hcore"], st["v"])               # solver.py:740-745
+        lindep = st.get("lindep")
+        sh = (lambda na: None) if lindep is None else (lambda na: st["lindep"][:na])
+        w1, mo, Cw = self._eigh_orth(st["Hcore"], st["v"], shift=sh(B))  # solver.py:740-745
         D, occ = self.density_matrix(mo, w1)
         tol = self.tol
         order = torch.arange(B, device=dev)
@@ -685,7 +718,7 @@ class Engine:
                 Dc = loc["Dc"][:na]
                 V2, E2 = self.fock2e(st, na, Dc, loc["Mo"][:na], loc["Oc"][:na], full=True)
                 Enew = _tr(H, Dc).real + E2
-                w1n, mon, cwn = self._eigh_orth(H + V2, st["v"][:na], loc["Cw"][:na])
+                w1n, mon, cwn = self._eigh_orth(H + V2, st["v"][:na], loc["Cw"][:na], sh(na))
                 loc["Cw"][:na] = cwn
                 Dn, On = self.density_matrix(mon, w1n)
                 loc["E_old"][:na] = loc["E"][:na]
@@ -699,14 +732,14 @@ class Engine:
             return loc["Mo"], loc["W1"], loc["E"], loc["nit"] != self.max_cycle, loc["nit"], order
 
         def step(Fin, na):
-            w1, mo, cw = self._eigh_orth(Fin, st["v"][:na], loc["Cw"][:na])
+            w1, mo, cw = self._eigh_orth(Fin, st["v"][:na], loc["Cw"][:na], sh(na))
             loc["Cw"][:na] = cw
             D, occ = self.density_matrix(mo, w1)
             V2, E2 = self.fock2e(st, na, D, mo, occ)
             H, S = st["Hcore"][:na], st["S"][:na]
             Enew = _tr(H, D).real + E2
             F = H + V2
-            e = S @ D @ F - F @ D @ S
+            e = self._errvec(st, na, S, D, F)
             if self.precision == "mixed":
                 st["err"][:na] = torch.linalg.norm(e, dim=(-2, -1))
             return mo, w1, Enew, F, e
@@ -727,7 +760,7 @@ class Engine:
         V2, E2 = self.fock2e(st, B, D, mo, occ)                           # scf.py:122-128
         H, S = st["Hcore"], st["S"]
         F0 = H + V2
-        e0 = S @ D @ F0 - F0 @ D @ S
+        e0 = self._errvec(st, B, S, D, F0)
         loc = {"Fk": F0.unsqueeze(1).repeat(1, m, 1, 1).contiguous(),
                "ek": e0.unsqueeze(1).repeat(1, m, 1, 1).contiguous(),
                "Bh": (e0 * e0).sum((-1, -2))[:, None, None].repeat(1, m, m).contiguous(),

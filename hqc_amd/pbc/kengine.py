@@ -98,15 +98,16 @@ class KEngine(Engine):
         S = torch.empty(B, c.nao, c.nao, dtype=c128, device=dev)
         T = torch.empty_like(S)
         K.overlap_kinetic_k(st["xpb"], st["kpt"], c.lattice, c.alpha, c.coef, c.comp, S, T)
-        w, u = torch.linalg.eigh(S)                                     # solver.py:934-935
         st["S"], st["T"] = S, T
-        st["v"] = u * w.rsqrt().unsqueeze(-2)
+        self._orthogonalizer(st, S)                                     # solver.py:934-935
         P = torch.empty(B, c.n, 3, c.g, c.na, c.nl, dtype=c128, device=dev)
         K.gto_tables_k(st["xpb"], st["kpt"], c.g, c.L, c.tmax_img, c.alpha, c.nl, P)
         phi = torch.zeros(B, 2 * c.nao_pad, c.ld, dtype=f64, device=dev)
         K.ao_values_k(P, c.coef, c.comp, c.nao_pad, c.ld, phi)
         st["phi"] = phi
         per = ["xpb", "kpt", "Lb", "dVb", "S", "T", "v", "phi", "vloc", "V", "Hcore"]
+        if "lindep" in st:
+            per.append("lindep")
         cb = torch.empty(B, c.g, c.g, c.g, dtype=c128, device=dev)
         K.vloc_G(st["xpb"], c.g, c.L, c.VG, cb)
         K.fft3d(cb, +1)
@@ -128,9 +129,11 @@ class KEngine(Engine):
         occ = self.occupations(w1)
         return (mo * occ.unsqueeze(-2)) @ mo.mH, occ
 
-    def _eigh_orth(self, F, v, Cw=None):
+    def _eigh_orth(self, F, v, Cw=None, shift=None):
         """F' = v^H F v, eigh of its Hermitian part, mo = v C' (hqc/pbc/scf.py:68-72)."""
         Fp = v.mH @ (F @ v)
+        if shift is not None:
+            Fp = Fp + torch.diag_embed(shift).to(Fp.dtype)
         w, C = torch.linalg.eigh(0.5 * (Fp + Fp.mH))
         return w, v @ C, C
 

@@ -26,7 +26,8 @@ def _sys(n=4, rs=1.5, basis="gth-dzvp", grid_length=0.3, seed=0):
     return R.System(n, L, rs, basis, grid_length=grid_length), xp * rs, L
 
 
-@pytest.mark.parametrize("nao_pad,g", [(16, 7), (48, 13), (112, 21), (176, 9)])
+@pytest.mark.parametrize("nao_pad,g", [(16, 7), (48, 13), (112, 21), (176, 9), (256, 13),
+                                       (640, 7)])
 def test_density_and_vmat_f64(K, nao_pad, g):
     print(f"[density/vmat f64] nao_pad={nao_pad} g={g}", flush=True)
     torch.manual_seed(0)
@@ -56,7 +57,7 @@ def test_density_and_vmat_f64(K, nao_pad, g):
     assert torch.equal(V, V.transpose(1, 2))
 
 
-@pytest.mark.parametrize("nao_pad,g", [(48, 13), (112, 21)])
+@pytest.mark.parametrize("nao_pad,g", [(48, 13), (112, 21), (224, 13), (272, 11)])
 def test_density_and_vmat_f32(K, nao_pad, g):
     print(f"[density/vmat f32] nao_pad={nao_pad} g={g}", flush=True)
     torch.manual_seed(1)

@@ -205,3 +205,19 @@ def test_batch_of_different_cells():
             Eref = ref[3] + R.ewald_pp(xp, L, rs)
             assert abs(E[i].item() - Eref) < 1e-8, (i, E[i].item(), Eref)
             assert np.allclose(F[i].cpu().numpy(), R.forces(s, xp), atol=1e-8)
+
+
+def test_ovlp_cut_matches_when_nothing_is_cut():
+    """ovlp_cut below the smallest overlap eigenvalue gives the same fixed point (the
+    DIIS residual is then measured in the orthogonal basis); above it the
+    removed directions become empty bands at the shift and the SCF still runs."""
+    from hqc_amd.pbc import make_solver
+    n, rs = 4, 1.5
+    xp, L = _xp(n, 3)
+    kw = dict(grid_length=0.3, dft=True, smearing=True, smearing_sigma=0.05, tol=1e-10)
+    a = make_solver(n, L, rs, "gth-dzvp", **kw)(xp)
+    b = make_solver(n, L, rs, "gth-dzvp", ovlp_cut=1e-12, **kw)(xp)
+    assert abs(a[3].item() - b[3].item()) < 1e-9           # same fixed point, SCF tol 1e-10
+    c = make_solver(n, L, rs, "gth-dzvp", ovlp_cut=0.2, **kw)(xp)
+    assert torch.isfinite(c[3]) and bool(c[8])
+    assert (c[2] > 1e3).sum().item() >= 1            # shifted directions (bands in Ry)
