@@ -41,6 +41,17 @@ struct vec4_of<float> {
   typedef f32x4 type;
 };
 
+// LDS row padding for the MFMA fragment reads.  Lane (i, kq) of a 16x16x4
+// fragment reads row i, column k0 + kq; the LDS serves 32 four-byte banks per
+// cycle, i.e. 16 doubles or 32 floats.  With a row stride of STR elements the
+// 16 rows of one cycle fall on distinct banks when 2*STR = 2 (mod 32) for
+// FP64 (STR = KC + 1) and STR = 2 (mod 32) for FP32 (STR = KC + 2, with the
+// two kq values of a cycle on adjacent banks).
+template <typename T>
+__host__ __device__ constexpr int lds_pad() {
+  return sizeof(T) == 4 ? 2 : 1;
+}
+
 // ---------------------------------------------------------------------------
 // Density.  Block = 4 waves, 64 grid columns (16 per wave).  For every group of
 // NTG row tiles the wave runs the full K loop over AOs: X tile = D(rows, :) *
@@ -172,6 +183,105 @@ __global__ __launch_bounds__(256) void k_orb_density(const T* __restrict__ phi,
   if (kq == 0) rho[(long)b * ld + col] = rsum;
 }
 
+// LDS variant (coefficients of up to 4 orbital tiles staged once per workgroup).
+// Workgroup = ODENS_CG groups of 256 grid columns, wave = 64 columns = 4 column
+// tiles per group, so every coefficient fragment read from LDS feeds 4 MFMAs
+// and every phi fragment feeds MT; phi fragments are prefetched two k-steps
+// ahead.
+#define ODENS_CT 4
+#define ODENS_CG 4          // column groups of 256 per workgroup (one staging of At)
+#define ODENS_LDS_MAX (96 * 1024)
+
+template <typename T, int MT>
+__global__ __launch_bounds__(256) void k_orb_density_lds(const T* __restrict__ phi,
+                                                         const double* __restrict__ At,
+                                                         int nao_pad, int m_pad, int ld,
+                                                         double* __restrict__ rho) {
+  typedef typename vec4_of<T>::type V4;
+  constexpr int MW = MT * 16;                   // orbitals per group
+  extern __shared__ unsigned char smem_raw[];
+  T* As = reinterpret_cast<T*>(smem_raw);       // [nao_pad][MW]
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const T* Pb = phi + (long)b * nao_pad * ld + l16;
+  const double* Ab = At + (long)b * nao_pad * m_pad;
+  const int KS = nao_pad >> 2;
+  // per column group: the density of this lane's column (before the kq reduction)
+  double rs[ODENS_CG][ODENS_CT];
+#pragma unroll
+  for (int g = 0; g < ODENS_CG; ++g)
+#pragma unroll
+    for (int c = 0; c < ODENS_CT; ++c) rs[g][c] = 0.0;
+  for (int mg = 0; mg < m_pad; mg += MW) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < nao_pad * MW; e += 256) {
+      const int k = e / MW, j = e - k * MW;
+      As[e] = (mg + j < m_pad) ? (T)Ab[(long)k * m_pad + mg + j] : (T)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < ODENS_CG; ++g) {
+      const long col0 = ((long)blockIdx.x * ODENS_CG + g) * 256 + w * 64;
+      if (col0 >= ld) continue;                 // wave-uniform (ld is a multiple of 64)
+      const T* P = Pb + col0;
+      V4 acc[MT][ODENS_CT];
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int c = 0; c < ODENS_CT; ++c) acc[i][c] = V4{0, 0, 0, 0};
+      T bq[2][ODENS_CT];
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int c = 0; c < ODENS_CT; ++c)
+          bq[q][c] = (q < KS) ? P[(long)(q * 4 + kq) * ld + c * 16] : (T)0;
+      for (int ks = 0; ks < KS; ks += 2) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int kk = ks + q;
+          if (kk < KS) {
+            const int k = kk * 4 + kq;
+            T a[MT];
+#pragma unroll
+            for (int i = 0; i < MT; ++i) a[i] = As[k * MW + i * 16 + l16];
+#pragma unroll
+            for (int i = 0; i < MT; ++i)
+#pragma unroll
+              for (int c = 0; c < ODENS_CT; ++c) acc[i][c] = mfma16(a[i], bq[q][c], acc[i][c]);
+            if (kk + 2 < KS) {
+#pragma unroll
+              for (int c = 0; c < ODENS_CT; ++c)
+                bq[q][c] = P[(long)((kk + 2) * 4 + kq) * ld + c * 16];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int c = 0; c < ODENS_CT; ++c)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const double y = (double)acc[i][c][e];
+            rs[g][c] += y * y;
+          }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < ODENS_CG; ++g) {
+    const long col0 = ((long)blockIdx.x * ODENS_CG + g) * 256 + w * 64;
+    if (col0 >= ld) continue;
+#pragma unroll
+    for (int c = 0; c < ODENS_CT; ++c) {
+      double v = rs[g][c];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (kq == 0) rho[(long)b * ld + col0 + c * 16 + l16] = v;
+    }
+  }
+}
+
 template <typename T>
 static int launch_orb_density(const T* phi, const double* At, int B, int nao_pad, int m_pad,
                               int ld, double* rho, void* stream) {
@@ -179,6 +289,41 @@ static int launch_orb_density(const T* phi, const double* At, int B, int nao_pad
       (ld & 63))
     return HQC_E_ARG;
   if (B == 0) return HQC_OK;
+  {
+    const int mt = min(m_pad >> 4, 4);
+    const size_t lds = (size_t)nao_pad * mt * 16 * sizeof(T);
+    if (lds <= ODENS_LDS_MAX) {
+      if (lds > 64 * 1024) {
+        const void* f = mt == 4 ? (const void*)k_orb_density_lds<T, 4>
+                      : mt == 3 ? (const void*)k_orb_density_lds<T, 3>
+                      : mt == 2 ? (const void*)k_orb_density_lds<T, 2>
+                                : (const void*)k_orb_density_lds<T, 1>;
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+            hipSuccess)
+          (void)hipGetLastError();
+      }
+      const dim3 grid((ld + 256 * ODENS_CG - 1) / (256 * ODENS_CG), B);
+      switch (mt) {
+        case 1:
+          hipLaunchKernelGGL((k_orb_density_lds<T, 1>), grid, dim3(256), lds,
+                             (hipStream_t)stream, phi, At, nao_pad, m_pad, ld, rho);
+          break;
+        case 2:
+          hipLaunchKernelGGL((k_orb_density_lds<T, 2>), grid, dim3(256), lds,
+                             (hipStream_t)stream, phi, At, nao_pad, m_pad, ld, rho);
+          break;
+        case 3:
+          hipLaunchKernelGGL((k_orb_density_lds<T, 3>), grid, dim3(256), lds,
+                             (hipStream_t)stream, phi, At, nao_pad, m_pad, ld, rho);
+          break;
+        default:
+          hipLaunchKernelGGL((k_orb_density_lds<T, 4>), grid, dim3(256), lds,
+                             (hipStream_t)stream, phi, At, nao_pad, m_pad, ld, rho);
+      }
+      HQC_CHECK_LAUNCH();
+      return HQC_OK;
+    }
+  }
 #define ODENS_CASE(G)                                                                      \
   case G:                                                                                  \
     hipLaunchKernelGGL((k_orb_density<T, G>), dim3(ld / 64, B), dim3(256), 0,               \
@@ -243,7 +388,7 @@ __global__ __launch_bounds__(256) void k_vmat_big(const T* __restrict__ phi,
   constexpr int VW = 16 / sizeof(T);            // elements per 16-byte vector
   constexpr int VPR = KC / VW;                  // vectors per row per chunk
   constexpr int NV = 2 * VBIG * VPR / 256;      // vectors per thread per chunk
-  constexpr int STR = KC + (sizeof(T) == 4 ? 4 : 2);
+  constexpr int STR = KC + lds_pad<T>();
   typedef T vec_t __attribute__((ext_vector_type(VW)));
   extern __shared__ unsigned char smem_raw[];
   T* Ls = reinterpret_cast<T*>(smem_raw);       // [2 buffers][2 operands][VBIG*STR]
@@ -355,9 +500,7 @@ __global__ __launch_bounds__(256) void k_vmat_pipe(const T* __restrict__ phi,
   typedef typename vec4_of<T>::type V4;
   constexpr int VW = 16 / sizeof(T);            // elements per 16-byte vector
   constexpr int VPR = KC / VW;                  // vectors per row of a chunk (16)
-  // LDS row stride chosen so that the 16 rows x 4 k of one MFMA fragment land
-  // on distinct 4-byte banks (KC+4 for FP32, KC+2 for FP64)
-  constexpr int STR = KC + (sizeof(T) == 4 ? 4 : 2);
+  constexpr int STR = KC + lds_pad<T>();
   typedef T vec_t __attribute__((ext_vector_type(VW)));
   extern __shared__ unsigned char smem_raw[];
   T* Ps = reinterpret_cast<T*>(smem_raw);                    // [2][nao_pad*STR]
@@ -530,7 +673,7 @@ static int launch_vmat(const T* phi, const double* v, int B, int nao_pad, int ld
   double* part = (double*)work;
   dim3 grid(nwg, ngroups, B);
   if (vmat_pipelined(nao_pad)) {
-    const size_t lds = 2 * ((size_t)nao_pad * (KC + (sizeof(T) == 4 ? 4 : 2)) + KC) * sizeof(T);
+    const size_t lds = 2 * ((size_t)nao_pad * (KC + lds_pad<T>()) + KC) * sizeof(T);
     constexpr int K = sizeof(T) == 4 ? 64 : 32;
 #define VMAT_PIPE_CASE(TP)                                                                  \
   case TP:                                                                                  \
@@ -556,7 +699,7 @@ static int launch_vmat(const T* phi, const double* v, int B, int nao_pad, int ld
     return HQC_OK;
   }
   constexpr int KB = sizeof(T) == 4 ? 32 : 16;
-  const size_t lds = ((size_t)4 * VBIG * (KB + (sizeof(T) == 4 ? 4 : 2)) + 2 * KB) * sizeof(T);
+  const size_t lds = ((size_t)4 * VBIG * (KB + lds_pad<T>()) + 2 * KB) * sizeof(T);
   if (hipFuncSetAttribute((const void*)k_vmat_big<T, KB>,
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
     (void)hipGetLastError();

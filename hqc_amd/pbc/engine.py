@@ -635,7 +635,18 @@ class Engine:
         return V2 + Vx, EH + Ex
 
     # --------------------------------------------------------------------- SCF
-    def _eigh_orth(self, F, v, Cw=None, shift=None):
+    EIG_TOL_MIN, EIG_TOL_MAX = 1e-14, 1e-8   # Jacobi stopping range (relative off-diagonal)
+    EIG_TOL_PER_RESIDUAL = 1e-4              # tol = this x the smallest DIIS residual
+
+    def _eig_tol(self, res):
+        """Jacobi tolerance for the next Roothaan step: the eigenvector error it leaves
+        (~ tol |F| / gap) stays far below the current DIIS residual; 1e-14 when the
+        residual is unknown."""
+        if res is None:
+            return self.EIG_TOL_MIN
+        return min(self.EIG_TOL_MAX, max(self.EIG_TOL_MIN, self.EIG_TOL_PER_RESIDUAL * res))
+
+    def _eigh_orth(self, F, v, Cw=None, shift=None, tol=None):
         """Roothaan step (hqc/pbc/scf.py:68-72): F' = v^T F v, eigh, mo = v C'.
 
         n_ao <= 110 uses the LDS Jacobi kernel, warm-started in the previous
@@ -650,10 +661,11 @@ class Engine:
         if self.c.nao <= K.EIGH_JACOBI_MAXN:
             sw = torch.zeros(F.shape[0], dtype=torch.int32, device=F.device)
             if Cw is not None:
-                w, U = K.eigh_jacobi(Cw.transpose(-1, -2).contiguous() @ (Fp @ Cw), sweeps=sw)
+                w, U = K.eigh_jacobi(Cw.transpose(-1, -2).contiguous() @ (Fp @ Cw), sweeps=sw,
+                                     tol=tol or self.EIG_TOL_MIN)
                 C = Cw @ U
             else:
-                w, C = K.eigh_jacobi(Fp, sweeps=sw)
+                w, C = K.eigh_jacobi(Fp, sweeps=sw, tol=tol or self.EIG_TOL_MIN)
             self.sweeps += sw.sum()
             self.eigh_calls += F.shape[0]
         else:
@@ -731,8 +743,8 @@ class Engine:
                 na = self._compact(st, keys, loc, order, keep, na)
             return loc["Mo"], loc["W1"], loc["E"], loc["nit"] != self.max_cycle, loc["nit"], order
 
-        def step(Fin, na):
-            w1, mo, cw = self._eigh_orth(Fin, st["v"][:na], loc["Cw"][:na], sh(na))
+        def step(Fin, na, tol):
+            w1, mo, cw = self._eigh_orth(Fin, st["v"][:na], loc["Cw"][:na], sh(na), tol)
             loc["Cw"][:na] = cw
             D, occ = self.density_matrix(mo, w1)
             V2, E2 = self.fock2e(st, na, D, mo, occ)
@@ -770,7 +782,9 @@ class Engine:
                "nit": torch.zeros(B, dtype=torch.long, device=dev)}
         na = B
         for _ in range(self.diis_start_cycle + m):                        # scf.py:130-172
-            mo_, w1_, Enew, F, e = step(loc["Fk"][:, -1], na)
+            mo_, w1_, Enew, F, e = step(loc["Fk"][:, -1], na,
+                                        self.EIG_TOL_MIN if lindep is not None else
+                                        self.EIG_TOL_MAX)
             push(loc, na, F, e)
             loc["Mo"][:], loc["W1"][:] = mo_, w1_
             loc["E_old"][:] = loc["E"]
@@ -779,10 +793,14 @@ class Engine:
         loc["E_old"][:] = loc["E"] - 1.0
         loc["nit"][:] = loop
 
+        res = [None]
+
         def cont(na):
             err = torch.linalg.norm(loc["ek"][:na, -1], dim=(-2, -1))
             dE = (loc["E"][:na] - loc["E_old"][:na]).abs()
-            return ((dE > tol) | (err > 10 * tol)) & (loop < self.max_cycle)
+            keep = ((dE > tol) | (err > 10 * tol)) & (loop < self.max_cycle)
+            res[0] = float(err.min().item()) if na else None
+            return keep
 
         na = self._compact(st, keys, loc, order, cont(na), na)
         # complex (k-point) error vectors give a complex symmetric B, as 'imn,jmn->ij'
@@ -801,7 +819,9 @@ class Engine:
             Fk = loc["Fk"][:na]
             Fg = (ck[:, :, None, None] * Fk).sum(1)
             Fg = (1 - self.diis_damp) * Fg + self.diis_damp * Fk[:, -1]
-            mo_a, w1_a, Enew, F, e_new = step(Fg, na)
+            # the overlap-cut shift dominates |F'|, so a relative tolerance would be loose
+            etol = self.EIG_TOL_MIN if lindep is not None else self._eig_tol(res[0])
+            mo_a, w1_a, Enew, F, e_new = step(Fg, na, etol)
             push(loc, na, F, e_new)
             loc["E_old"][:na] = loc["E"][:na]
             loc["E"][:na] = Enew

@@ -129,7 +129,7 @@ class KEngine(Engine):
         occ = self.occupations(w1)
         return (mo * occ.unsqueeze(-2)) @ mo.mH, occ
 
-    def _eigh_orth(self, F, v, Cw=None, shift=None):
+    def _eigh_orth(self, F, v, Cw=None, shift=None, tol=None):
         """F' = v^H F v, eigh of its Hermitian part, mo = v C' (hqc/pbc/scf.py:68-72)."""
         Fp = v.mH @ (F @ v)
         if shift is not None:

@@ -83,7 +83,9 @@ def test_density_and_vmat_f32(K, nao_pad, g):
 
 
 @pytest.mark.parametrize("nao_pad,m,dt", [(48, 20, torch.float64), (112, 60, torch.float64),
-                                           (112, 60, torch.float32)])
+                                           (112, 60, torch.float32), (112, 100, torch.float32),
+                                           (64, 140, torch.float64), (224, 100, torch.float64),
+                                           (640, 64, torch.float64)])
 def test_orb_density(K, nao_pad, m, dt):
     """rho from At = C sqrt(f) equals the D-form density with D = C f C^T."""
     torch.manual_seed(2)
