@@ -10,7 +10,7 @@ defaults otherwise); energy (E + V_pp) plus analytic forces for every
 geometry.  Inputs are already on the GPU when the timed region starts.
 
 `value` is the throughput of --path (default: the mixed-precision batched
-path, batch 64).  Beside it the run measures, untimed for `value`:
+path, batch 256).  Beside it the run measures, untimed for `value`:
   * the FP64 path on the same batch, whose results also give the accuracy of
     the mixed path (max |dE| per atom, |dF|, |d bands| against FP64);
   * the plain FP64 path, one geometry per call (seeds 0..3);
@@ -114,14 +114,15 @@ def time_orb_density(pes, B, reps=20):
     return ms, flops, keep
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_54H_dzv_b64.txt")
+TRAFFIC_BATCH = 256
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"pmc_traffic_54H_dzv_b{TRAFFIC_BATCH}.txt")
 
 
 def measured_traffic(kernel_prefix, B):
-    """HBM bytes per launch of a kernel at this workload with batch 64, from the
-    calibrated rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE table kept with the profiles
+    """HBM bytes per launch of a kernel at this workload with batch TRAFFIC_BATCH, from
+    the calibrated rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE table kept with the profiles
     (None for other batch sizes or when the table is absent)."""
-    if B != 64 or not os.path.isfile(TRAFFIC_FILE):
+    if B != TRAFFIC_BATCH or not os.path.isfile(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
         parts = line.split()
@@ -256,7 +257,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--path", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
@@ -314,6 +315,9 @@ def main():
         k["achieved"] = k["flops_per_launch"] / (k["ms_per_launch"] * 1e-3) / 1e12
         k["frac"] = k["achieved"] / k["peak"]
     dom = max(cand, key=lambda k: k["ms_per_launch"] * k["calls_per_step"])
+    w_main = st["w1"].clone()
+    del pes, st, eng
+    torch.cuda.empty_cache()
     out = {
         "metric": METRIC, "value": B / dt, "unit": "geometries/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3,
@@ -333,7 +337,7 @@ def main():
     if not a.no_compare:
         other = "fp64" if a.path == "mixed" else "mixed"
         pes2, dt2, E2, F2 = run_path(other, x, 1, 1)
-        w_a, w_b = st["w1"], pes2.solver.last["w1"]
+        w_a, w_b = w_main, pes2.solver.last["w1"]
         out["compare"] = {
             "against": other, "batched_value": B / dt2,
             "max_dE_per_atom_Ha": (E - E2).abs().max().item() / RY / N_ATOMS,
@@ -342,6 +346,7 @@ def main():
             "n_iter_mean_other": pes2.solver.last["nit"].double().mean().item(),
             "all_converged_other": bool(pes2.solver.last["conv"].all().item())}
         del pes2
+        torch.cuda.empty_cache()
         plain = plain_fp64()
         out["compare"]["fp64_plain_value"] = plain
         out["speedup_vs_fp64_plain"] = out["value"] / plain

@@ -25,13 +25,13 @@ _SIGS = {
     "hqc_gto_tables": [_P, _I, _I, _I, _D, _I, _P, _I, _I, _P, _P],
     "hqc_ao_values_f64": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
     "hqc_ao_values_f32": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
-    "hqc_ao_density_f64": [_P, _P, _I, _I, _I, _P, _P, _P],
-    "hqc_ao_density_f32": [_P, _P, _I, _I, _I, _P, _P],
-    "hqc_orb_density_f64": [_P, _P, _I, _I, _I, _I, _P, _P],
-    "hqc_orb_density_f32": [_P, _P, _I, _I, _I, _I, _P, _P],
+    "hqc_ao_density_f64": [_P, _P, _P, _I, _I, _I, _P, _P, _P],
+    "hqc_ao_density_f32": [_P, _P, _P, _I, _I, _I, _P, _P],
+    "hqc_orb_density_f64": [_P, _P, _P, _I, _I, _I, _I, _P, _P],
+    "hqc_orb_density_f32": [_P, _P, _P, _I, _I, _I, _I, _P, _P],
     "hqc_ao_vmat_workspace": [_I, _I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
-    "hqc_ao_vmat_f64": [_P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
-    "hqc_ao_vmat_f32": [_P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
+    "hqc_ao_vmat_f64": [_P, _P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
+    "hqc_ao_vmat_f32": [_P, _P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
     "hqc_pair_pack": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_split_f64": [_P, _I, _I, _I, _I, _I, _P, _P],
@@ -163,27 +163,40 @@ def ao_points(xp, xe, kpt, L, tmax, alpha, nl, coef, comp, out):
           _ptr(comp, I32), nloc, _ptr(out, None, "out"), _stream())
 
 
-def ao_density(phi, D, rho, X=None):
-    B, nao_pad, ld = phi.shape
+def _bidx(phi, bidx, B):
+    """Geometry index list (int32 on the device) of a subset of phi's batch, or None."""
+    if bidx is None:
+        assert phi.shape[0] == B
+        return None
+    assert bidx.dtype == I32 and bidx.shape == (B,) and bidx.is_cuda
+    return bidx
+
+
+def ao_density(phi, D, rho, X=None, bidx=None):
+    """rho (B, ld) [, X (B, nao_pad, ld)] = D-form density of phi[bidx] (bidx None: all)."""
+    nao_pad, ld = phi.shape[1], phi.shape[2]
+    B = D.shape[0]
     assert D.shape == (B, nao_pad, nao_pad) and rho.shape == (B, ld)
+    bi = _bidx(phi, bidx, B)
     if phi.dtype == F64:
         if X is not None:
-            assert X.shape == phi.shape
-        _call("hqc_ao_density_f64", _ptr(phi, F64, "phi"), _ptr(D, F64, "D"), B, nao_pad, ld,
-              _ptr(rho, F64, "rho"), _ptr(X, F64, "X"), _stream())
+            assert X.shape == (B, nao_pad, ld)
+        _call("hqc_ao_density_f64", _ptr(phi, F64, "phi"), _ptr(bi, I32, "bidx"), _ptr(D, F64, "D"),
+              B, nao_pad, ld, _ptr(rho, F64, "rho"), _ptr(X, F64, "X"), _stream())
     else:
         assert X is None
-        _call("hqc_ao_density_f32", _ptr(phi, F32, "phi"), _ptr(D, F64, "D"), B, nao_pad, ld,
-              _ptr(rho, F64, "rho"), _stream())
+        _call("hqc_ao_density_f32", _ptr(phi, F32, "phi"), _ptr(bi, I32, "bidx"), _ptr(D, F64, "D"),
+              B, nao_pad, ld, _ptr(rho, F64, "rho"), _stream())
 
 
-def orb_density(phi, At, rho):
-    B, nao_pad, ld = phi.shape
-    m_pad = At.shape[2]
+def orb_density(phi, At, rho, bidx=None):
+    nao_pad, ld = phi.shape[1], phi.shape[2]
+    B, m_pad = At.shape[0], At.shape[2]
     assert At.shape == (B, nao_pad, m_pad) and rho.shape == (B, ld)
+    bi = _bidx(phi, bidx, B)
     name = "hqc_orb_density_f64" if phi.dtype == F64 else "hqc_orb_density_f32"
-    _call(name, _ptr(phi, None, "phi"), _ptr(At, F64, "At"), B, nao_pad, m_pad, ld,
-          _ptr(rho, F64, "rho"), _stream())
+    _call(name, _ptr(phi, None, "phi"), _ptr(bi, I32, "bidx"), _ptr(At, F64, "At"), B, nao_pad,
+          m_pad, ld, _ptr(rho, F64, "rho"), _stream())
 
 
 def vmat_workspace(B, nao_pad, ld, prec):
@@ -192,13 +205,15 @@ def vmat_workspace(B, nao_pad, ld, prec):
     return n.value
 
 
-def ao_vmat(phi, v, dV, V, work):
-    B, nao_pad, ld = phi.shape
+def ao_vmat(phi, v, dV, V, work, bidx=None):
+    nao_pad, ld = phi.shape[1], phi.shape[2]
+    B = v.shape[0]
     assert v.shape == (B, ld) and V.shape == (B, nao_pad, nao_pad)
+    bi = _bidx(phi, bidx, B)
     name = "hqc_ao_vmat_f64" if phi.dtype == F64 else "hqc_ao_vmat_f32"
-    _call(name, _ptr(phi, None, "phi"), _ptr(v, F64, "v"), B, nao_pad, ld, float(dV),
-          _ptr(V, F64, "V"), _ptr(work, None, "work"), work.numel() * work.element_size(),
-          _stream())
+    _call(name, _ptr(phi, None, "phi"), _ptr(bi, I32, "bidx"), _ptr(v, F64, "v"), B, nao_pad, ld,
+          float(dV), _ptr(V, F64, "V"), _ptr(work, None, "work"),
+          work.numel() * work.element_size(), _stream())
 
 
 def fft3d(c, sign):

@@ -59,7 +59,7 @@ __host__ __device__ constexpr int lds_pad() {
 // give the A operand coalesced); the epilogue forms sum_p phi_p X_p per column.
 // ---------------------------------------------------------------------------
 template <typename T, int NTG>
-__global__ __launch_bounds__(256) void k_density(const T* __restrict__ phi,
+__global__ __launch_bounds__(256) void k_density(const T* __restrict__ phi, const int* __restrict__ bidx,
                                                  const double* __restrict__ D, int nao_pad,
                                                  int ld, double* __restrict__ rho,
                                                  double* __restrict__ X) {
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void k_density(const T* __restrict__ phi,
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
   const long col = (long)blockIdx.x * 64 + w * 16 + l16;
-  const T* P = phi + (long)b * nao_pad * ld;
+  const T* P = phi + (long)(bidx ? bidx[b] : b) * nao_pad * ld;
   const double* Db = D + (long)b * nao_pad * nao_pad;
   const int NT = nao_pad >> 4, KS = nao_pad >> 2;
   double rsum = 0.0;
@@ -105,14 +105,14 @@ __global__ __launch_bounds__(256) void k_density(const T* __restrict__ phi,
 }
 
 template <typename T>
-static int launch_density(const T* phi, const double* D, int B, int nao_pad, int ld,
+static int launch_density(const T* phi, const int* bidx, const double* D, int B, int nao_pad, int ld,
                           double* rho, double* X, void* stream) {
   if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63)) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
 #define DENS_CASE(G)                                                                       \
   case G:                                                                                  \
     hipLaunchKernelGGL((k_density<T, G>), dim3(ld / 64, B), dim3(256), 0, (hipStream_t)stream, \
-                       phi, D, nao_pad, ld, rho, X);                                       \
+                       phi, bidx, D, nao_pad, ld, rho, X);                                 \
     break;
   switch (min(nao_pad >> 4, 8)) {
     DENS_CASE(1) DENS_CASE(2) DENS_CASE(3) DENS_CASE(4) DENS_CASE(5) DENS_CASE(6) DENS_CASE(7)
@@ -123,14 +123,14 @@ static int launch_density(const T* phi, const double* D, int B, int nao_pad, int
   return HQC_OK;
 }
 
-extern "C" int hqc_ao_density_f64(const double* phi, const double* D, int B, int nao_pad,
-                                  int ld, double* rho, double* X, void* stream) {
-  return launch_density<double>(phi, D, B, nao_pad, ld, rho, X, stream);
+extern "C" int hqc_ao_density_f64(const double* phi, const int* bidx, const double* D, int B,
+                                  int nao_pad, int ld, double* rho, double* X, void* stream) {
+  return launch_density<double>(phi, bidx, D, B, nao_pad, ld, rho, X, stream);
 }
 
-extern "C" int hqc_ao_density_f32(const float* phi, const double* D, int B, int nao_pad,
-                                  int ld, double* rho, void* stream) {
-  return launch_density<float>(phi, D, B, nao_pad, ld, rho, nullptr, stream);
+extern "C" int hqc_ao_density_f32(const float* phi, const int* bidx, const double* D, int B,
+                                  int nao_pad, int ld, double* rho, void* stream) {
+  return launch_density<float>(phi, bidx, D, B, nao_pad, ld, rho, nullptr, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -140,7 +140,7 @@ extern "C" int hqc_ao_density_f32(const float* phi, const double* D, int B, int 
 // epilogue squares the accumulator instead of re-reading phi.
 // ---------------------------------------------------------------------------
 template <typename T, int NTG>
-__global__ __launch_bounds__(256) void k_orb_density(const T* __restrict__ phi,
+__global__ __launch_bounds__(256) void k_orb_density(const T* __restrict__ phi, const int* __restrict__ bidx,
                                                      const double* __restrict__ At, int nao_pad,
                                                      int m_pad, int ld,
                                                      double* __restrict__ rho) {
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void k_orb_density(const T* __restrict__ phi,
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
   const long col = (long)blockIdx.x * 64 + w * 16 + l16;
-  const T* P = phi + (long)b * nao_pad * ld;
+  const T* P = phi + (long)(bidx ? bidx[b] : b) * nao_pad * ld;
   const double* Ab = At + (long)b * nao_pad * m_pad;
   const int NT = m_pad >> 4, KS = nao_pad >> 2;
   double rsum = 0.0;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void k_orb_density(const T* __restrict__ phi,
 #define ODENS_LDS_MAX (96 * 1024)
 
 template <typename T, int MT>
-__global__ __launch_bounds__(256) void k_orb_density_lds(const T* __restrict__ phi,
+__global__ __launch_bounds__(256) void k_orb_density_lds(const T* __restrict__ phi, const int* __restrict__ bidx,
                                                          const double* __restrict__ At,
                                                          int nao_pad, int m_pad, int ld,
                                                          double* __restrict__ rho) {
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) void k_orb_density_lds(const T* __restrict__ p
   const int b = blockIdx.y;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
-  const T* Pb = phi + (long)b * nao_pad * ld + l16;
+  const T* Pb = phi + (long)(bidx ? bidx[b] : b) * nao_pad * ld + l16;
   const double* Ab = At + (long)b * nao_pad * m_pad;
   const int KS = nao_pad >> 2;
   // per column group: the density of this lane's column (before the kq reduction)
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_orb_density_lds(const T* __restrict__ p
 }
 
 template <typename T>
-static int launch_orb_density(const T* phi, const double* At, int B, int nao_pad, int m_pad,
+static int launch_orb_density(const T* phi, const int* bidx, const double* At, int B, int nao_pad, int m_pad,
                               int ld, double* rho, void* stream) {
   if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || m_pad <= 0 || (m_pad & 15) || ld <= 0 ||
       (ld & 63))
@@ -306,19 +306,19 @@ static int launch_orb_density(const T* phi, const double* At, int B, int nao_pad
       switch (mt) {
         case 1:
           hipLaunchKernelGGL((k_orb_density_lds<T, 1>), grid, dim3(256), lds,
-                             (hipStream_t)stream, phi, At, nao_pad, m_pad, ld, rho);
+                             (hipStream_t)stream, phi, bidx, At, nao_pad, m_pad, ld, rho);
           break;
         case 2:
           hipLaunchKernelGGL((k_orb_density_lds<T, 2>), grid, dim3(256), lds,
-                             (hipStream_t)stream, phi, At, nao_pad, m_pad, ld, rho);
+                             (hipStream_t)stream, phi, bidx, At, nao_pad, m_pad, ld, rho);
           break;
         case 3:
           hipLaunchKernelGGL((k_orb_density_lds<T, 3>), grid, dim3(256), lds,
-                             (hipStream_t)stream, phi, At, nao_pad, m_pad, ld, rho);
+                             (hipStream_t)stream, phi, bidx, At, nao_pad, m_pad, ld, rho);
           break;
         default:
           hipLaunchKernelGGL((k_orb_density_lds<T, 4>), grid, dim3(256), lds,
-                             (hipStream_t)stream, phi, At, nao_pad, m_pad, ld, rho);
+                             (hipStream_t)stream, phi, bidx, At, nao_pad, m_pad, ld, rho);
       }
       HQC_CHECK_LAUNCH();
       return HQC_OK;
@@ -327,7 +327,7 @@ static int launch_orb_density(const T* phi, const double* At, int B, int nao_pad
 #define ODENS_CASE(G)                                                                      \
   case G:                                                                                  \
     hipLaunchKernelGGL((k_orb_density<T, G>), dim3(ld / 64, B), dim3(256), 0,               \
-                       (hipStream_t)stream, phi, At, nao_pad, m_pad, ld, rho);             \
+                       (hipStream_t)stream, phi, bidx, At, nao_pad, m_pad, ld, rho);             \
     break;
   switch (min(m_pad >> 4, 8)) {
     ODENS_CASE(1) ODENS_CASE(2) ODENS_CASE(3) ODENS_CASE(4) ODENS_CASE(5) ODENS_CASE(6)
@@ -339,14 +339,14 @@ static int launch_orb_density(const T* phi, const double* At, int B, int nao_pad
   return HQC_OK;
 }
 
-extern "C" int hqc_orb_density_f64(const double* phi, const double* At, int B, int nao_pad,
-                                   int m_pad, int ld, double* rho, void* stream) {
-  return launch_orb_density<double>(phi, At, B, nao_pad, m_pad, ld, rho, stream);
+extern "C" int hqc_orb_density_f64(const double* phi, const int* bidx, const double* At, int B,
+                                   int nao_pad, int m_pad, int ld, double* rho, void* stream) {
+  return launch_orb_density<double>(phi, bidx, At, B, nao_pad, m_pad, ld, rho, stream);
 }
 
-extern "C" int hqc_orb_density_f32(const float* phi, const double* At, int B, int nao_pad,
-                                   int m_pad, int ld, double* rho, void* stream) {
-  return launch_orb_density<float>(phi, At, B, nao_pad, m_pad, ld, rho, stream);
+extern "C" int hqc_orb_density_f32(const float* phi, const int* bidx, const double* At, int B,
+                                   int nao_pad, int m_pad, int ld, double* rho, void* stream) {
+  return launch_orb_density<float>(phi, bidx, At, B, nao_pad, m_pad, ld, rho, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -380,7 +380,7 @@ __device__ __forceinline__ void tile_rc(int t, int NT, int& rt, int& ct) {
 #define VBIG 128
 
 template <typename T, int KC>
-__global__ __launch_bounds__(256) void k_vmat_big(const T* __restrict__ phi,
+__global__ __launch_bounds__(256) void k_vmat_big(const T* __restrict__ phi, const int* __restrict__ bidx,
                                                   const double* __restrict__ v, int nao_pad,
                                                   int ld, int ntiles, int nwg,
                                                   double* __restrict__ part) {
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(256) void k_vmat_big(const T* __restrict__ phi,
   const int wr = w >> 1, wc = w & 1;
   const bool active = !(I == J && wr > wc);
   const int l16 = lane & 15, kq = lane >> 4;
-  const T* P = phi + (long)b * nao_pad * ld;
+  const T* P = phi + (long)(bidx ? bidx[b] : b) * nao_pad * ld;
   const double* vb = v + (long)b * ld;
   vec_t stage[NV];
   double vstage = 0.0;
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void k_vmat_big(const T* __restrict__ phi,
 #define VMAT_PIPE_MAXV 8
 
 template <typename T, int KC, int TPW>
-__global__ __launch_bounds__(256) void k_vmat_pipe(const T* __restrict__ phi,
+__global__ __launch_bounds__(256) void k_vmat_pipe(const T* __restrict__ phi, const int* __restrict__ bidx,
                                                    const double* __restrict__ v, int nao_pad,
                                                    int ld, int ntiles, int nwg,
                                                    double* __restrict__ part) {
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(256) void k_vmat_pipe(const T* __restrict__ phi,
     rto[t] = (rt * 16 + l16) * STR;
     cto[t] = (ct * 16 + l16) * STR;
   }
-  const T* P = phi + (long)b * nao_pad * ld;
+  const T* P = phi + (long)(bidx ? bidx[b] : b) * nao_pad * ld;
   const double* vb = v + (long)b * ld;
   vec_t stage[VMAT_PIPE_MAXV];
   double vstage = 0.0;
@@ -661,7 +661,7 @@ extern "C" int hqc_ao_vmat_workspace(int B, int nao_pad, int ld, int prec, size_
 }
 
 template <typename T>
-static int launch_vmat(const T* phi, const double* v, int B, int nao_pad, int ld, double dV,
+static int launch_vmat(const T* phi, const int* bidx, const double* v, int B, int nao_pad, int ld, double dV,
                        double* V, void* work, size_t work_bytes, int prec, void* stream) {
   if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63)) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
@@ -677,7 +677,7 @@ static int launch_vmat(const T* phi, const double* v, int B, int nao_pad, int ld
     constexpr int K = sizeof(T) == 4 ? 64 : 32;
 #define VMAT_PIPE_CASE(TP)                                                                  \
   case TP:                                                                                  \
-    hipLaunchKernelGGL((k_vmat_pipe<T, K, TP>), grid, dim3(256), lds, s, phi, v, nao_pad, ld, \
+    hipLaunchKernelGGL((k_vmat_pipe<T, K, TP>), grid, dim3(256), lds, s, phi, bidx, v, nao_pad, ld, \
                        ntiles, nwg, part);                                                  \
     break;
     switch (vmat_pipe_tpw(nao_pad)) {
@@ -703,7 +703,7 @@ static int launch_vmat(const T* phi, const double* v, int B, int nao_pad, int ld
   if (hipFuncSetAttribute((const void*)k_vmat_big<T, KB>,
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
     (void)hipGetLastError();
-  hipLaunchKernelGGL((k_vmat_big<T, KB>), grid, dim3(256), lds, s, phi, v, nao_pad, ld, ntiles,
+  hipLaunchKernelGGL((k_vmat_big<T, KB>), grid, dim3(256), lds, s, phi, bidx, v, nao_pad, ld, ntiles,
                      nwg, part);
   HQC_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_vmat_reduce<T>, dim3(ntiles, B), dim3(256), 0, s, part, nao_pad, ntiles,
@@ -712,14 +712,14 @@ static int launch_vmat(const T* phi, const double* v, int B, int nao_pad, int ld
   return HQC_OK;
 }
 
-extern "C" int hqc_ao_vmat_f64(const double* phi, const double* v, int B, int nao_pad, int ld,
-                               double dV, double* V, void* work, size_t work_bytes,
-                               void* stream) {
-  return launch_vmat<double>(phi, v, B, nao_pad, ld, dV, V, work, work_bytes, 64, stream);
+extern "C" int hqc_ao_vmat_f64(const double* phi, const int* bidx, const double* v, int B,
+                               int nao_pad, int ld, double dV, double* V, void* work,
+                               size_t work_bytes, void* stream) {
+  return launch_vmat<double>(phi, bidx, v, B, nao_pad, ld, dV, V, work, work_bytes, 64, stream);
 }
 
-extern "C" int hqc_ao_vmat_f32(const float* phi, const double* v, int B, int nao_pad, int ld,
-                               double dV, double* V, void* work, size_t work_bytes,
-                               void* stream) {
-  return launch_vmat<float>(phi, v, B, nao_pad, ld, dV, V, work, work_bytes, 32, stream);
+extern "C" int hqc_ao_vmat_f32(const float* phi, const int* bidx, const double* v, int B,
+                               int nao_pad, int ld, double dV, double* V, void* work,
+                               size_t work_bytes, void* stream) {
+  return launch_vmat<float>(phi, bidx, v, B, nao_pad, ld, dV, V, work, work_bytes, 32, stream);
 }

@@ -134,16 +134,15 @@ class Engine:
     error exceeds MIXED_SAFETY times the current DIIS residual per matrix
     element, so the iteration converges to the FP64 fixed point.  While the DIIS
     residual is still large (>= MIXED_DIRECT_ERR, the fixed-point warm-up) rho
-    and V are taken in FP32 outright.  When only a few geometries (<= MIXED_SPLIT
-    of the batch) fail their bound, the batch takes the FP32 increment and just
-    those are recomputed in FP64.  Eigensolves, DIIS, occupations, FFTs, xc and
-    every energy sum stay FP64."""
+    and V are taken in FP32 outright.  The choice is per geometry: the kernels
+    read a subset of the batch in place through a geometry index list, so each
+    precision class costs one launch over its own geometries.  Eigensolves,
+    DIIS, occupations, FFTs, xc and every energy sum stay FP64."""
 
     MIXED_EPS = 2e-6        # effective relative error of an FP32-accumulated increment
     MIXED_SAFETY = 1e-2     # potential increment: error per element <= 1e-2 residual / n_ao
     MIXED_RHO_SAFETY = 1.0  # density increment: |v| x relative rho error <= residual
     MIXED_DIRECT_ERR = 1e-1  # residual above which rho and V are taken in FP32 outright
-    MIXED_SPLIT = 0.125      # largest failing share re-anchored alone (else all in FP64)
 
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
@@ -217,14 +216,15 @@ class Engine:
         out[:, :c.nao, :c.nao] = M
         return out
 
-    def vmat(self, phi, v, dV=None):
+    def vmat(self, phi, v, dV=None, bidx=None):
         """dV * phi diag(v) phi^T for a batch -> (B, nao, nao); dV a float (None: the
-        cell's) or a (B,) tensor of per-geometry volume elements."""
+        cell's) or a (B,) tensor of per-geometry volume elements; bidx (B,) int32
+        selects the geometries of phi (None: phi holds exactly the batch)."""
         if dV is None:
-            return self.vmat_dv(phi, v, self.c.dV)
+            return self.vmat_dv(phi, v, self.c.dV, bidx)
         if torch.is_tensor(dV):
-            return self.vmat_dv(phi, v * dV[:, None], 1.0)
-        return self.vmat_dv(phi, v, dV)
+            return self.vmat_dv(phi, v * dV[:, None], 1.0, bidx)
+        return self.vmat_dv(phi, v, dV, bidx)
 
     @staticmethod
     def _dv(st, na):
@@ -235,12 +235,12 @@ class Engine:
     def _geo(st, na):
         return {k: st[k][:na] for k in ("Lb", "dVb", "VGb")} if st.get("hetero") else None
 
-    def vmat_dv(self, phi, v, dV):
+    def vmat_dv(self, phi, v, dV, bidx=None):
         c = self.c
-        B = phi.shape[0]
+        B = v.shape[0]
         prec = 64 if phi.dtype == torch.float64 else 32
         V = torch.empty(B, c.nao_pad, c.nao_pad, dtype=torch.float64, device=c.device)
-        K.ao_vmat(phi, v, dV, V, self._workspace(B, prec))
+        K.ao_vmat(phi, v.contiguous(), dV, V, self._workspace(B, prec), bidx)
         return V[:, :c.nao, :c.nao]
 
     def density(self, phi, D, X=None):
@@ -251,9 +251,9 @@ class Engine:
 
     OCC_CUT = 1e-15   # orbitals below this occupation contribute < 1e-15 relative to rho
 
-    def orb_density(self, phi, mo, occ):
+    def orb_density(self, phi, mo, occ, bidx=None):
         """rho = sum_i f_i psi_i^2 over the orbitals with f_i > OCC_CUT (equal to the
-        D form with D = C f C^T up to that cut)."""
+        D form with D = C f C^T up to that cut); bidx as in vmat."""
         c = self.c
         B = mo.shape[0]
         keep = max(int((occ > self.OCC_CUT).sum(-1).max().item()), 1)
@@ -261,7 +261,7 @@ class Engine:
         At = torch.zeros(B, c.nao_pad, m_pad, dtype=torch.float64, device=c.device)
         At[:, :c.nao, :keep] = mo[:, :, :keep] * occ[:, None, :keep].clamp(min=0).sqrt()
         rho = torch.zeros(B, c.ld, dtype=torch.float64, device=c.device)
-        K.orb_density(phi, At, rho)
+        K.orb_density(phi, At, rho, bidx)
         return rho
 
     def poisson(self, rho, save=False, geo=None):
@@ -523,26 +523,20 @@ class Engine:
         Sg = st["Sgrid"][:na]
         return Kmat + st["c0b"][:na, None, None] * (Sg @ D @ Sg)
 
-    def _mixed_plan(self, direct, inc, na):
-        """Batch-level choice: 'direct' when every geometry is still in the large-residual
-        phase, otherwise FP32 increments for the batch with an FP64 re-anchor of the
-        geometries that fail their bound, when those are at most MIXED_SPLIT of the
-        batch; else FP64 for the whole batch.  Returns (mode, FP64 index or None)."""
-        if bool(direct.all().item()):
-            return "direct", None
-        fail = ~inc
-        nf = int(fail.sum().item())
-        if nf == 0:
-            return "inc", None
-        if nf <= max(1, int(self.MIXED_SPLIT * na)):
-            return "inc", torch.nonzero(fail).flatten()
-        return "fp64", None
+    @staticmethod
+    def _subset(mask):
+        """(index tensor int32 or None when the mask covers the batch, count)."""
+        n = int(mask.sum().item())
+        if n == mask.numel():
+            return None, n
+        return torch.nonzero(mask).flatten().to(torch.int32), n
 
     def _mixed_density(self, st, na, D, mo, occ):
-        """rho for geometries [0, na) in the mixed mode: FP32 orbital form outright
-        while the DIIS residual is large (err >= MIXED_DIRECT_ERR), FP32 increments
-        rho_a + rho(D - D_a) where the increment bound holds, FP64 (and a new anchor)
-        elsewhere (see _mixed_plan)."""
+        """rho for geometries [0, na) in the mixed mode, chosen per geometry: the FP32
+        orbital form outright while its DIIS residual is large (err >= MIXED_DIRECT_ERR),
+        FP32 increments rho_a + rho(D - D_a) where the increment bound holds, FP64
+        (and a new anchor) elsewhere.  Each class is one launch over its geometries,
+        read in place from the batch arrays through a geometry index list."""
         c = self.c
         sel = lambda key: st[key][:na]
         err = sel("err")
@@ -550,35 +544,50 @@ class Engine:
         dD = D - sel("Da")
         # bound on the relative density error of the FP32 increment
         rel = self.MIXED_EPS * torch.linalg.norm(dD, dim=(-2, -1)) / sel("kappa")
-        inc = (rel <= 1e-2) & (rel * sel("vmax") <= self.MIXED_RHO_SAFETY * err)
-        mode, idx = self._mixed_plan(direct, inc, na)
-        if mode == "direct":
+        inc = ~direct & (rel <= 1e-2) & (rel * sel("vmax") <= self.MIXED_RHO_SAFETY * err)
+        full = ~(direct | inc)
+        rho = torch.empty(na, c.ld, dtype=torch.float64, device=c.device)
+        idx, n = self._subset(direct)
+        if n:
             # orbital form sum_i f_i psi_i^2: non-negative in any precision, unlike the
             # D form, whose FP32 rounding can turn a tiny density negative
+            r = self.orb_density(st["phi_lp"] if n < na else sel("phi_lp"),
+                                 mo if idx is None else mo[idx.long()],
+                                 occ if idx is None else occ[idx.long()], idx)
+            self._scatter(rho, idx, r)
             self.stats["rho32"] += 1
-            return self.orb_density(sel("phi_lp"), mo, occ)
-        if mode == "fp64":
-            idx = torch.arange(na, device=c.device)
-            rho = r64 = self.orb_density(sel("phi"), mo, occ)
-        else:
-            rho = torch.zeros(na, c.ld, dtype=torch.float64, device=c.device)
-            K.ao_density(sel("phi_lp"), self._pad(dD), rho)
-            rho += sel("rhoa")
+        idx, n = self._subset(inc)
+        if n:
+            il = None if idx is None else idx.long()
+            r = torch.zeros(n, c.ld, dtype=torch.float64, device=c.device)
+            K.ao_density(st["phi_lp"] if n < na else sel("phi_lp"),
+                         self._pad(dD if il is None else dD[il]), r, bidx=idx)
+            r += sel("rhoa") if il is None else st["rhoa"][il]
+            self._scatter(rho, idx, r)
             self.stats["rho32"] += 1
-            if idx is None:
-                return rho
-            r64 = self.orb_density(st["phi"][idx], mo[idx], occ[idx])
-            rho[idx] = r64
-        self.stats["rho64"] += 1
-        st["Da"][idx] = D[idx]
-        st["rhoa"][idx] = r64
-        st["kappa"][idx] = (r64[:, :c.N] / st["phisq"][idx]).amin(-1)
+        idx, n = self._subset(full)
+        if n:
+            il = torch.arange(na, device=c.device) if idx is None else idx.long()
+            r64 = self.orb_density(st["phi"] if n < na else sel("phi"), mo[il], occ[il], idx)
+            self._scatter(rho, idx, r64)
+            st["Da"][il] = D[il]
+            st["rhoa"][il] = r64
+            st["kappa"][il] = (r64[:, :c.N] / st["phisq"][il]).amin(-1)
+            self.stats["rho64"] += 1
         return rho
 
+    @staticmethod
+    def _scatter(dst, idx, val):
+        if idx is None:
+            dst.copy_(val)
+        else:
+            dst[idx.long()] = val
+
     def _mixed_vmat(self, st, na, v):
-        """V(v) for geometries [0, na) in the mixed mode, planned like the density:
+        """V(v) for geometries [0, na) in the mixed mode, per geometry like the density:
         FP32 V(v) outright in the large-residual phase, V_a + V(v - v_a) in FP32 where
-        the increment bound holds, FP64 (and a new anchor) elsewhere."""
+        the increment bound holds (both in one FP32 launch), FP64 (and a new anchor)
+        elsewhere."""
         c = self.c
         sel = lambda key: st[key][:na]
         err = sel("err")
@@ -586,27 +595,31 @@ class Engine:
         direct = err >= self.MIXED_DIRECT_ERR
         dv = v - sel("va")
         tol_e = self.MIXED_SAFETY * err / c.nao                 # per-element error budget
-        inc = sel("va_set") & (self.MIXED_EPS * dv.abs().amax(-1) <= tol_e)
-        mode, idx = self._mixed_plan(direct, inc, na)
+        inc = ~direct & sel("va_set") & (self.MIXED_EPS * dv.abs().amax(-1) <= tol_e)
+        low = direct | inc
         dVs = self._dv(st, na)
-        if mode == "direct":
+        V2 = torch.empty(na, c.nao, c.nao, dtype=torch.float64, device=c.device)
+        idx, n = self._subset(low)
+        if n:
+            il = None if idx is None else idx.long()
+            vv = torch.where(direct[:, None], v, dv)
+            d = dVs if not torch.is_tensor(dVs) or il is None else dVs[il]
+            V32 = self.vmat(st["phi_lp"] if n < na else sel("phi_lp"),
+                            vv if il is None else vv[il], d, idx)
+            V32 = V32 + torch.where(inc[:, None, None], sel("Va"), 0.0)[
+                slice(None) if il is None else il]
+            self._scatter(V2, idx, V32)
             self.stats["v32"] += 1
-            return self.vmat(sel("phi_lp"), v, dVs)
-        if mode == "fp64":
-            idx = torch.arange(na, device=c.device)
-            V2 = V64 = self.vmat(sel("phi"), v, dVs)
-        else:
-            V2 = sel("Va") + self.vmat(sel("phi_lp"), dv, dVs)
-            self.stats["v32"] += 1
-            if idx is None:
-                return V2
-            V64 = self.vmat(st["phi"][idx], v[idx].contiguous(),
-                            dVs[idx] if torch.is_tensor(dVs) else dVs)
-            V2[idx] = V64
-        self.stats["v64"] += 1
-        st["va"][idx] = v[idx]
-        st["Va"][idx] = V64
-        st["va_set"][idx] = True
+        idx, n = self._subset(~low)
+        if n:
+            il = torch.arange(na, device=c.device) if idx is None else idx.long()
+            d = dVs if not torch.is_tensor(dVs) else dVs[il]
+            V64 = self.vmat(st["phi"] if n < na else sel("phi"), v[il], d, idx)
+            self._scatter(V2, idx, V64)
+            st["va"][il] = v[il]
+            st["Va"][il] = V64
+            st["va_set"][il] = True
+            self.stats["v64"] += 1
         return V2
 
     def fock2e(self, st, na, D, mo, occ, full=False):

@@ -37,22 +37,22 @@ class KEngine(Engine):
         self._kwork = None
 
     # ------------------------------------------------------------- contractions
-    def vmat_dv(self, phi, v, dV):
+    def vmat_dv(self, phi, v, dV, bidx=None):
         """dV phi^H diag(v) phi (B, nao, nao) complex from stacked phi (B, 2 nao_pad, ld)."""
         c = self.c
-        B = phi.shape[0]
+        B = v.shape[0]
         n2 = 2 * c.nao_pad
         need = K.vmat_workspace(B, n2, c.ld, 64)
         if self._kwork is None or self._kwork.numel() * 8 < need:
             self._kwork = torch.empty(max(need // 8, 1), dtype=torch.float64, device=c.device)
         M = torch.empty(B, n2, n2, dtype=torch.float64, device=c.device)
-        K.ao_vmat(phi, v, dV, M, self._kwork)
+        K.ao_vmat(phi, v.contiguous(), dV, M, self._kwork, bidx)
         n, p = c.nao, c.nao_pad
         rr, ii = M[:, :n, :n], M[:, p:p + n, p:p + n]
         ri, ir = M[:, :n, p:p + n], M[:, p:p + n, :n]
         return torch.complex(rr + ii, ri - ir)
 
-    def orb_density(self, phi, mo, occ):
+    def orb_density(self, phi, mo, occ, bidx=None):
         """rho = sum_i f_i |psi_i|^2 over orbitals with f_i > OCC_CUT, psi_i = sum_p C_pi phi_p."""
         c = self.c
         B = mo.shape[0]
@@ -67,7 +67,7 @@ class KEngine(Engine):
         At[:, :n, m_pad:m_pad + keep] = Ci
         At[:, p:p + n, m_pad:m_pad + keep] = Cr
         rho = torch.zeros(B, c.ld, dtype=torch.float64, device=c.device)
-        K.orb_density(phi, At, rho)
+        K.orb_density(phi, At, rho, bidx)
         return rho
 
     def stacked_dm(self, D):

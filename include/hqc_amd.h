@@ -88,26 +88,32 @@ int hqc_ao_points(const double* xp, const double* xe, const double* kpt, int B, 
                   double L, int tmax, const double* alpha, int na, int nl, const double* coef,
                   const int* comp, int nloc, double* out, void* stream);
 
-/* Electron density on the grid and (optionally) X = D phi.
+/* Geometry selection, common to the density and potential-matrix entry points:
+ * bidx (B) int32 or NULL.  Entry b reads the AO values of geometry bidx[b] of
+ * phi (which may hold more geometries than B, e.g. a whole SCF batch, so a
+ * subset runs without copying phi); every other operand and every output is
+ * indexed by b.  NULL means bidx[b] = b.
+ *
+ * Electron density on the grid and (optionally) X = D phi.
  * Replaces the 'pr,qr,pq->r' contraction of Exc_Vxc_integral
  * (hqc/pbc/solver.py:690) and the density used by hartree_rhoG (:627).
  *   rho[b,r] = sum_pq D[b,p,q] phi[b,p,r] phi[b,q,r]    (D symmetric)
  * FP64 MFMA (v_mfma_f64_16x16x4f64).  rho (B,ld); X (B,nao_pad,ld) or NULL. */
-int hqc_ao_density_f64(const double* phi, const double* D, int B, int nao_pad,
-                       int ld, double* rho, double* X, void* stream);
+int hqc_ao_density_f64(const double* phi, const int* bidx, const double* D, int B,
+                       int nao_pad, int ld, double* rho, double* X, void* stream);
 /* FP32 operands on v_mfma_f32_16x16x4f32 with FP32 accumulation over the AO
  * index; D is rounded to FP32; rho is written in FP64. */
-int hqc_ao_density_f32(const float* phi, const double* D, int B, int nao_pad,
-                       int ld, double* rho, void* stream);
+int hqc_ao_density_f32(const float* phi, const int* bidx, const double* D, int B,
+                       int nao_pad, int ld, double* rho, void* stream);
 
 /* Density in orbital form, rho[b,r] = sum_i (sum_k At[b,k,i] phi[b,k,r])^2,
  * At = C sqrt(f) (nao_pad x m_pad, m_pad a multiple of 16, zero-padded) over
  * the orbitals with non-negligible occupation; equal to the D form above for
  * D = C f C^T.  Same reference operation (hqc/pbc/solver.py:690). */
-int hqc_orb_density_f64(const double* phi, const double* At, int B, int nao_pad, int m_pad,
-                        int ld, double* rho, void* stream);
-int hqc_orb_density_f32(const float* phi, const double* At, int B, int nao_pad, int m_pad,
-                        int ld, double* rho, void* stream);
+int hqc_orb_density_f64(const double* phi, const int* bidx, const double* At, int B,
+                        int nao_pad, int m_pad, int ld, double* rho, void* stream);
+int hqc_orb_density_f32(const float* phi, const int* bidx, const double* At, int B,
+                        int nao_pad, int m_pad, int ld, double* rho, void* stream);
 
 /* Grid potential matrix V[b] = dV * phi diag(v[b]) phi^T (symmetric, full).
  * Replaces the 'pr,qr,r->pq' contraction (hqc/pbc/solver.py:693) and, through
@@ -115,12 +121,12 @@ int hqc_orb_density_f32(const float* phi, const double* At, int B, int nao_pad, 
  * J (:628).  FP64 MFMA; per-chunk partial tiles are reduced in a fixed order
  * (deterministic).  work: device scratch of hqc_ao_vmat_workspace bytes. */
 int hqc_ao_vmat_workspace(int B, int nao_pad, int ld, int prec, size_t* bytes);
-int hqc_ao_vmat_f64(const double* phi, const double* v, int B, int nao_pad, int ld,
-                    double dV, double* V, void* work, size_t work_bytes, void* stream);
+int hqc_ao_vmat_f64(const double* phi, const int* bidx, const double* v, int B, int nao_pad,
+                    int ld, double dV, double* V, void* work, size_t work_bytes, void* stream);
 /* FP32 phi and v on v_mfma_f32_16x16x4f32 with FP32 accumulation inside a
  * block, FP64 accumulation across blocks; V written in FP64. */
-int hqc_ao_vmat_f32(const float* phi, const double* v, int B, int nao_pad, int ld,
-                    double dV, double* V, void* work, size_t work_bytes, void* stream);
+int hqc_ao_vmat_f32(const float* phi, const int* bidx, const double* v, int B, int nao_pad,
+                    int ld, double dV, double* V, void* work, size_t work_bytes, void* stream);
 
 /* Batched in-place 3-D complex FFT of B grids g^3 (any g <= 128; mixed-radix
  * Stockham passes in LDS, one launch per axis).  sign = -1 forward,

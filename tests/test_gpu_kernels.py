@@ -375,3 +375,39 @@ def test_potential_energy_protons_electrons(K):
         ref = R.potential_energy(x[w], 10.0, G, L, rs)
         for a, r in zip((vpp[w], vep[w], vee[w]), ref):
             assert abs(a.item() - r) < 1e-11 * max(1.0, abs(r))
+
+
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_geometry_index_lists(K, dt):
+    """Density, orbital density and potential matrix on a subset of the batch read in
+    place through bidx equal the same kernels on a gathered copy."""
+    torch.manual_seed(5)
+    B, nao_pad, g = 5, 48, 9
+    N = g ** 3
+    ld = (N + 63) // 64 * 64
+    phi = torch.zeros(B, nao_pad, ld, dtype=torch.float64, device=DEV)
+    phi[:, :, :N] = torch.randn(B, nao_pad, N, dtype=torch.float64, device=DEV)
+    phi = phi.to(dt).contiguous()
+    idx = torch.tensor([3, 0, 4], dtype=torch.int32, device=DEV)
+    sub = phi[idx.long()].contiguous()
+    A = torch.randn(3, nao_pad, nao_pad, dtype=torch.float64, device=DEV)
+    D = (A + A.transpose(1, 2)).contiguous()
+    r1 = torch.zeros(3, ld, dtype=torch.float64, device=DEV)
+    r2 = torch.zeros_like(r1)
+    K.ao_density(phi, D, r1, bidx=idx)
+    K.ao_density(sub, D, r2)
+    assert torch.equal(r1, r2)
+    At = torch.randn(3, nao_pad, 32, dtype=torch.float64, device=DEV)
+    K.orb_density(phi, At, r1, bidx=idx)
+    K.orb_density(sub, At, r2)
+    assert torch.equal(r1, r2)
+    v = torch.zeros(3, ld, dtype=torch.float64, device=DEV)
+    v[:, :N] = torch.randn(3, N, dtype=torch.float64, device=DEV)
+    V1 = torch.empty(3, nao_pad, nao_pad, dtype=torch.float64, device=DEV)
+    V2 = torch.empty_like(V1)
+    prec = 64 if dt == torch.float64 else 32
+    work = torch.empty(K.vmat_workspace(3, nao_pad, ld, prec) // 8 + 1, dtype=torch.float64,
+                       device=DEV)
+    K.ao_vmat(phi, v, 0.7, V1, work, bidx=idx)
+    K.ao_vmat(sub, v, 0.7, V2, work)
+    assert torch.equal(V1, V2)
