@@ -1,6 +1,8 @@
 // Batched 3-D complex FFT for the odd (and even) grids of the GPW solver.
 //
-// One launch per axis.  A block stages `LPB` lines of length g in LDS
+// Up to g = 69 the y and z axes are done together on whole planes in LDS
+// (k_fft_plane), then the x axis by lines; larger grids go axis by axis.  A
+// line block stages `LPB` lines of length g in LDS
 // (position-major, line-minor so that strided axes load coalesced), runs the
 // mixed-radix Stockham passes between two LDS buffers, and writes the lines
 // back in place.  Radix butterflies are specialised for 2,3,4,5,7,11,13; any
@@ -21,10 +23,11 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Stockham pass with compile-time radix R (see the header comment).
+// Stockham pass with compile-time radix R (see the header comment).  Element
+// (position p, line) of the lpb lines lives at p*ps + line*ls in LDS.
 template <int R>
 __device__ void pass_r(const double2* __restrict__ in, double2* __restrict__ out,
-                       const double2* __restrict__ tw, int N, int Ns, int lpb) {
+                       const double2* __restrict__ tw, int N, int Ns, int lpb, int ps, int ls) {
   const int NR = N / R;
   const int step = N / (Ns * R);
   for (int it = threadIdx.x; it < NR * lpb; it += blockDim.x) {
@@ -32,7 +35,7 @@ __device__ void pass_r(const double2* __restrict__ in, double2* __restrict__ out
     const int k = j % Ns;
     double2 v[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = in[(j + r * NR) * lpb + line];
+    for (int r = 0; r < R; ++r) v[r] = in[(j + r * NR) * ps + line * ls];
 #pragma unroll
     for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[k * r * step]);
     const int idx = (j / Ns) * Ns * R + k;
@@ -45,14 +48,15 @@ __device__ void pass_r(const double2* __restrict__ in, double2* __restrict__ out
         acc.x += v[r].x * wr.x - v[r].y * wr.y;
         acc.y += v[r].x * wr.y + v[r].y * wr.x;
       }
-      out[(idx + kk * Ns) * lpb + line] = acc;
+      out[(idx + kk * Ns) * ps + line * ls] = acc;
     }
   }
 }
 
 // Generic radix: one work item per output element.
 __device__ void pass_generic(const double2* __restrict__ in, double2* __restrict__ out,
-                             const double2* __restrict__ tw, int N, int Ns, int R, int lpb) {
+                             const double2* __restrict__ tw, int N, int Ns, int R, int lpb,
+                             int ps, int ls) {
   const int NR = N / R;
   const int step = N / (Ns * R);
   for (int it = threadIdx.x; it < N * lpb; it += blockDim.x) {
@@ -62,15 +66,83 @@ __device__ void pass_generic(const double2* __restrict__ in, double2* __restrict
     const int k = j % Ns;
     double2 acc = make_double2(0.0, 0.0);
     for (int r = 0; r < R; ++r) {
-      double2 x = in[(j + r * NR) * lpb + line];
+      double2 x = in[(j + r * NR) * ps + line * ls];
       if (r) x = cmul(x, tw[k * r * step]);
       const double2 w = tw[((r * kk) % R) * NR];
       acc.x += x.x * w.x - x.y * w.y;
       acc.y += x.x * w.y + x.y * w.x;
     }
     const int idx = (j / Ns) * Ns * R + k;
-    out[(idx + kk * Ns) * lpb + line] = acc;
+    out[(idx + kk * Ns) * ps + line * ls] = acc;
   }
+}
+
+// A whole transform of prime length N > 13 in one pass (direct DFT, the only
+// stage).  Work item = (line, block of PR outputs): the inputs of the line are
+// read once each and accumulated into PR outputs; the twiddle index of output
+// q after input r is (r * q) mod N, stepped by q with one conditional subtract.
+#define FFT_PRIME_PR 8
+
+__device__ void pass_prime(const double2* __restrict__ in, double2* __restrict__ out,
+                           const double2* __restrict__ tw, int N, int lpb, int ps, int ls) {
+  const int nblk = (N + FFT_PRIME_PR - 1) / FFT_PRIME_PR;
+  for (int it = threadIdx.x; it < nblk * lpb; it += blockDim.x) {
+    const int line = it % lpb, q0 = (it / lpb) * FFT_PRIME_PR;
+    double2 acc[FFT_PRIME_PR];
+    int idx[FFT_PRIME_PR];
+#pragma unroll
+    for (int j = 0; j < FFT_PRIME_PR; ++j) {
+      acc[j] = make_double2(0.0, 0.0);
+      idx[j] = 0;
+    }
+    for (int r = 0; r < N; ++r) {
+      const double2 x = in[r * ps + line * ls];
+#pragma unroll
+      for (int j = 0; j < FFT_PRIME_PR; ++j) {
+        const double2 w = tw[idx[j]];
+        acc[j].x += x.x * w.x - x.y * w.y;
+        acc[j].y += x.x * w.y + x.y * w.x;
+        // (r + 1) q mod N for q = q0 + j; outputs past N (discarded) keep q = 0
+        const int q = q0 + j < N ? q0 + j : 0;
+        const int t = idx[j] + q;
+        idx[j] = t >= N ? t - N : t;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < FFT_PRIME_PR; ++j)
+      if (q0 + j < N) out[(q0 + j) * ps + line * ls] = acc[j];
+  }
+}
+
+// All Stockham passes of one transform of length N over lpb lines; returns the
+// buffer that holds the result (in or out).
+__device__ double2* run_passes(double2* in, double2* out, const double2* __restrict__ tw,
+                               const FftPlan& plan, int N, int lpb, int ps, int ls) {
+  if (plan.nst == 1 && plan.rad[0] > 13) {
+    pass_prime(in, out, tw, N, lpb, ps, ls);
+    __syncthreads();
+    return out;
+  }
+  int Ns = 1;
+  for (int s = 0; s < plan.nst; ++s) {
+    const int R = plan.rad[s];
+    switch (R) {
+      case 2: pass_r<2>(in, out, tw, N, Ns, lpb, ps, ls); break;
+      case 3: pass_r<3>(in, out, tw, N, Ns, lpb, ps, ls); break;
+      case 4: pass_r<4>(in, out, tw, N, Ns, lpb, ps, ls); break;
+      case 5: pass_r<5>(in, out, tw, N, Ns, lpb, ps, ls); break;
+      case 7: pass_r<7>(in, out, tw, N, Ns, lpb, ps, ls); break;
+      case 11: pass_r<11>(in, out, tw, N, Ns, lpb, ps, ls); break;
+      case 13: pass_r<13>(in, out, tw, N, Ns, lpb, ps, ls); break;
+      default: pass_generic(in, out, tw, N, Ns, R, lpb, ps, ls); break;
+    }
+    __syncthreads();
+    double2* t = in;
+    in = out;
+    out = t;
+    Ns *= R;
+  }
+  return in;
 }
 
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(double2* __restrict__ data, int g,
@@ -107,27 +179,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(double2* __restrict__ 
     bufA[p * lpb + line] = val;
   }
   __syncthreads();
-  double2* in = bufA;
-  double2* out = bufB;
-  int Ns = 1;
-  for (int s = 0; s < plan.nst; ++s) {
-    const int R = plan.rad[s];
-    switch (R) {
-      case 2: pass_r<2>(in, out, tw, g, Ns, lpb); break;
-      case 3: pass_r<3>(in, out, tw, g, Ns, lpb); break;
-      case 4: pass_r<4>(in, out, tw, g, Ns, lpb); break;
-      case 5: pass_r<5>(in, out, tw, g, Ns, lpb); break;
-      case 7: pass_r<7>(in, out, tw, g, Ns, lpb); break;
-      case 11: pass_r<11>(in, out, tw, g, Ns, lpb); break;
-      case 13: pass_r<13>(in, out, tw, g, Ns, lpb); break;
-      default: pass_generic(in, out, tw, g, Ns, R, lpb); break;
-    }
-    __syncthreads();
-    double2* t = in;
-    in = out;
-    out = t;
-    Ns *= R;
-  }
+  double2* in = run_passes(bufA, bufB, tw, plan, g, lpb, lpb, 1);
   for (int e = threadIdx.x; e < g * lpb; e += blockDim.x) {
     int p, line;
     if (axis == 2) {
@@ -143,6 +195,36 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(double2* __restrict__ 
       base[lb + p * stride] = in[p * lpb + line];
     }
   }
+}
+
+// The two contiguous axes in one launch: a workgroup loads the whole (y, z)
+// plane of one x (g^2 contiguous points, coalesced), transforms along z
+// (lines y: element (z, y) at y*g + z) and then along y (lines z: element
+// (y, z) at y*g + z) in LDS, and writes the plane back.  Together with the
+// strided x pass this reads and writes the grid twice instead of three times.
+#define FFT_PLANE_THREADS 1024
+#define FFT_PLANE_MAX_LDS (156 * 1024)
+
+__global__ __launch_bounds__(FFT_PLANE_THREADS) void k_fft_plane(double2* __restrict__ data,
+                                                                 int g, FftPlan plan,
+                                                                 int sign) {
+  extern __shared__ double2 sm[];
+  const int gg = g * g;
+  double2* bufA = sm;
+  double2* bufB = sm + gg;
+  double2* tw = bufB + gg;
+  double2* plane = data + ((long)blockIdx.y * g + blockIdx.x) * gg;
+  for (int m = threadIdx.x; m < g; m += blockDim.x) {
+    double s, c;
+    sincospi((double)(2 * m) / (double)g, &s, &c);
+    tw[m] = make_double2(c, sign < 0 ? -s : s);
+  }
+  for (int e = threadIdx.x; e < gg; e += blockDim.x) bufA[e] = plane[e];
+  __syncthreads();
+  double2* r = run_passes(bufA, bufB, tw, plan, g, g, 1, g);          // along z
+  double2* o = (r == bufA) ? bufB : bufA;
+  r = run_passes(r, o, tw, plan, g, g, g, 1);                          // along y
+  for (int e = threadIdx.x; e < gg; e += blockDim.x) plane[e] = r[e];
 }
 
 static int make_plan(int g, FftPlan& plan) {
@@ -172,11 +254,27 @@ extern "C" int hqc_fft3d_c2c_f64(double* data, int B, int g, int sign, void* str
   if (B == 0 || g == 1) return HQC_OK;
   FftPlan plan;
   if (make_plan(g, plan) != HQC_OK) return HQC_E_SIZE;
-  const int lpb = 16;
   const long lines = (long)g * g;
-  const size_t lds = (size_t)(2 * g * lpb + g) * sizeof(double2);
-  dim3 grid(hqc_div_up(lines, lpb), B);
-  for (int axis = 2; axis >= 0; --axis) {
+  const size_t plane_lds = (size_t)(2 * g * g + g) * sizeof(double2);
+  int first_axis = 2;
+  if (plane_lds <= FFT_PLANE_MAX_LDS) {
+    if (plane_lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void*)k_fft_plane, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)plane_lds) != hipSuccess)
+      (void)hipGetLastError();
+    hipLaunchKernelGGL(k_fft_plane, dim3(g, B), dim3(FFT_PLANE_THREADS), plane_lds,
+                       (hipStream_t)stream, (double2*)data, g, plan, sign);
+    HQC_CHECK_LAUNCH();
+    first_axis = 0;
+  }
+  // remaining axes line by line; more lines per block on the strided x axis
+  for (int axis = first_axis; axis >= 0; --axis) {
+    // lines per block: 32 on the strided x axis (longer coalesced segments), 16
+    // otherwise, and never more than 64 KB of LDS
+    const int fit = (int)((64 * 1024 / sizeof(double2) - g) / (2 * g));
+    const int lpb = max(1, min(axis == 0 ? 32 : 16, fit));
+    const size_t lds = (size_t)(2 * g * lpb + g) * sizeof(double2);
+    dim3 grid(hqc_div_up(lines, lpb), B);
     hipLaunchKernelGGL(k_fft_axis, grid, dim3(FFT_THREADS), lds, (hipStream_t)stream,
                        (double2*)data, g, axis, lpb, plan, sign);
     HQC_CHECK_LAUNCH();

@@ -107,7 +107,8 @@ def test_orb_density(K, nao_pad, m, dt):
     assert torch.allclose(rho, rref, rtol=tol, atol=tol * rref.abs().max().item())
 
 
-@pytest.mark.parametrize("g", [1, 2, 5, 7, 13, 24, 25, 27, 33, 48, 49, 55, 61, 64, 65])
+@pytest.mark.parametrize("g", [1, 2, 5, 7, 13, 17, 24, 25, 27, 31, 33, 48, 49, 51, 55, 57, 61,
+                               64, 65, 69, 71, 97, 127, 128])
 def test_fft3d(K, g):
     print(f"[fft3d] g={g}", flush=True)
     torch.manual_seed(g)
