@@ -1,4 +1,5 @@
-// AO x grid contractions on MFMA: the two per-iteration hot kernels.
+// AO x grid contractions: the per-iteration hot kernels (MFMA, and the vector
+// ALUs for the FP64 orbital-form density).
 //
 //   density:  rho(r) = sum_pq D_pq phi_p(r) phi_q(r)          (solver.py:690)
 //   vmat:     V_pq   = dV sum_r phi_p(r) v(r) phi_q(r)        (solver.py:693)
@@ -282,6 +283,95 @@ __global__ __launch_bounds__(256) void k_orb_density_lds(const T* __restrict__ p
   }
 }
 
+// FP64 orbital density on the vector ALUs.  On gfx950 a stream of v_fma_f64
+// sustains more FP64 throughput than the 16x16x4 f64 MFMA (measured: this
+// kernel 8.5 ms against 10.9 ms for the MFMA + LDS form at 54 H dzv, batch
+// 256), and the left operand sqrt(f_i) C_pi is the same for every lane of a
+// wave: it is read with scalar loads and enters each FMA as the scalar operand,
+// so it costs neither LDS bandwidth nor vector registers.  Workgroup = NW waves
+// over the same 64*VD_COLS grid columns (stride 64 between a lane's columns, so
+// every load is one coalesced 512-byte row segment); wave w owns orbitals
+// [mg + MG w, mg + MG (w + 1)), MG | m_pad.  The waves read the same phi rows, so
+// HBM delivers phi once and L1 serves the other reads.  The per-wave partial
+// sums sum_i psi_i^2 are added through LDS in a fixed order.
+#define VD_COLS 2
+#define VD_PF 4
+
+template <int MG, int NW>
+__global__ __launch_bounds__(64 * NW) void k_orb_density_valu(const double* __restrict__ phi,
+                                                              const int* __restrict__ bidx,
+                                                              const double* __restrict__ At,
+                                                              int nao_pad, int m_pad, int ld,
+                                                              double* __restrict__ rho) {
+  constexpr int NC = 64 * VD_COLS;              // columns per workgroup
+  __shared__ double red[NW][NC];
+  const int b = blockIdx.y;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const long col0 = (long)blockIdx.x * NC;
+  const double* P = phi + (long)(bidx ? bidx[b] : b) * nao_pad * ld + col0 + lane;
+  const double* Ab = At + (long)b * nao_pad * m_pad;
+  // a column block past ld (last workgroup only) re-reads block 0 instead, so
+  // every load stays inside its row; those columns are never stored
+  int off[VD_COLS];
+#pragma unroll
+  for (int j = 0; j < VD_COLS; ++j) off[j] = (col0 + 64 * j < ld) ? 64 * j : 0;
+  double rs[VD_COLS];
+#pragma unroll
+  for (int j = 0; j < VD_COLS; ++j) rs[j] = 0.0;
+  const int klast = nao_pad - 1;
+  for (int mg = 0; mg < m_pad; mg += NW * MG) {
+    const int o0 = mg + w * MG;
+    if (o0 >= m_pad) continue;                  // wave-uniform
+    const double* A = Ab + o0;
+    double acc[MG][VD_COLS];
+#pragma unroll
+    for (int i = 0; i < MG; ++i)
+#pragma unroll
+      for (int j = 0; j < VD_COLS; ++j) acc[i][j] = 0.0;
+    // nao_pad is a multiple of 16, so the k loop runs whole groups of VD_PF;
+    // the loads past the last row re-read the last row (values unused)
+    double pb[VD_PF][VD_COLS];
+#pragma unroll
+    for (int q = 0; q < VD_PF; ++q)
+#pragma unroll
+      for (int j = 0; j < VD_COLS; ++j) pb[q][j] = P[(long)q * ld + off[j]];
+    for (int k0 = 0; k0 < nao_pad; k0 += VD_PF) {
+#pragma unroll
+      for (int q = 0; q < VD_PF; ++q) {
+        const int k = k0 + q;
+        const int kp = min(k + VD_PF, klast);
+        double a[MG];
+#pragma unroll
+        for (int i = 0; i < MG; ++i) a[i] = A[(long)k * m_pad + i];
+        double p[VD_COLS];
+#pragma unroll
+        for (int j = 0; j < VD_COLS; ++j) {
+          p[j] = pb[q][j];
+          pb[q][j] = P[(long)kp * ld + off[j]];
+        }
+#pragma unroll
+        for (int i = 0; i < MG; ++i)
+#pragma unroll
+          for (int j = 0; j < VD_COLS; ++j) acc[i][j] = fma(a[i], p[j], acc[i][j]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < MG; ++i)
+#pragma unroll
+      for (int j = 0; j < VD_COLS; ++j) rs[j] = fma(acc[i][j], acc[i][j], rs[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < VD_COLS; ++j) red[w][lane + 64 * j] = rs[j];
+  __syncthreads();
+  for (int t = threadIdx.x; t < NC; t += 64 * NW) {
+    double s = 0.0;
+#pragma unroll
+    for (int v = 0; v < NW; ++v) s += red[v][t];
+    if (col0 + t < ld) rho[(long)b * ld + col0 + t] = s;
+  }
+}
+
 template <typename T>
 static int launch_orb_density(const T* phi, const int* bidx, const double* At, int B, int nao_pad, int m_pad,
                               int ld, double* rho, void* stream) {
@@ -289,6 +379,18 @@ static int launch_orb_density(const T* phi, const int* bidx, const double* At, i
       (ld & 63))
     return HQC_E_ARG;
   if (B == 0) return HQC_OK;
+  if (sizeof(T) == 8) {
+    const dim3 grid((ld + 64 * VD_COLS - 1) / (64 * VD_COLS), B);
+    const double* ph = reinterpret_cast<const double*>(phi);
+    if (m_pad % 32 == 0)
+      hipLaunchKernelGGL((k_orb_density_valu<32, 2>), grid, dim3(128), 0, (hipStream_t)stream, ph,
+                         bidx, At, nao_pad, m_pad, ld, rho);
+    else
+      hipLaunchKernelGGL((k_orb_density_valu<16, 4>), grid, dim3(256), 0, (hipStream_t)stream, ph,
+                         bidx, At, nao_pad, m_pad, ld, rho);
+    HQC_CHECK_LAUNCH();
+    return HQC_OK;
+  }
   {
     const int mt = min(m_pad >> 4, 4);
     const size_t lds = (size_t)nao_pad * mt * 16 * sizeof(T);

@@ -137,12 +137,18 @@ class Engine:
     and V are taken in FP32 outright.  The choice is per geometry: the kernels
     read a subset of the batch in place through a geometry index list, so each
     precision class costs one launch over its own geometries.  Eigensolves,
-    DIIS, occupations, FFTs, xc and every energy sum stay FP64."""
+    DIIS, occupations, FFTs, xc and every energy sum stay FP64.
+
+    Density increments are off by default (MIXED_RHO_INCREMENTS): the D form
+    costs 2 n_ao^2 flops per grid point against 2 n_ao n_occ for the FP64
+    orbital form, and with n_occ ~ n_ao / 2 the FP32 MFMA rate (2x FP64) does
+    not pay that back, so past the warm-up rho is taken in FP64 orbital form."""
 
     MIXED_EPS = 2e-6        # effective relative error of an FP32-accumulated increment
     MIXED_SAFETY = 1e-2     # potential increment: error per element <= 1e-2 residual / n_ao
     MIXED_RHO_SAFETY = 1.0  # density increment: |v| x relative rho error <= residual
     MIXED_DIRECT_ERR = 1e-1  # residual above which rho and V are taken in FP32 outright
+    MIXED_RHO_INCREMENTS = False  # FP32 D-form density increments (else FP64 orbital form)
 
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
@@ -545,6 +551,8 @@ class Engine:
         # bound on the relative density error of the FP32 increment
         rel = self.MIXED_EPS * torch.linalg.norm(dD, dim=(-2, -1)) / sel("kappa")
         inc = ~direct & (rel <= 1e-2) & (rel * sel("vmax") <= self.MIXED_RHO_SAFETY * err)
+        if not self.MIXED_RHO_INCREMENTS:
+            inc = torch.zeros_like(inc)
         full = ~(direct | inc)
         rho = torch.empty(na, c.ld, dtype=torch.float64, device=c.device)
         idx, n = self._subset(direct)

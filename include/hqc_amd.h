@@ -109,7 +109,9 @@ int hqc_ao_density_f32(const float* phi, const int* bidx, const double* D, int B
 /* Density in orbital form, rho[b,r] = sum_i (sum_k At[b,k,i] phi[b,k,r])^2,
  * At = C sqrt(f) (nao_pad x m_pad, m_pad a multiple of 16, zero-padded) over
  * the orbitals with non-negligible occupation; equal to the D form above for
- * D = C f C^T.  Same reference operation (hqc/pbc/solver.py:690). */
+ * D = C f C^T.  Same reference operation (hqc/pbc/solver.py:690).
+ * f64: FP64 FMAs on the vector ALUs with At read through scalar loads;
+ * f32: FP32 phi and At on v_mfma_f32_16x16x4f32, rho accumulated in FP64. */
 int hqc_orb_density_f64(const double* phi, const int* bidx, const double* At, int B,
                         int nao_pad, int m_pad, int ld, double* rho, void* stream);
 int hqc_orb_density_f32(const float* phi, const int* bidx, const double* At, int B,

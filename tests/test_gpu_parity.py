@@ -107,9 +107,11 @@ def test_batched_equals_single():
         assert abs(pes(xps[b]).item() - Eb[b]) < 1e-10
 
 
-def test_mixed_precision_matches_fp64():
+@pytest.mark.parametrize("rho_inc", [False, True], ids=["rho_orbital64", "rho_increments32"])
+def test_mixed_precision_matches_fp64(rho_inc):
     """Defect-corrected FP32 increments converge to the FP64 answer within the
-    north-star thresholds (dE <= 1 uHa/atom, dF <= 1e-5 Ha/Bohr, dbands <= 1e-6 Ha)."""
+    accuracy bar (dE <= 1 uHa/atom, dF <= 1e-5 Ha/Bohr, dbands <= 1e-6 Ha), with
+    the density past the warm-up in FP64 orbital form or as FP32 D-form increments."""
     from hqc_amd.pbc import make_pes
     n, rs = 16, 1.25
     L = (4 / 3 * np.pi * n) ** (1 / 3)
@@ -117,6 +119,7 @@ def test_mixed_precision_matches_fp64():
     kw = dict(grid_length=0.2, dft=True, smearing=True, smearing_sigma=0.0316681, tol=1e-8)
     p64 = make_pes(n, L, rs, "gth-dzv", precision="fp64", **kw)
     pmx = make_pes(n, L, rs, "gth-dzv", precision="mixed", **kw)
+    pmx.solver.engine.MIXED_RHO_INCREMENTS = rho_inc
     E64, F64 = p64.energy_and_forces(xps)
     Emx, Fmx = pmx.energy_and_forces(xps)
     assert bool(pmx.solver.last["conv"].all())
