@@ -42,6 +42,7 @@ GRID_LENGTH = 0.16
 TOL = 1e-8
 # dense MFMA TFLOP/s; MI355X's FP64 vector FMA peak is the same 78.6 TFLOP/s
 MI355X_PEAK = {"f64": 78.6, "f32": 157.3}
+MI355X_HBM_GBS = 8000.0
 
 
 def workload_xp(B, seed0=0):
@@ -58,16 +59,33 @@ def make(precision):
                     search_method="bisect", tol=TOL, precision=precision)
 
 
+def vmat_kind(pes):
+    """Which potential-matrix kernel the path's low-precision (or only) V runs on."""
+    eng = pes.solver.engine
+    if eng.precision == "fp64":
+        return "f64"
+    return "h2" if eng.MIXED_SPLIT16 and eng.c.nao_pad <= 128 else "f32"
+
+
 def time_vmat(pes, B, reps=20):
     """Average duration of the AO x grid potential-matrix kernel at the workload
-    shape, from HIP events on the stream it is launched on (torch's current)."""
+    shape, from HIP events on the stream it is launched on (torch's current):
+    FP64 for the FP64 path, the FP16 split-product kernel (or FP32) for the mixed
+    path's increments."""
     import torch
+    from hqc_amd import _lib as K
     eng = pes.solver.engine
     c = eng.c
-    dt = torch.float64 if eng.precision == "fp64" else torch.float32
-    phi = torch.rand(B, c.nao_pad, c.ld, dtype=torch.float64, device="cuda").to(dt)
+    kind = vmat_kind(pes)
+    phi = torch.rand(B, c.nao_pad, c.ld, dtype=torch.float64, device="cuda")
     phi[:, c.nao:] = 0
     phi[:, :, c.N:] = 0
+    if kind == "h2":
+        ph = torch.empty(B, c.nao_pad, c.ld, dtype=torch.int32, device="cuda")
+        K.phi_split_f16(phi, eng._phi_scale, ph)
+        phi = ph
+    elif kind == "f32":
+        phi = phi.float()
     v = torch.zeros(B, c.ld, dtype=torch.float64, device="cuda")
     v[:, :c.N] = torch.rand(B, c.N, dtype=torch.float64, device="cuda")
     for _ in range(3):
@@ -193,9 +211,18 @@ def kernel_probe(name, B, reps=5):
             dst.copy_(src)
         torch.cuda.synchronize()
         return
-    pes = make("mixed" if name == "vmat_f32" else "fp64")
+    pes = make("mixed" if name in ("vmat_f32", "vmat_h2") else "fp64")
     c = pes.solver.engine.c
-    if name in ("vmat_f64", "vmat_f32"):
+    if name == "vmat_h2":
+        phi = torch.rand(B, c.nao_pad, c.ld, dtype=torch.float64, device="cuda")
+        ph = torch.empty(B, c.nao_pad, c.ld, dtype=torch.int32, device="cuda")
+        pes.solver.engine._phi_scale = 2.0 ** 13         # |phi| < 1
+        K.phi_split_f16(phi, pes.solver.engine._phi_scale, ph)
+        del phi
+        v = torch.rand(B, c.ld, dtype=torch.float64, device="cuda")
+        for _ in range(reps):
+            pes.solver.engine.vmat(ph, v)
+    elif name in ("vmat_f64", "vmat_f32"):
         dt = torch.float64 if name == "vmat_f64" else torch.float32
         phi = torch.rand(B, c.nao_pad, c.ld, dtype=torch.float64, device="cuda").to(dt)
         v = torch.rand(B, c.ld, dtype=torch.float64, device="cuda")
@@ -264,7 +291,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP64 batched and plain runs (accuracy and speedups)")
     ap.add_argument("--kernel-probe", default=None,
-                    help="vmat_f64|vmat_f32|orb_density|copy: launch that kernel only")
+                    help="vmat_f64|vmat_f32|vmat_h2|orb_density|copy: launch that kernel only")
     ap.add_argument("--summarize-trace", default=None, metavar="DIR")
     ap.add_argument("--summarize-pmc", default=None, metavar="DIR")
     a = ap.parse_args()
@@ -295,25 +322,40 @@ def main():
     stats = dict(eng.stats)
     sweeps = eng.sweeps.item() / eng.eigh_calls if eng.eigh_calls else None
     # dominant kernels: potential matrix (FP64 or FP32) and FP64 orbital density
+    kind = vmat_kind(pes)
     ms_v, fl_v = time_vmat(pes, B)
     ms_d, fl_d, keep = time_orb_density(pes, B)
-    prec_v = "f64" if a.path == "fp64" else "f32"
     n_vmat = stats["v64"] if a.path == "fp64" else stats["v32"]
     n_dens64 = stats["rho64"]
-    vname = "k_vmat_pipe<" + ("double" if prec_v == "f64" else "float")
+    nsteps = max(a.steps + a.warmup, 1)
+    c = eng.c
+    if kind == "h2":
+        # FP16 pairs (4 bytes per phi value) + v (8 bytes per point) per launch: HBM roof
+        vname = "k_vmat_h2"
+        vk = {"kernel": "k_vmat_h2 (dV phi diag(v) phi^T from FP16 hi/lo split products, "
+                        "FP32 accumulation)", "bound": "hbm", "unit": "GB/s",
+              "bytes_per_launch": B * c.ld * (4.0 * c.nao_pad + 8.0),
+              "peak": MI355X_HBM_GBS}
+    else:
+        prec_v = "f64" if kind == "f64" else "f32"
+        vname = "k_vmat_pipe<" + ("double" if prec_v == "f64" else "float")
+        vk = {"kernel": vname + "> (dV phi diag(v) phi^T, symmetric: n(n+1)N flops)",
+              "bound": "mfma", "unit": "TFLOP/s", "peak": MI355X_PEAK[prec_v]}
+    vk.update({"ms_per_launch": ms_v, "flops_per_launch": fl_v,
+               "calls_per_step": n_vmat / nsteps, "traffic": measured_traffic(vname, B)})
     cand = [
-        {"kernel": vname + "> (dV phi diag(v) phi^T, symmetric: n(n+1)N flops)",
-         "ms_per_launch": ms_v,
-         "flops_per_launch": fl_v, "peak": MI355X_PEAK[prec_v],
-         "calls_per_step": n_vmat / max(a.steps + a.warmup, 1),
-         "traffic": measured_traffic(vname, B)},
+        vk,
         {"kernel": "k_orb_density_valu (sum_i f_i (C^T phi)_i^2, FP64 vector FMA)",
+         "bound": "mfma", "unit": "TFLOP/s",
          "ms_per_launch": ms_d, "flops_per_launch": fl_d, "peak": MI355X_PEAK["f64"],
-         "orbitals": keep, "calls_per_step": n_dens64 / max(a.steps + a.warmup, 1),
+         "orbitals": keep, "calls_per_step": n_dens64 / nsteps,
          "traffic": measured_traffic("k_orb_density_valu", B)},
     ]
     for k in cand:
-        k["achieved"] = k["flops_per_launch"] / (k["ms_per_launch"] * 1e-3) / 1e12
+        if k["unit"] == "GB/s":
+            k["achieved"] = k["bytes_per_launch"] / (k["ms_per_launch"] * 1e-3) / 1e9
+        else:
+            k["achieved"] = k["flops_per_launch"] / (k["ms_per_launch"] * 1e-3) / 1e12
         k["frac"] = k["achieved"] / k["peak"]
     dom = max(cand, key=lambda k: k["ms_per_launch"] * k["calls_per_step"])
     w_main = st["w1"].clone()
@@ -323,7 +365,8 @@ def main():
         "metric": METRIC, "value": B / dt, "unit": "geometries/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.path == "fp64" else "f64+f32(increments)",
+        "dtype": "f64" if a.path == "fp64" else
+                 ("f64+f16x2(split increments)" if kind == "h2" else "f64+f32(increments)"),
         "data": "synthetic: uniform-random protons, numpy default_rng(seed), seeds 0..B-1",
         "config": {"workload": f"54 H rs=1.25 gth-dzv LDA(vwn) Fermi T=1e4K, grid 49^3 "
                                f"(grid_length 0.16), tol 1e-8, energy+forces, batch {B}",
@@ -331,8 +374,8 @@ def main():
                    "contractions": stats, "jacobi_sweeps_per_eigh": sweeps},
         # compute-bound roof: MFMA for the potential matrix, FP64 vector FMAs (same
         # 78.6 TFLOP/s peak) for the orbital-form density
-        "roofline": {"bound": "mfma", "kernel": dom["kernel"], "achieved": dom["achieved"],
-                     "peak": dom["peak"], "unit": "TFLOP/s", "frac": dom["frac"],
+        "roofline": {"bound": dom["bound"], "kernel": dom["kernel"], "achieved": dom["achieved"],
+                     "peak": dom["peak"], "unit": dom["unit"], "frac": dom["frac"],
                      "traffic": dom["traffic"], "ms_per_launch": dom["ms_per_launch"],
                      "flops_per_launch": dom["flops_per_launch"]},
         "kernels": cand,

@@ -32,6 +32,9 @@ _SIGS = {
     "hqc_ao_vmat_workspace": [_I, _I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
     "hqc_ao_vmat_f64": [_P, _P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_ao_vmat_f32": [_P, _P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
+    "hqc_phi_split_f16": [_P, ctypes.c_long, _D, _P, _P],
+    "hqc_ao_vmat_h2_workspace": [_I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
+    "hqc_ao_vmat_h2": [_P, _P, _P, _I, _I, _I, _D, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
     "hqc_pair_pack": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_split_f64": [_P, _I, _I, _I, _I, _I, _P, _P],
@@ -213,6 +216,31 @@ def ao_vmat(phi, v, dV, V, work, bidx=None):
     name = "hqc_ao_vmat_f64" if phi.dtype == F64 else "hqc_ao_vmat_f32"
     _call(name, _ptr(phi, None, "phi"), _ptr(bi, I32, "bidx"), _ptr(v, F64, "v"), B, nao_pad, ld,
           float(dV), _ptr(V, F64, "V"), _ptr(work, None, "work"),
+          work.numel() * work.element_size(), _stream())
+
+
+def phi_split_f16(phi, scale, out):
+    """out (int32 view of FP16 pairs) <- (hi, lo) with scale*phi = hi + lo."""
+    assert phi.dtype == F64 and out.dtype == I32 and out.shape == phi.shape
+    assert phi.is_contiguous() and out.is_contiguous()
+    _call("hqc_phi_split_f16", _ptr(phi, F64, "phi"), phi.numel(),
+          float(scale), _ptr(out, I32, "out"), _stream())
+
+
+def vmat_h2_workspace(B, nao_pad, ld):
+    n = ctypes.c_size_t(0)
+    _call("hqc_ao_vmat_h2_workspace", B, nao_pad, ld, ctypes.byref(n))
+    return n.value
+
+
+def ao_vmat_h2(ph, v, scale, dV, V, work, bidx=None):
+    """V = dV phi diag(v) phi^T from the FP16 pairs of phi_split_f16 (split products)."""
+    nao_pad, ld = ph.shape[1], ph.shape[2]
+    B = v.shape[0]
+    assert ph.dtype == I32 and v.shape == (B, ld) and V.shape == (B, nao_pad, nao_pad)
+    bi = _bidx(ph, bidx, B)
+    _call("hqc_ao_vmat_h2", _ptr(ph, I32, "ph"), _ptr(bi, I32, "bidx"), _ptr(v, F64, "v"), B,
+          nao_pad, ld, float(scale), float(dV), _ptr(V, F64, "V"), _ptr(work, None, "work"),
           work.numel() * work.element_size(), _stream())
 
 

@@ -825,3 +825,264 @@ extern "C" int hqc_ao_vmat_f32(const float* phi, const int* bidx, const double* 
                                size_t work_bytes, void* stream) {
   return launch_vmat<float>(phi, bidx, v, B, nao_pad, ld, dV, V, work, work_bytes, 32, stream);
 }
+
+// ---------------------------------------------------------------------------
+// Split-precision potential matrix: FP16 products, FP32 accumulation.
+// phi is kept as FP16 pairs (hi, lo) with s phi = hi + lo to 22 significant
+// bits (s = 2^k chosen by the caller so that |s phi| <= 2^14; hqc_phi_split_f16).
+// While a chunk is staged in LDS, v phi is formed in FP32 from the pair and
+// split the same way, with a per-geometry power-of-two scale sv (|sv v| <= 1).
+// Every 16x16 tile takes hi*hi + hi*lo + lo*hi per 32 grid points on
+// v_mfma_f32_16x16x32_f16 (operands: 8 consecutive k per lane, the same k order
+// for both; result layout as v_mfma_f32_16x16x4f32, measured); the dropped lo*lo
+// term is below 2^-22 of the product, so the error is that of FP32 accumulation.
+// Chunking, tiles per wave and the deterministic reduction are k_vmat_pipe's;
+// phi in HBM costs 4 bytes per value, as FP32.
+// ---------------------------------------------------------------------------
+typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+#define VH_KC 64
+// one LDS buffer (64.5 KB at nao_pad 112) lets two workgroups share a CU, so one
+// workgroup's FP16 splitting overlaps the other's MFMAs (3.3 ms against 3.9 ms
+// double-buffered at 54 H dzv, batch 256)
+#define VH_NBUF 1
+
+__device__ __forceinline__ unsigned pack_h2(_Float16 hi, _Float16 lo) {
+  return (unsigned)__builtin_bit_cast(unsigned short, hi) |
+         ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
+}
+
+__global__ __launch_bounds__(256) void k_phi_split_f16(const double* __restrict__ phi, long n,
+                                                       double s, unsigned* __restrict__ out) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const double x = phi[i] * s;
+    const _Float16 hi = (_Float16)x;
+    const _Float16 lo = (_Float16)(x - (double)hi);
+    out[i] = pack_h2(hi, lo);
+  }
+}
+
+// sv[b] = 2^-ceil(log2 max_r |v[b,r]|) (1 for v = 0)
+__global__ __launch_bounds__(256) void k_vscale(const double* __restrict__ v, int ld,
+                                                float* __restrict__ sv) {
+  __shared__ double red[4];
+  const int b = blockIdx.x;
+  double m = 0.0;
+  for (int r = threadIdx.x; r < ld; r += 256) m = fmax(m, fabs(v[(long)b * ld + r]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    sv[b] = m > 0.0 ? (float)exp2(-ceil(log2(m))) : 1.0f;
+  }
+}
+
+template <int TPW, int NBUF>
+__global__ __launch_bounds__(256) void k_vmat_h2(const unsigned* __restrict__ ph,
+                                                 const int* __restrict__ bidx,
+                                                 const double* __restrict__ v,
+                                                 const float* __restrict__ sv, double sphi,
+                                                 int nao_pad, int ld, int ntiles, int nwg,
+                                                 double* __restrict__ part) {
+  constexpr int KC = VH_KC;
+  constexpr int STR = KC + 8;         // halves: 144-byte rows, so the 16-byte fragment
+                                      // reads of 8 consecutive rows hit distinct banks
+  constexpr int VPR = KC / 4;         // 16-byte vectors (4 pairs) per row of a chunk
+  extern __shared__ unsigned char smem_raw[];
+  _Float16* Ls = reinterpret_cast<_Float16*>(smem_raw);  // [2 buffers][Ahi,Alo,Bhi,Blo]
+  const long plane = (long)nao_pad * STR;
+  const int wg = blockIdx.x, grp = blockIdx.y, b = blockIdx.z;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int NT = nao_pad >> 4;
+  const int NV = nao_pad * VPR / 256;  // nao_pad / 16
+  const int cv = threadIdx.x % VPR;    // the same 4 columns of every chunk for this thread
+  const int tbase = grp * 4 * TPW + w * TPW;
+  int rto[TPW], cto[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    int rt, ct;
+    tile_rc(min(tbase + t, ntiles - 1), NT, rt, ct);
+    rto[t] = (rt * 16 + l16) * STR + 8 * kq;
+    cto[t] = (ct * 16 + l16) * STR + 8 * kq;
+  }
+  const unsigned* P = ph + (long)(bidx ? bidx[b] : b) * nao_pad * ld;
+  const double* vb = v + (long)b * ld;
+  const float svb = sv[b];
+  u32x4 stage[VMAT_PIPE_MAXV];
+  double vst[4];
+  auto gload = [&](long c0) {
+#pragma unroll
+    for (int j = 0; j < VMAT_PIPE_MAXV; ++j) {
+      if (j < NV) {
+        const int row = (threadIdx.x + 256 * j) / VPR;
+        stage[j] = *reinterpret_cast<const u32x4*>(P + (long)row * ld + c0 + cv * 4);
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) vst[e] = vb[c0 + cv * 4 + e];
+  };
+  auto lstore = [&](int buf) {
+    _Float16* Ahi = Ls + (long)buf * 4 * plane;
+    _Float16* Alo = Ahi + plane;
+    _Float16* Bhi = Alo + plane;
+    _Float16* Blo = Bhi + plane;
+    float vs[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) vs[e] = (float)(vst[e] * (double)svb);
+#pragma unroll
+    for (int j = 0; j < VMAT_PIPE_MAXV; ++j) {
+      if (j < NV) {
+        const int row = (threadIdx.x + 256 * j) / VPR;
+        const int off = row * STR + cv * 4;
+        h16x4 ah, al, bh, bl;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const unsigned u = stage[j][e];
+          const _Float16 hi = __builtin_bit_cast(_Float16, (unsigned short)(u & 0xffffu));
+          const _Float16 lo = __builtin_bit_cast(_Float16, (unsigned short)(u >> 16));
+          ah[e] = hi;
+          al[e] = lo;
+          // y is pinned as an FP32 register value (an empty asm, no memory access):
+          // left free, the compiler folds the FP16 rounding of hi into the multiply
+          // (v_fma_mix, one rounding of the exact product) in some places and not
+          // in others, and hi + lo then misses y by an FP16 ulp whenever the two
+          // roundings differ
+          float y = vs[e] * ((float)hi + (float)lo);
+          asm volatile("" : "+v"(y));
+          const _Float16 yh = (_Float16)y;
+          bh[e] = yh;
+          bl[e] = (_Float16)(y - (float)yh);
+        }
+        *reinterpret_cast<h16x4*>(Ahi + off) = ah;
+        *reinterpret_cast<h16x4*>(Alo + off) = al;
+        *reinterpret_cast<h16x4*>(Bhi + off) = bh;
+        *reinterpret_cast<h16x4*>(Blo + off) = bl;
+      }
+    }
+  };
+  f32x4 acc[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) acc[t] = f32x4{0, 0, 0, 0};
+  long c0 = (long)wg * KC;
+  const long cstep = (long)nwg * KC;
+  if (c0 < ld) {
+    gload(c0);
+    lstore(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (; c0 < ld; c0 += cstep) {
+    const bool more = c0 + cstep < ld;
+    if (more) gload(c0 + cstep);
+    const _Float16* Ahi = Ls + (long)buf * 4 * plane;
+    const _Float16* Alo = Ahi + plane;
+    const _Float16* Bhi = Alo + plane;
+    const _Float16* Blo = Bhi + plane;
+#pragma unroll
+    for (int ks = 0; ks < KC / 32; ++ks) {
+#pragma unroll
+      for (int t = 0; t < TPW; ++t) {
+        const h16x8 ah = *reinterpret_cast<const h16x8*>(Ahi + rto[t] + 32 * ks);
+        const h16x8 al = *reinterpret_cast<const h16x8*>(Alo + rto[t] + 32 * ks);
+        const h16x8 bh = *reinterpret_cast<const h16x8*>(Bhi + cto[t] + 32 * ks);
+        const h16x8 bl = *reinterpret_cast<const h16x8*>(Blo + cto[t] + 32 * ks);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[t], 0, 0, 0);
+      }
+    }
+    if (NBUF == 2) {
+      if (more) lstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    } else {                          // one buffer: two blocks fit a CU
+      __syncthreads();
+      if (more) lstore(0);
+      __syncthreads();
+    }
+  }
+  const double inv = 1.0 / (sphi * sphi * (double)svb);
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int tile = tbase + t;
+    if (tile < ntiles) {
+      double* o = part + (((long)b * ntiles + tile) * nwg + wg) * 256 + lane * 4;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) o[e] = (double)acc[t][e] * inv;
+    }
+  }
+}
+
+extern "C" int hqc_phi_split_f16(const double* phi, long n, double scale, unsigned* out,
+                                 void* stream) {
+  if (n < 0 || !(scale > 0.0)) return HQC_E_ARG;
+  if (n == 0) return HQC_OK;
+  const long blocks = hqc_div_up(n, 256);
+  hipLaunchKernelGGL(k_phi_split_f16, dim3((unsigned)(blocks < 65536 ? blocks : 65536)),
+                     dim3(256), 0, (hipStream_t)stream, phi, n, scale, out);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+static size_t vmat_h2_part_bytes(int B, int nao_pad, int ld) {
+  int KC, ntiles, ngroups, nwg;
+  vmat_shape(B, nao_pad, ld, 32, KC, ntiles, ngroups, nwg);
+  return ((size_t)B * ntiles * nwg * 256 * sizeof(double) + 255) / 256 * 256;
+}
+
+extern "C" int hqc_ao_vmat_h2_workspace(int B, int nao_pad, int ld, size_t* bytes) {
+  if (!bytes || B < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63))
+    return HQC_E_ARG;
+  if (!vmat_pipelined(nao_pad)) return HQC_E_SIZE;
+  *bytes = vmat_h2_part_bytes(B, nao_pad, ld) + (size_t)B * sizeof(float);
+  return HQC_OK;
+}
+
+extern "C" int hqc_ao_vmat_h2(const unsigned* ph, const int* bidx, const double* v, int B,
+                              int nao_pad, int ld, double phi_scale, double dV, double* V,
+                              void* work, size_t work_bytes, void* stream) {
+  if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63) || !(phi_scale > 0.0))
+    return HQC_E_ARG;
+  if (!vmat_pipelined(nao_pad)) return HQC_E_SIZE;
+  if (B == 0) return HQC_OK;
+  int KC, ntiles, ngroups, nwg;
+  vmat_shape(B, nao_pad, ld, 32, KC, ntiles, ngroups, nwg);
+  const size_t pbytes = vmat_h2_part_bytes(B, nao_pad, ld);
+  if (!work || work_bytes < pbytes + (size_t)B * sizeof(float)) return HQC_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)work;
+  float* sv = (float*)((char*)work + pbytes);
+  hipLaunchKernelGGL(k_vscale, dim3(B), dim3(256), 0, s, v, ld, sv);
+  HQC_CHECK_LAUNCH();
+  const size_t lds = (size_t)VH_NBUF * 4 * nao_pad * (VH_KC + 8) * sizeof(_Float16);
+  const dim3 grid(nwg, ngroups, B);
+#define VMAT_H2_CASE(TP)                                                                     \
+  case TP: {                                                                                 \
+    if (hipFuncSetAttribute((const void*)k_vmat_h2<TP, VH_NBUF>,                             \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) \
+      (void)hipGetLastError();                                                               \
+    hipLaunchKernelGGL((k_vmat_h2<TP, VH_NBUF>), grid, dim3(256), lds, s, ph, bidx, v, sv,    \
+                       phi_scale, nao_pad, ld, ntiles, nwg, part);                           \
+  } break;
+  switch (vmat_pipe_tpw(nao_pad)) {
+    VMAT_H2_CASE(1)
+    VMAT_H2_CASE(2)
+    VMAT_H2_CASE(3)
+    VMAT_H2_CASE(4)
+    VMAT_H2_CASE(5)
+    VMAT_H2_CASE(6)
+    VMAT_H2_CASE(7)
+    default:
+    VMAT_H2_CASE(8)
+  }
+#undef VMAT_H2_CASE
+  HQC_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_vmat_reduce<float>, dim3(ntiles, B), dim3(256), 0, s, part, nao_pad, ntiles,
+                     nwg, dV, V);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}

@@ -128,7 +128,9 @@ class Engine:
     precision='fp64': every contraction in FP64 (MFMA f64).
     precision='mixed': defect correction.  rho(D) and V(v) are linear, so the
     engine keeps FP64 anchors (D_a, rho_a) and (v_a, V_a) and evaluates only the
-    increments rho(D - D_a) and V(v - v_a) with FP32 AO values on FP32 MFMA.
+    increments rho(D - D_a) and V(v - v_a) in low precision: V from FP16 (hi, lo)
+    split products with FP32 accumulation on MFMA (FP32 MFMA for n_ao > 128), rho
+    with FP32 AO values on FP32 MFMA.
     The increment error is proportional to the increment, which shrinks as the
     SCF converges; an FP64 re-anchor is forced whenever the estimated increment
     error exceeds MIXED_SAFETY times the current DIIS residual per matrix
@@ -149,6 +151,10 @@ class Engine:
     MIXED_RHO_SAFETY = 1.0  # density increment: |v| x relative rho error <= residual
     MIXED_DIRECT_ERR = 1e-1  # residual above which rho and V are taken in FP32 outright
     MIXED_RHO_INCREMENTS = False  # FP32 D-form density increments (else FP64 orbital form)
+    # low-precision V from FP16 (hi, lo) split products on MFMA (n_ao <= 128; FP32
+    # MFMA beyond), whose 22-bit operands give ~4x the FP32 increment error
+    MIXED_SPLIT16 = True
+    MIXED_EPS_SPLIT16 = 8e-6
 
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
@@ -207,10 +213,12 @@ class Engine:
     # ------------------------------------------------------------------ utils
     def _workspace(self, B, prec):
         # one grow-only buffer per precision; calls of any batch size share it
-        nbytes = K.vmat_workspace(B, self.c.nao_pad, self.c.ld, prec)
+        nbytes = (K.vmat_h2_workspace(B, self.c.nao_pad, self.c.ld) if prec == 16 else
+                  K.vmat_workspace(B, self.c.nao_pad, self.c.ld, prec))
         buf = self._work.get(prec)
         if buf is None or buf.numel() * 8 < nbytes:
-            buf = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=self.c.device)
+            buf = torch.empty(max((nbytes + 7) // 8, 1), dtype=torch.float64,
+                              device=self.c.device)
             self._work[prec] = buf
         return buf
 
@@ -244,9 +252,13 @@ class Engine:
     def vmat_dv(self, phi, v, dV, bidx=None):
         c = self.c
         B = v.shape[0]
-        prec = 64 if phi.dtype == torch.float64 else 32
         V = torch.empty(B, c.nao_pad, c.nao_pad, dtype=torch.float64, device=c.device)
-        K.ao_vmat(phi, v.contiguous(), dV, V, self._workspace(B, prec), bidx)
+        if phi.dtype == torch.int32:                  # FP16 pairs: split-precision products
+            K.ao_vmat_h2(phi, v.contiguous(), self._phi_scale, dV, V, self._workspace(B, 16),
+                         bidx)
+        else:
+            prec = 64 if phi.dtype == torch.float64 else 32
+            K.ao_vmat(phi, v.contiguous(), dV, V, self._workspace(B, prec), bidx)
         return V[:, :c.nao, :c.nao]
 
     def density(self, phi, D, X=None):
@@ -424,6 +436,13 @@ class Engine:
         dVs = self._dv(st, B)
         if mixed:
             st["phi_lp"] = phi32
+            if self.MIXED_SPLIT16 and c.nao_pad <= 128:
+                # FP16 (hi, lo) pairs of s*phi, s = 2^k with |s phi| <= 2^14
+                amax = float(phi.abs().amax().item())
+                self._phi_scale = 2.0 ** math.floor(math.log2(2.0 ** 14 / max(amax, 1e-300)))
+                st["phi_h2"] = torch.empty(B, c.nao_pad, c.ld, dtype=torch.int32, device=dev)
+                K.phi_split_f16(phi, self._phi_scale, st["phi_h2"])
+                per.append("phi_h2")
             st["phisq"] = (phi[:, :c.nao, :c.N] ** 2).sum(1)             # sum_p phi_p(r)^2
             st["Da"] = torch.zeros(B, c.nao, c.nao, dtype=f64, device=dev)
             st["rhoa"] = torch.zeros(B, c.ld, dtype=f64, device=dev)
@@ -603,7 +622,9 @@ class Engine:
         direct = err >= self.MIXED_DIRECT_ERR
         dv = v - sel("va")
         tol_e = self.MIXED_SAFETY * err / c.nao                 # per-element error budget
-        inc = ~direct & sel("va_set") & (self.MIXED_EPS * dv.abs().amax(-1) <= tol_e)
+        lp = "phi_h2" if "phi_h2" in st else "phi_lp"
+        eps = self.MIXED_EPS_SPLIT16 if lp == "phi_h2" else self.MIXED_EPS
+        inc = ~direct & sel("va_set") & (eps * dv.abs().amax(-1) <= tol_e)
         low = direct | inc
         dVs = self._dv(st, na)
         V2 = torch.empty(na, c.nao, c.nao, dtype=torch.float64, device=c.device)
@@ -612,7 +633,7 @@ class Engine:
             il = None if idx is None else idx.long()
             vv = torch.where(direct[:, None], v, dv)
             d = dVs if not torch.is_tensor(dVs) or il is None else dVs[il]
-            V32 = self.vmat(st["phi_lp"] if n < na else sel("phi_lp"),
+            V32 = self.vmat(st[lp] if n < na else sel(lp),
                             vv if il is None else vv[il], d, idx)
             V32 = V32 + torch.where(inc[:, None, None], sel("Va"), 0.0)[
                 slice(None) if il is None else il]

@@ -44,7 +44,8 @@ class KEngine(Engine):
         n2 = 2 * c.nao_pad
         need = K.vmat_workspace(B, n2, c.ld, 64)
         if self._kwork is None or self._kwork.numel() * 8 < need:
-            self._kwork = torch.empty(max(need // 8, 1), dtype=torch.float64, device=c.device)
+            self._kwork = torch.empty(max((need + 7) // 8, 1), dtype=torch.float64,
+                                      device=c.device)
         M = torch.empty(B, n2, n2, dtype=torch.float64, device=c.device)
         K.ao_vmat(phi, v.contiguous(), dV, M, self._kwork, bidx)
         n, p = c.nao, c.nao_pad

@@ -129,6 +129,18 @@ int hqc_ao_vmat_f64(const double* phi, const int* bidx, const double* v, int B, 
  * block, FP64 accumulation across blocks; V written in FP64. */
 int hqc_ao_vmat_f32(const float* phi, const int* bidx, const double* v, int B, int nao_pad,
                     int ld, double dV, double* V, void* work, size_t work_bytes, void* stream);
+/* Split-precision form of the same contraction (nao_pad <= 128).
+ * hqc_phi_split_f16: out[i] = (hi, lo) FP16 pair (hi in the low half) with
+ *   scale*phi[i] = hi + lo to 22 significant bits; scale = 2^k, |scale phi| <= 2^14.
+ * hqc_ao_vmat_h2: V[b] = dV phi diag(v[b]) phi^T from those pairs; v phi is split
+ *   the same way in LDS with a per-geometry power-of-two scale, and each tile takes
+ *   hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16 with FP32 accumulation
+ *   (relative error ~2^-22 of sum |phi phi v|).  work: hqc_ao_vmat_h2_workspace bytes. */
+int hqc_phi_split_f16(const double* phi, long n, double scale, unsigned* out, void* stream);
+int hqc_ao_vmat_h2_workspace(int B, int nao_pad, int ld, size_t* bytes);
+int hqc_ao_vmat_h2(const unsigned* ph, const int* bidx, const double* v, int B, int nao_pad,
+                   int ld, double phi_scale, double dV, double* V, void* work,
+                   size_t work_bytes, void* stream);
 
 /* Batched in-place 3-D complex FFT of B grids g^3 (any g <= 128; mixed-radix
  * Stockham passes in LDS, one launch per axis).  sign = -1 forward,

@@ -82,6 +82,45 @@ def test_density_and_vmat_f32(K, nao_pad, g):
     assert torch.allclose(V, Vref, rtol=2e-5)
 
 
+@pytest.mark.parametrize("nao_pad,g,B", [(16, 5, 1), (48, 13, 3), (112, 21, 2), (128, 9, 2)])
+def test_vmat_split16(K, nao_pad, g, B):
+    """Split-precision V (FP16 hi/lo products, FP32 accumulation) against FP64:
+    error within 2^-19 of sum |phi_p phi_q v| per element (operands carry 22 bits,
+    products accumulate in FP32 over a block's chunks), for signed v of very
+    different size per geometry, and through a geometry index list."""
+    print(f"[vmat split16] nao_pad={nao_pad} g={g} B={B}", flush=True)
+    torch.manual_seed(3)
+    N = g ** 3
+    ld = (N + 63) // 64 * 64
+    phi = torch.zeros(B, nao_pad, ld, dtype=torch.float64, device=DEV)
+    phi[:, :, :N] = torch.randn(B, nao_pad, N, dtype=torch.float64, device=DEV)
+    v = torch.zeros(B, ld, dtype=torch.float64, device=DEV)
+    v[:, :N] = torch.randn(B, N, dtype=torch.float64, device=DEV)
+    v *= torch.logspace(0, -9, B, dtype=torch.float64, device=DEV)[:, None]
+    scale = 2.0 ** math.floor(math.log2(2.0 ** 14 / phi.abs().max().item()))
+    ph = torch.empty(B, nao_pad, ld, dtype=torch.int32, device=DEV)
+    K.phi_split_f16(phi, scale, ph)
+    work = torch.empty(K.vmat_h2_workspace(B, nao_pad, ld) // 8 + 1, dtype=torch.float64,
+                       device=DEV)
+    V = torch.empty(B, nao_pad, nao_pad, dtype=torch.float64, device=DEV)
+    K.ao_vmat_h2(ph, v, scale, 0.5, V, work)
+    Vref = 0.5 * (phi * v[:, None, :]) @ phi.transpose(1, 2)
+    bound = 0.5 * (phi.abs() * v.abs()[:, None, :]) @ phi.abs().transpose(1, 2)
+    torch.cuda.synchronize()
+    r = ((V - Vref).abs() / bound.clamp(min=1e-300)).max().item()
+    print(f"  max |V - Vref| / sum|phi phi v| = {r:.3e} (2^{math.log2(max(r, 1e-300)):.1f})",
+          flush=True)
+    assert ((V - Vref).abs() <= 2.0 ** -19 * bound + 1e-300).all()
+    assert torch.equal(V, V.transpose(1, 2))
+    # geometry index list: entries read phi geometries in reverse order
+    bidx = torch.arange(B - 1, -1, -1, dtype=torch.int32, device=DEV)
+    V2 = torch.empty_like(V)
+    K.ao_vmat_h2(ph, v, scale, 0.5, V2, work, bidx)
+    Vref2 = 0.5 * (phi.flip(0) * v[:, None, :]) @ phi.flip(0).transpose(1, 2)
+    bound2 = 0.5 * (phi.flip(0).abs() * v.abs()[:, None, :]) @ phi.flip(0).abs().transpose(1, 2)
+    assert ((V2 - Vref2).abs() <= 2.0 ** -19 * bound2 + 1e-300).all()
+
+
 @pytest.mark.parametrize("nao_pad,m,dt", [(48, 20, torch.float64), (112, 60, torch.float64),
                                            (112, 60, torch.float32), (112, 100, torch.float32),
                                            (64, 140, torch.float64), (224, 100, torch.float64),

@@ -107,8 +107,9 @@ def test_batched_equals_single():
         assert abs(pes(xps[b]).item() - Eb[b]) < 1e-10
 
 
-@pytest.mark.parametrize("rho_inc", [False, True], ids=["rho_orbital64", "rho_increments32"])
-def test_mixed_precision_matches_fp64(rho_inc):
+@pytest.mark.parametrize("rho_inc,split16", [(False, False), (True, False), (False, True)],
+                         ids=["v_fp32", "rho_increments32", "v_split16"])
+def test_mixed_precision_matches_fp64(rho_inc, split16):
     """Defect-corrected FP32 increments converge to the FP64 answer within the
     accuracy bar (dE <= 1 uHa/atom, dF <= 1e-5 Ha/Bohr, dbands <= 1e-6 Ha), with
     the density past the warm-up in FP64 orbital form or as FP32 D-form increments."""
@@ -120,6 +121,7 @@ def test_mixed_precision_matches_fp64(rho_inc):
     p64 = make_pes(n, L, rs, "gth-dzv", precision="fp64", **kw)
     pmx = make_pes(n, L, rs, "gth-dzv", precision="mixed", **kw)
     pmx.solver.engine.MIXED_RHO_INCREMENTS = rho_inc
+    pmx.solver.engine.MIXED_SPLIT16 = split16
     E64, F64 = p64.energy_and_forces(xps)
     Emx, Fmx = pmx.energy_and_forces(xps)
     assert bool(pmx.solver.last["conv"].all())
@@ -127,6 +129,24 @@ def test_mixed_precision_matches_fp64(rho_inc):
     assert (F64 - Fmx).abs().max().item() <= 1e-5
     assert (p64.solver.last["w1"] - pmx.solver.last["w1"]).abs().max().item() <= 1e-6
     assert pmx.solver.engine.stats["v32"] > 0 and pmx.solver.engine.stats["rho32"] > 0
+    assert pmx.solver.last is not None
+
+
+def test_mixed_single_geometry_split16():
+    """A fresh mixed engine whose first split-product launch has an odd batch (one
+    geometry): the workspace is sized in whole doubles (it ends in B floats)."""
+    from hqc_amd.pbc import make_pes
+    n, rs = 16, 1.25
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    xp = np.random.default_rng(5).uniform(0, L, (1, n, 3))
+    kw = dict(grid_length=0.2, dft=True, smearing=True, smearing_sigma=0.0316681, tol=1e-8)
+    pmx = make_pes(n, L, rs, "gth-dzv", precision="mixed", **kw)
+    assert pmx.solver.engine.MIXED_SPLIT16
+    E, F = pmx.energy_and_forces(xp)
+    p64 = make_pes(n, L, rs, "gth-dzv", precision="fp64", **kw)
+    E64, F64 = p64.energy_and_forces(xp)
+    assert abs(E.item() - E64.item()) / 2 / n <= 1e-6
+    assert (F - F64).abs().max().item() <= 1e-5
 
 
 KCASES = [
