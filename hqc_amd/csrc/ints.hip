@@ -28,7 +28,8 @@ __device__ __forceinline__ void hermite(double mu, double R, double* e, int nmax
 // unit-normalised integral and are skipped.
 #define INT_EXP_CUT 46.0
 
-// One thread per (b, A, Bat, o, q).  CPLX: k-point version, every lattice term
+// One thread per (b, A, Bat, o, q) with (A, o) <= (Bat, q); each writes its entry
+// and the mirrored one.  CPLX: k-point version, every lattice term
 // carries the Bloch phase e^{-i k.T} (hqc/pbc/solver.py:270-271) and the
 // outputs are complex (interleaved re, im).
 template <bool CPLX>
@@ -48,6 +49,8 @@ __global__ __launch_bounds__(256) void k_overlap_kinetic(
   t1 /= n;
   const int A = (int)(t1 % n);
   const int b = (int)(t1 / n);
+  // each unordered pair once: the (Bat,q | A,o) entry is the mirror written below
+  if (Bat < A || (Bat == A && q < o)) return;
   const double* xA = xp + ((long)b * n + A) * 3;
   const double* xB = xp + ((long)b * n + Bat) * 3;
   const double d0 = xA[0] - xB[0], d1 = xA[1] - xB[1], d2 = xA[2] - xB[2];
@@ -142,13 +145,42 @@ __global__ __launch_bounds__(256) void k_overlap_kinetic(
   }
   const long nao = (long)n * nloc;
   const long e = ((long)b * nao + (long)A * nloc + o) * nao + (long)Bat * nloc + q;
+  // mirror (Bat,q | A,o): both functions are real, so the integral is symmetric and
+  // the lattice sum turns into its complex conjugate; the derivative moves to the
+  // other centre, which flips its sign (the integrals depend on R_A - R_B only)
+  const long em = ((long)b * nao + (long)Bat * nloc + q) * nao + (long)A * nloc + o;
+  if (em == e) {
+    // diagonal element: written once (its derivative vanishes up to rounding)
+    if (!CPLX) {
+      S[e] = s[0];
+      Tm[e] = t[0];
+    } else {
+      S[2 * e] = s[0];
+      S[2 * e + 1] = s[1];
+      Tm[2 * e] = t[0];
+      Tm[2 * e + 1] = t[1];
+    }
+    if (deriv) {
+      for (int c = 0; c < 3; ++c) {
+        for (int part = 0; part < (CPLX ? 2 : 1); ++part) {
+          dS[(CPLX ? 2 : 1) * (e * 3 + c) + part] = ds[part][c];
+          dT[(CPLX ? 2 : 1) * (e * 3 + c) + part] = dt[part][c];
+        }
+      }
+    }
+    return;
+  }
   if (!CPLX) {
     S[e] = s[0];
     Tm[e] = t[0];
+    S[em] = s[0];
+    Tm[em] = t[0];
     if (deriv) {
       for (int c = 0; c < 3; ++c) {
         dS[e * 3 + c] = ds[0][c];
         dT[e * 3 + c] = dt[0][c];
+        dS[em * 3 + c] = -ds[0][c];
+        dT[em * 3 + c] = -dt[0][c];
       }
     }
   } else {
@@ -156,12 +188,20 @@ __global__ __launch_bounds__(256) void k_overlap_kinetic(
     S[2 * e + 1] = s[1];
     Tm[2 * e] = t[0];
     Tm[2 * e + 1] = t[1];
+    S[2 * em] = s[0];
+    S[2 * em + 1] = -s[1];
+    Tm[2 * em] = t[0];
+    Tm[2 * em + 1] = -t[1];
     if (deriv) {
       for (int c = 0; c < 3; ++c) {
         dS[2 * (e * 3 + c)] = ds[0][c];
         dS[2 * (e * 3 + c) + 1] = ds[1][c];
         dT[2 * (e * 3 + c)] = dt[0][c];
         dT[2 * (e * 3 + c) + 1] = dt[1][c];
+        dS[2 * (em * 3 + c)] = -ds[0][c];
+        dS[2 * (em * 3 + c) + 1] = ds[1][c];
+        dT[2 * (em * 3 + c)] = -dt[0][c];
+        dT[2 * (em * 3 + c) + 1] = dt[1][c];
       }
     }
   }

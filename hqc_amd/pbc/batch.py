@@ -71,7 +71,7 @@ class BatchPES:
                                              for i in part]), dtype=torch.float64,
                                    device=eng.c.device)
                 Lt = torch.tensor(Lb[part], dtype=torch.float64, device=eng.c.device)
-                st = eng.solve(xpb, Lb=Lt)
+                st = eng.solve(xpb, Lb=Lt, forces=True)
                 Fb = compute_forces(eng, st, self.kappa, self.Gmax)
                 Eb = st["E"] * RY + st["Epp"]
                 for j, i in enumerate(part):

@@ -375,13 +375,15 @@ class Engine:
         else:
             st["v"] = u * w.rsqrt().unsqueeze(-2)
 
-    def setup(self, xpb, kpt=None, Lb=None):
+    def setup(self, xpb, kpt=None, Lb=None, forces=False):
         """Per-geometry constants for xp (B, n, 3) in Bohr on the device.  Every
         tensor whose leading axis is the geometry is listed in st['per'] so the
         SCF can keep the active geometries as a contiguous prefix.
 
         Lb (B,) Bohr gives each geometry its own cell length (all on this cell's
-        grid); the L-dependent set-up kernels then run once per distinct length."""
+        grid); the L-dependent set-up kernels then run once per distinct length.
+        forces=True also keeps the S/T centre derivatives (st['dS'], st['dT']) from
+        the same lattice sums, for the force pass."""
         c = self.c
         B = xpb.shape[0]
         dev, f64 = c.device, torch.float64
@@ -400,6 +402,8 @@ class Engine:
             per.append("VGb")
         S = torch.empty(B, c.nao, c.nao, dtype=f64, device=dev)
         T = torch.empty_like(S)
+        dS = torch.empty(B, c.nao, c.nao, 3, dtype=f64, device=dev) if forces else None
+        dT = torch.empty_like(dS) if forces else None
         phi = torch.zeros(B, c.nao_pad, c.ld, dtype=f64, device=dev)
         mixed = self.precision == "mixed"
         phi32 = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float32, device=dev) if mixed else None
@@ -409,9 +413,17 @@ class Engine:
             nb = x.shape[0]
             Sg = S if idx is None else torch.empty(nb, c.nao, c.nao, dtype=f64, device=dev)
             Tg = T if idx is None else torch.empty_like(Sg)
-            K.overlap_kinetic(x, cst.lattice, c.alpha, c.coef, c.comp, Sg, Tg)
+            dSg = dTg = None
+            if forces:
+                dSg = dS if idx is None else torch.empty(nb, c.nao, c.nao, 3, dtype=f64,
+                                                         device=dev)
+                dTg = dT if idx is None else torch.empty_like(dSg)
+            K.overlap_kinetic(x, cst.lattice, c.alpha, c.coef, c.comp, Sg, Tg, dSg, dTg)
             self._put(S, idx, Sg)
             self._put(T, idx, Tg)
+            if forces:
+                self._put(dS, idx, dSg)
+                self._put(dT, idx, dTg)
             P = torch.empty(nb, c.n, 3, c.g, c.na, c.nl, dtype=f64, device=dev)
             K.gto_tables(x, c.g, cst.L, cst.tmax_img, c.alpha, c.nl, P)
             pg = phi if idx is None else torch.zeros(nb, c.nao_pad, c.ld, dtype=f64, device=dev)
@@ -429,6 +441,9 @@ class Engine:
             K.complex_real(cb, c.ld, vg)
             self._put(vl, idx, vg)
         st["S"], st["T"] = S, T
+        if forces:
+            st["dS"], st["dT"] = dS, dT
+            per += ["dS", "dT"]
         self._orthogonalizer(st, S)
         if "lindep" in st:
             per.append("lindep")
@@ -874,13 +889,13 @@ class Engine:
         return loc["Mo"], loc["W1"], loc["E"], loc["nit"] != self.max_cycle, loc["nit"], order
 
     # ---------------------------------------------------------------- results
-    def solve(self, xpb, kpt=None, Lb=None):
+    def solve(self, xpb, kpt=None, Lb=None, forces=False):
         """Full solver output for a batch (B, n, 3) Bohr.
 
         Per-geometry device state in st stays in slot order (st['order'][i] is
         the geometry held by slot i); the user-facing results mo, D, w1, E, conv,
         nit, Ki, Vep, Vee, Se, occ are returned in input order."""
-        st = self.setup(xpb, kpt, Lb)
+        st = self.setup(xpb, kpt, Lb, forces)
         mo, w1, E, conv, nit, order = self.scf(st)
         B = st["B"]
         D, occ = self.density_matrix(mo, w1)                              # solver.py:758-765

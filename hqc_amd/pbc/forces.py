@@ -67,18 +67,22 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
     D, occ, w1, mo = sl["D"], sl["occ"], sl["w1"], sl["mo"]
     groups = engine._cell_groups(B, st["Lb"] if st.get("hetero") else None)
     sub, put = engine._sub, engine._put
-    dS = torch.empty(B, c.nao, c.nao, 3, dtype=f64, device=dev)
-    dT = torch.empty_like(dS)
-    for cst, idx in groups:
-        x = sub(st["xpb"], idx)
-        nb = x.shape[0]
-        S = torch.empty(nb, c.nao, c.nao, dtype=f64, device=dev)
-        T = torch.empty_like(S)
-        dSg = dS if idx is None else torch.empty(nb, c.nao, c.nao, 3, dtype=f64, device=dev)
-        dTg = dT if idx is None else torch.empty_like(dSg)
-        K.overlap_kinetic(x, cst.lattice, c.alpha, c.coef, c.comp, S, T, dSg, dTg)
-        put(dS, idx, dSg)
-        put(dT, idx, dTg)
+    if "dS" in st:                          # kept by a set-up with forces=True
+        dS, dT = st["dS"], st["dT"]
+    else:
+        dS = torch.empty(B, c.nao, c.nao, 3, dtype=f64, device=dev)
+        dT = torch.empty_like(dS)
+        for cst, idx in groups:
+            x = sub(st["xpb"], idx)
+            nb = x.shape[0]
+            S = torch.empty(nb, c.nao, c.nao, dtype=f64, device=dev)
+            T = torch.empty_like(S)
+            dSg = dS if idx is None else torch.empty(nb, c.nao, c.nao, 3, dtype=f64,
+                                                     device=dev)
+            dTg = dT if idx is None else torch.empty_like(dSg)
+            K.overlap_kinetic(x, cst.lattice, c.alpha, c.coef, c.comp, S, T, dSg, dTg)
+            put(dS, idx, dSg)
+            put(dT, idx, dTg)
     W = (mo * (occ * w1).unsqueeze(-2)) @ mo.transpose(-1, -2)
     atom = c.atom_of_ao
     other = (atom[:, None] != atom[None, :]).to(f64)
@@ -128,11 +132,15 @@ def compute_forces_k(engine, st, kappa=10, Gmax=15):
     f64, c128 = torch.float64, torch.complex128
     sl = st["slot"]
     D, occ, w1, mo = sl["D"], sl["occ"], sl["w1"], sl["mo"]
-    S = torch.empty(B, c.nao, c.nao, dtype=c128, device=dev)
-    T = torch.empty_like(S)
-    dS = torch.empty(B, c.nao, c.nao, 3, dtype=c128, device=dev)
-    dT = torch.empty_like(dS)
-    K.overlap_kinetic_k(st["xpb"], st["kpt"], c.lattice, c.alpha, c.coef, c.comp, S, T, dS, dT)
+    if "dS" in st:                          # kept by a set-up with forces=True
+        dS, dT = st["dS"], st["dT"]
+    else:
+        S = torch.empty(B, c.nao, c.nao, dtype=c128, device=dev)
+        T = torch.empty_like(S)
+        dS = torch.empty(B, c.nao, c.nao, 3, dtype=c128, device=dev)
+        dT = torch.empty_like(dS)
+        K.overlap_kinetic_k(st["xpb"], st["kpt"], c.lattice, c.alpha, c.coef, c.comp, S, T,
+                            dS, dT)
     W = (mo * (occ * w1).unsqueeze(-2)) @ mo.mH
     atom = c.atom_of_ao
     other = (atom[:, None] != atom[None, :]).to(f64)

@@ -84,7 +84,7 @@ class KEngine(Engine):
         return M
 
     # ------------------------------------------------------------------ set-up
-    def setup(self, xpb, kpt=None, Lb=None):
+    def setup(self, xpb, kpt=None, Lb=None, forces=False):
         if Lb is not None:
             raise NotImplementedError("k-point batches share one cell length")
         if kpt is None:
@@ -98,8 +98,13 @@ class KEngine(Engine):
         st["dVb"] = torch.full((B,), c.dV, dtype=f64, device=dev)
         S = torch.empty(B, c.nao, c.nao, dtype=c128, device=dev)
         T = torch.empty_like(S)
-        K.overlap_kinetic_k(st["xpb"], st["kpt"], c.lattice, c.alpha, c.coef, c.comp, S, T)
+        dS = torch.empty(B, c.nao, c.nao, 3, dtype=c128, device=dev) if forces else None
+        dT = torch.empty_like(dS) if forces else None
+        K.overlap_kinetic_k(st["xpb"], st["kpt"], c.lattice, c.alpha, c.coef, c.comp, S, T,
+                            dS, dT)
         st["S"], st["T"] = S, T
+        if forces:
+            st["dS"], st["dT"] = dS, dT
         self._orthogonalizer(st, S)                                     # solver.py:934-935
         P = torch.empty(B, c.n, 3, c.g, c.na, c.nl, dtype=c128, device=dev)
         K.gto_tables_k(st["xpb"], st["kpt"], c.g, c.L, c.tmax_img, c.alpha, c.nl, P)
@@ -107,6 +112,8 @@ class KEngine(Engine):
         K.ao_values_k(P, c.coef, c.comp, c.nao_pad, c.ld, phi)
         st["phi"] = phi
         per = ["xpb", "kpt", "Lb", "dVb", "S", "T", "v", "phi", "vloc", "V", "Hcore"]
+        if forces:
+            per += ["dS", "dT"]
         if "lindep" in st:
             per.append("lindep")
         cb = torch.empty(B, c.g, c.g, c.g, dtype=c128, device=dev)

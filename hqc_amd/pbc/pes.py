@@ -42,13 +42,13 @@ class PES:
         return out[0] if self.mode == "default" else out
 
     def energy_and_forces(self, xp, kpt=None):
-        st, single, x = self.solver.run(xp, kpt)
+        st, single, x = self.solver.run(xp, kpt, forces=True)
         F = compute_forces(self.solver.engine, st, self.kappa, self.Gmax)
         Etot = st["E"] * RY + st["Epp"]
         return (Etot[0], F[0]) if single else (Etot, F)
 
     def free_energy_and_forces(self, x, kpt=None):
-        st, _, x = self.solver.run(x, kpt)
+        st, _, x = self.solver.run(x, kpt, forces=True)
         F = compute_forces(self.solver.engine, st, self.kappa, self.Gmax)
         A = st["E"] * RY - RY * self.solver.engine.sigma * st["Se"] + st["Epp"]
         return A, F

@@ -62,10 +62,11 @@ class GPWSolver:
         assert k.shape == (B, 3)
         return k.contiguous()
 
-    def run(self, xp, kpt=None):
-        """Solve a batch; returns (state dict, single flag, xp tensor in rs units)."""
+    def run(self, xp, kpt=None, forces=False):
+        """Solve a batch; returns (state dict, single flag, xp tensor in rs units).
+        forces=True keeps what the force pass reuses (S/T centre derivatives)."""
         x, single = _as_batch(xp, self.n, self.cell.device)
-        st = self.engine.solve(x * self.rs, self._kpts(kpt, x.shape[0]))
+        st = self.engine.solve(x * self.rs, self._kpts(kpt, x.shape[0]), forces=forces)
         self.last = st
         return st, single, x
 

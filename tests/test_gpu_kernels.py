@@ -200,6 +200,27 @@ def test_overlap_kinetic(K):
             assert np.allclose(a[0].cpu().numpy(), r, rtol=1e-12, atol=1e-13)
 
 
+def test_overlap_kinetic_batch(K):
+    """Two geometries in one launch (each thread writes its pair and the mirrored
+    one): both match the oracle, including the centre derivatives."""
+    s, xa, L = _sys(n=3, basis="gth-dzvp", seed=1)
+    _, xb, _ = _sys(n=3, basis="gth-dzvp", seed=2)
+    b = s.basis
+    nao = s.n_ao
+    S = torch.empty(2, nao, nao, dtype=torch.float64, device=DEV)
+    T = torch.empty_like(S)
+    dS = torch.empty(2, nao, nao, 3, dtype=torch.float64, device=DEV)
+    dT = torch.empty_like(dS)
+    K.overlap_kinetic(torch.tensor(np.stack([xa, xb]), device=DEV),
+                      torch.tensor(s.lattice, device=DEV).contiguous(),
+                      torch.tensor(b.alpha, device=DEV), torch.tensor(b.coef, device=DEV),
+                      torch.tensor(b.comp, dtype=torch.int32, device=DEV), S, T, dS, dT)
+    for i, xp in enumerate((xa, xb)):
+        refs = R.overlap_kinetic(s, xp, deriv=True)
+        for a, r in zip((S, T, dS, dT), refs):
+            assert np.allclose(a[i].cpu().numpy(), r, rtol=1e-12, atol=1e-13)
+
+
 def test_kpoint_tables_values_and_integrals(K):
     """Bloch-phase tables, stacked complex AO values and complex S/T (+ derivatives)
     against the oracle at a general k-point."""
