@@ -34,7 +34,7 @@ _SIGS = {
     "hqc_ao_vmat_f32": [_P, _P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_phi_split_f16": [_P, ctypes.c_long, _D, _P, _P],
     "hqc_ao_vmat_h2_workspace": [_I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
-    "hqc_ao_vmat_h2": [_P, _P, _P, _I, _I, _I, _D, _D, _P, _P, ctypes.c_size_t, _P],
+    "hqc_ao_vmat_h2": [_P, _P, _P, _P, _I, _I, _I, _D, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
     "hqc_pair_pack": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_split_f64": [_P, _I, _I, _I, _I, _I, _P, _P],
@@ -44,7 +44,8 @@ _SIGS = {
     "hqc_complex_real": [_P, _I, _I, _I, _P, _P],
     "hqc_poisson_mul": [_P, _P, _I, _I, _I, _P, _P],
     "hqc_xc_blocks": [_I],
-    "hqc_xc_eval": [_P, _P, _I, _I, _I, _I, _P, _P, _P],
+    "hqc_xc_eval": [_P, _P, _I, _I, _I, _I, _I, _P, _P, _P],
+    "hqc_real_pair_to_complex": [_P, _I, _I, _I, _D, _P, _P],
     "hqc_grad_G": [_P, _I, _I, _D, _P, _D, _P, _P],
     "hqc_div_G": [_P, _I, _I, _D, _P, _D, _P, _P],
     "hqc_gga_blocks": [_I],
@@ -233,15 +234,17 @@ def vmat_h2_workspace(B, nao_pad, ld):
     return n.value
 
 
-def ao_vmat_h2(ph, v, scale, dV, V, work, bidx=None):
-    """V = dV phi diag(v) phi^T from the FP16 pairs of phi_split_f16 (split products)."""
+def ao_vmat_h2(ph, v, scale, dV, V, work, bidx=None, vmax=None):
+    """V = dV phi diag(v) phi^T from the FP16 pairs of phi_split_f16 (split products);
+    vmax (B,) = max_r |v| per entry if the caller has it (else computed)."""
     nao_pad, ld = ph.shape[1], ph.shape[2]
     B = v.shape[0]
     assert ph.dtype == I32 and v.shape == (B, ld) and V.shape == (B, nao_pad, nao_pad)
+    assert vmax is None or (vmax.shape == (B,) and vmax.is_contiguous())
     bi = _bidx(ph, bidx, B)
-    _call("hqc_ao_vmat_h2", _ptr(ph, I32, "ph"), _ptr(bi, I32, "bidx"), _ptr(v, F64, "v"), B,
-          nao_pad, ld, float(scale), float(dV), _ptr(V, F64, "V"), _ptr(work, None, "work"),
-          work.numel() * work.element_size(), _stream())
+    _call("hqc_ao_vmat_h2", _ptr(ph, I32, "ph"), _ptr(bi, I32, "bidx"), _ptr(v, F64, "v"),
+          _ptr(vmax, F64, "vmax"), B, nao_pad, ld, float(scale), float(dV), _ptr(V, F64, "V"),
+          _ptr(work, None, "work"), work.numel() * work.element_size(), _stream())
 
 
 def fft3d(c, sign):
@@ -342,6 +345,14 @@ def real_to_complex(x, g, s, out):
           _stream())
 
 
+def real_pair_to_complex(x, g, s, out):
+    """out ((B+1)/2, g, g, g) complex128 <- s (x[2p] + i x[2p+1])."""
+    B, ld = x.shape
+    assert out.dtype == torch.complex128 and out.shape == ((B + 1) // 2, g, g, g)
+    _call("hqc_real_pair_to_complex", _ptr(x, F64, "x"), B, g, ld, float(s),
+          _ptr(out, None, "out"), _stream())
+
+
 def complex_real(c, ld, out):
     B, g = c.shape[0], c.shape[1]
     assert out.shape == (B, ld)
@@ -361,10 +372,13 @@ def xc_blocks(g):
     return lib().hqc_xc_blocks(g)
 
 
-def xc_eval(rho, vh, g, xc_id, vout, part):
+def xc_eval(rho, vh, g, xc_id, vout, part, vh_paired=False):
+    """vh: complex (B, g, g, g), or ((B+1)/2, g, g, g) geometry pairs (Re, Im) with
+    vh_paired (the output of a transform of real_pair_to_complex)."""
     B, ld = rho.shape
-    _call("hqc_xc_eval", _ptr(rho, F64, "rho"), _ptr(vh, None, "vh"), B, g, ld, int(xc_id),
-          _ptr(vout, F64, "vout"), _ptr(part, F64, "part"), _stream())
+    assert vh.dtype == torch.complex128 and vh.shape[0] == ((B + 1) // 2 if vh_paired else B)
+    _call("hqc_xc_eval", _ptr(rho, F64, "rho"), _ptr(vh, None, "vh"), int(bool(vh_paired)), B,
+          g, ld, int(xc_id), _ptr(vout, F64, "vout"), _ptr(part, F64, "part"), _stream())
 
 
 def occupations(w, nelec, sigma, method, search, max_cycle, tol, mu_shift, occ, mu):

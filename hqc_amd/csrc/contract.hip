@@ -831,7 +831,8 @@ extern "C" int hqc_ao_vmat_f32(const float* phi, const int* bidx, const double* 
 // phi is kept as FP16 pairs (hi, lo) with s phi = hi + lo to 22 significant
 // bits (s = 2^k chosen by the caller so that |s phi| <= 2^14; hqc_phi_split_f16).
 // While a chunk is staged in LDS, v phi is formed in FP32 from the pair and
-// split the same way, with a per-geometry power-of-two scale sv (|sv v| <= 1).
+// split the same way, with a per-geometry power-of-two scale sv (|sv v| <= 1)
+// from max_r |v| (the caller's, or a reduction pass of this entry point).
 // Every 16x16 tile takes hi*hi + hi*lo + lo*hi per 32 grid points on
 // v_mfma_f32_16x16x32_f16 (operands: 8 consecutive k per lane, the same k order
 // for both; result layout as v_mfma_f32_16x16x4f32, measured); the dropped lo*lo
@@ -864,9 +865,9 @@ __global__ __launch_bounds__(256) void k_phi_split_f16(const double* __restrict_
   }
 }
 
-// sv[b] = 2^-ceil(log2 max_r |v[b,r]|) (1 for v = 0)
-__global__ __launch_bounds__(256) void k_vscale(const double* __restrict__ v, int ld,
-                                                float* __restrict__ sv) {
+// vm[b] = max_r |v[b,r]|
+__global__ __launch_bounds__(256) void k_vmax(const double* __restrict__ v, int ld,
+                                              double* __restrict__ vm) {
   __shared__ double red[4];
   const int b = blockIdx.x;
   double m = 0.0;
@@ -874,17 +875,19 @@ __global__ __launch_bounds__(256) void k_vscale(const double* __restrict__ v, in
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    sv[b] = m > 0.0 ? (float)exp2(-ceil(log2(m))) : 1.0f;
-  }
+  if (threadIdx.x == 0) vm[b] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// power-of-two scale with |s v| <= 1 (1 for v = 0)
+__device__ __forceinline__ float vscale_of(double m) {
+  return m > 0.0 ? (float)exp2(-ceil(log2(m))) : 1.0f;
 }
 
 template <int TPW, int NBUF>
 __global__ __launch_bounds__(256) void k_vmat_h2(const unsigned* __restrict__ ph,
                                                  const int* __restrict__ bidx,
                                                  const double* __restrict__ v,
-                                                 const float* __restrict__ sv, double sphi,
+                                                 const double* __restrict__ vmax, double sphi,
                                                  int nao_pad, int ld, int ntiles, int nwg,
                                                  double* __restrict__ part) {
   constexpr int KC = VH_KC;
@@ -911,7 +914,7 @@ __global__ __launch_bounds__(256) void k_vmat_h2(const unsigned* __restrict__ ph
   }
   const unsigned* P = ph + (long)(bidx ? bidx[b] : b) * nao_pad * ld;
   const double* vb = v + (long)b * ld;
-  const float svb = sv[b];
+  const float svb = vscale_of(vmax[b]);
   u32x4 stage[VMAT_PIPE_MAXV];
   double vst[4];
   auto gload = [&](long c0) {
@@ -1038,13 +1041,13 @@ extern "C" int hqc_ao_vmat_h2_workspace(int B, int nao_pad, int ld, size_t* byte
   if (!bytes || B < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63))
     return HQC_E_ARG;
   if (!vmat_pipelined(nao_pad)) return HQC_E_SIZE;
-  *bytes = vmat_h2_part_bytes(B, nao_pad, ld) + (size_t)B * sizeof(float);
+  *bytes = vmat_h2_part_bytes(B, nao_pad, ld) + (size_t)B * sizeof(double);
   return HQC_OK;
 }
 
-extern "C" int hqc_ao_vmat_h2(const unsigned* ph, const int* bidx, const double* v, int B,
-                              int nao_pad, int ld, double phi_scale, double dV, double* V,
-                              void* work, size_t work_bytes, void* stream) {
+extern "C" int hqc_ao_vmat_h2(const unsigned* ph, const int* bidx, const double* v,
+                              const double* vmax, int B, int nao_pad, int ld, double phi_scale,
+                              double dV, double* V, void* work, size_t work_bytes, void* stream) {
   if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63) || !(phi_scale > 0.0))
     return HQC_E_ARG;
   if (!vmat_pipelined(nao_pad)) return HQC_E_SIZE;
@@ -1052,12 +1055,15 @@ extern "C" int hqc_ao_vmat_h2(const unsigned* ph, const int* bidx, const double*
   int KC, ntiles, ngroups, nwg;
   vmat_shape(B, nao_pad, ld, 32, KC, ntiles, ngroups, nwg);
   const size_t pbytes = vmat_h2_part_bytes(B, nao_pad, ld);
-  if (!work || work_bytes < pbytes + (size_t)B * sizeof(float)) return HQC_E_ARG;
+  if (!work || work_bytes < pbytes + (size_t)B * sizeof(double)) return HQC_E_ARG;
   hipStream_t s = (hipStream_t)stream;
   double* part = (double*)work;
-  float* sv = (float*)((char*)work + pbytes);
-  hipLaunchKernelGGL(k_vscale, dim3(B), dim3(256), 0, s, v, ld, sv);
-  HQC_CHECK_LAUNCH();
+  if (!vmax) {
+    double* vm = (double*)((char*)work + pbytes);
+    hipLaunchKernelGGL(k_vmax, dim3(B), dim3(256), 0, s, v, ld, vm);
+    HQC_CHECK_LAUNCH();
+    vmax = vm;
+  }
   const size_t lds = (size_t)VH_NBUF * 4 * nao_pad * (VH_KC + 8) * sizeof(_Float16);
   const dim3 grid(nwg, ngroups, B);
 #define VMAT_H2_CASE(TP)                                                                     \
@@ -1065,7 +1071,7 @@ extern "C" int hqc_ao_vmat_h2(const unsigned* ph, const int* bidx, const double*
     if (hipFuncSetAttribute((const void*)k_vmat_h2<TP, VH_NBUF>,                             \
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) \
       (void)hipGetLastError();                                                               \
-    hipLaunchKernelGGL((k_vmat_h2<TP, VH_NBUF>), grid, dim3(256), lds, s, ph, bidx, v, sv,    \
+    hipLaunchKernelGGL((k_vmat_h2<TP, VH_NBUF>), grid, dim3(256), lds, s, ph, bidx, v, vmax,  \
                        phi_scale, nao_pad, ld, ntiles, nwg, part);                           \
   } break;
   switch (vmat_pipe_tpw(nao_pad)) {

@@ -56,6 +56,19 @@ __global__ void k_r2c(const double* __restrict__ in, int N, int ld, double s,
   out[(long)b * N + r] = make_double2(s * in[(long)b * ld + r], 0.0);
 }
 
+// two real grids per complex grid: out[p] = s (in[2p] + i in[2p+1]) (the last
+// imaginary part 0 for odd B); with a real, even kernel such as VG the transform,
+// multiply and inverse transform keep the two grids apart in Re and Im
+__global__ void k_r2c_pair(const double* __restrict__ in, int B, int N, int ld, double s,
+                           double2* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (r >= N) return;
+  const double re = s * in[(long)(2 * p) * ld + r];
+  const double im = 2 * p + 1 < B ? s * in[(long)(2 * p + 1) * ld + r] : 0.0;
+  out[(long)p * N + r] = make_double2(re, im);
+}
+
 __global__ void k_c2r(const double2* __restrict__ in, int N, int ld, double* __restrict__ out) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
@@ -82,6 +95,17 @@ extern "C" int hqc_real_to_complex(const double* in, int B, int g, int ld, doubl
   const int N = g * g * g;
   hipLaunchKernelGGL(k_r2c, dim3(hqc_div_up(N, 256), B), dim3(256), 0, (hipStream_t)stream, in, N,
                      ld, s, (double2*)out);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+extern "C" int hqc_real_pair_to_complex(const double* in, int B, int g, int ld, double s,
+                                        double* out, void* stream) {
+  if (B < 0 || g <= 0 || ld < g * g * g) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  const int N = g * g * g;
+  hipLaunchKernelGGL(k_r2c_pair, dim3(hqc_div_up(N, 256), (B + 1) / 2), dim3(256), 0,
+                     (hipStream_t)stream, in, B, N, ld, s, (double2*)out);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }
@@ -162,14 +186,18 @@ __device__ __forceinline__ void xc_point(double rho, int xc_id, double& eps, dou
 
 __global__ __launch_bounds__(256) void k_xc_eval(const double* __restrict__ rho,
                                                  const double2* __restrict__ vh, int N, int ld,
-                                                 int xc_id, double* __restrict__ vout,
+                                                 int xc_id, int paired,
+                                                 double* __restrict__ vout,
                                                  double* __restrict__ part) {
   __shared__ double red[4];
   const int b = blockIdx.y;
+  // paired: geometry b's v_H is Re (b even) or Im (b odd) of complex grid b/2
+  const double* vhb = reinterpret_cast<const double*>(vh) +
+                      (paired ? (long)(b >> 1) * N * 2 + (b & 1) : (long)b * N * 2);
   double se = 0.0, sh = 0.0;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
     const double p = rho[(long)b * ld + r];
-    const double h = vh[(long)b * N + r].x;
+    const double h = vhb[2 * (long)r];
     double eps = 0.0, v = 0.0;
     if (xc_id != 0) xc_point(p, xc_id, eps, v);
     vout[(long)b * ld + r] = h + v;
@@ -189,12 +217,12 @@ extern "C" int hqc_xc_blocks(int g) {
   return XC_BLOCKS;
 }
 
-extern "C" int hqc_xc_eval(const double* rho, const double* vh, int B, int g, int ld, int xc_id,
-                           double* vout, double* part, void* stream) {
+extern "C" int hqc_xc_eval(const double* rho, const double* vh, int vh_paired, int B, int g,
+                           int ld, int xc_id, double* vout, double* part, void* stream) {
   if (B < 0 || g <= 0 || ld < g * g * g || xc_id < 0 || xc_id > 5) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
   hipLaunchKernelGGL(k_xc_eval, dim3(XC_BLOCKS, B), dim3(256), 0, (hipStream_t)stream, rho,
-                     (const double2*)vh, g * g * g, ld, xc_id, vout, part);
+                     (const double2*)vh, g * g * g, ld, xc_id, vh_paired ? 1 : 0, vout, part);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }

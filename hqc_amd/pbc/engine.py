@@ -230,15 +230,17 @@ class Engine:
         out[:, :c.nao, :c.nao] = M
         return out
 
-    def vmat(self, phi, v, dV=None, bidx=None):
+    def vmat(self, phi, v, dV=None, bidx=None, vmax=None):
         """dV * phi diag(v) phi^T for a batch -> (B, nao, nao); dV a float (None: the
         cell's) or a (B,) tensor of per-geometry volume elements; bidx (B,) int32
-        selects the geometries of phi (None: phi holds exactly the batch)."""
+        selects the geometries of phi (None: phi holds exactly the batch); vmax (B,)
+        max_r |v| if known (used by the split-product kernel)."""
         if dV is None:
-            return self.vmat_dv(phi, v, self.c.dV, bidx)
+            return self.vmat_dv(phi, v, self.c.dV, bidx, vmax)
         if torch.is_tensor(dV):
-            return self.vmat_dv(phi, v * dV[:, None], 1.0, bidx)
-        return self.vmat_dv(phi, v, dV, bidx)
+            return self.vmat_dv(phi, v * dV[:, None], 1.0, bidx,
+                                None if vmax is None else vmax * dV)
+        return self.vmat_dv(phi, v, dV, bidx, vmax)
 
     @staticmethod
     def _dv(st, na):
@@ -249,13 +251,13 @@ class Engine:
     def _geo(st, na):
         return {k: st[k][:na] for k in ("Lb", "dVb", "VGb")} if st.get("hetero") else None
 
-    def vmat_dv(self, phi, v, dV, bidx=None):
+    def vmat_dv(self, phi, v, dV, bidx=None, vmax=None):
         c = self.c
         B = v.shape[0]
         V = torch.empty(B, c.nao_pad, c.nao_pad, dtype=torch.float64, device=c.device)
         if phi.dtype == torch.int32:                  # FP16 pairs: split-precision products
             K.ao_vmat_h2(phi, v.contiguous(), self._phi_scale, dV, V, self._workspace(B, 16),
-                         bidx)
+                         bidx, None if vmax is None else vmax.contiguous())
         else:
             prec = 64 if phi.dtype == torch.float64 else 32
             K.ao_vmat(phi, v.contiguous(), dV, V, self._workspace(B, prec), bidx)
@@ -282,12 +284,23 @@ class Engine:
         K.orb_density(phi, At, rho, bidx)
         return rho
 
-    def poisson(self, rho, save=False, geo=None):
+    def poisson(self, rho, save=False, geo=None, paired=False):
         """v_H on the grid (complex buffer, real part = v_H) and optionally
         rho~(G) = dV FFT(rho).  geo: per-geometry (Lb, dVb, VGb) of a batch of
-        different cell sizes (None: the cell's constants)."""
+        different cell sizes (None: the cell's constants).  paired (only without
+        save and geo): geometries 2p, 2p+1 share one complex grid as Re and Im, so a
+        batch takes half the transforms; v_H of geometry b is then Re or Im of grid b/2."""
         c = self.c
         B = rho.shape[0]
+        if paired:
+            assert not save and geo is None
+            cbuf = torch.empty((B + 1) // 2, c.g, c.g, c.g, dtype=torch.complex128,
+                               device=c.device)
+            K.real_pair_to_complex(rho, c.g, c.dV, cbuf)
+            K.fft3d(cbuf, -1)
+            K.poisson_mul(cbuf, c.VG)
+            K.fft3d(cbuf, +1)
+            return cbuf, None
         cbuf = torch.empty(B, c.g, c.g, c.g, dtype=torch.complex128, device=c.device)
         if geo is None:
             K.real_to_complex(rho, c.g, c.dV, cbuf)
@@ -311,11 +324,12 @@ class Engine:
         derivative, the exact derivative of the discrete functional."""
         c = self.c
         B, g = rho.shape[0], c.g
-        vh, rg = self.poisson(rho, save=self.gga or save_rg, geo=geo)
+        paired = not (self.gga or save_rg) and geo is None and B > 1
+        vh, rg = self.poisson(rho, save=self.gga or save_rg, geo=geo, paired=paired)
         v = torch.zeros(B, c.ld, dtype=torch.float64, device=c.device)
         if not self.gga:
             part = torch.empty(B, K.xc_blocks(g), 2, dtype=torch.float64, device=c.device)
-            K.xc_eval(rho, vh, g, self.xc_id, v, part)
+            K.xc_eval(rho, vh, g, self.xc_id, v, part, vh_paired=paired)
             return v, part.sum(1), rg
         grad = torch.empty(B, 3, g, g, g, dtype=torch.complex128, device=c.device)
         if geo is None:
@@ -627,9 +641,9 @@ class Engine:
 
     def _mixed_vmat(self, st, na, v):
         """V(v) for geometries [0, na) in the mixed mode, per geometry like the density:
-        FP32 V(v) outright in the large-residual phase, V_a + V(v - v_a) in FP32 where
-        the increment bound holds (both in one FP32 launch), FP64 (and a new anchor)
-        elsewhere."""
+        low-precision V(v) outright in the large-residual phase, V_a + V(v - v_a) in low
+        precision where the increment bound holds (both in one launch of the FP16
+        split-product kernel, or FP32), FP64 (and a new anchor) elsewhere."""
         c = self.c
         sel = lambda key: st[key][:na]
         err = sel("err")
@@ -639,7 +653,8 @@ class Engine:
         tol_e = self.MIXED_SAFETY * err / c.nao                 # per-element error budget
         lp = "phi_h2" if "phi_h2" in st else "phi_lp"
         eps = self.MIXED_EPS_SPLIT16 if lp == "phi_h2" else self.MIXED_EPS
-        inc = ~direct & sel("va_set") & (eps * dv.abs().amax(-1) <= tol_e)
+        dvmax = dv.abs().amax(-1)
+        inc = ~direct & sel("va_set") & (eps * dvmax <= tol_e)
         low = direct | inc
         dVs = self._dv(st, na)
         V2 = torch.empty(na, c.nao, c.nao, dtype=torch.float64, device=c.device)
@@ -647,9 +662,11 @@ class Engine:
         if n:
             il = None if idx is None else idx.long()
             vv = torch.where(direct[:, None], v, dv)
+            vvm = torch.where(direct, st["vmax"][:na], dvmax)
             d = dVs if not torch.is_tensor(dVs) or il is None else dVs[il]
             V32 = self.vmat(st[lp] if n < na else sel(lp),
-                            vv if il is None else vv[il], d, idx)
+                            vv if il is None else vv[il], d, idx,
+                            vvm if il is None else vvm[il])
             V32 = V32 + torch.where(inc[:, None, None], sel("Va"), 0.0)[
                 slice(None) if il is None else il]
             self._scatter(V2, idx, V32)

@@ -37,8 +37,9 @@ class KEngine(Engine):
         self._kwork = None
 
     # ------------------------------------------------------------- contractions
-    def vmat_dv(self, phi, v, dV, bidx=None):
-        """dV phi^H diag(v) phi (B, nao, nao) complex from stacked phi (B, 2 nao_pad, ld)."""
+    def vmat_dv(self, phi, v, dV, bidx=None, vmax=None):
+        """dV phi^H diag(v) phi (B, nao, nao) complex from stacked phi (B, 2 nao_pad, ld)
+        (FP64; vmax is not used)."""
         c = self.c
         B = v.shape[0]
         n2 = 2 * c.nao_pad

@@ -133,14 +133,15 @@ int hqc_ao_vmat_f32(const float* phi, const int* bidx, const double* v, int B, i
  * hqc_phi_split_f16: out[i] = (hi, lo) FP16 pair (hi in the low half) with
  *   scale*phi[i] = hi + lo to 22 significant bits; scale = 2^k, |scale phi| <= 2^14.
  * hqc_ao_vmat_h2: V[b] = dV phi diag(v[b]) phi^T from those pairs; v phi is split
- *   the same way in LDS with a per-geometry power-of-two scale, and each tile takes
+ *   the same way in LDS with a per-geometry power-of-two scale from vmax[b] =
+ *   max_r |v[b,r]| (device array, or NULL: computed here), and each tile takes
  *   hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16 with FP32 accumulation
  *   (relative error ~2^-22 of sum |phi phi v|).  work: hqc_ao_vmat_h2_workspace bytes. */
 int hqc_phi_split_f16(const double* phi, long n, double scale, unsigned* out, void* stream);
 int hqc_ao_vmat_h2_workspace(int B, int nao_pad, int ld, size_t* bytes);
-int hqc_ao_vmat_h2(const unsigned* ph, const int* bidx, const double* v, int B, int nao_pad,
-                   int ld, double phi_scale, double dV, double* V, void* work,
-                   size_t work_bytes, void* stream);
+int hqc_ao_vmat_h2(const unsigned* ph, const int* bidx, const double* v, const double* vmax,
+                   int B, int nao_pad, int ld, double phi_scale, double dV, double* V,
+                   void* work, size_t work_bytes, void* stream);
 
 /* Batched in-place 3-D complex FFT of B grids g^3 (any g <= 128; mixed-radix
  * Stockham passes in LDS, one launch per axis).  sign = -1 forward,
@@ -178,6 +179,12 @@ int hqc_vloc_G(const double* xp, int B, int n, int g, double L, const double* VG
 /* Real grid (B,ld) -> complex (B,g^3) scaled by s: out = (s*in, 0). */
 int hqc_real_to_complex(const double* in, int B, int g, int ld, double s, double* out,
                         void* stream);
+/* Two real grids per complex grid: out ((B+1)/2, g^3) = s (in[2p] + i in[2p+1]),
+ * imaginary part 0 for the last grid of an odd B.  With a real, even G-space
+ * kernel (VG) the FFT, multiply and inverse FFT keep the two results in Re and
+ * Im: one transform pair per two geometries. */
+int hqc_real_pair_to_complex(const double* in, int B, int g, int ld, double s, double* out,
+                             void* stream);
 /* Complex (B,g^3) -> real part into (B,ld). */
 int hqc_complex_real(const double* in, int B, int g, int ld, double* out, void* stream);
 /* Poisson kernel in G space: c[b,G] *= VG[G] (VG (g^3) shared, or (B,g^3) one
@@ -190,13 +197,15 @@ int hqc_poisson_mul(double* c, const double* VG, int B, int g, int vg_batched, d
  * Replaces exc_functional / vxc_functional (hqc/pbc/solver.py:669-678,
  * hqc/tools/excor.py) with closed-form derivatives.
  *   xc_id: 0 none (HF), 1 lda+vwn, 2 lda only, 3 lda+hl71, 4 lda+bh72, 5 lda+gl77
- * rho (B,ld); vh complex (B,g^3) from the inverse FFT (real part = v_H);
+ * rho (B,ld); vh complex (B,g^3) from the inverse FFT (real part = v_H), or
+ * with vh_paired the ((B+1)/2, g^3) grids of hqc_real_pair_to_complex (v_H of
+ * geometry b = Re (b even) or Im (b odd) of grid b/2);
  * vout (B,ld) = v_H + v_xc (padding left untouched);
  * part (B, nblk, 2) per-block sums of rho*eps_xc and rho*v_H; nblk is
  * returned by hqc_xc_blocks. */
 int hqc_xc_blocks(int g);
-int hqc_xc_eval(const double* rho, const double* vh, int B, int g, int ld, int xc_id,
-                double* vout, double* part, void* stream);
+int hqc_xc_eval(const double* rho, const double* vh, int vh_paired, int B, int g, int ld,
+                int xc_id, double* vout, double* part, void* stream);
 
 /* GGA (PBE) on the grid; new in this build (the reference accepts only LDA
  * exchange, hqc/tools/excor.py:14-17).  Spectral derivatives on the odd grid:

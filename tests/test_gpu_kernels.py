@@ -112,6 +112,10 @@ def test_vmat_split16(K, nao_pad, g, B):
           flush=True)
     assert ((V - Vref).abs() <= 2.0 ** -19 * bound + 1e-300).all()
     assert torch.equal(V, V.transpose(1, 2))
+    # the caller's max |v| gives the same scales, hence the same result
+    V3 = torch.empty_like(V)
+    K.ao_vmat_h2(ph, v, scale, 0.5, V3, work, vmax=v.abs().amax(-1))
+    assert torch.equal(V3, V)
     # geometry index list: entries read phi geometries in reverse order
     bidx = torch.arange(B - 1, -1, -1, dtype=torch.int32, device=DEV)
     V2 = torch.empty_like(V)
@@ -300,6 +304,25 @@ def test_xc_eval(K, xc):
     sums = part.sum(1)[0].cpu().numpy()
     assert np.isclose(sums[0], np.sum(rho_np * eps), rtol=1e-12)
     assert np.isclose(sums[1], np.sum(rho_np * vh_np), rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("B", [2, 3])
+def test_paired_poisson_matches_single(K, B):
+    """Two geometries per complex FFT (Re, Im) give the same v_H + v_xc and energy
+    sums as one transform per geometry (the Coulomb kernel is real and even on the
+    odd grid), also for an odd batch."""
+    from hqc_amd.pbc import make_pes
+    n = 4
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    eng = make_pes(n, L, 1.5, "gth-dzv", grid_length=0.3, dft=True).solver.engine
+    c = eng.c
+    torch.manual_seed(B)
+    rho = torch.zeros(B, c.ld, dtype=torch.float64, device=DEV)
+    rho[:, :c.N] = torch.rand(B, c.N, dtype=torch.float64, device=DEV) + 1e-3
+    v1, E1, _ = eng.xc_potential(rho)                  # paired
+    v2, E2, _ = eng.xc_potential(rho, save_rg=True)    # one grid per geometry
+    assert torch.allclose(v1, v2, rtol=1e-12, atol=1e-12 * v2.abs().max().item())
+    assert torch.allclose(E1, E2, rtol=1e-12, atol=1e-12)
 
 
 def test_pbe_points(K):
