@@ -33,6 +33,7 @@ _SIGS = {
     "hqc_ao_vmat_f64": [_P, _P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_ao_vmat_f32": [_P, _P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_phi_split_f16": [_P, ctypes.c_long, _D, _P, _P],
+    "hqc_orb_density_h2": [_P, _P, _P, _I, _I, _I, _I, _D, _P, _P],
     "hqc_ao_vmat_h2_workspace": [_I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
     "hqc_ao_vmat_h2": [_P, _P, _P, _P, _I, _I, _I, _D, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
@@ -193,11 +194,17 @@ def ao_density(phi, D, rho, X=None, bidx=None):
               B, nao_pad, ld, _ptr(rho, F64, "rho"), _stream())
 
 
-def orb_density(phi, At, rho, bidx=None):
+def orb_density(phi, At, rho, bidx=None, phi_scale=None):
+    """phi FP64, FP32, or int32 FP16 pairs of phi_split_f16 (then phi_scale is its scale)."""
     nao_pad, ld = phi.shape[1], phi.shape[2]
     B, m_pad = At.shape[0], At.shape[2]
     assert At.shape == (B, nao_pad, m_pad) and rho.shape == (B, ld)
     bi = _bidx(phi, bidx, B)
+    if phi.dtype == I32:
+        _call("hqc_orb_density_h2", _ptr(phi, I32, "phi"), _ptr(bi, I32, "bidx"),
+              _ptr(At, F64, "At"), B, nao_pad, m_pad, ld, float(phi_scale),
+              _ptr(rho, F64, "rho"), _stream())
+        return
     name = "hqc_orb_density_f64" if phi.dtype == F64 else "hqc_orb_density_f32"
     _call(name, _ptr(phi, None, "phi"), _ptr(bi, I32, "bidx"), _ptr(At, F64, "At"), B, nao_pad,
           m_pad, ld, _ptr(rho, F64, "rho"), _stream())
@@ -406,13 +413,15 @@ def jacobi_pairs(n, device):
     return _PAIRS[key]
 
 
-def eigh_jacobi(A, max_sweeps=30, tol=1e-14, sweeps=None):
-    """Ascending eigenvalues and column eigenvectors of symmetric A (B, n, n), n <= 110."""
+def eigh_jacobi(A, max_sweeps=30, tol=1e-14, sweeps=None, pairs=None):
+    """Ascending eigenvalues and column eigenvectors of symmetric A (B, n, n), n <= 110.
+    pairs: a schedule table (jacobi_pairs) or None for the same schedule computed on
+    the device."""
     B, n, _ = A.shape
     A = A.contiguous()
     w = torch.empty(B, n, dtype=F64, device=A.device)
     V = torch.empty(B, n, n, dtype=F64, device=A.device)
-    _call("hqc_eigh_jacobi", _ptr(A, F64, "A"), B, n, _ptr(jacobi_pairs(n, A.device), I32),
+    _call("hqc_eigh_jacobi", _ptr(A, F64, "A"), B, n, _ptr(pairs, I32, "pairs"),
           int(max_sweeps), float(tol), _ptr(w, F64, "w"), _ptr(V, F64, "V"),
           _ptr(sweeps, I32, "sweeps"), _stream())
     return w, V

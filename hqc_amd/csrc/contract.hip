@@ -1092,3 +1092,163 @@ extern "C" int hqc_ao_vmat_h2(const unsigned* ph, const int* bidx, const double*
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Orbital-form density from the FP16 pairs of hqc_phi_split_f16:
+//   rho[b,r] = sum_i (sum_k At[b,k,i] phi[b,k,r])^2
+// with At split the same way (per-geometry power-of-two scale, |s At| <= 2^14),
+// hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16, FP32 accumulation of psi,
+// FP64 accumulation of psi^2.  Workgroup = 4 waves x 64 grid columns (4 column
+// tiles per wave) over ODENS_CG column groups; the split At of up to 64
+// orbitals is staged in LDS once per workgroup, orbital-major so one 16-byte
+// read gives a lane its 8 consecutive AOs; the phi pairs of 8 AO rows come from
+// global memory per lane (rows past nao_pad re-read the last row, whose At
+// rows are zero).  Relative error of psi ~2^-22 of sum_k |At phi|.
+// ---------------------------------------------------------------------------
+#define OH_CT 4
+
+__device__ __forceinline__ void split_h2(double x, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)x;
+  lo = (_Float16)(x - (double)hi);
+}
+
+template <int MT>
+__global__ __launch_bounds__(256) void k_orb_density_h2(const unsigned* __restrict__ ph,
+                                                        const int* __restrict__ bidx,
+                                                        const double* __restrict__ At,
+                                                        int nao_pad, int m_pad, int ld,
+                                                        double sphi, double* __restrict__ rho) {
+  constexpr int MW = MT * 16;
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ double red[4];
+  const int KP = (nao_pad + 31) & ~31;            // AO count padded to the MFMA depth
+  const int STR = KP + 8;                         // halves; 16 (mod 128) bytes per row
+  _Float16* Ah = reinterpret_cast<_Float16*>(smem_raw);   // [MW][STR]
+  _Float16* Al = Ah + MW * STR;
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const unsigned* Pb = ph + (long)(bidx ? bidx[b] : b) * nao_pad * ld;
+  const double* Ab = At + (long)b * nao_pad * m_pad;
+  // per-geometry scale of At: |sA At| <= 2^14
+  double m = 0.0;
+  for (int e = threadIdx.x; e < nao_pad * m_pad; e += 256) m = fmax(m, fabs(Ab[e]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (lane == 0) red[w] = m;
+  __syncthreads();
+  m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  const double sA = m > 0.0 ? exp2(14.0 - ceil(log2(m))) : 1.0;
+  const double inv = 1.0 / (sA * sphi * sA * sphi);
+  double rs[ODENS_CG][OH_CT];
+#pragma unroll
+  for (int g = 0; g < ODENS_CG; ++g)
+#pragma unroll
+    for (int c = 0; c < OH_CT; ++c) rs[g][c] = 0.0;
+  for (int mg = 0; mg < m_pad; mg += MW) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < MW * KP; e += 256) {
+      const int i = e / KP, k = e - i * KP;
+      _Float16 hi = (_Float16)0, lo = (_Float16)0;
+      if (k < nao_pad && mg + i < m_pad) split_h2(Ab[(long)k * m_pad + mg + i] * sA, hi, lo);
+      Ah[i * STR + k] = hi;
+      Al[i * STR + k] = lo;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < ODENS_CG; ++g) {
+      const long col0 = ((long)blockIdx.x * ODENS_CG + g) * 256 + w * 64;
+      if (col0 >= ld) continue;                   // wave-uniform (ld is a multiple of 64)
+      const unsigned* P = Pb + col0 + l16;
+      f32x4 acc[MT][OH_CT];
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int c = 0; c < OH_CT; ++c) acc[i][c] = f32x4{0, 0, 0, 0};
+      for (int k0 = 0; k0 < KP; k0 += 32) {
+        unsigned u[OH_CT][8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int row = min(k0 + 8 * kq + e, nao_pad - 1);
+#pragma unroll
+          for (int c = 0; c < OH_CT; ++c) u[c][e] = P[(long)row * ld + c * 16];
+        }
+        h16x8 ah[MT], al[MT];
+#pragma unroll
+        for (int i = 0; i < MT; ++i) {
+          ah[i] = *reinterpret_cast<const h16x8*>(Ah + (i * 16 + l16) * STR + k0 + 8 * kq);
+          al[i] = *reinterpret_cast<const h16x8*>(Al + (i * 16 + l16) * STR + k0 + 8 * kq);
+        }
+#pragma unroll
+        for (int c = 0; c < OH_CT; ++c) {
+          h16x8 bh, bl;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            bh[e] = __builtin_bit_cast(_Float16, (unsigned short)(u[c][e] & 0xffffu));
+            bl[e] = __builtin_bit_cast(_Float16, (unsigned short)(u[c][e] >> 16));
+          }
+#pragma unroll
+          for (int i = 0; i < MT; ++i) {
+            acc[i][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh, acc[i][c], 0, 0, 0);
+            acc[i][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl, acc[i][c], 0, 0, 0);
+            acc[i][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh, acc[i][c], 0, 0, 0);
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int c = 0; c < OH_CT; ++c)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const double y = (double)acc[i][c][e];
+            rs[g][c] += y * y;
+          }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < ODENS_CG; ++g) {
+    const long col0 = ((long)blockIdx.x * ODENS_CG + g) * 256 + w * 64;
+    if (col0 >= ld) continue;
+#pragma unroll
+    for (int c = 0; c < OH_CT; ++c) {
+      double v = rs[g][c];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (kq == 0) rho[(long)b * ld + col0 + c * 16 + l16] = v * inv;
+    }
+  }
+}
+
+extern "C" int hqc_orb_density_h2(const unsigned* ph, const int* bidx, const double* At, int B,
+                                  int nao_pad, int m_pad, int ld, double phi_scale, double* rho,
+                                  void* stream) {
+  if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || m_pad <= 0 || (m_pad & 15) || ld <= 0 ||
+      (ld & 63) || !(phi_scale > 0.0))
+    return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  const int mt = min(m_pad >> 4, 4);
+  const int KP = (nao_pad + 31) & ~31;
+  const size_t lds = (size_t)2 * mt * 16 * (KP + 8) * sizeof(_Float16);
+  if (lds > 96 * 1024) return HQC_E_SIZE;
+  const dim3 grid((ld + 256 * ODENS_CG - 1) / (256 * ODENS_CG), B);
+  hipStream_t s = (hipStream_t)stream;
+#define OH2_CASE(M)                                                                          \
+  case M: {                                                                                  \
+    if (lds > 64 * 1024 &&                                                                   \
+        hipFuncSetAttribute((const void*)k_orb_density_h2<M>,                                \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) \
+      (void)hipGetLastError();                                                               \
+    hipLaunchKernelGGL((k_orb_density_h2<M>), grid, dim3(256), lds, s, ph, bidx, At, nao_pad, \
+                       m_pad, ld, phi_scale, rho);                                           \
+  } break;
+  switch (mt) {
+    OH2_CASE(1)
+    OH2_CASE(2)
+    OH2_CASE(3)
+    default:
+    OH2_CASE(4)
+  }
+#undef OH2_CASE
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}

@@ -65,6 +65,10 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
   __shared__ int s_done;
   if (tid == 0) s_fro2 = fro2;
   __syncthreads();
+  // threshold Jacobi: rotations with |a_pq| <= thr are skipped.  If every
+  // off-diagonal element is below thr, sum_{p<q} a_pq^2 < n^2/2 thr^2 =
+  // tol^2 |A|_F^2 / 2 and the stopping test passes, so the skip never stalls it
+  const double thr = tol * sqrt(s_fro2) / n;
   // static split of the V update: vstep threads per row (blockDim >= n)
   const int vstep = blockDim.x / n;
   const int vrow = vstep > 0 ? tid % n : n;
@@ -86,9 +90,20 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
     __syncthreads();
     if (s_done) break;
     for (int r = 0; r < npad - 1; ++r) {
-      const int* prow = pairs + r * npad;
+      const int* prow = pairs ? pairs + r * npad : nullptr;
       for (int k = tid; k < np; k += blockDim.x) {
-        int p = prow[2 * k], q = prow[2 * k + 1];
+        int p, q;
+        if (prow) {
+          p = prow[2 * k];
+          q = prow[2 * k + 1];
+        } else {
+          // circle method: index 0 fixed, the others rotated r places; pair k joins
+          // positions k and npad-1-k (the schedule hqc_amd's host code builds)
+          const int m = npad - 1;
+          auto at = [&](int j) { return j == 0 ? 0 : 1 + ((j - 1 - r) % m + m) % m; };
+          p = at(k);
+          q = at(npad - 1 - k);
+        }
         if (p > q) {
           const int t = p;
           p = q;
@@ -99,7 +114,7 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
         double c = 1.0, s = 0.0;
         if (q < n) {
           const double apq = Ap[pk(p, q, n)];
-          if (apq != 0.0) {
+          if (fabs(apq) > thr) {
             const double app = Ap[pk(p, p, n)], aqq = Ap[pk(q, q, n)];
             const double theta = (aqq - app) / (2.0 * apq);
             const double t = (theta >= 0.0 ? 1.0 : -1.0) /
@@ -129,6 +144,7 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
           }
           continue;
         }
+        if (sI == 0.0 && sJ == 0.0) continue;     // both rotations skipped: block unchanged
         // generic block: rows {pI,qI}, cols {pJ,qJ}; M' = R_I^T M R_J
         const double m00 = getA(Ap, pI, pJ, n);
         const double m01 = vqJ ? getA(Ap, pI, qJ, n) : 0.0;

@@ -281,7 +281,8 @@ class Engine:
         At = torch.zeros(B, c.nao_pad, m_pad, dtype=torch.float64, device=c.device)
         At[:, :c.nao, :keep] = mo[:, :, :keep] * occ[:, None, :keep].clamp(min=0).sqrt()
         rho = torch.zeros(B, c.ld, dtype=torch.float64, device=c.device)
-        K.orb_density(phi, At, rho, bidx)
+        K.orb_density(phi, At, rho, bidx,
+                      self._phi_scale if phi.dtype == torch.int32 else None)
         return rho
 
     def poisson(self, rho, save=False, geo=None, paired=False):
@@ -420,7 +421,10 @@ class Engine:
         dT = torch.empty_like(dS) if forces else None
         phi = torch.zeros(B, c.nao_pad, c.ld, dtype=f64, device=dev)
         mixed = self.precision == "mixed"
-        phi32 = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float32, device=dev) if mixed else None
+        use_h2 = mixed and self.MIXED_SPLIT16 and c.nao_pad <= 128
+        # FP32 AO values: the FP32 kernels' operand, unless FP16 pairs serve them all
+        need32 = mixed and (not use_h2 or self.MIXED_RHO_INCREMENTS)
+        phi32 = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float32, device=dev) if need32 else None
         vl = torch.zeros(B, c.ld, dtype=f64, device=dev)
         for cst, idx in groups:
             x = self._sub(st["xpb"], idx)
@@ -443,7 +447,7 @@ class Engine:
             pg = phi if idx is None else torch.zeros(nb, c.nao_pad, c.ld, dtype=f64, device=dev)
             K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, pg)
             self._put(phi, idx, pg)
-            if mixed:
+            if need32:
                 pg32 = phi32 if idx is None else torch.zeros(nb, c.nao_pad, c.ld,
                                                               dtype=torch.float32, device=dev)
                 K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, pg32)
@@ -464,8 +468,10 @@ class Engine:
         st["phi"] = phi
         dVs = self._dv(st, B)
         if mixed:
-            st["phi_lp"] = phi32
-            if self.MIXED_SPLIT16 and c.nao_pad <= 128:
+            if need32:
+                st["phi_lp"] = phi32
+                per.append("phi_lp")
+            if use_h2:
                 # FP16 (hi, lo) pairs of s*phi, s = 2^k with |s phi| <= 2^14
                 amax = float(phi.abs().amax().item())
                 self._phi_scale = 2.0 ** math.floor(math.log2(2.0 ** 14 / max(amax, 1e-300)))
@@ -481,7 +487,7 @@ class Engine:
             st["vmax"] = torch.zeros(B, dtype=f64, device=dev)
             st["va_set"] = torch.zeros(B, dtype=torch.bool, device=dev)
             st["err"] = torch.full((B,), float("inf"), dtype=f64, device=dev)
-            per += ["phi_lp", "phisq", "Da", "rhoa", "kappa", "va", "Va", "vmax", "va_set", "err"]
+            per += ["phisq", "Da", "rhoa", "kappa", "va", "Va", "vmax", "va_set", "err"]
         st["vloc"] = vl
         st["V"] = self.vmat(phi, vl, dVs)
         st["Hcore"] = T + st["V"]
@@ -607,7 +613,8 @@ class Engine:
         if n:
             # orbital form sum_i f_i psi_i^2: non-negative in any precision, unlike the
             # D form, whose FP32 rounding can turn a tiny density negative
-            r = self.orb_density(st["phi_lp"] if n < na else sel("phi_lp"),
+            lp = "phi_h2" if "phi_h2" in st else "phi_lp"
+            r = self.orb_density(st[lp] if n < na else sel(lp),
                                  mo if idx is None else mo[idx.long()],
                                  occ if idx is None else occ[idx.long()], idx)
             self._scatter(rho, idx, r)

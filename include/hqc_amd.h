@@ -116,6 +116,13 @@ int hqc_orb_density_f64(const double* phi, const int* bidx, const double* At, in
                         int nao_pad, int m_pad, int ld, double* rho, void* stream);
 int hqc_orb_density_f32(const float* phi, const int* bidx, const double* At, int B,
                         int nao_pad, int m_pad, int ld, double* rho, void* stream);
+/* h2: from the FP16 pairs of hqc_phi_split_f16 (phi_scale as there); At is split
+ * the same way with a per-geometry power-of-two scale, psi = At^T phi takes
+ * hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16 with FP32 accumulation
+ * (relative error ~2^-22 of sum_k |At phi|), rho accumulated in FP64. */
+int hqc_orb_density_h2(const unsigned* ph, const int* bidx, const double* At, int B,
+                       int nao_pad, int m_pad, int ld, double phi_scale, double* rho,
+                       void* stream);
 
 /* Grid potential matrix V[b] = dV * phi diag(v[b]) phi^T (symmetric, full).
  * Replaces the 'pr,qr,r->pq' contraction (hqc/pbc/solver.py:693) and, through
@@ -241,8 +248,11 @@ int hqc_occupations(const double* w, int B, int nmo, int nelec, double sigma, in
  * (hqc/pbc/scf.py:69,138,198, hqc/pbc/solver.py:741).
  * A (B,n,n) symmetric; pairs ((npad-1) x npad ints, npad = n rounded up to
  * even) is a round-robin pairing schedule: every row lists npad/2 disjoint
- * pairs and every pair occurs once per sweep.  Stops when the off-diagonal
- * Frobenius norm <= tol * ||A||_F or after max_sweeps.  w (B,n) ascending;
+ * pairs and every pair occurs once per sweep; NULL selects the circle method
+ * computed on the device (index 0 fixed, the rest rotated one place per round).
+ * Rotations with |a_pq| <= tol ||A||_F / n are skipped (threshold Jacobi).
+ * Stops when the off-diagonal Frobenius norm <= tol * ||A||_F or after
+ * max_sweeps.  w (B,n) ascending;
  * V (B,n,n) eigenvectors as columns; sweeps (B) or NULL.
  * hqc_eigh_lds_bytes(n) is the LDS one block needs. */
 size_t hqc_eigh_lds_bytes(int n);

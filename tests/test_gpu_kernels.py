@@ -150,6 +150,41 @@ def test_orb_density(K, nao_pad, m, dt):
     assert torch.allclose(rho, rref, rtol=tol, atol=tol * rref.abs().max().item())
 
 
+@pytest.mark.parametrize("nao_pad,m", [(16, 8), (48, 20), (112, 60), (112, 100), (64, 140)])
+def test_orb_density_split16(K, nao_pad, m):
+    """Orbital density from FP16 (hi, lo) pairs with split coefficients: within
+    2^-18 sum_i (sum_k |At_ki phi_k|)^2 of the FP64 value (psi carries ~2^-22)."""
+    print(f"[orb density split16] nao_pad={nao_pad} m={m}", flush=True)
+    torch.manual_seed(5)
+    g = 11
+    N = g ** 3
+    ld = (N + 63) // 64 * 64
+    B = 2
+    phi = torch.zeros(B, nao_pad, ld, dtype=torch.float64, device=DEV)
+    phi[:, :, :N] = torch.randn(B, nao_pad, N, dtype=torch.float64, device=DEV)
+    m_pad = (m + 15) // 16 * 16
+    At = torch.zeros(B, nao_pad, m_pad, dtype=torch.float64, device=DEV)
+    At[:, :, :m] = torch.randn(B, nao_pad, m, dtype=torch.float64, device=DEV)
+    At[1] *= 1e-3                                   # a different coefficient scale
+    scale = 2.0 ** math.floor(math.log2(2.0 ** 14 / phi.abs().max().item()))
+    ph = torch.empty(B, nao_pad, ld, dtype=torch.int32, device=DEV)
+    K.phi_split_f16(phi, scale, ph)
+    rho = torch.zeros(B, ld, dtype=torch.float64, device=DEV)
+    K.orb_density(ph, At, rho, phi_scale=scale)
+    psi = At.transpose(1, 2) @ phi
+    rref = (psi ** 2).sum(1)
+    bound = ((At.abs().transpose(1, 2) @ phi.abs()) ** 2).sum(1)
+    torch.cuda.synchronize()
+    assert ((rho - rref).abs() <= 2.0 ** -18 * bound + 1e-300).all()
+    # through a geometry index list
+    bidx = torch.tensor([1, 0], dtype=torch.int32, device=DEV)
+    rho2 = torch.zeros_like(rho)
+    K.orb_density(ph, At, rho2, bidx, phi_scale=scale)
+    rref2 = ((At.transpose(1, 2) @ phi.flip(0)) ** 2).sum(1)
+    bound2 = ((At.abs().transpose(1, 2) @ phi.flip(0).abs()) ** 2).sum(1)
+    assert ((rho2 - rref2).abs() <= 2.0 ** -18 * bound2 + 1e-300).all()
+
+
 @pytest.mark.parametrize("g", [1, 2, 5, 7, 13, 17, 24, 25, 27, 31, 33, 48, 49, 51, 55, 57, 61,
                                64, 65, 69, 71, 97, 127, 128])
 def test_fft3d(K, g):
@@ -411,6 +446,9 @@ def test_eigh_jacobi(K, n):
     w2, U = K.eigh_jacobi(Aw, sweeps=sw)
     assert torch.allclose(w2, wr, atol=1e-12 * scale)
     assert torch.allclose(A @ (C @ U), (C @ U) * w2[:, None, :], atol=1e-11 * scale)
+    # the schedule computed on the device is the host table's, bit for bit
+    w3, V3 = K.eigh_jacobi(A, pairs=K.jacobi_pairs(n, A.device))
+    assert torch.equal(w3, w) and torch.equal(V3, V)
 
 
 @pytest.mark.parametrize("gamma", [True, False])
