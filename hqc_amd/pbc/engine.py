@@ -473,7 +473,7 @@ class Engine:
                 per.append("phi_lp")
             if use_h2:
                 # FP16 (hi, lo) pairs of s*phi, s = 2^k with |s phi| <= 2^14
-                amax = float(phi.abs().amax().item())
+                amax = float(torch.linalg.vector_norm(phi, float("inf")).item())
                 self._phi_scale = 2.0 ** math.floor(math.log2(2.0 ** 14 / max(amax, 1e-300)))
                 st["phi_h2"] = torch.empty(B, c.nao_pad, c.ld, dtype=torch.int32, device=dev)
                 K.phi_split_f16(phi, self._phi_scale, st["phi_h2"])
@@ -837,17 +837,23 @@ class Engine:
                 st["err"][:na] = torch.linalg.norm(e, dim=(-2, -1))
             return mo, w1, Enew, F, e
 
+        # the DIIS window is a ring: the newest (F, e) overwrites the oldest slot, so
+        # nothing is shifted; the extrapolation is invariant under a reordering of the
+        # window (the bordered Pulay system permutes with it)
+        last = [self.diis_space - 1]
+
         def push(loc, na, F, e):
-            """Shift the DIIS window and the Pulay matrix B_ij = <e_i, e_j>."""
-            Fk, ek, Bh = loc["Fk"][:na], loc["ek"][:na], loc["Bh"][:na]
-            Fk_new = torch.cat([Fk[:, 1:], F.unsqueeze(1)], 1)
-            ek_new = torch.cat([ek[:, 1:], e.unsqueeze(1)], 1)
-            d = (ek_new * e.unsqueeze(1)).sum((-1, -2))                  # (na, m)
-            B_new = torch.empty_like(Bh)
-            B_new[:, :-1, :-1] = Bh[:, 1:, 1:]
-            B_new[:, -1, :] = d
-            B_new[:, :, -1] = d
-            loc["Fk"][:na], loc["ek"][:na], loc["Bh"][:na] = Fk_new, ek_new, B_new
+            """Newest (F, e) into the ring and row/column of B_ij = <e_i, e_j>."""
+            m = self.diis_space
+            s = (last[0] + 1) % m
+            loc["Fk"][:na, s] = F
+            loc["ek"][:na, s] = e
+            # elementwise product + sum: a batched GEMV with K = n_ao^2 gets a slow
+            # library kernel here
+            d = (loc["ek"][:na] * e.unsqueeze(1)).sum((-1, -2))
+            loc["Bh"][:na, s, :] = d
+            loc["Bh"][:na, :, s] = d
+            last[0] = s
 
         m = self.diis_space
         V2, E2 = self.fock2e(st, B, D, mo, occ)                           # scf.py:122-128
@@ -863,7 +869,7 @@ class Engine:
                "nit": torch.zeros(B, dtype=torch.long, device=dev)}
         na = B
         for _ in range(self.diis_start_cycle + m):                        # scf.py:130-172
-            mo_, w1_, Enew, F, e = step(loc["Fk"][:, -1], na,
+            mo_, w1_, Enew, F, e = step(loc["Fk"][:, last[0]], na,
                                         self.EIG_TOL_MIN if lindep is not None else
                                         self.EIG_TOL_MAX)
             push(loc, na, F, e)
@@ -877,7 +883,7 @@ class Engine:
         res = [None]
 
         def cont(na):
-            err = torch.linalg.norm(loc["ek"][:na, -1], dim=(-2, -1))
+            err = torch.linalg.norm(loc["ek"][:na, last[0]], dim=(-2, -1))
             dE = (loc["E"][:na] - loc["E_old"][:na]).abs()
             keep = ((dE > tol) | (err > 10 * tol)) & (loop < self.max_cycle)
             res[0] = float(err.min().item()) if na else None
@@ -899,7 +905,8 @@ class Engine:
             ck = torch.linalg.solve(h, rhs)[:, :m, 0]
             Fk = loc["Fk"][:na]
             Fg = (ck[:, :, None, None] * Fk).sum(1)
-            Fg = (1 - self.diis_damp) * Fg + self.diis_damp * Fk[:, -1]
+            if self.diis_damp:
+                Fg = (1 - self.diis_damp) * Fg + self.diis_damp * Fk[:, last[0]]
             # the overlap-cut shift dominates |F'|, so a relative tolerance would be loose
             etol = self.EIG_TOL_MIN if lindep is not None else self._eig_tol(res[0])
             mo_a, w1_a, Enew, F, e_new = step(Fg, na, etol)
