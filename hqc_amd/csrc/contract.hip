@@ -461,6 +461,9 @@ extern "C" int hqc_orb_density_f32(const float* phi, const int* bidx, const doub
 // ---------------------------------------------------------------------------
 #define VMAT_TPW 8
 #define VMAT_TARGET_BLOCKS 1024
+#ifndef VH_TARGET_BLOCKS
+#define VH_TARGET_BLOCKS 2048   // split-product kernel: workgroups to aim for per launch
+#endif
 
 __device__ __forceinline__ void tile_rc(int t, int NT, int& rt, int& ct) {
   int r = 0, rem = t;
@@ -736,10 +739,10 @@ static void vmat_shape(int B, int nao_pad, int ld, int prec, int& KC, int& ntile
   ntiles = NT * (NT + 1) / 2;
   long target_blocks;
   if (vmat_pipelined(nao_pad)) {
-    KC = prec == 32 ? 64 : 32;                 // 16 vectors of 16 B per row
+    KC = prec == 64 ? 32 : 64;                 // 16 vectors of 16 B per row (16: 64 pairs)
     const int tpw = vmat_pipe_tpw(nao_pad);
     ngroups = (ntiles + 4 * tpw - 1) / (4 * tpw);
-    target_blocks = 2048;
+    target_blocks = prec == 16 ? VH_TARGET_BLOCKS : 2048;
   } else {
     KC = prec == 32 ? 32 : 16;                 // 8 vectors of 16 B per row
     const int NB = (nao_pad + VBIG - 1) / VBIG;
@@ -1033,7 +1036,7 @@ extern "C" int hqc_phi_split_f16(const double* phi, long n, double scale, unsign
 
 static size_t vmat_h2_part_bytes(int B, int nao_pad, int ld) {
   int KC, ntiles, ngroups, nwg;
-  vmat_shape(B, nao_pad, ld, 32, KC, ntiles, ngroups, nwg);
+  vmat_shape(B, nao_pad, ld, 16, KC, ntiles, ngroups, nwg);
   return ((size_t)B * ntiles * nwg * 256 * sizeof(double) + 255) / 256 * 256;
 }
 
@@ -1053,7 +1056,7 @@ extern "C" int hqc_ao_vmat_h2(const unsigned* ph, const int* bidx, const double*
   if (!vmat_pipelined(nao_pad)) return HQC_E_SIZE;
   if (B == 0) return HQC_OK;
   int KC, ntiles, ngroups, nwg;
-  vmat_shape(B, nao_pad, ld, 32, KC, ntiles, ngroups, nwg);
+  vmat_shape(B, nao_pad, ld, 16, KC, ntiles, ngroups, nwg);
   const size_t pbytes = vmat_h2_part_bytes(B, nao_pad, ld);
   if (!work || work_bytes < pbytes + (size_t)B * sizeof(double)) return HQC_E_ARG;
   hipStream_t s = (hipStream_t)stream;

@@ -230,6 +230,29 @@ def test_batch_of_different_cells():
             assert np.allclose(F[i].cpu().numpy(), R.forces(s, xp), atol=1e-8)
 
 
+def test_mixed_batch_of_different_cells():
+    """precision='mixed' on a batch of several cell lengths on one grid (per-geometry
+    volume elements scale the split-product increments) against the FP64 batch:
+    the accuracy bar per geometry."""
+    from hqc_amd.pbc import make_pes_batch
+    rng = np.random.default_rng(21)
+    specs = [(16, 1.20), (16, 1.25), (16, 1.28), (16, 1.22)]
+    xps, rss = [], []
+    for n, rs in specs:
+        L = (4 / 3 * np.pi * n) ** (1 / 3)
+        xps.append(rng.uniform(0, L, (n, 3)))
+        rss.append(rs)
+    kw = dict(dft=True, smearing=True, smearing_sigma=0.0316681, tol=1e-8)
+    E64, F64, _ = make_pes_batch("gth-dzv", grid_length=0.2, precision="fp64",
+                                 **kw).energy_and_forces(xps, rss)
+    pes = make_pes_batch("gth-dzv", grid_length=0.2, precision="mixed", **kw)
+    Emx, Fmx, info = pes.energy_and_forces(xps, rss)
+    assert max(info["buckets"].values()) >= 2                  # lengths share a grid
+    for i, (n, _) in enumerate(specs):
+        assert abs(Emx[i].item() - E64[i].item()) / 2 / n <= 1e-6
+        assert (Fmx[i] - F64[i]).abs().max().item() <= 1e-5
+
+
 def test_ovlp_cut_matches_when_nothing_is_cut():
     """ovlp_cut below the smallest overlap eigenvalue gives the same fixed point (the
     DIIS residual is then measured in the orthogonal basis); above it the
