@@ -149,7 +149,11 @@ class Engine:
     MIXED_EPS = 2e-6        # effective relative error of an FP32-accumulated increment
     MIXED_SAFETY = 1e-2     # potential increment: error per element <= 1e-2 residual / n_ao
     MIXED_RHO_SAFETY = 1.0  # density increment: |v| x relative rho error <= residual
-    MIXED_DIRECT_ERR = 1e-1  # residual above which rho and V are taken in FP32 outright
+    MIXED_DIRECT_ERR = 1e-1  # residual above which V is taken in low precision outright
+    # the same for the density (orbital form): the split-FP16 density leaves a DIIS
+    # residual floor of ~3e-6..1e-5 at 54 H dzv, so the switch to FP64 stays an order
+    # of magnitude above it (3e-6 stalled geometries at max_cycle)
+    MIXED_RHO_DIRECT_ERR = 1e-4
     MIXED_RHO_INCREMENTS = False  # FP32 D-form density increments (else FP64 orbital form)
     # low-precision V from FP16 (hi, lo) split products on MFMA (n_ao <= 128; FP32
     # MFMA beyond).  Its measured error is that of FP32 MFMA; the increment constant
@@ -602,7 +606,7 @@ class Engine:
         c = self.c
         sel = lambda key: st[key][:na]
         err = sel("err")
-        direct = err >= self.MIXED_DIRECT_ERR
+        direct = err >= self.MIXED_RHO_DIRECT_ERR
         dD = D - sel("Da")
         # bound on the relative density error of the FP32 increment
         rel = self.MIXED_EPS * torch.linalg.norm(dD, dim=(-2, -1)) / sel("kappa")
