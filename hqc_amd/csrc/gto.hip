@@ -331,31 +331,39 @@ extern "C" int hqc_ao_points(const double* xp, const double* xe, const double* k
 // and accumulates Z_p(r) grad phi_p(r) with Z = 2 v X (+ Zx).  Each block
 // writes its three partial sums; the host adds them in a fixed order.
 // ---------------------------------------------------------------------------
-#define HQC_FORCE_BLOCKS 64
+#define HQC_FORCE_BLOCKS 16
 
 __global__ __launch_bounds__(256) void k_grid_forces(
     const double* __restrict__ P, int n, int g, int na, int nl,
     const double* __restrict__ alpha, const double* __restrict__ coef,
     const int* __restrict__ comp, int nloc, int nao_pad, int ld,
     const double* __restrict__ X, const double* __restrict__ v,
-    const double* __restrict__ Zx, double dV, double* __restrict__ part) {
+    const double* __restrict__ Zx, double dV, int tab_lds, double* __restrict__ part) {
   __shared__ double s_coef[HQC_MAX_NA * HQC_MAX_NLOC];
   __shared__ double s_alpha[HQC_MAX_NA];
   __shared__ int s_comp[HQC_MAX_NLOC];
   __shared__ double red[4];
+  extern __shared__ double s_tab[];   // this atom's 1-D tables when they fit (tab_lds)
   for (int i = threadIdx.x; i < na * nloc; i += blockDim.x) s_coef[i] = coef[i];
   for (int i = threadIdx.x; i < na; i += blockDim.x) s_alpha[i] = alpha[i];
   for (int i = threadIdx.x; i < nloc; i += blockDim.x) s_comp[i] = comp[i];
-  __syncthreads();
   const int A = blockIdx.y, b = blockIdx.z;
   const int N = g * g * g;
   const long tab = (long)(b * n + A) * 3 * g * na * nl;
+  // the workgroup's atom tables (3 g na nl doubles, 14 KB at 49^3 dzv) staged in LDS:
+  // read through L1 they thrash when the workgroups on a CU serve different atoms
+  const double* T = P + tab;
+  if (tab_lds) {
+    for (int i = threadIdx.x; i < 3 * g * na * nl; i += blockDim.x) s_tab[i] = P[tab + i];
+    T = s_tab;
+  }
+  __syncthreads();
   double f[3] = {0.0, 0.0, 0.0};
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < N; r += gridDim.x * blockDim.x) {
     const int iz = r % g, iy = (r / g) % g, ix = r / (g * g);
-    const double* Pd[3] = {P + tab + (long)(0 * g + ix) * na * nl,
-                           P + tab + (long)(1 * g + iy) * na * nl,
-                           P + tab + (long)(2 * g + iz) * na * nl};
+    const double* Pd[3] = {T + (long)(0 * g + ix) * na * nl,
+                           T + (long)(1 * g + iy) * na * nl,
+                           T + (long)(2 * g + iz) * na * nl};
     const double vr = v ? 2.0 * v[(long)b * ld + r] : 0.0;
     for (int o = 0; o < nloc; ++o) {
       const long row = (long)b * nao_pad + (long)A * nloc + o;
@@ -406,9 +414,11 @@ extern "C" int hqc_grid_forces(const double* P, int B, int n, int g, int na, int
   if (nl < 3) return HQC_E_ARG;   // gradients need l_max + 2 powers (>= 2 for s)
   if (v && !X) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
-  hipLaunchKernelGGL(k_grid_forces, dim3(HQC_FORCE_BLOCKS, n, B), dim3(256), 0,
-                     (hipStream_t)stream, P, n, g, na, nl, alpha, coef, comp, nloc, nao_pad,
-                     ld, X, v, Zx, dV, part);
+  const size_t tab_bytes = (size_t)3 * g * na * nl * sizeof(double);
+  const int tab_lds = tab_bytes <= 48 * 1024;
+  hipLaunchKernelGGL(k_grid_forces, dim3(HQC_FORCE_BLOCKS, n, B), dim3(256),
+                     tab_lds ? tab_bytes : 0, (hipStream_t)stream, P, n, g, na, nl, alpha, coef,
+                     comp, nloc, nao_pad, ld, X, v, Zx, dV, tab_lds, part);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }
