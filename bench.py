@@ -41,7 +41,7 @@ SIGMA = 1.0 / (157888.088922572 / T_K) / 2.0
 GRID_LENGTH = 0.16
 TOL = 1e-8
 # dense MFMA TFLOP/s; MI355X's FP64 vector FMA peak is the same 78.6 TFLOP/s
-MI355X_PEAK = {"f64": 78.6, "f32": 157.3}
+MI355X_PEAK = {"f64": 78.6, "f32": 157.3, "i8": 5000.0}   # TFLOP/s (int8: TOP/s)
 MI355X_HBM_GBS = 8000.0
 
 
@@ -104,9 +104,11 @@ def time_vmat(pes, B, reps=20):
     return ms, flops
 
 
-def time_orb_density(pes, B, reps=20):
-    """Average duration of the orbital-form density kernel (rho = sum_i f_i psi_i^2,
-    FP64) at the workload shape with the occupations of the last run."""
+def time_orb_density(pes, B, q8=False, reps=20):
+    """Average duration of the accurate orbital-form density kernel (rho = sum_i f_i
+    psi_i^2) at the workload shape with the occupations of the last run: FP64 vector
+    FMAs, or (q8) int8 digits of phi with exact integer products.  Returns (ms,
+    flops, orbitals, phi bytes per launch, int8 MFMA ops per launch)."""
     import torch
     from hqc_amd import _lib as K
     eng = pes.solver.engine
@@ -119,18 +121,34 @@ def time_orb_density(pes, B, reps=20):
     At[:, :c.nao, :keep] = mo[:, :, :keep] * occ[:, None, :keep].clamp(min=0).sqrt()
     phi = torch.rand(B, c.nao_pad, c.ld, dtype=torch.float64, device="cuda")
     rho = torch.zeros(B, c.ld, dtype=torch.float64, device="cuda")
+    ns = eng.Q8_SLICES
+    if q8:
+        q = torch.empty(B, c.ld * ns * c.nao_pad, dtype=torch.int8, device="cuda")
+        pe = torch.empty(B, c.ld, dtype=torch.int16, device="cuda")
+        K.phi_slice_i8(phi, ns, q, pe)
+        del phi
+        run = lambda: K.orb_density_q8(q, pe, At, rho, ns)
+    else:
+        run = lambda: K.orb_density(phi, At, rho)
     for _ in range(3):
-        K.orb_density(phi, At, rho)
+        run()
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
     for _ in range(reps):
-        K.orb_density(phi, At, rho)
+        run()
     e1.record(s)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
     flops = 2.0 * (c.nao + 1) * keep * c.N * B       # psi = C^T phi, then sum f psi^2
-    return ms, flops, keep
+    # int8 digits: ns bytes per phi value + a 2-byte exponent per point in, rho out
+    nbytes = B * c.ld * (ns * c.nao_pad + 2.0 + 8.0)
+    # digit pairs (k, l) with l < ns, k <= 5 - l, each a 16x16x64 product over the AO
+    # depth padded to 64 and the orbitals padded to 16
+    pairs = sum(6 - l for l in range(ns))
+    kdepth = (c.nao_pad + 63) // 64 * 64
+    ops = 2.0 * m_pad * kdepth * c.ld * B * pairs
+    return ms, flops, keep, nbytes, ops
 
 
 TRAFFIC_BATCH = 256
@@ -235,6 +253,17 @@ def kernel_probe(name, B, reps=5):
         rho = torch.zeros(B, c.ld, dtype=torch.float64, device="cuda")
         for _ in range(reps):
             K.orb_density(phi, At, rho)
+    elif name == "orb_density_q8":
+        keep, ns = 64, pes.solver.engine.Q8_SLICES
+        At = torch.rand(B, c.nao_pad, keep, dtype=torch.float64, device="cuda")
+        phi = torch.rand(B, c.nao_pad, c.ld, dtype=torch.float64, device="cuda")
+        q = torch.empty(B, c.ld * ns * c.nao_pad, dtype=torch.int8, device="cuda")
+        pe = torch.empty(B, c.ld, dtype=torch.int16, device="cuda")
+        K.phi_slice_i8(phi, ns, q, pe)
+        del phi
+        rho = torch.zeros(B, c.ld, dtype=torch.float64, device="cuda")
+        for _ in range(reps):
+            K.orb_density_q8(q, pe, At, rho, ns)
     else:
         raise SystemExit(f"unknown kernel {name}")
     torch.cuda.synchronize()
@@ -324,9 +353,10 @@ def main():
     # dominant kernels: potential matrix (FP64 or FP32) and FP64 orbital density
     kind = vmat_kind(pes)
     ms_v, fl_v = time_vmat(pes, B)
-    ms_d, fl_d, keep = time_orb_density(pes, B)
+    q8 = a.path == "mixed" and stats.get("rhoq8", 0) > 0
+    ms_d, fl_d, keep, by_d, ops_d = time_orb_density(pes, B, q8)
     n_vmat = stats["v64"] if a.path == "fp64" else stats["v32"]
-    n_dens64 = stats["rho64"]
+    n_dens64 = stats["rhoq8"] if q8 else stats["rho64"]
     nsteps = max(a.steps + a.warmup, 1)
     c = eng.c
     if kind == "h2":
@@ -343,14 +373,22 @@ def main():
               "bound": "mfma", "unit": "TFLOP/s", "peak": MI355X_PEAK[prec_v]}
     vk.update({"ms_per_launch": ms_v, "flops_per_launch": fl_v,
                "calls_per_step": n_vmat / nsteps, "traffic": measured_traffic(vname, B)})
-    cand = [
-        vk,
-        {"kernel": "k_orb_density_valu (sum_i f_i (C^T phi)_i^2, FP64 vector FMA)",
-         "bound": "mfma", "unit": "TFLOP/s",
-         "ms_per_launch": ms_d, "flops_per_launch": fl_d, "peak": MI355X_PEAK["f64"],
-         "orbitals": keep, "calls_per_step": n_dens64 / nsteps,
-         "traffic": measured_traffic("k_orb_density_valu", B)},
-    ]
+    if q8:
+        # int8 digits (4 bytes per phi value) streamed once per launch: HBM roof; the
+        # int8 MFMA work of the kept digit pairs is reported beside it
+        dk = {"kernel": "k_orb_density_q8 (sum_i f_i (C^T phi)_i^2 from int8 digits, "
+                        "exact int32 products)", "bound": "hbm", "unit": "GB/s",
+              "bytes_per_launch": by_d, "peak": MI355X_HBM_GBS,
+              "int8_mfma_tops": ops_d / (ms_d * 1e-3) / 1e12,
+              "int8_mfma_frac": ops_d / (ms_d * 1e-3) / 1e12 / MI355X_PEAK["i8"],
+              "traffic": measured_traffic("k_orb_density_q8", B)}
+    else:
+        dk = {"kernel": "k_orb_density_valu (sum_i f_i (C^T phi)_i^2, FP64 vector FMA)",
+              "bound": "mfma", "unit": "TFLOP/s", "peak": MI355X_PEAK["f64"],
+              "traffic": measured_traffic("k_orb_density_valu", B)}
+    dk.update({"ms_per_launch": ms_d, "flops_per_launch": fl_d, "orbitals": keep,
+               "calls_per_step": n_dens64 / nsteps})
+    cand = [vk, dk]
     for k in cand:
         if k["unit"] == "GB/s":
             k["achieved"] = k["bytes_per_launch"] / (k["ms_per_launch"] * 1e-3) / 1e9
@@ -366,7 +404,8 @@ def main():
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if a.path == "fp64" else
-                 ("f64+f16x2(split increments)" if kind == "h2" else "f64+f32(increments)"),
+                 ("f64+f16x2(split increments)" if kind == "h2" else "f64+f32(increments)")
+                 + ("+i8(digit density)" if q8 else ""),
         "data": "synthetic: uniform-random protons, numpy default_rng(seed), seeds 0..B-1",
         "config": {"workload": f"54 H rs=1.25 gth-dzv LDA(vwn) Fermi T=1e4K, grid 49^3 "
                                f"(grid_length 0.16), tol 1e-8, energy+forces, batch {B}",

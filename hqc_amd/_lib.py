@@ -36,6 +36,8 @@ _SIGS = {
     "hqc_orb_density_h2": [_P, _P, _P, _I, _I, _I, _I, _D, _P, _P],
     "hqc_ao_vmat_h2_workspace": [_I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
     "hqc_ao_vmat_h2": [_P, _P, _P, _P, _I, _I, _I, _D, _D, _P, _P, ctypes.c_size_t, _P],
+    "hqc_phi_slice_i8": [_P, _I, _I, _I, _I, _P, _P, _P],
+    "hqc_orb_density_q8": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P],
     "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
     "hqc_pair_pack": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_split_f64": [_P, _I, _I, _I, _I, _I, _P, _P],
@@ -233,6 +235,32 @@ def phi_split_f16(phi, scale, out):
     assert phi.is_contiguous() and out.is_contiguous()
     _call("hqc_phi_split_f16", _ptr(phi, F64, "phi"), phi.numel(),
           float(scale), _ptr(out, I32, "out"), _stream())
+
+
+Q8_MAX_NAO_PAD = 128
+
+
+def phi_slice_i8(phi, nslice, q, pe):
+    """q (G, ld*nslice*nao_pad) int8 digits and pe (G, ld) int16 exponents of phi
+    (G, nao_pad, ld) FP64, in the tile layout the int8 density kernel reads."""
+    G, nao_pad, ld = phi.shape
+    assert q.dtype == torch.int8 and q.shape == (G, ld * nslice * nao_pad)
+    assert pe.dtype == torch.int16 and pe.shape == (G, ld)
+    _call("hqc_phi_slice_i8", _ptr(phi, F64, "phi"), G, nao_pad, ld, nslice,
+          _ptr(q, torch.int8, "q"), _ptr(pe, torch.int16, "pe"), _stream())
+
+
+def orb_density_q8(q, pe, At, rho, nslice, bidx=None):
+    """rho (B, ld) = sum_i (At[:, :, i]^T phi)^2 from the int8 digits of phi_slice_i8
+    (exact integer accumulation; bidx selects the geometries of q and pe)."""
+    B, nao_pad, m_pad = At.shape
+    ld = rho.shape[1]
+    assert rho.shape == (B, ld) and q.shape[1] == ld * nslice * nao_pad and pe.shape[1] == ld
+    assert q.shape[0] == pe.shape[0]
+    bi = _bidx(q, bidx, B)
+    _call("hqc_orb_density_q8", _ptr(q, torch.int8, "q"), _ptr(pe, torch.int16, "pe"),
+          _ptr(bi, I32, "bidx"), _ptr(At, F64, "At"), B, nao_pad, m_pad, ld, nslice,
+          _ptr(rho, F64, "rho"), _stream())
 
 
 def vmat_h2_workspace(B, nao_pad, ld):

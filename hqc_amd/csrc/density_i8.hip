@@ -1,0 +1,293 @@
+// Orbital-form density rho(r) = sum_i (sum_mu At[mu][i] phi_mu(r))^2 from int8
+// slices with exact integer accumulation on the int8 matrix cores
+// (v_mfma_i32_16x16x64_i8).  It serves the mixed path's accurate density; the
+// reference evaluates the same quantity in the D form, rho = einsum('pr,qr,pq->r',
+// phi, phi, D) with D = C f C^T (hqc/pbc/solver.py:690); here At = C sqrt(f).
+//
+// Slicing.  A value x of a vector whose largest magnitude is below 2^e is cut
+// into int8 digits by
+//     y_0 = x 2^(6-e),  x_s = rint(y_s),  y_(s+1) = 128 (y_s - x_s),
+// every step exact in FP64, |x_s| <= 64, and
+//     x = 2^(e-6) sum_s 2^(-7 s) x_s + r,   |r| <= 2^(e-7 S)  for S digits.
+// phi is sliced once per geometry with a per-grid-point exponent (S_phi digits,
+// a fixed perturbation of the basis: smooth in the SCF state); the orbital
+// coefficients are sliced per call with a per-orbital exponent into T+1 digits,
+// and every digit product with k + l <= T is accumulated exactly in int32 per
+// level d = k + l (|level| <= 4 * 128 * 64 * 64 < 2^31).  What changes with the
+// SCF state from call to call (coefficient rounding, dropped levels) is then
+// ~2^(-7(T+1)) = 2^-42 of max|At_i| max|phi(r)| per AO.  psi_i(r) =
+// 2^(e_r + e_i - 12) sum_d 2^(-7 d) P_d, combined in FP64 by Horner's rule.
+//
+// Layout of the phi slices (per geometry, ld/16 tiles of 16 grid points):
+//     q[tile][s][mb][16 points][16 AOs]   (NB = nao_pad/16 AO blocks mb),
+// so a lane's 16-byte operand (16 consecutive AOs of one point) is one load and
+// a wave's 64 lanes read 1 KB contiguously; pe[point] holds the exponent e_r.
+#include "common.h"
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4q __attribute__((ext_vector_type(4)));
+
+#define Q8_T 5            // highest digit level kept: T + 1 coefficient digits
+#define Q8_SC (Q8_T + 1)
+#define Q8_MAXKS 2        // nao_pad <= 128 (two K steps of 64 AOs)
+#define Q8_NW 4           // waves per workgroup
+#define Q8_TARGET_BLOCKS 2048
+
+__device__ __forceinline__ int q8_exponent(double m) {
+  // smallest e with m < 2^e (0 for m = 0)
+  return m > 0.0 ? ilogb(m) + 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// phi [G][nao_pad][ld] (FP64) -> q (layout above) and pe [G][ld]
+// grid (ld/64, G), 256 threads: point p = 64 blockIdx.x + lane, AO blocks
+// mb = w, w + 4 of that point per thread
+// ---------------------------------------------------------------------------
+template <int SP>
+__global__ __launch_bounds__(256) void k_phi_slice_i8(const double* __restrict__ phi, int nao_pad,
+                                                      int ld, signed char* __restrict__ q,
+                                                      short* __restrict__ pe) {
+  __shared__ double red[4][64];
+  const int NB = nao_pad >> 4;
+  const int g = blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long p = (long)blockIdx.x * 64 + lane;
+  const double* P = phi + (long)g * nao_pad * ld + p;
+  double v[2][16];
+  double m = 0.0;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int mb = w + 4 * t;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      v[t][i] = mb < NB ? P[(long)(16 * mb + i) * ld] : 0.0;
+      m = fmax(m, fabs(v[t][i]));
+    }
+  }
+  red[w][lane] = m;
+  __syncthreads();
+  m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+  const int e = q8_exponent(m);
+  if (w == 0) pe[(long)g * ld + p] = (short)e;
+  signed char* Q = q + ((long)g * ld + (p & ~15L)) * SP * nao_pad + (p & 15) * 16;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int mb = w + 4 * t;
+    if (mb >= NB) continue;
+    unsigned word[SP][4];
+#pragma unroll
+    for (int s = 0; s < SP; ++s)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) word[s][c] = 0u;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      double y = ldexp(v[t][i], 6 - e);
+#pragma unroll
+      for (int s = 0; s < SP; ++s) {
+        const double d = rint(y);
+        word[s][i >> 2] |= ((unsigned)(int)d & 0xffu) << (8 * (i & 3));
+        y = (y - d) * 128.0;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < SP; ++s)
+      *reinterpret_cast<u32x4q*>(Q + ((long)s * NB + mb) * 256) =
+          u32x4q{word[s][0], word[s][1], word[s][2], word[s][3]};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// rho[b][r] = sum_i psi_i(r)^2 over the m_pad columns of At[b] ([nao_pad][m_pad]).
+// grid (nwg, B), Q8_NW waves; per group of up to 64 orbitals the workgroup
+// slices At into LDS (digit-major, rows of 64 KS + 16 bytes so the 16 lanes of
+// a fragment read hit distinct banks), then every wave walks its own tiles of
+// 16 grid points: the phi operands of the next tile are loaded while the
+// current one runs through MT orbital tiles x KS K-steps x the kept digit pairs.
+// ---------------------------------------------------------------------------
+template <int SP, int KS>
+__global__ __launch_bounds__(64 * Q8_NW) void k_orb_density_q8(
+    const signed char* __restrict__ q, const short* __restrict__ pe, const int* __restrict__ bidx,
+    const double* __restrict__ At, int nao_pad, int m_pad, int ld, double* __restrict__ rho) {
+  constexpr int STRA = 64 * KS + 16;
+  constexpr int PLANE = 64 * STRA;              // one digit of 64 orbitals
+  extern __shared__ unsigned char smem_q8[];
+  signed char* As = reinterpret_cast<signed char*>(smem_q8);          // [Q8_SC][64][STRA]
+  double* wo = reinterpret_cast<double*>(smem_q8 + Q8_SC * PLANE);   // [64] 2^(2 e_i)
+  double* red = wo + 64;                                              // [4][64]
+  const int NB = nao_pad >> 4;
+  const int NPT = ld >> 4;
+  const int b = blockIdx.y;
+  const long gb = bidx ? bidx[b] : b;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = lane & 15, kq = lane >> 4;
+  const double* Ab = At + (long)b * nao_pad * m_pad;
+  const signed char* Qb = q + gb * (long)ld * SP * nao_pad + j * 16;
+  const short* Pe = pe + gb * ld;
+  double* Rb = rho + (long)b * ld;
+  const int wstride = gridDim.x * Q8_NW;
+  const int pt0 = blockIdx.x * Q8_NW + w;
+
+  auto loadB = [&](int pt, u32x4q (&Bf)[SP][KS], int& er) {
+    const signed char* base = Qb + (long)pt * SP * NB * 256;
+#pragma unroll
+    for (int s = 0; s < SP; ++s)
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int blk = 4 * ks + kq;
+        // AO blocks past nao_pad read the last block (in bounds) and are zeroed
+        const u32x4q x =
+            *reinterpret_cast<const u32x4q*>(base + ((long)s * NB + min(blk, NB - 1)) * 256);
+        Bf[s][ks] = blk < NB ? x : u32x4q{0u, 0u, 0u, 0u};
+      }
+    er = Pe[pt * 16 + j];
+  };
+
+  for (int og = 0; og < m_pad; og += 64) {
+    const int MW = min(64, m_pad - og);         // a multiple of 16
+    __syncthreads();                            // previous group's fragments are read
+    {
+      // per-orbital exponent: orbital i = lane, AO rows w, w + 4, ...
+      double m = 0.0;
+      if (lane < MW)
+        for (int mu = w; mu < nao_pad; mu += 4) m = fmax(m, fabs(Ab[(long)mu * m_pad + og + lane]));
+      red[w * 64 + lane] = m;
+    }
+    __syncthreads();
+    {
+      const double m = fmax(fmax(red[lane], red[64 + lane]), fmax(red[128 + lane], red[192 + lane]));
+      const int e = q8_exponent(m);
+      if (w == 0) wo[lane] = ldexp(1.0, 2 * e);
+      // digits of 4 consecutive AOs per task: orbital i = lane, AO quad from w
+      for (int quad = w; quad < 16 * KS; quad += 4) {
+        unsigned word[Q8_SC];
+#pragma unroll
+        for (int s = 0; s < Q8_SC; ++s) word[s] = 0u;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int mu = 4 * quad + c;
+          double y = (lane < MW && mu < nao_pad) ? ldexp(Ab[(long)mu * m_pad + og + lane], 6 - e) : 0.0;
+#pragma unroll
+          for (int s = 0; s < Q8_SC; ++s) {
+            const double d = rint(y);
+            word[s] |= ((unsigned)(int)d & 0xffu) << (8 * c);
+            y = (y - d) * 128.0;
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < Q8_SC; ++s)
+          *reinterpret_cast<unsigned*>(As + s * PLANE + lane * STRA + 4 * quad) = word[s];
+      }
+    }
+    __syncthreads();
+    const int MT = MW >> 4;
+    int pt = pt0;
+    u32x4q Bc[SP][KS], Bn[SP][KS];
+#pragma unroll
+    for (int s = 0; s < SP; ++s)
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) Bn[s][ks] = u32x4q{0u, 0u, 0u, 0u};
+    int erc = 0, ern = 0;
+    if (pt < NPT) loadB(pt, Bc, erc);
+    while (pt < NPT) {                          // wave-uniform
+      const int pn = pt + wstride;
+      if (pn < NPT) loadB(pn, Bn, ern);
+      double racc = 0.0;
+      for (int t = 0; t < MT; ++t) {
+        i32x4 acc[Q8_T + 1];
+#pragma unroll
+        for (int d = 0; d <= Q8_T; ++d) acc[d] = i32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          i32x4 a[Q8_SC];
+#pragma unroll
+          for (int k = 0; k < Q8_SC; ++k)
+            a[k] = *reinterpret_cast<const i32x4*>(As + k * PLANE + (16 * t + j) * STRA + 64 * ks +
+                                                   16 * kq);
+#pragma unroll
+          for (int l = 0; l < SP; ++l) {
+            const i32x4 bl = __builtin_bit_cast(i32x4, Bc[l][ks]);
+#pragma unroll
+            for (int k = 0; k + l <= Q8_T; ++k)
+              acc[k + l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[k], bl, acc[k + l], 0, 0, 0);
+          }
+        }
+        // lane holds orbitals 16 t + 4 kq + e (e = 0..3) at point j
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          double Q = (double)acc[Q8_T][e];
+#pragma unroll
+          for (int d = Q8_T - 1; d >= 0; --d) Q = fma(Q, 0.0078125, (double)acc[d][e]);
+          racc = fma(wo[16 * t + 4 * kq + e] * Q, Q, racc);
+        }
+      }
+      racc += __shfl_xor(racc, 16, 64);
+      racc += __shfl_xor(racc, 32, 64);
+      if (kq == 0) {
+        const double r = ldexp(racc, 2 * erc - 24);
+        const long idx = (long)pt * 16 + j;
+        Rb[idx] = og == 0 ? r : Rb[idx] + r;
+      }
+#pragma unroll
+      for (int s = 0; s < SP; ++s)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) Bc[s][ks] = Bn[s][ks];
+      erc = ern;
+      pt = pn;
+    }
+  }
+}
+
+static size_t q8_lds_bytes(int KS) {
+  return (size_t)Q8_SC * 64 * (64 * KS + 16) + 64 * sizeof(double) + 256 * sizeof(double);
+}
+
+extern "C" int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, int nslice,
+                                signed char* q, short* pe, void* stream) {
+  if (G < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63) || !phi || !q || !pe)
+    return HQC_E_ARG;
+  if (nao_pad > 64 * Q8_MAXKS) return HQC_E_SIZE;
+  if (G == 0) return HQC_OK;
+  const dim3 grid(ld / 64, G);
+  hipStream_t s = (hipStream_t)stream;
+  switch (nslice) {
+    case 3: hipLaunchKernelGGL(k_phi_slice_i8<3>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe); break;
+    case 4: hipLaunchKernelGGL(k_phi_slice_i8<4>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe); break;
+    case 5: hipLaunchKernelGGL(k_phi_slice_i8<5>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe); break;
+    default: return HQC_E_ARG;
+  }
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
+extern "C" int hqc_orb_density_q8(const signed char* q, const short* pe, const int* bidx,
+                                  const double* At, int B, int nao_pad, int m_pad, int ld,
+                                  int nslice, double* rho, void* stream) {
+  if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || m_pad <= 0 || (m_pad & 15) || ld <= 0 ||
+      (ld & 63) || nslice < 3 || nslice > 5)
+    return HQC_E_ARG;
+  if (nao_pad > 64 * Q8_MAXKS) return HQC_E_SIZE;
+  if (B == 0) return HQC_OK;
+  if (!q || !pe || !At || !rho) return HQC_E_ARG;
+  const int KS = nao_pad > 64 ? 2 : 1;
+  const size_t lds = q8_lds_bytes(KS);
+  const int npt = ld >> 4;
+  long nwg = Q8_TARGET_BLOCKS / B;
+  const long maxwg = (npt + Q8_NW - 1) / Q8_NW;
+  if (nwg > maxwg) nwg = maxwg;
+  if (nwg < 1) nwg = 1;
+  const dim3 grid((unsigned)nwg, B);
+  hipStream_t s = (hipStream_t)stream;
+#define Q8_CASE(SP_, KS_)                                                                      \
+  if (nslice == SP_ && KS == KS_) {                                                            \
+    if (lds > 64 * 1024 &&                                                                     \
+        hipFuncSetAttribute((const void*)k_orb_density_q8<SP_, KS_>,                           \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) \
+      (void)hipGetLastError();                                                                 \
+    hipLaunchKernelGGL((k_orb_density_q8<SP_, KS_>), grid, dim3(64 * Q8_NW), lds, s, q, pe,    \
+                       bidx, At, nao_pad, m_pad, ld, rho);                                     \
+  }
+  Q8_CASE(3, 1) Q8_CASE(3, 2) Q8_CASE(4, 1) Q8_CASE(4, 2) Q8_CASE(5, 1) Q8_CASE(5, 2)
+#undef Q8_CASE
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}

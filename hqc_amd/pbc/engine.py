@@ -144,7 +144,8 @@ class Engine:
     Density increments are off by default (MIXED_RHO_INCREMENTS): the D form
     costs 2 n_ao^2 flops per grid point against 2 n_ao n_occ for the FP64
     orbital form, and with n_occ ~ n_ao / 2 the FP32 MFMA rate (2x FP64) does
-    not pay that back, so past the warm-up rho is taken in FP64 orbital form."""
+    not pay that back, so past the warm-up rho is taken in the accurate orbital
+    form: int8 digit products with exact integer accumulation (MIXED_Q8), or FP64."""
 
     MIXED_EPS = 2e-6        # effective relative error of an FP32-accumulated increment
     MIXED_SAFETY = 1e-2     # potential increment: error per element <= 1e-2 residual / n_ao
@@ -161,6 +162,12 @@ class Engine:
     # same throughput at 54 H dzv)
     MIXED_SPLIT16 = True
     MIXED_EPS_SPLIT16 = 8e-6
+    # past the warm-up the density comes from int8 digits of phi with exact integer
+    # products (n_ao <= 128; FP64 vector FMAs beyond).  4 digits: phi is fixed to
+    # 2^-28 of its per-point maximum (a basis perturbation that is the same on every
+    # call); the coefficient digits add ~2^-42 per AO, so no residual floor appears
+    MIXED_Q8 = True
+    Q8_SLICES = 4
 
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
@@ -211,7 +218,7 @@ class Engine:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
         # kernel launches: FP64 / FP32 densities and potential matrices
-        self.stats = {"rho64": 0, "rho32": 0, "v64": 0, "v32": 0}
+        self.stats = {"rho64": 0, "rhoq8": 0, "rho32": 0, "v64": 0, "v32": 0}
         self.sweeps = torch.zeros((), dtype=torch.long, device=cell.device)
         self.eigh_calls = 0
         self._work = {}
@@ -277,18 +284,32 @@ class Engine:
 
     OCC_CUT = 1e-15   # orbitals below this occupation contribute < 1e-15 relative to rho
 
-    def orb_density(self, phi, mo, occ, bidx=None):
-        """rho = sum_i f_i psi_i^2 over the orbitals with f_i > OCC_CUT (equal to the
-        D form with D = C f C^T up to that cut); bidx as in vmat."""
+    def _at(self, mo, occ):
+        """At = C sqrt(f) (B, nao_pad, m_pad) over the orbitals with f > OCC_CUT."""
         c = self.c
         B = mo.shape[0]
         keep = max(int((occ > self.OCC_CUT).sum(-1).max().item()), 1)
         m_pad = _round_up(keep, 16)
         At = torch.zeros(B, c.nao_pad, m_pad, dtype=torch.float64, device=c.device)
         At[:, :c.nao, :keep] = mo[:, :, :keep] * occ[:, None, :keep].clamp(min=0).sqrt()
-        rho = torch.zeros(B, c.ld, dtype=torch.float64, device=c.device)
+        return At
+
+    def orb_density(self, phi, mo, occ, bidx=None):
+        """rho = sum_i f_i psi_i^2 over the orbitals with f_i > OCC_CUT (equal to the
+        D form with D = C f C^T up to that cut); bidx as in vmat."""
+        c = self.c
+        At = self._at(mo, occ)
+        rho = torch.zeros(mo.shape[0], c.ld, dtype=torch.float64, device=c.device)
         K.orb_density(phi, At, rho, bidx,
                       self._phi_scale if phi.dtype == torch.int32 else None)
+        return rho
+
+    def orb_density_q8(self, st, mo, occ, bidx=None):
+        """orb_density from the int8 digits of phi (st['phi_q8'], st['phi_e8'])."""
+        c = self.c
+        At = self._at(mo, occ)
+        rho = torch.empty(mo.shape[0], c.ld, dtype=torch.float64, device=c.device)
+        K.orb_density_q8(st["phi_q8"], st["phi_e8"], At, rho, self.Q8_SLICES, bidx)
         return rho
 
     def poisson(self, rho, save=False, geo=None, paired=False):
@@ -415,7 +436,14 @@ class Engine:
         st = {"xpb": xpb.contiguous(), "B": B, "hetero": hetero, "dV": c.dV}
         st["Lb"] = Lb if hetero else torch.full((B,), c.L, dtype=f64, device=dev)
         st["dVb"] = (st["Lb"] / c.g) ** 3
-        per = ["xpb", "Lb", "dVb", "S", "T", "v", "phi", "vloc", "V", "Hcore"]
+        # AO values stay in geometry order while the SCF compacts the slots (gidx maps
+        # slot -> geometry) for DFT; the exchange path moves them with the slots
+        inplace = self.dft
+        per = ["xpb", "Lb", "dVb", "S", "T", "v", "vloc", "V", "Hcore"]
+        phi_keys = ["phi"]
+        st["gidx"] = torch.arange(B, dtype=torch.int32, device=dev)
+        if inplace:
+            per.append("gidx")
         if hetero:
             st["VGb"] = torch.empty(B, c.N, dtype=f64, device=dev)
             for cst, idx in groups:
@@ -476,24 +504,33 @@ class Engine:
         if mixed:
             if need32:
                 st["phi_lp"] = phi32
-                per.append("phi_lp")
+                phi_keys.append("phi_lp")
             if use_h2:
                 # FP16 (hi, lo) pairs of s*phi, s = 2^k with |s phi| <= 2^14
                 amax = float(torch.linalg.vector_norm(phi, float("inf")).item())
                 self._phi_scale = 2.0 ** math.floor(math.log2(2.0 ** 14 / max(amax, 1e-300)))
                 st["phi_h2"] = torch.empty(B, c.nao_pad, c.ld, dtype=torch.int32, device=dev)
                 K.phi_split_f16(phi, self._phi_scale, st["phi_h2"])
-                per.append("phi_h2")
-            st["phisq"] = (phi[:, :c.nao, :c.N] ** 2).sum(1)             # sum_p phi_p(r)^2
-            st["Da"] = torch.zeros(B, c.nao, c.nao, dtype=f64, device=dev)
-            st["rhoa"] = torch.zeros(B, c.ld, dtype=f64, device=dev)
-            st["kappa"] = torch.zeros(B, dtype=f64, device=dev)          # min rho_a/phisq
+                phi_keys.append("phi_h2")
+            if self.MIXED_Q8 and c.nao_pad <= K.Q8_MAX_NAO_PAD:
+                # int8 digits of phi for the accurate density (exact integer products)
+                ns = self.Q8_SLICES
+                st["phi_q8"] = torch.empty(B, c.ld * ns * c.nao_pad, dtype=torch.int8, device=dev)
+                st["phi_e8"] = torch.empty(B, c.ld, dtype=torch.int16, device=dev)
+                K.phi_slice_i8(phi, ns, st["phi_q8"], st["phi_e8"])
+                phi_keys += ["phi_q8", "phi_e8"]
+            if self.MIXED_RHO_INCREMENTS:
+                st["phisq"] = (phi[:, :c.nao, :c.N] ** 2).sum(1)         # sum_p phi_p(r)^2
+                st["Da"] = torch.zeros(B, c.nao, c.nao, dtype=f64, device=dev)
+                st["rhoa"] = torch.zeros(B, c.ld, dtype=f64, device=dev)
+                st["kappa"] = torch.zeros(B, dtype=f64, device=dev)      # min rho_a/phisq
+                per += ["phisq", "Da", "rhoa", "kappa"]
             st["va"] = torch.zeros(B, c.ld, dtype=f64, device=dev)
             st["Va"] = torch.zeros(B, c.nao, c.nao, dtype=f64, device=dev)
             st["vmax"] = torch.zeros(B, dtype=f64, device=dev)
             st["va_set"] = torch.zeros(B, dtype=torch.bool, device=dev)
             st["err"] = torch.full((B,), float("inf"), dtype=f64, device=dev)
-            per += ["phisq", "Da", "rhoa", "kappa", "va", "Va", "vmax", "va_set", "err"]
+            per += ["va", "Va", "vmax", "va_set", "err"]
         st["vloc"] = vl
         st["V"] = self.vmat(phi, vl, dVs)
         st["Hcore"] = T + st["V"]
@@ -508,6 +545,8 @@ class Engine:
                 self._put(st["VGh"], idx, cst.VGh.expand(nb, c.ld))
                 self._put(st["c0b"], idx, torch.full((nb,), cst.c0, dtype=f64, device=dev))
             per += ["Sgrid", "VGh", "c0b"]
+        if not inplace:
+            per += phi_keys
         st["per"] = per
         return st
 
@@ -590,6 +629,13 @@ class Engine:
         return Kmat + st["c0b"][:na, None, None] * (Sg @ D @ Sg)
 
     @staticmethod
+    def _gix(st, na, idx=None):
+        """int32 geometry index (into the AO-value arrays) of the slots [0, na), or of
+        the subset idx of them."""
+        g = st["gidx"][:na]
+        return g if idx is None else g[idx.long()]
+
+    @staticmethod
     def _subset(mask):
         """(index tensor int32 or None when the mask covers the batch, count)."""
         n = int(mask.sum().item())
@@ -598,21 +644,23 @@ class Engine:
         return torch.nonzero(mask).flatten().to(torch.int32), n
 
     def _mixed_density(self, st, na, D, mo, occ):
-        """rho for geometries [0, na) in the mixed mode, chosen per geometry: the FP32
-        orbital form outright while its DIIS residual is large (err >= MIXED_DIRECT_ERR),
-        FP32 increments rho_a + rho(D - D_a) where the increment bound holds, FP64
-        (and a new anchor) elsewhere.  Each class is one launch over its geometries,
-        read in place from the batch arrays through a geometry index list."""
+        """rho for geometries [0, na) in the mixed mode, chosen per geometry: the
+        low-precision orbital form outright while its DIIS residual is large
+        (err >= MIXED_RHO_DIRECT_ERR), FP32 increments rho_a + rho(D - D_a) where the
+        increment bound holds (off by default), the accurate orbital form (int8 digit
+        products, or FP64; a new anchor) elsewhere.  Each class is one launch over its
+        geometries, read in place from the AO-value arrays through a geometry index list."""
         c = self.c
         sel = lambda key: st[key][:na]
         err = sel("err")
         direct = err >= self.MIXED_RHO_DIRECT_ERR
-        dD = D - sel("Da")
-        # bound on the relative density error of the FP32 increment
-        rel = self.MIXED_EPS * torch.linalg.norm(dD, dim=(-2, -1)) / sel("kappa")
-        inc = ~direct & (rel <= 1e-2) & (rel * sel("vmax") <= self.MIXED_RHO_SAFETY * err)
-        if not self.MIXED_RHO_INCREMENTS:
-            inc = torch.zeros_like(inc)
+        if "rhoa" in st:
+            dD = D - sel("Da")
+            # bound on the relative density error of the FP32 increment
+            rel = self.MIXED_EPS * torch.linalg.norm(dD, dim=(-2, -1)) / sel("kappa")
+            inc = ~direct & (rel <= 1e-2) & (rel * sel("vmax") <= self.MIXED_RHO_SAFETY * err)
+        else:
+            inc = torch.zeros_like(direct)
         full = ~(direct | inc)
         rho = torch.empty(na, c.ld, dtype=torch.float64, device=c.device)
         idx, n = self._subset(direct)
@@ -620,29 +668,34 @@ class Engine:
             # orbital form sum_i f_i psi_i^2: non-negative in any precision, unlike the
             # D form, whose FP32 rounding can turn a tiny density negative
             lp = "phi_h2" if "phi_h2" in st else "phi_lp"
-            r = self.orb_density(st[lp] if n < na else sel(lp),
-                                 mo if idx is None else mo[idx.long()],
-                                 occ if idx is None else occ[idx.long()], idx)
+            r = self.orb_density(st[lp], mo if idx is None else mo[idx.long()],
+                                 occ if idx is None else occ[idx.long()], self._gix(st, na, idx))
             self._scatter(rho, idx, r)
             self.stats["rho32"] += 1
         idx, n = self._subset(inc)
         if n:
             il = None if idx is None else idx.long()
             r = torch.zeros(n, c.ld, dtype=torch.float64, device=c.device)
-            K.ao_density(st["phi_lp"] if n < na else sel("phi_lp"),
-                         self._pad(dD if il is None else dD[il]), r, bidx=idx)
+            K.ao_density(st["phi_lp"], self._pad(dD if il is None else dD[il]), r,
+                         bidx=self._gix(st, na, idx))
             r += sel("rhoa") if il is None else st["rhoa"][il]
             self._scatter(rho, idx, r)
             self.stats["rho32"] += 1
         idx, n = self._subset(full)
         if n:
             il = torch.arange(na, device=c.device) if idx is None else idx.long()
-            r64 = self.orb_density(st["phi"] if n < na else sel("phi"), mo[il], occ[il], idx)
+            gi = self._gix(st, na, idx)
+            if "phi_q8" in st:
+                r64 = self.orb_density_q8(st, mo[il], occ[il], gi)
+                self.stats["rhoq8"] += 1
+            else:
+                r64 = self.orb_density(st["phi"], mo[il], occ[il], gi)
+                self.stats["rho64"] += 1
             self._scatter(rho, idx, r64)
-            st["Da"][il] = D[il]
-            st["rhoa"][il] = r64
-            st["kappa"][il] = (r64[:, :c.N] / st["phisq"][il]).amin(-1)
-            self.stats["rho64"] += 1
+            if "rhoa" in st:                                 # anchor of the increments
+                st["Da"][il] = D[il]
+                st["rhoa"][il] = r64
+                st["kappa"][il] = (r64[:, :c.N] / st["phisq"][il]).amin(-1)
         return rho
 
     @staticmethod
@@ -677,8 +730,7 @@ class Engine:
             vv = torch.where(direct[:, None], v, dv)
             vvm = torch.where(direct, st["vmax"][:na], dvmax)
             d = dVs if not torch.is_tensor(dVs) or il is None else dVs[il]
-            V32 = self.vmat(st[lp] if n < na else sel(lp),
-                            vv if il is None else vv[il], d, idx,
+            V32 = self.vmat(st[lp], vv if il is None else vv[il], d, self._gix(st, na, idx),
                             vvm if il is None else vvm[il])
             V32 = V32 + torch.where(inc[:, None, None], sel("Va"), 0.0)[
                 slice(None) if il is None else il]
@@ -688,7 +740,7 @@ class Engine:
         if n:
             il = torch.arange(na, device=c.device) if idx is None else idx.long()
             d = dVs if not torch.is_tensor(dVs) else dVs[il]
-            V64 = self.vmat(st["phi"] if n < na else sel("phi"), v[il], d, idx)
+            V64 = self.vmat(st["phi"], v[il], d, self._gix(st, na, idx))
             self._scatter(V2, idx, V64)
             st["va"][il] = v[il]
             st["Va"][il] = V64
@@ -699,16 +751,15 @@ class Engine:
     def fock2e(self, st, na, D, mo, occ, full=False):
         """(J + V_xc or J - K/2, E_H + E_xc or E_H + E_x) for geometries [0, na)."""
         c = self.c
-        sel = lambda key: st[key][:na]
-        phi = sel("phi")
         mixed = self.precision == "mixed" and not full
         dVs = self._dv(st, na)
         geo = self._geo(st, na)
         if not mixed:
-            rho = self.orb_density(phi, mo, occ)
+            gi = self._gix(st, na)
+            rho = self.orb_density(st["phi"], mo, occ, gi)
             self.stats["rho64"] += 1
             v, E, _ = self.xc_potential(rho, geo=geo)
-            V2 = self.vmat(phi, v, dVs)
+            V2 = self.vmat(st["phi"], v, dVs, gi)
             self.stats["v64"] += 1
         else:
             rho = self._mixed_density(st, na, D, mo, occ)

@@ -92,7 +92,7 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
     # grid adjoint (FP64 phi regardless of the SCF precision)
     X = torch.zeros(B, c.nao_pad, c.ld, dtype=f64, device=dev)
     rho = torch.zeros(B, c.ld, dtype=f64, device=dev)
-    K.ao_density(st["phi"], engine._pad(D), rho, X)
+    K.ao_density(st["phi"], engine._pad(D), rho, X, bidx=st["gidx"])
     v, _, rg = engine.xc_potential(rho, save_rg=True, geo=engine._geo(st, B))
     veff = (v + st["vloc"]).contiguous()
     Zx = None if engine.dft else _exchange_adjoint(engine, st)
@@ -150,7 +150,7 @@ def compute_forces_k(engine, st, kappa=10, Gmax=15):
     F.index_add_(1, atom, t)
     X = torch.zeros(B, 2 * c.nao_pad, c.ld, dtype=f64, device=dev)
     rho = torch.zeros(B, c.ld, dtype=f64, device=dev)
-    K.ao_density(st["phi"], engine.stacked_dm(D), rho, X)
+    K.ao_density(st["phi"], engine.stacked_dm(D), rho, X, bidx=st["gidx"])
     v, _, rg = engine.xc_potential(rho, save_rg=True)
     veff = (v + st["vloc"]).contiguous()
     P = torch.empty(B, c.n, 3, c.g, c.na, 3, dtype=c128, device=dev)

@@ -112,7 +112,10 @@ class KEngine(Engine):
         phi = torch.zeros(B, 2 * c.nao_pad, c.ld, dtype=f64, device=dev)
         K.ao_values_k(P, c.coef, c.comp, c.nao_pad, c.ld, phi)
         st["phi"] = phi
-        per = ["xpb", "kpt", "Lb", "dVb", "S", "T", "v", "phi", "vloc", "V", "Hcore"]
+        # as in the Gamma engine: DFT keeps phi in geometry order, slot -> geometry in gidx
+        st["gidx"] = torch.arange(B, dtype=torch.int32, device=dev)
+        per = ["xpb", "kpt", "Lb", "dVb", "S", "T", "v", "vloc", "V", "Hcore"]
+        per.append("gidx" if self.dft else "phi")
         if forces:
             per += ["dS", "dT"]
         if "lindep" in st:

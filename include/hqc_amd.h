@@ -123,6 +123,23 @@ int hqc_orb_density_f32(const float* phi, const int* bidx, const double* At, int
 int hqc_orb_density_h2(const unsigned* ph, const int* bidx, const double* At, int B,
                        int nao_pad, int m_pad, int ld, double phi_scale, double* rho,
                        void* stream);
+/* q8: from int8 digits with exact integer accumulation (Ozaki-style slicing).
+ * hqc_phi_slice_i8 cuts phi (G,nao_pad,ld) FP64, once per geometry, into
+ * nslice (3..5) int8 digits per value under a per-grid-point power-of-two
+ * exponent: q (G, ld*nslice*nao_pad) bytes in tiles of 16 points
+ * [tile][digit][AO block of 16][point][AO], pe (G,ld) the exponents.
+ * hqc_orb_density_q8 cuts At per call into 6 digits per orbital (per-orbital
+ * exponent), forms every digit product of level <= 5 on
+ * v_mfma_i32_16x16x64_i8 with int32 accumulation (exact), combines the levels
+ * in FP64 and sums psi^2 in FP64.  Error: the fixed phi rounding,
+ * 2^-(7 nslice) of max_k |phi_k(r)| per value (a perturbation of the basis
+ * that does not change between calls), plus ~2^-42 of max|At_i| max|phi(r)|
+ * per AO from the coefficient digits.  nao_pad <= 128 (HQC_E_SIZE beyond). */
+int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, int nslice,
+                     signed char* q, short* pe, void* stream);
+int hqc_orb_density_q8(const signed char* q, const short* pe, const int* bidx,
+                       const double* At, int B, int nao_pad, int m_pad, int ld, int nslice,
+                       double* rho, void* stream);
 
 /* Grid potential matrix V[b] = dV * phi diag(v[b]) phi^T (symmetric, full).
  * Replaces the 'pr,qr,r->pq' contraction (hqc/pbc/solver.py:693) and, through

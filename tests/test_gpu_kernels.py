@@ -7,6 +7,7 @@ import pytest
 import torch
 
 from oracle import cpu_ref as R
+from q8_digits import q8_density_digits
 
 pytestmark = pytest.mark.gpu
 
@@ -183,6 +184,56 @@ def test_orb_density_split16(K, nao_pad, m):
     rref2 = ((At.transpose(1, 2) @ phi.flip(0)) ** 2).sum(1)
     bound2 = ((At.abs().transpose(1, 2) @ phi.flip(0).abs()) ** 2).sum(1)
     assert ((rho2 - rref2).abs() <= 2.0 ** -18 * bound2 + 1e-300).all()
+
+
+@pytest.mark.parametrize("nao_pad,m,ns", [(16, 8, 4), (48, 20, 3), (112, 60, 4), (112, 100, 5),
+                                          (128, 140, 4)])
+def test_orb_density_q8(K, nao_pad, m, ns):
+    """Orbital density from int8 digits (exact integer products): equal to the host
+    restatement of the digit algorithm to FP64 rounding, and within the digit
+    error bound of the FP64 density,
+        |rho~ - rho| <= sum_i (2|psi_i| + d_i) d_i,
+        d_i(r) = nao_pad max_k|At_ki| max_k|phi_k(r)| (2^(1-7 ns) + 2^-36)."""
+    print(f"[orb density q8] nao_pad={nao_pad} m={m} ns={ns}", flush=True)
+    torch.manual_seed(7)
+    g = 11
+    N = g ** 3
+    ld = (N + 63) // 64 * 64
+    B = 2
+    phi = torch.zeros(B, nao_pad, ld, dtype=torch.float64, device=DEV)
+    phi[:, :, :N] = torch.randn(B, nao_pad, N, dtype=torch.float64, device=DEV)
+    # per-point magnitudes over 30 binary orders, some points all zero
+    phi[:, :, :N] *= torch.exp2(-torch.randint(0, 30, (B, 1, N), device=DEV).double())
+    phi[:, :, 5:9] = 0.0
+    m_pad = (m + 15) // 16 * 16
+    At = torch.zeros(B, nao_pad, m_pad, dtype=torch.float64, device=DEV)
+    At[:, :, :m] = torch.randn(B, nao_pad, m, dtype=torch.float64, device=DEV)
+    At[:, :, :m] *= torch.exp(-0.2 * torch.arange(m, device=DEV, dtype=torch.float64))
+    At[1] *= 1e-3
+    At[0, :, m // 2] = 0.0                              # an empty orbital column
+    q = torch.empty(B, ld * ns * nao_pad, dtype=torch.int8, device=DEV)
+    pe = torch.empty(B, ld, dtype=torch.int16, device=DEV)
+    K.phi_slice_i8(phi, ns, q, pe)
+    rho = torch.full((B, ld), float("nan"), dtype=torch.float64, device=DEV)
+    K.orb_density_q8(q, pe, At, rho, ns)
+    torch.cuda.synchronize()
+    assert torch.isfinite(rho).all()
+    ph, A = phi.cpu().numpy(), At.cpu().numpy()
+    emul = q8_density_digits(ph, A, ns)
+    r = rho.cpu().numpy()
+    assert np.allclose(r, emul, rtol=1e-13, atol=1e-300)
+    psi = np.matmul(A.transpose(0, 2, 1), ph)                    # (B, m, ld)
+    rref = (psi ** 2).sum(1)
+    d = (nao_pad * np.abs(A).max(1)[:, :, None] * np.abs(ph).max(1)[:, None, :]
+         * (2.0 ** (1 - 7 * ns) + 2.0 ** -36))
+    bound = ((2 * np.abs(psi) + d) * d).sum(1)
+    assert (np.abs(r - rref) <= bound).all()
+    # through a geometry index list: slot 0 reads geometry 1 and slot 1 geometry 0
+    bidx = torch.tensor([1, 0], dtype=torch.int32, device=DEV)
+    rho2 = torch.empty_like(rho)
+    K.orb_density_q8(q, pe, At, rho2, ns, bidx)
+    emul2 = q8_density_digits(ph[::-1].copy(), A, ns)
+    assert np.allclose(rho2.cpu().numpy(), emul2, rtol=1e-13, atol=1e-300)
 
 
 @pytest.mark.parametrize("g", [1, 2, 5, 7, 13, 17, 24, 25, 27, 31, 33, 48, 49, 51, 55, 57, 61,

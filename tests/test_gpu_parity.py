@@ -107,12 +107,15 @@ def test_batched_equals_single():
         assert abs(pes(xps[b]).item() - Eb[b]) < 1e-10
 
 
-@pytest.mark.parametrize("rho_inc,split16", [(False, False), (True, False), (False, True)],
-                         ids=["v_fp32", "rho_increments32", "v_split16"])
-def test_mixed_precision_matches_fp64(rho_inc, split16):
-    """Defect-corrected FP32 increments converge to the FP64 answer within the
-    accuracy bar (dE <= 1 uHa/atom, dF <= 1e-5 Ha/Bohr, dbands <= 1e-6 Ha), with
-    the density past the warm-up in FP64 orbital form or as FP32 D-form increments."""
+@pytest.mark.parametrize("rho_inc,split16,q8",
+                         [(False, False, False), (True, False, False), (False, True, False),
+                          (False, True, True)],
+                         ids=["v_fp32", "rho_increments32", "v_split16", "v_split16_rho_q8"])
+def test_mixed_precision_matches_fp64(rho_inc, split16, q8):
+    """Defect-corrected low-precision increments converge to the FP64 answer within
+    the accuracy bar (dE <= 1 uHa/atom, dF <= 1e-5 Ha/Bohr, dbands <= 1e-6 Ha), with
+    the density past the warm-up in FP64 orbital form, from int8 digits (q8), or as
+    FP32 D-form increments."""
     from hqc_amd.pbc import make_pes
     n, rs = 16, 1.25
     L = (4 / 3 * np.pi * n) ** (1 / 3)
@@ -122,6 +125,7 @@ def test_mixed_precision_matches_fp64(rho_inc, split16):
     pmx = make_pes(n, L, rs, "gth-dzv", precision="mixed", **kw)
     pmx.solver.engine.MIXED_RHO_INCREMENTS = rho_inc
     pmx.solver.engine.MIXED_SPLIT16 = split16
+    pmx.solver.engine.MIXED_Q8 = q8
     E64, F64 = p64.energy_and_forces(xps)
     Emx, Fmx = pmx.energy_and_forces(xps)
     assert bool(pmx.solver.last["conv"].all())
@@ -129,7 +133,42 @@ def test_mixed_precision_matches_fp64(rho_inc, split16):
     assert (F64 - Fmx).abs().max().item() <= 1e-5
     assert (p64.solver.last["w1"] - pmx.solver.last["w1"]).abs().max().item() <= 1e-6
     assert pmx.solver.engine.stats["v32"] > 0 and pmx.solver.engine.stats["rho32"] > 0
+    assert (pmx.solver.engine.stats["rhoq8"] > 0) == q8
     assert pmx.solver.last is not None
+
+
+def test_compacted_batch_keeps_phi_in_place():
+    """A DFT batch whose geometries converge at different iterations: the SCF
+    permutes its slots while the AO values stay in geometry order (read through
+    the slot -> geometry index), and energies and forces equal the one-geometry
+    runs; the same with the mixed path's int8-digit density."""
+    from hqc_amd.pbc import make_pes
+    n, rs = 8, 1.3
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    xps = np.random.default_rng(31).uniform(0, L, (5, n, 3))
+    for prec in ("fp64", "mixed"):
+        print(f"[phi in place] {prec}", flush=True)
+        # a tight Fermi-level search: the default 1e-7 leaves a residual floor whose
+        # stopping iteration, and energy to ~1e-9, depends on rounding
+        pes = make_pes(n, L, rs, "gth-dzv", grid_length=0.25, dft=True, smearing=True,
+                       smearing_sigma=0.05, tol=1e-10, search_tol=1e-13, search_cycle=200,
+                       max_cycle=300, precision=prec)
+        E, F = pes.energy_and_forces(xps)
+        st = pes.solver.last
+        assert "phi" not in st["per"] and "gidx" in st["per"]
+        order = st["order"].cpu().numpy()
+        assert (order != np.arange(5)).any()                  # slots were permuted
+        assert (st["gidx"].cpu().numpy() == order).all()
+        if prec == "mixed":
+            assert pes.solver.engine.stats["rhoq8"] > 0
+        # FP64 is batch-invariant to the SCF tolerance; the mixed path's FP16 split
+        # scale is taken over the batch, so there batch and single runs agree to
+        # the mixed accuracy bar (1 uHa/atom = 2e-6 Ry/atom, 1e-5 Ha/Bohr)
+        tol_e, tol_f = (1e-9, 1e-7) if prec == "fp64" else (2e-6 * n, 1e-5)
+        for b in (0, 3):
+            Eb, Fb = pes.energy_and_forces(xps[b])
+            assert abs(Eb.item() - E[b].item()) < tol_e
+            assert (Fb.reshape(n, 3) - F[b]).abs().max().item() < tol_f
 
 
 def test_mixed_single_geometry_split16():
