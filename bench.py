@@ -163,7 +163,8 @@ def measured_traffic(kernel_prefix, B):
         return None
     for line in open(TRAFFIC_FILE):
         parts = line.split()
-        if len(parts) >= 5 and not line.startswith("#") and line[13:].startswith(kernel_prefix):
+        if (len(parts) >= 5 and not line.startswith("#")
+                and line.split(None, 1)[1].startswith(kernel_prefix)):
             return float(parts[-1])
     return None
 

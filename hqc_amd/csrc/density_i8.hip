@@ -31,7 +31,9 @@ typedef unsigned u32x4q __attribute__((ext_vector_type(4)));
 #define Q8_SC (Q8_T + 1)
 #define Q8_MAXKS 2        // nao_pad <= 128 (two K steps of 64 AOs)
 #define Q8_NW 4           // waves per workgroup
+#ifndef Q8_TARGET_BLOCKS
 #define Q8_TARGET_BLOCKS 2048
+#endif
 
 __device__ __forceinline__ int q8_exponent(double m) {
   // smallest e with m < 2^e (0 for m = 0)
@@ -127,18 +129,17 @@ __global__ __launch_bounds__(64 * Q8_NW) void k_orb_density_q8(
   const int wstride = gridDim.x * Q8_NW;
   const int pt0 = blockIdx.x * Q8_NW + w;
 
+  // AO blocks past nao_pad re-read the last block: their coefficient digits are
+  // zero, so the (exact integer) products vanish whatever these bytes hold, and
+  // the loaded values are not touched before the MFMAs that use them
   auto loadB = [&](int pt, u32x4q (&Bf)[SP][KS], int& er) {
     const signed char* base = Qb + (long)pt * SP * NB * 256;
 #pragma unroll
     for (int s = 0; s < SP; ++s)
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const int blk = 4 * ks + kq;
-        // AO blocks past nao_pad read the last block (in bounds) and are zeroed
-        const u32x4q x =
-            *reinterpret_cast<const u32x4q*>(base + ((long)s * NB + min(blk, NB - 1)) * 256);
-        Bf[s][ks] = blk < NB ? x : u32x4q{0u, 0u, 0u, 0u};
-      }
+      for (int ks = 0; ks < KS; ++ks)
+        Bf[s][ks] = *reinterpret_cast<const u32x4q*>(
+            base + ((long)s * NB + min(4 * ks + kq, NB - 1)) * 256);
     er = Pe[pt * 16 + j];
   };
 
@@ -180,17 +181,7 @@ __global__ __launch_bounds__(64 * Q8_NW) void k_orb_density_q8(
     }
     __syncthreads();
     const int MT = MW >> 4;
-    int pt = pt0;
-    u32x4q Bc[SP][KS], Bn[SP][KS];
-#pragma unroll
-    for (int s = 0; s < SP; ++s)
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) Bn[s][ks] = u32x4q{0u, 0u, 0u, 0u};
-    int erc = 0, ern = 0;
-    if (pt < NPT) loadB(pt, Bc, erc);
-    while (pt < NPT) {                          // wave-uniform
-      const int pn = pt + wstride;
-      if (pn < NPT) loadB(pn, Bn, ern);
+    auto tile = [&](const u32x4q (&Bf)[SP][KS], int er, int pt) {
       double racc = 0.0;
       for (int t = 0; t < MT; ++t) {
         i32x4 acc[Q8_T + 1];
@@ -205,7 +196,7 @@ __global__ __launch_bounds__(64 * Q8_NW) void k_orb_density_q8(
                                                    16 * kq);
 #pragma unroll
           for (int l = 0; l < SP; ++l) {
-            const i32x4 bl = __builtin_bit_cast(i32x4, Bc[l][ks]);
+            const i32x4 bl = __builtin_bit_cast(i32x4, Bf[l][ks]);
 #pragma unroll
             for (int k = 0; k + l <= Q8_T; ++k)
               acc[k + l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[k], bl, acc[k + l], 0, 0, 0);
@@ -223,16 +214,26 @@ __global__ __launch_bounds__(64 * Q8_NW) void k_orb_density_q8(
       racc += __shfl_xor(racc, 16, 64);
       racc += __shfl_xor(racc, 32, 64);
       if (kq == 0) {
-        const double r = ldexp(racc, 2 * erc - 24);
+        const double r = ldexp(racc, 2 * er - 24);
         const long idx = (long)pt * 16 + j;
         Rb[idx] = og == 0 ? r : Rb[idx] + r;
       }
-#pragma unroll
-      for (int s = 0; s < SP; ++s)
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) Bc[s][ks] = Bn[s][ks];
-      erc = ern;
-      pt = pn;
+    };
+    // two operand buffers in turn: the next tile's loads are in flight while the
+    // current tile runs
+    u32x4q B0[SP][KS], B1[SP][KS];
+    int er0 = 0, er1 = 0;
+    int pt = pt0;
+    if (pt < NPT) loadB(pt, B0, er0);
+    while (pt < NPT) {                          // wave-uniform
+      const int p1 = pt + wstride;
+      if (p1 < NPT) loadB(p1, B1, er1);
+      tile(B0, er0, pt);
+      if (p1 >= NPT) break;
+      const int p2 = p1 + wstride;
+      if (p2 < NPT) loadB(p2, B0, er0);
+      tile(B1, er1, p1);
+      pt = p2;
     }
   }
 }
