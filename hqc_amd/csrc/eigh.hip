@@ -28,6 +28,44 @@ __device__ __forceinline__ double getA(const double* Ap, int i, int j, int n) {
 
 #define EIG_THREADS 1024
 
+// slot j of the circle-method schedule in round r: index 0 stays, the other
+// m = npad - 1 indices move one place per round (the host table's order)
+__device__ __forceinline__ int circle_at(int j, int r, int m) {
+  if (j == 0) return 0;
+  const int x = j - 1 - r;                  // in (-m, m)
+  return 1 + (x < 0 ? x + m : x);
+}
+
+template <bool TABLE>
+__device__ __forceinline__ void round_pair(const int* __restrict__ prow, int k, int r, int npad,
+                                           int& p, int& q) {
+  int a, b;
+  if (TABLE) {
+    a = prow[2 * k];
+    b = prow[2 * k + 1];
+  } else {
+    a = circle_at(k, r, npad - 1);
+    b = circle_at(npad - 1 - k, r, npad - 1);
+  }
+  p = min(a, b);
+  q = max(a, b);
+}
+
+// (c, s) of the rotation that zeroes a_pq: t = tan(angle) is the root of
+// t^2 + 2 theta t - 1 = 0 of smaller magnitude, theta = (a_qq - a_pp) / (2 a_pq).
+// Only c and s enter the update, and c = 1/sqrt(1 + t^2), s = t c is an exact
+// rotation for any t, so t comes from the hardware reciprocal; the rotated block
+// is then formed from (c, s) as R^T M R, so A and V stay consistent.
+__device__ __forceinline__ double2 jacobi_cs(double app, double aqq, double apq) {
+  const double d = aqq - app;
+  const double h = sqrt(d * d + 4.0 * apq * apq);
+  double t = 2.0 * fabs(apq) * __builtin_amdgcn_rcp(fabs(d) + h);
+  if (d != 0.0 && ((d < 0.0) != (apq < 0.0))) t = -t;   // sign of theta
+  const double c = rsqrt(1.0 + t * t);
+  return make_double2(c, t * c);
+}
+
+template <bool TABLE>
 __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __restrict__ Ain, int n,
                                                      int npad, const int* __restrict__ pairs,
                                                      int max_sweeps, double tol,
@@ -36,33 +74,44 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
                                                      int* __restrict__ sweeps_out) {
   extern __shared__ double sm[];
   const int np = npad / 2;
+  const int m = npad - 1;                         // rounds per sweep
+  const int na = n * (n + 1) / 2;
   double* Ap = sm;                                // n(n+1)/2
-  double* V = Ap + n * (n + 1) / 2;               // n*n
-  double* cs = V + n * n;                         // 2*np: (c, s) per pair
-  double* red = cs + 2 * np;                      // EIG_THREADS/64 partial sums
-  int* pr = reinterpret_cast<int*>(red + EIG_THREADS / 64);  // 2*np: current pairing
-  unsigned short* blk = reinterpret_cast<unsigned short*>(pr + 2 * np);  // (I,J) of each block
+  double* V = Ap + na;                            // n*n
+  const int cso = (na + n * n + 1) & ~1;          // 16-byte aligned
+  double2* cs = reinterpret_cast<double2*>(sm + cso);   // np: (c, s) per pair
+  double* red = sm + cso + 2 * np;                // EIG_THREADS/64 partial sums
+  __shared__ double s_fro2;
+  __shared__ int s_done, s_rot;
   const int nblk = np * (np + 1) / 2;
-  for (int e = threadIdx.x; e < np; e += blockDim.x) {
-    // blocks of row I start at I*np - I*(I-1)/2
-    const int start = e * np - (e * (e - 1)) / 2;
-    for (int J = e; J < np; ++J) blk[start + (J - e)] = (unsigned short)((e << 8) | J);
+  const int tid = threadIdx.x;
+  // this thread's blocks (I <= J) of the pair-by-pair partition, fixed for the
+  // whole solve: e = tid and tid + EIG_THREADS in row-major order of the triangle
+  int bI[2], bJ[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int e = tid + u * EIG_THREADS;
+    int I = 0, start = 0;
+    while (I < np && start + (np - I) <= e) {     // row I holds blocks (I, I..np-1)
+      start += np - I;
+      ++I;
+    }
+    bI[u] = e < nblk ? I : -1;
+    bJ[u] = I + (e - start);
   }
   const int b = blockIdx.x;
   const double* A = Ain + (long)b * n * n;
-  const int tid = threadIdx.x;
   for (int e = tid; e < n * n; e += blockDim.x) {
     const int i = e / n, j = e - i * n;
     if (i <= j) Ap[pk(i, j, n)] = 0.5 * (A[i * n + j] + A[j * n + i]);
     V[e] = (i == j) ? 1.0 : 0.0;          // column-major: V[col*n + row]
   }
+  if (tid == 0) s_rot = 0;
   __syncthreads();
   // scale for the stopping test
   double loc = 0.0;
-  for (int e = tid; e < n * (n + 1) / 2; e += blockDim.x) loc += Ap[e] * Ap[e];
+  for (int e = tid; e < na; e += blockDim.x) loc += Ap[e] * Ap[e];
   const double fro2 = block_sum(loc, red);
-  __shared__ double s_fro2;
-  __shared__ int s_done;
   if (tid == 0) s_fro2 = fro2;
   __syncthreads();
   // threshold Jacobi: rotations with |a_pq| <= thr are skipped.  If every
@@ -89,63 +138,40 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
     if (tid == 0) s_done = (off <= tol * tol * s_fro2) ? 1 : 0;
     __syncthreads();
     if (s_done) break;
-    for (int r = 0; r < npad - 1; ++r) {
-      const int* prow = pairs ? pairs + r * npad : nullptr;
-      for (int k = tid; k < np; k += blockDim.x) {
+    for (int r = 0; r < m; ++r) {
+      const int* prow = TABLE ? pairs + r * npad : nullptr;
+      const int gr = sweep * m + r + 1;           // this round's id
+      if (tid < np) {
         int p, q;
-        if (prow) {
-          p = prow[2 * k];
-          q = prow[2 * k + 1];
-        } else {
-          // circle method: index 0 fixed, the others rotated r places; pair k joins
-          // positions k and npad-1-k (the schedule hqc_amd's host code builds)
-          const int m = npad - 1;
-          auto at = [&](int j) { return j == 0 ? 0 : 1 + ((j - 1 - r) % m + m) % m; };
-          p = at(k);
-          q = at(npad - 1 - k);
-        }
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
-        pr[2 * k] = p;
-        pr[2 * k + 1] = q;
-        double c = 1.0, s = 0.0;
+        round_pair<TABLE>(prow, tid, r, npad, p, q);
+        double2 rot = make_double2(1.0, 0.0);
         if (q < n) {
           const double apq = Ap[pk(p, q, n)];
           if (fabs(apq) > thr) {
-            const double app = Ap[pk(p, p, n)], aqq = Ap[pk(q, q, n)];
-            const double theta = (aqq - app) / (2.0 * apq);
-            const double t = (theta >= 0.0 ? 1.0 : -1.0) /
-                             (fabs(theta) + sqrt(theta * theta + 1.0));
-            c = 1.0 / sqrt(t * t + 1.0);
-            s = t * c;
+            rot = jacobi_cs(Ap[pk(p, p, n)], Ap[pk(q, q, n)], apq);
+            s_rot = gr;
           }
         }
-        cs[2 * k] = c;
-        cs[2 * k + 1] = s;
+        cs[tid] = rot;
       }
       __syncthreads();
+      // a round without a rotation leaves A and V as they are (uniform branch:
+      // s_rot only changes again after every thread has passed this point)
+      if (s_rot != gr) continue;
       // A <- J^T A J by 2x2 blocks (I <= J over pairs)
-      for (int e = tid; e < nblk; e += blockDim.x) {
-        const int I = blk[e] >> 8, J = blk[e] & 255;
-        const int pI = pr[2 * I], qI = pr[2 * I + 1], pJ = pr[2 * J], qJ = pr[2 * J + 1];
-        const double cI = cs[2 * I], sI = cs[2 * I + 1], cJ = cs[2 * J], sJ = cs[2 * J + 1];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int I = bI[u], J = bJ[u];
+        if (I < 0) continue;
+        const double2 rI = cs[I], rJ = cs[J];
+        if (rI.y == 0.0 && rJ.y == 0.0) continue;   // both rotations skipped
+        int pI, qI, pJ, qJ;
+        round_pair<TABLE>(prow, I, r, npad, pI, qI);
+        round_pair<TABLE>(prow, J, r, npad, pJ, qJ);
         const bool vqI = qI < n, vqJ = qJ < n;
-        if (I == J) {
-          if (!vqI) continue;
-          const double app = Ap[pk(pI, pI, n)], aqq = Ap[pk(qI, qI, n)], apq = Ap[pk(pI, qI, n)];
-          if (sI != 0.0) {
-            const double t = sI / cI;
-            Ap[pk(pI, pI, n)] = app - t * apq;
-            Ap[pk(qI, qI, n)] = aqq + t * apq;
-            Ap[pk(pI, qI, n)] = 0.0;
-          }
-          continue;
-        }
-        if (sI == 0.0 && sJ == 0.0) continue;     // both rotations skipped: block unchanged
-        // generic block: rows {pI,qI}, cols {pJ,qJ}; M' = R_I^T M R_J
+        const double cI = rI.x, sI = rI.y, cJ = rJ.x, sJ = rJ.y;
+        // rows {pI,qI}, cols {pJ,qJ}; M' = R_I^T M R_J (for I == J the
+        // symmetric diagonal block, whose off-diagonal comes out ~0)
         const double m00 = getA(Ap, pI, pJ, n);
         const double m01 = vqJ ? getA(Ap, pI, qJ, n) : 0.0;
         const double m10 = vqI ? getA(Ap, qI, pJ, n) : 0.0;
@@ -160,6 +186,13 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
           if (i <= j) Ap[pk(i, j, n)] = val;
           else Ap[pk(j, i, n)] = val;
         };
+        if (I == J) {
+          if (!vqI) continue;
+          Ap[pk(pI, pI, n)] = n00;
+          Ap[pk(qI, qI, n)] = n11;
+          Ap[pk(pI, qI, n)] = 0.5 * (n01 + n10);
+          continue;
+        }
         put(pI, pJ, n00);
         if (vqJ) put(pI, qJ, n01);
         if (vqI) put(qI, pJ, n10);
@@ -168,13 +201,14 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
       // V <- V J: thread owns row vrow and every vstep-th pair from vj0
       if (vrow_ok) {
         for (int J = vj0; J < np; J += vstep) {
-          const int pJ = pr[2 * J], qJ = pr[2 * J + 1];
+          const double2 rJ = cs[J];
+          if (rJ.y == 0.0) continue;
+          int pJ, qJ;
+          round_pair<TABLE>(prow, J, r, npad, pJ, qJ);
           if (qJ >= n) continue;
-          const double c = cs[2 * J], s = cs[2 * J + 1];
-          if (s == 0.0) continue;
           const double x = V[pJ * n + vrow], y = V[qJ * n + vrow];
-          V[pJ * n + vrow] = c * x - s * y;
-          V[qJ * n + vrow] = s * x + c * y;
+          V[pJ * n + vrow] = rJ.x * x - rJ.y * y;
+          V[qJ * n + vrow] = rJ.y * x + rJ.x * y;
         }
       }
       __syncthreads();
@@ -197,8 +231,8 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
 extern "C" size_t hqc_eigh_lds_bytes(int n) {
   const int npad = n + (n & 1);
   const int np = npad / 2;
-  return (size_t)(n * (n + 1) / 2 + n * n + npad + EIG_THREADS / 64) * sizeof(double) +
-         (size_t)npad * sizeof(int) + (size_t)(np * (np + 1) / 2) * sizeof(unsigned short);
+  const int cso = (n * (n + 1) / 2 + n * n + 1) & ~1;
+  return (size_t)(cso + 2 * np + EIG_THREADS / 64) * sizeof(double);
 }
 
 extern "C" int hqc_eigh_jacobi(const double* A, int B, int n, const int* pairs, int max_sweeps,
@@ -208,15 +242,20 @@ extern "C" int hqc_eigh_jacobi(const double* A, int B, int n, const int* pairs, 
   if (B == 0) return HQC_OK;
   const int npad = n + (n & 1);
   const size_t lds = hqc_eigh_lds_bytes(n);
+  const void* kern = pairs ? (const void*)k_jacobi_eigh<true> : (const void*)k_jacobi_eigh<false>;
   if (lds > 64 * 1024) {
     // raise the dynamic-LDS limit where the runtime asks for it; a runtime that
     // does not support the attribute leaves the launch check below to decide
-    if (hipFuncSetAttribute((const void*)k_jacobi_eigh,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+        hipSuccess)
       (void)hipGetLastError();
   }
-  hipLaunchKernelGGL(k_jacobi_eigh, dim3(B), dim3(EIG_THREADS), lds, (hipStream_t)stream, A, n, npad,
-                     pairs, max_sweeps, tol, w, V, sweeps);
+  if (pairs)
+    hipLaunchKernelGGL(k_jacobi_eigh<true>, dim3(B), dim3(EIG_THREADS), lds, (hipStream_t)stream, A,
+                       n, npad, pairs, max_sweeps, tol, w, V, sweeps);
+  else
+    hipLaunchKernelGGL(k_jacobi_eigh<false>, dim3(B), dim3(EIG_THREADS), lds, (hipStream_t)stream,
+                       A, n, npad, pairs, max_sweeps, tol, w, V, sweeps);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }
