@@ -351,6 +351,10 @@ def main():
     conv = bool(st["conv"].all().item())
     stats = dict(eng.stats)
     sweeps = eng.sweeps.item() / eng.eigh_calls if eng.eigh_calls else None
+    # DIIS iterations of the last call and how many geometries each one carried
+    act = list(eng.active)
+    tail = {"iterations": len(act), "geometry_iterations": int(sum(act)),
+            "iterations_below_16_geometries": sum(1 for a in act if a < 16)}
     # dominant kernels: potential matrix (FP64 or FP32) and FP64 orbital density
     kind = vmat_kind(pes)
     ms_v, fl_v = time_vmat(pes, B)
@@ -411,7 +415,8 @@ def main():
         "config": {"workload": f"54 H rs=1.25 gth-dzv LDA(vwn) Fermi T=1e4K, grid 49^3 "
                                f"(grid_length 0.16), tol 1e-8, energy+forces, batch {B}",
                    "path": a.path, "batch": B, "n_iter_mean": nit, "all_converged": conv,
-                   "contractions": stats, "jacobi_sweeps_per_eigh": sweeps},
+                   "contractions": stats, "jacobi_sweeps_per_eigh": sweeps,
+                   "diis_loop": tail},
         # compute-bound roof: MFMA for the potential matrix, FP64 vector FMAs (same
         # 78.6 TFLOP/s peak) for the orbital-form density
         "roofline": {"bound": dom["bound"], "kernel": dom["kernel"], "achieved": dom["achieved"],

@@ -31,9 +31,11 @@ typedef unsigned u32x4q __attribute__((ext_vector_type(4)));
 #define Q8_SC (Q8_T + 1)
 #define Q8_MAXKS 2        // nao_pad <= 128 (two K steps of 64 AOs)
 #define Q8_NW 4           // waves per workgroup
-#ifndef Q8_TARGET_BLOCKS
+// point tiles per wave at least: the coefficient digits are cut once per
+// workgroup, so a few geometries spread over too many workgroups pay that
+// prologue over and over (4 geometries: 0.117 -> 0.096 ms)
+#define Q8_MIN_TILES 8
 #define Q8_TARGET_BLOCKS 2048
-#endif
 
 __device__ __forceinline__ int q8_exponent(double m) {
   // smallest e with m < 2^e (0 for m = 0)
@@ -273,7 +275,7 @@ extern "C" int hqc_orb_density_q8(const signed char* q, const short* pe, const i
   const size_t lds = q8_lds_bytes(KS);
   const int npt = ld >> 4;
   long nwg = Q8_TARGET_BLOCKS / B;
-  const long maxwg = (npt + Q8_NW - 1) / Q8_NW;
+  const long maxwg = (npt + Q8_NW * Q8_MIN_TILES - 1) / (Q8_NW * Q8_MIN_TILES);
   if (nwg > maxwg) nwg = maxwg;
   if (nwg < 1) nwg = 1;
   const dim3 grid((unsigned)nwg, B);

@@ -221,6 +221,7 @@ class Engine:
         self.stats = {"rho64": 0, "rhoq8": 0, "rho32": 0, "v64": 0, "v32": 0}
         self.sweeps = torch.zeros((), dtype=torch.long, device=cell.device)
         self.eigh_calls = 0
+        self.active = []
         self._work = {}
 
     # ------------------------------------------------------------------ utils
@@ -637,11 +638,13 @@ class Engine:
 
     @staticmethod
     def _subset(mask):
-        """(index tensor int32 or None when the mask covers the batch, count)."""
-        n = int(mask.sum().item())
+        """(index tensor int32 or None when the mask covers the batch, count); one
+        host round trip (the size of nonzero's result)."""
+        idx = torch.nonzero(mask).flatten()
+        n = idx.numel()
         if n == mask.numel():
             return None, n
-        return torch.nonzero(mask).flatten().to(torch.int32), n
+        return idx.to(torch.int32), n
 
     def _mixed_density(self, st, na, D, mo, occ):
         """rho for geometries [0, na) in the mixed mode, chosen per geometry: the
@@ -672,7 +675,7 @@ class Engine:
                                  occ if idx is None else occ[idx.long()], self._gix(st, na, idx))
             self._scatter(rho, idx, r)
             self.stats["rho32"] += 1
-        idx, n = self._subset(inc)
+        idx, n = self._subset(inc) if "rhoa" in st else (None, 0)
         if n:
             il = None if idx is None else idx.long()
             r = torch.zeros(n, c.ld, dtype=torch.float64, device=c.device)
@@ -823,17 +826,20 @@ class Engine:
 
     @staticmethod
     def _compact(st, keys, loc, order, keep, na):
-        """Move the geometries flagged in keep (bool over [0, na)) to a prefix by
-        swapping them with the inactive ones inside it; returns the new prefix."""
-        new_na = int(keep.sum().item())
+        """Move the geometries flagged in keep (bool over [0, na), a device tensor or
+        a host list) to a prefix by swapping them with the inactive ones inside it;
+        returns the new prefix."""
+        if torch.is_tensor(keep):
+            keep = keep.tolist()
+        new_na = sum(keep)
         if new_na == na:
             return na
-        slots = torch.arange(na, device=keep.device)
-        holes = slots[:new_na][~keep[:new_na]]
-        movers = slots[new_na:][keep[new_na:]]
-        if holes.numel():
-            ia = torch.cat([holes, movers])
-            ib = torch.cat([movers, holes])
+        holes = [i for i in range(new_na) if not keep[i]]
+        movers = [i for i in range(new_na, na) if keep[i]]
+        if holes:
+            dev = order.device
+            ia = torch.tensor(holes + movers, dtype=torch.long).pin_memory().to(dev, non_blocking=True)
+            ib = torch.tensor(movers + holes, dtype=torch.long).pin_memory().to(dev, non_blocking=True)
             for k in keys:
                 st[k][ia] = st[k][ib]
             for k in loc:
@@ -940,13 +946,17 @@ class Engine:
         res = [None]
 
         def cont(na):
+            """Host list of the geometries to keep; one round trip for the residuals
+            and energy changes together."""
             err = torch.linalg.norm(loc["ek"][:na, last[0]], dim=(-2, -1))
             dE = (loc["E"][:na] - loc["E_old"][:na]).abs()
-            keep = ((dE > tol) | (err > 10 * tol)) & (loop < self.max_cycle)
-            res[0] = float(err.min().item()) if na else None
-            return keep
+            e_h, d_h = torch.stack([err, dE]).tolist() if na else ([], [])
+            res[0] = min(e_h) if na else None
+            more = loop < self.max_cycle
+            return [more and (d > tol or e > 10 * tol) for e, d in zip(e_h, d_h)]
 
         na = self._compact(st, keys, loc, order, cont(na), na)
+        self.active = []          # geometries in each DIIS iteration (host ints, for reports)
         # complex (k-point) error vectors give a complex symmetric B, as 'imn,jmn->ij'
         # does in the reference (no conjugate)
         eye = torch.eye(m, dtype=loc["Bh"].dtype, device=dev)
@@ -959,7 +969,7 @@ class Engine:
             h[:, m, :m] = -1.0
             rhs = torch.zeros(na, m + 1, 1, dtype=Bm.dtype, device=dev)
             rhs[:, m, 0] = -1.0
-            ck = torch.linalg.solve(h, rhs)[:, :m, 0]
+            ck = torch.linalg.solve_ex(h, rhs)[0][:, :m, 0]      # no host check of info
             Fk = loc["Fk"][:na]
             Fg = (ck[:, :, None, None] * Fk).sum(1)
             if self.diis_damp:
@@ -973,6 +983,7 @@ class Engine:
             loc["Mo"][:na], loc["W1"][:na] = mo_a, w1_a
             loop += 1
             loc["nit"][:na] = loop
+            self.active.append(na)
             na = self._compact(st, keys, loc, order, cont(na), na)
         return loc["Mo"], loc["W1"], loc["E"], loc["nit"] != self.max_cycle, loc["nit"], order
 
