@@ -285,30 +285,33 @@ class Engine:
 
     OCC_CUT = 1e-15   # orbitals below this occupation contribute < 1e-15 relative to rho
 
-    def _at(self, mo, occ):
-        """At = C sqrt(f) (B, nao_pad, m_pad) over the orbitals with f > OCC_CUT."""
+    def _at(self, mo, occ, keep=None):
+        """At = C sqrt(f) (B, nao_pad, m_pad) over the orbitals with f > OCC_CUT (keep:
+        their largest count over the batch, if known)."""
         c = self.c
         B = mo.shape[0]
-        keep = max(int((occ > self.OCC_CUT).sum(-1).max().item()), 1)
+        if keep is None:
+            keep = int((occ > self.OCC_CUT).sum(-1).max().item())
+        keep = max(keep, 1)
         m_pad = _round_up(keep, 16)
         At = torch.zeros(B, c.nao_pad, m_pad, dtype=torch.float64, device=c.device)
         At[:, :c.nao, :keep] = mo[:, :, :keep] * occ[:, None, :keep].clamp(min=0).sqrt()
         return At
 
-    def orb_density(self, phi, mo, occ, bidx=None):
+    def orb_density(self, phi, mo, occ, bidx=None, keep=None):
         """rho = sum_i f_i psi_i^2 over the orbitals with f_i > OCC_CUT (equal to the
-        D form with D = C f C^T up to that cut); bidx as in vmat."""
+        D form with D = C f C^T up to that cut); bidx as in vmat, keep as in _at."""
         c = self.c
-        At = self._at(mo, occ)
+        At = self._at(mo, occ, keep)
         rho = torch.zeros(mo.shape[0], c.ld, dtype=torch.float64, device=c.device)
         K.orb_density(phi, At, rho, bidx,
                       self._phi_scale if phi.dtype == torch.int32 else None)
         return rho
 
-    def orb_density_q8(self, st, mo, occ, bidx=None):
+    def orb_density_q8(self, st, mo, occ, bidx=None, keep=None):
         """orb_density from the int8 digits of phi (st['phi_q8'], st['phi_e8'])."""
         c = self.c
-        At = self._at(mo, occ)
+        At = self._at(mo, occ, keep)
         rho = torch.empty(mo.shape[0], c.ld, dtype=torch.float64, device=c.device)
         K.orb_density_q8(st["phi_q8"], st["phi_e8"], At, rho, self.Q8_SLICES, bidx)
         return rho
@@ -454,12 +457,18 @@ class Engine:
         T = torch.empty_like(S)
         dS = torch.empty(B, c.nao, c.nao, 3, dtype=f64, device=dev) if forces else None
         dT = torch.empty_like(dS) if forces else None
-        phi = torch.zeros(B, c.nao_pad, c.ld, dtype=f64, device=dev)
+        def ao_buffer(nb, dtype):
+            # AO values are written for the real AOs at the real grid points only
+            buf = torch.empty(nb, c.nao_pad, c.ld, dtype=dtype, device=dev)
+            buf[:, c.nao:, :] = 0
+            buf[:, :c.nao, c.N:] = 0
+            return buf
+        phi = ao_buffer(B, f64)
         mixed = self.precision == "mixed"
         use_h2 = mixed and self.MIXED_SPLIT16 and c.nao_pad <= 128
         # FP32 AO values: the FP32 kernels' operand, unless FP16 pairs serve them all
         need32 = mixed and (not use_h2 or self.MIXED_RHO_INCREMENTS)
-        phi32 = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float32, device=dev) if need32 else None
+        phi32 = ao_buffer(B, torch.float32) if need32 else None
         vl = torch.zeros(B, c.ld, dtype=f64, device=dev)
         for cst, idx in groups:
             x = self._sub(st["xpb"], idx)
@@ -479,12 +488,11 @@ class Engine:
                 self._put(dT, idx, dTg)
             P = torch.empty(nb, c.n, 3, c.g, c.na, c.nl, dtype=f64, device=dev)
             K.gto_tables(x, c.g, cst.L, cst.tmax_img, c.alpha, c.nl, P)
-            pg = phi if idx is None else torch.zeros(nb, c.nao_pad, c.ld, dtype=f64, device=dev)
+            pg = phi if idx is None else ao_buffer(nb, f64)
             K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, pg)
             self._put(phi, idx, pg)
             if need32:
-                pg32 = phi32 if idx is None else torch.zeros(nb, c.nao_pad, c.ld,
-                                                              dtype=torch.float32, device=dev)
+                pg32 = phi32 if idx is None else ao_buffer(nb, torch.float32)
                 K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, pg32)
                 self._put(phi32, idx, pg32)
             cb = torch.empty(nb, c.g, c.g, c.g, dtype=torch.complex128, device=dev)
@@ -637,6 +645,24 @@ class Engine:
         return g if idx is None else g[idx.long()]
 
     @staticmethod
+    def _split(mask, extra=()):
+        """((index, count) of the True entries, (index, count) of the False entries,
+        host values of the integer tensors in extra) from one host round trip; an
+        index is int32 on the device, or None when it covers the whole mask."""
+        n = mask.numel()
+        h = torch.cat([mask.reshape(-1).to(torch.int64)] +
+                      [t.reshape(-1).to(torch.int64) for t in extra]).tolist()
+        yes = [i for i in range(n) if h[i]]
+        no = [i for i in range(n) if not h[i]]
+
+        def up(lst):
+            if len(lst) == n:
+                return None
+            return torch.tensor(lst, dtype=torch.int32).pin_memory().to(mask.device,
+                                                                         non_blocking=True)
+        return (up(yes), len(yes)), (up(no), len(no)), h[n:]
+
+    @staticmethod
     def _subset(mask):
         """(index tensor int32 or None when the mask covers the batch, count); one
         host round trip (the size of nonzero's result)."""
@@ -663,19 +689,26 @@ class Engine:
             rel = self.MIXED_EPS * torch.linalg.norm(dD, dim=(-2, -1)) / sel("kappa")
             inc = ~direct & (rel <= 1e-2) & (rel * sel("vmax") <= self.MIXED_RHO_SAFETY * err)
         else:
-            inc = torch.zeros_like(direct)
-        full = ~(direct | inc)
+            inc = None
         rho = torch.empty(na, c.ld, dtype=torch.float64, device=c.device)
-        idx, n = self._subset(direct)
+        if inc is None:
+            # both classes and the occupied-orbital count in one round trip
+            kc = (occ > self.OCC_CUT).sum(-1).amax()
+            (idx, n), full_sub, (keep,) = self._split(direct, (kc,))
+        else:
+            full = ~(direct | inc)
+            idx, n = self._subset(direct)
+            keep = None
         if n:
             # orbital form sum_i f_i psi_i^2: non-negative in any precision, unlike the
             # D form, whose FP32 rounding can turn a tiny density negative
             lp = "phi_h2" if "phi_h2" in st else "phi_lp"
             r = self.orb_density(st[lp], mo if idx is None else mo[idx.long()],
-                                 occ if idx is None else occ[idx.long()], self._gix(st, na, idx))
+                                 occ if idx is None else occ[idx.long()], self._gix(st, na, idx),
+                                 keep)
             self._scatter(rho, idx, r)
             self.stats["rho32"] += 1
-        idx, n = self._subset(inc) if "rhoa" in st else (None, 0)
+        idx, n = self._subset(inc) if inc is not None else (None, 0)
         if n:
             il = None if idx is None else idx.long()
             r = torch.zeros(n, c.ld, dtype=torch.float64, device=c.device)
@@ -684,15 +717,15 @@ class Engine:
             r += sel("rhoa") if il is None else st["rhoa"][il]
             self._scatter(rho, idx, r)
             self.stats["rho32"] += 1
-        idx, n = self._subset(full)
+        idx, n = full_sub if inc is None else self._subset(full)
         if n:
             il = torch.arange(na, device=c.device) if idx is None else idx.long()
             gi = self._gix(st, na, idx)
             if "phi_q8" in st:
-                r64 = self.orb_density_q8(st, mo[il], occ[il], gi)
+                r64 = self.orb_density_q8(st, mo[il], occ[il], gi, keep)
                 self.stats["rhoq8"] += 1
             else:
-                r64 = self.orb_density(st["phi"], mo[il], occ[il], gi)
+                r64 = self.orb_density(st["phi"], mo[il], occ[il], gi, keep)
                 self.stats["rho64"] += 1
             self._scatter(rho, idx, r64)
             if "rhoa" in st:                                 # anchor of the increments
@@ -727,7 +760,7 @@ class Engine:
         low = direct | inc
         dVs = self._dv(st, na)
         V2 = torch.empty(na, c.nao, c.nao, dtype=torch.float64, device=c.device)
-        idx, n = self._subset(low)
+        (idx, n), high_sub, _ = self._split(low)
         if n:
             il = None if idx is None else idx.long()
             vv = torch.where(direct[:, None], v, dv)
@@ -739,7 +772,7 @@ class Engine:
                 slice(None) if il is None else il]
             self._scatter(V2, idx, V32)
             self.stats["v32"] += 1
-        idx, n = self._subset(~low)
+        idx, n = high_sub
         if n:
             il = torch.arange(na, device=c.device) if idx is None else idx.long()
             d = dVs if not torch.is_tensor(dVs) else dVs[il]

@@ -90,8 +90,9 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
     F = torch.zeros(B, c.n, 3, dtype=f64, device=dev)
     F.index_add_(1, atom, t)
     # grid adjoint (FP64 phi regardless of the SCF precision)
-    X = torch.zeros(B, c.nao_pad, c.ld, dtype=f64, device=dev)
-    rho = torch.zeros(B, c.ld, dtype=f64, device=dev)
+    # the D-form kernel writes every row and grid column of X and rho
+    X = torch.empty(B, c.nao_pad, c.ld, dtype=f64, device=dev)
+    rho = torch.empty(B, c.ld, dtype=f64, device=dev)
     K.ao_density(st["phi"], engine._pad(D), rho, X, bidx=st["gidx"])
     v, _, rg = engine.xc_potential(rho, save_rg=True, geo=engine._geo(st, B))
     veff = (v + st["vloc"]).contiguous()
@@ -148,8 +149,8 @@ def compute_forces_k(engine, st, kappa=10, Gmax=15):
                - (D * other).conj().unsqueeze(-1) * dT).sum(2).real
     F = torch.zeros(B, c.n, 3, dtype=f64, device=dev)
     F.index_add_(1, atom, t)
-    X = torch.zeros(B, 2 * c.nao_pad, c.ld, dtype=f64, device=dev)
-    rho = torch.zeros(B, c.ld, dtype=f64, device=dev)
+    X = torch.empty(B, 2 * c.nao_pad, c.ld, dtype=f64, device=dev)
+    rho = torch.empty(B, c.ld, dtype=f64, device=dev)
     K.ao_density(st["phi"], engine.stacked_dm(D), rho, X, bidx=st["gidx"])
     v, _, rg = engine.xc_potential(rho, save_rg=True)
     veff = (v + st["vloc"]).contiguous()

@@ -54,11 +54,14 @@ class KEngine(Engine):
         ri, ir = M[:, :n, p:p + n], M[:, p:p + n, :n]
         return torch.complex(rr + ii, ri - ir)
 
-    def orb_density(self, phi, mo, occ, bidx=None):
-        """rho = sum_i f_i |psi_i|^2 over orbitals with f_i > OCC_CUT, psi_i = sum_p C_pi phi_p."""
+    def orb_density(self, phi, mo, occ, bidx=None, keep=None):
+        """rho = sum_i f_i |psi_i|^2 over orbitals with f_i > OCC_CUT, psi_i = sum_p C_pi phi_p
+        (keep: their largest count over the batch, if known)."""
         c = self.c
         B = mo.shape[0]
-        keep = max(int((occ > self.OCC_CUT).sum(-1).max().item()), 1)
+        if keep is None:
+            keep = int((occ > self.OCC_CUT).sum(-1).max().item())
+        keep = max(keep, 1)
         m_pad = _round_up(keep, 16)
         n, p = c.nao, c.nao_pad
         At = torch.zeros(B, 2 * p, 2 * m_pad, dtype=torch.float64, device=c.device)
