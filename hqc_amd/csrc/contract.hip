@@ -913,11 +913,12 @@ __global__ __launch_bounds__(256) void k_vmat_h2(const unsigned* __restrict__ ph
   const int NV = nao_pad * VPR / 256;  // nao_pad / 16
   const int cv = threadIdx.x % VPR;    // the same 4 columns of every chunk for this thread
   const int tbase = grp * 4 * TPW + w * TPW;
-  int rto[TPW], cto[TPW];
+  int rto[TPW], cto[TPW], rtile[TPW];
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
     int rt, ct;
     tile_rc(min(tbase + t, ntiles - 1), NT, rt, ct);
+    rtile[t] = rt;                    // wave-uniform
     rto[t] = (rt * 16 + l16) * STR + 8 * kq;
     cto[t] = (ct * 16 + l16) * STR + 8 * kq;
   }
@@ -996,10 +997,15 @@ __global__ __launch_bounds__(256) void k_vmat_h2(const unsigned* __restrict__ ph
     const _Float16* Blo = Bhi + plane;
 #pragma unroll
     for (int ks = 0; ks < KC / 32; ++ks) {
+      // a wave's tiles run along rows of the upper triangle, so consecutive tiles
+      // mostly share their row tile: its phi fragments are read from LDS once
+      h16x8 ah, al;
 #pragma unroll
       for (int t = 0; t < TPW; ++t) {
-        const h16x8 ah = *reinterpret_cast<const h16x8*>(Ahi + rto[t] + 32 * ks);
-        const h16x8 al = *reinterpret_cast<const h16x8*>(Alo + rto[t] + 32 * ks);
+        if (t == 0 || rtile[t] != rtile[t - 1]) {
+          ah = *reinterpret_cast<const h16x8*>(Ahi + rto[t] + 32 * ks);
+          al = *reinterpret_cast<const h16x8*>(Alo + rto[t] + 32 * ks);
+        }
         const h16x8 bh = *reinterpret_cast<const h16x8*>(Bhi + cto[t] + 32 * ks);
         const h16x8 bl = *reinterpret_cast<const h16x8*>(Blo + cto[t] + 32 * ks);
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[t], 0, 0, 0);
