@@ -59,6 +59,7 @@ _SIGS = {
     "hqc_eigh_jacobi": [_P, _I, _I, _P, _I, _D, _P, _P, _P, _P],
     "hqc_overlap_kinetic": [_P, _I, _I, _P, _I, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P],
     "hqc_ewald_pairs": [_P, _I, _I, _P, _P, _I, _D, _D, _P, _P],
+    "hqc_ewald_recip": [_P, _I, _I, _P, _P, _I, _P, _P, _P, _P],
     "hqc_force_blocks": [_I],
     "hqc_gto_tables_k": [_P, _P, _I, _I, _I, _D, _I, _P, _I, _I, _P, _P],
     "hqc_ao_values_k": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
@@ -481,6 +482,20 @@ def ewald_pairs(rij, Gv, gk, kappa, g0, out):
     _call("hqc_ewald_pairs", _ptr(rij, F64, "rij"), B, npair, _ptr(Gv, F64, "Gv"),
           _ptr(gk, F64, "gk"), Gv.shape[0], float(kappa), float(g0), _ptr(out, F64, "out"),
           _stream())
+
+
+def ewald_recip(frac, Gv, gk):
+    """(E (B), grad (B, n, 3)): the reciprocal pair sums of ewald_pairs summed over
+    pairs, and their gradient in fractional coordinates, from structure factors."""
+    B, n, _ = frac.shape
+    nG = Gv.shape[0]
+    E = torch.empty(B, dtype=F64, device=frac.device)
+    grad = torch.empty(B, n, 3, dtype=F64, device=frac.device)
+    work = torch.empty(B, nG, 2, dtype=F64, device=frac.device)
+    _call("hqc_ewald_recip", _ptr(frac, F64, "frac"), B, n, _ptr(Gv, F64, "Gv"),
+          _ptr(gk, F64, "gk"), nG, _ptr(E, F64, "E"), _ptr(grad, F64, "grad"),
+          _ptr(work, F64, "work"), _stream())
+    return E, grad
 
 
 def force_blocks(g):

@@ -438,4 +438,80 @@ extern "C" int hqc_ewald_pairs(const double* rij, int B, int npair, const double
   return HQC_OK;
 }
 
+// Reciprocal part of the same Ewald sum through structure factors: with
+// theta_j = 2 pi G.r_j and S(G) = sum_j e^{i theta_j},
+//   sum_{i<j} sum_G w_G cos(2 pi G.r_ij)  =  1/2 sum_G w_G (|S(G)|^2 - n),
+//   d/dr_i of it                          = -2 pi sum_G w_G G Im(e^{i theta_i} S(G)^*),
+// n N_G phase evaluations per geometry instead of n(n-1)/2 N_G.
+__global__ __launch_bounds__(256) void k_ewald_sf(const double* __restrict__ frac, int n,
+                                                  const double* __restrict__ Gv, int nG,
+                                                  double* __restrict__ S) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (k >= nG) return;
+  const double G0 = Gv[3 * k], G1 = Gv[3 * k + 1], G2 = Gv[3 * k + 2];
+  const double* r = frac + (long)b * n * 3;
+  double sc = 0.0, ss = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double sn, cs;
+    sincos(2.0 * M_PI * (G0 * r[3 * j] + G1 * r[3 * j + 1] + G2 * r[3 * j + 2]), &sn, &cs);
+    sc += cs;
+    ss += sn;
+  }
+  S[((long)b * nG + k) * 2] = sc;
+  S[((long)b * nG + k) * 2 + 1] = ss;
+}
+
+// one workgroup per (atom i, geometry): the gradient of atom i, and for i = 0
+// the energy sum; fixed-order block reductions
+__global__ __launch_bounds__(256) void k_ewald_recip(const double* __restrict__ frac, int n,
+                                                     const double* __restrict__ Gv,
+                                                     const double* __restrict__ gk, int nG,
+                                                     const double* __restrict__ S,
+                                                     double* __restrict__ E,
+                                                     double* __restrict__ grad) {
+  __shared__ double red[4];
+  const int i = blockIdx.x, b = blockIdx.y;
+  const double* r = frac + ((long)b * n + i) * 3;
+  const double rx = r[0], ry = r[1], rz = r[2];
+  const double* Sb = S + (long)b * nG * 2;
+  double gx = 0.0, gy = 0.0, gz = 0.0, e = 0.0;
+  for (int k = threadIdx.x; k < nG; k += blockDim.x) {
+    const double G0 = Gv[3 * k], G1 = Gv[3 * k + 1], G2 = Gv[3 * k + 2];
+    const double sc = Sb[2 * k], ss = Sb[2 * k + 1], w = gk[k];
+    double sn, cs;
+    sincos(2.0 * M_PI * (G0 * rx + G1 * ry + G2 * rz), &sn, &cs);
+    const double t = -2.0 * M_PI * w * (sn * sc - cs * ss);
+    gx += t * G0;
+    gy += t * G1;
+    gz += t * G2;
+    if (i == 0) e += w * (sc * sc + ss * ss - (double)n);
+  }
+  gx = block_sum(gx, red);
+  gy = block_sum(gy, red);
+  gz = block_sum(gz, red);
+  if (i == 0) e = block_sum(e, red);
+  if (threadIdx.x == 0) {
+    double* g = grad + ((long)b * n + i) * 3;
+    g[0] = gx;
+    g[1] = gy;
+    g[2] = gz;
+    if (i == 0) E[b] = 0.5 * e;
+  }
+}
+
+extern "C" int hqc_ewald_recip(const double* frac, int B, int n, const double* Gv, const double* gk,
+                               int nG, double* E, double* grad, double* work, void* stream) {
+  if (B < 0 || n <= 0 || nG <= 0 || !frac || !Gv || !gk || !E || !grad || !work)
+    return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_ewald_sf, dim3(hqc_div_up(nG, 256), B), dim3(256), 0, s, frac, n, Gv, nG,
+                     work);
+  HQC_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ewald_recip, dim3(n, B), dim3(256), 0, s, frac, n, Gv, gk, nG, work, E,
+                     grad);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
 extern "C" int hqc_version(void) { return 100; }

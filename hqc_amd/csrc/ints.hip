@@ -41,16 +41,23 @@ __global__ __launch_bounds__(256) void k_overlap_kinetic(
     double* __restrict__ dT) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= total) return;
+  // threads run over the unordered atom pairs A <= Bat only (no lane waits out
+  // the lattice loop for a mirrored entry); (Bat,q | A,o) is written as the mirror
   const int q = (int)(idx % nloc);
   long t1 = idx / nloc;
   const int o = (int)(t1 % nloc);
   t1 /= nloc;
-  const int Bat = (int)(t1 % n);
-  t1 /= n;
-  const int A = (int)(t1 % n);
-  const int b = (int)(t1 / n);
-  // each unordered pair once: the (Bat,q | A,o) entry is the mirror written below
-  if (Bat < A || (Bat == A && q < o)) return;
+  const long npair = (long)n * (n + 1) / 2;
+  const int pr = (int)(t1 % npair);
+  const int b = (int)(t1 / npair);
+  // row A of the upper triangle holds pairs [A n - A(A-1)/2, (A+1) n - A(A+1)/2)
+  int A = (int)((2.0 * n + 1.0 - sqrt((2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * pr)) * 0.5);
+  if (A < 0) A = 0;
+  if (A > n - 1) A = n - 1;
+  while (A > 0 && A * n - A * (A - 1) / 2 > pr) --A;
+  while (A < n - 1 && (A + 1) * n - (A + 1) * A / 2 <= pr) ++A;
+  const int Bat = A + (pr - (A * n - A * (A - 1) / 2));
+  if (Bat == A && q < o) return;
   const double* xA = xp + ((long)b * n + A) * 3;
   const double* xB = xp + ((long)b * n + Bat) * 3;
   const double d0 = xA[0] - xB[0], d1 = xA[1] - xB[1], d2 = xA[2] - xB[2];
@@ -216,7 +223,7 @@ static int launch_ovlp(const double* xp, const double* kpt, int B, int n, const 
   if (CPLX && !kpt) return HQC_E_ARG;
   if (na > INT_MAX_NA || nloc > INT_MAX_NLOC) return HQC_E_SIZE;
   if (B == 0) return HQC_OK;
-  const long total = (long)B * n * n * nloc * nloc;
+  const long total = (long)B * (n * (n + 1) / 2) * nloc * nloc;
   hipLaunchKernelGGL(k_overlap_kinetic<CPLX>, dim3(hqc_div_up(total, 256)), dim3(256), 0,
                      (hipStream_t)stream, xp, kpt, total, n, lattice, nT, alpha, na, coef, comp,
                      nloc, S, T, dS, dT);

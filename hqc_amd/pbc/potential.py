@@ -39,23 +39,33 @@ def madelung(kappa, G):
 
 
 def ewald_pp(xp, L, rs, kappa=10, Gmax=15, forces=False):
-    """xp (B, n, 3) rs units on the GPU -> E (B) Ry [, F (B, n, 3) Ha/Bohr]."""
+    """xp (B, n, 3) rs units on the GPU -> E (B) Ry [, F (B, n, 3) Ha/Bohr].
+
+    The reference's pair sum (potential.py:35-67): per pair erfc(kappa d)/d + sum_G
+    w_G cos(2 pi G.r_ij) - pi/kappa^2 in fractional coordinates.  The real-space
+    terms are taken per pair; the G sums summed over the pairs come from structure
+    factors (hqc_ewald_recip), n N_G phases per geometry instead of n^2/2 N_G."""
     B, n, _ = xp.shape
     dev = xp.device
     x = xp - L * torch.floor(xp / L)
+    frac = (x / L).contiguous()
     i, j = torch.triu_indices(n, n, offset=1, device=dev)
-    rij = (x[:, i] - x[:, j]) / L
-    rij = (rij - torch.round(rij)).contiguous()
+    rij = frac[:, i] - frac[:, j]
+    rij = rij - torch.round(rij)
     G = kpoints(3, Gmax, dev)
     gk = _weights(G, kappa).contiguous()
-    out = torch.empty(B, rij.shape[1], 4, dtype=torch.float64, device=dev)
-    K.ewald_pairs(rij, G, gk, float(kappa), -math.pi / kappa ** 2, out)
-    E = 2 * out[..., 0].sum(-1) / L / rs + n * madelung(kappa, G) / L / rs
+    Erec, grec = K.ewald_recip(frac, G, gk)
+    d = torch.linalg.vector_norm(rij, dim=-1)
+    er = torch.special.erfc(kappa * d)
+    g0 = -math.pi / kappa ** 2
+    psum = (er / d).sum(-1) + Erec + i.numel() * g0
+    E = 2 * psum / L / rs + n * madelung(kappa, G) / L / rs
     if not forces:
         return E
     Lb = L * rs
-    grad = out[..., 1:] / Lb ** 2
-    F = torch.zeros(B, n, 3, dtype=torch.float64, device=dev)
+    dsr = -(er / (d * d) + 2.0 * kappa / math.sqrt(math.pi) * torch.exp(-(kappa * d) ** 2) / d)
+    grad = (dsr / d).unsqueeze(-1) * rij / Lb ** 2                 # d(pair)/d r_ij
+    F = -grec / Lb ** 2
     F.index_add_(1, i, -grad)
     F.index_add_(1, j, grad)
     return E, F

@@ -300,6 +300,13 @@ int hqc_overlap_kinetic_k(const double* xp, const double* kpt, int B, int n,
  * vectors with weights gk (nG); out (B,npair,4) = (psi, dpsi/dr_x,y,z). */
 int hqc_ewald_pairs(const double* rij, int B, int npair, const double* Gv, const double* gk,
                     int nG, double kappa, double g0, double* out, void* stream);
+/* The reciprocal sums of hqc_ewald_pairs summed over the pairs, by structure
+ * factors (n N_G phases per geometry instead of n(n-1)/2 N_G): frac (B,n,3)
+ * fractional coordinates, E (B) = sum_{i<j} sum_G gk cos(2 pi G.r_ij),
+ * grad (B,n,3) = d E / d frac_i; work holds B*nG*2 doubles.  Same reference
+ * sum (hqc/pbc/potential.py:35-67) as the pair kernel, in another order. */
+int hqc_ewald_recip(const double* frac, int B, int n, const double* Gv, const double* gk, int nG,
+                    double* E, double* grad, double* work, void* stream);
 
 /* Forces: grid adjoint of the AO contractions (new; the reference has none).
  *   F[b,A,c] = dV * sum_r sum_{p in A} Z_p(r) d phi_p(r)/d r_c,

@@ -475,6 +475,30 @@ def test_ewald(K):
         assert np.allclose(F[b].cpu().numpy(), Fr, atol=1e-11)
 
 
+def test_ewald_recip_matches_pair_kernel(K):
+    """Structure-factor reciprocal sums equal the pair kernel's G sums summed over
+    the pairs, and their gradient the pair gradients gathered per atom."""
+    from hqc_amd.pbc.potential import _weights, kpoints
+    n = 54
+    rng = np.random.default_rng(8)
+    frac = torch.tensor(rng.uniform(0, 1, (3, n, 3)), device=DEV)
+    G = kpoints(3, 15, DEV)
+    gk = _weights(G, 10.0).contiguous()
+    E, grad = K.ewald_recip(frac, G, gk)
+    i, j = torch.triu_indices(n, n, offset=1, device=DEV)
+    rij = frac[:, i] - frac[:, j]
+    rij = (rij - torch.round(rij)).contiguous()
+    out = torch.empty(3, rij.shape[1], 4, dtype=torch.float64, device=DEV)
+    # kappa large: the real-space terms vanish, g0 = 0 leaves the G sums alone
+    K.ewald_pairs(rij, G, gk, 1e3, 0.0, out)
+    Ep = out[..., 0].sum(-1)
+    gp = torch.zeros(3, n, 3, dtype=torch.float64, device=DEV)
+    gp.index_add_(1, i, out[..., 1:])
+    gp.index_add_(1, j, -out[..., 1:])
+    assert torch.allclose(E, Ep, rtol=1e-12, atol=1e-10)
+    assert torch.allclose(grad, gp, rtol=1e-10, atol=1e-9)
+
+
 @pytest.mark.parametrize("n", [2, 7, 8, 33, 107, 108])
 def test_eigh_jacobi(K, n):
     """Jacobi eigensolver vs torch.linalg.eigh: eigenvalues, residual, orthogonality;
