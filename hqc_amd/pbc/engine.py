@@ -152,16 +152,18 @@ class Engine:
     MIXED_RHO_SAFETY = 1.0  # density increment: |v| x relative rho error <= residual
     MIXED_DIRECT_ERR = 1e-1  # residual above which V is taken in low precision outright
     # the same for the density (orbital form): the split-FP16 density leaves a DIIS
-    # residual floor of ~3e-6..1e-5 at 54 H dzv, so the switch to FP64 stays an order
-    # of magnitude above it (3e-6 stalled geometries at max_cycle)
+    # residual floor of ~3e-6..1e-5 at 54 H dzv, so the switch to the accurate form
+    # stays an order of magnitude above it (3e-6 stalled geometries at max_cycle;
+    # 1e-3 runs within 1% of 1e-4 but put the bands of the geometries left at
+    # max_cycle 5e-7 Ha off FP64 against 1e-7)
     MIXED_RHO_DIRECT_ERR = 1e-4
     MIXED_RHO_INCREMENTS = False  # FP32 D-form density increments (else FP64 orbital form)
     # low-precision V from FP16 (hi, lo) split products on MFMA (n_ao <= 128; FP32
-    # MFMA beyond).  Its measured error is that of FP32 MFMA; the increment constant
-    # is 4x the FP32 one (4e-6 and 2e-6 cut FP64 re-anchors but add SCF cycles: the
-    # same throughput at 54 H dzv)
+    # MFMA beyond).  Its measured error is ~1.3e-6 of max|V|; the increment constant
+    # is 3x that (54 H dzv, batch 256: 8e-6 anchors twice as often for 3% less
+    # throughput, 1e-6, below the real error, stalls geometries and loses 18%)
     MIXED_SPLIT16 = True
-    MIXED_EPS_SPLIT16 = 8e-6
+    MIXED_EPS_SPLIT16 = 4e-6
     # past the warm-up the density comes from int8 digits of phi with exact integer
     # products (n_ao <= 128; FP64 vector FMAs beyond).  4 digits: phi is fixed to
     # 2^-28 of its per-point maximum (a basis perturbation that is the same on every
