@@ -111,7 +111,8 @@ __global__ __launch_bounds__(256) void k_phi_slice_i8(const double* __restrict__
 template <int SP, int KS>
 __global__ __launch_bounds__(64 * Q8_NW) void k_orb_density_q8(
     const signed char* __restrict__ q, const short* __restrict__ pe, const int* __restrict__ bidx,
-    const double* __restrict__ At, int nao_pad, int m_pad, int ld, double* __restrict__ rho) {
+    const double* __restrict__ At, int nao_pad, int m_pad, int ld, int spt,
+    double* __restrict__ rho) {
   constexpr int STRA = 64 * KS + 16;
   constexpr int PLANE = 64 * STRA;              // one digit of 64 orbitals
   extern __shared__ unsigned char smem_q8[];
@@ -125,7 +126,8 @@ __global__ __launch_bounds__(64 * Q8_NW) void k_orb_density_q8(
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int j = lane & 15, kq = lane >> 4;
   const double* Ab = At + (long)b * nao_pad * m_pad;
-  const signed char* Qb = q + gb * (long)ld * SP * nao_pad + j * 16;
+  // spt digits are stored per value; the first SP of them are used
+  const signed char* Qb = q + gb * (long)ld * spt * nao_pad + j * 16;
   const short* Pe = pe + gb * ld;
   double* Rb = rho + (long)b * ld;
   const int wstride = gridDim.x * Q8_NW;
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(64 * Q8_NW) void k_orb_density_q8(
   // zero, so the (exact integer) products vanish whatever these bytes hold, and
   // the loaded values are not touched before the MFMAs that use them
   auto loadB = [&](int pt, u32x4q (&Bf)[SP][KS], int& er) {
-    const signed char* base = Qb + (long)pt * SP * NB * 256;
+    const signed char* base = Qb + (long)pt * spt * NB * 256;
 #pragma unroll
     for (int s = 0; s < SP; ++s)
 #pragma unroll
@@ -318,9 +320,9 @@ extern "C" int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, i
 
 extern "C" int hqc_orb_density_q8(const signed char* q, const short* pe, const int* bidx,
                                   const double* At, int B, int nao_pad, int m_pad, int ld,
-                                  int nslice, double* rho, void* stream) {
+                                  int nslice, int nuse, double* rho, void* stream) {
   if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || m_pad <= 0 || (m_pad & 15) || ld <= 0 ||
-      (ld & 63) || nslice < 3 || nslice > 5)
+      (ld & 63) || nslice < 3 || nslice > 5 || nuse < 3 || nuse > nslice)
     return HQC_E_ARG;
   if (nao_pad > 64 * Q8_MAXKS) return HQC_E_SIZE;
   if (B == 0) return HQC_OK;
@@ -335,13 +337,13 @@ extern "C" int hqc_orb_density_q8(const signed char* q, const short* pe, const i
   const dim3 grid((unsigned)nwg, B);
   hipStream_t s = (hipStream_t)stream;
 #define Q8_CASE(SP_, KS_)                                                                      \
-  if (nslice == SP_ && KS == KS_) {                                                            \
+  if (nuse == SP_ && KS == KS_) {                                                              \
     if (lds > 64 * 1024 &&                                                                     \
         hipFuncSetAttribute((const void*)k_orb_density_q8<SP_, KS_>,                           \
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) \
       (void)hipGetLastError();                                                                 \
     hipLaunchKernelGGL((k_orb_density_q8<SP_, KS_>), grid, dim3(64 * Q8_NW), lds, s, q, pe,    \
-                       bidx, At, nao_pad, m_pad, ld, rho);                                     \
+                       bidx, At, nao_pad, m_pad, ld, nslice, rho);                             \
   }
   Q8_CASE(3, 1) Q8_CASE(3, 2) Q8_CASE(4, 1) Q8_CASE(4, 2) Q8_CASE(5, 1) Q8_CASE(5, 2)
 #undef Q8_CASE

@@ -170,6 +170,9 @@ class Engine:
     # call); the coefficient digits add ~2^-42 per AO, so no residual floor appears
     MIXED_Q8 = True
     Q8_SLICES = 4
+    # the warm-up density from the first 3 digits (2^-21 per value, a fixed basis
+    # with no residual floor; 0: the split-FP16 form)
+    Q8_WARMUP_DIGITS = 3
 
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
@@ -310,12 +313,13 @@ class Engine:
                       self._phi_scale if phi.dtype == torch.int32 else None)
         return rho
 
-    def orb_density_q8(self, st, mo, occ, bidx=None, keep=None):
-        """orb_density from the int8 digits of phi (st['phi_q8'], st['phi_e8'])."""
+    def orb_density_q8(self, st, mo, occ, bidx=None, keep=None, nuse=None):
+        """orb_density from the int8 digits of phi (st['phi_q8'], st['phi_e8']); nuse
+        of the stored digits (all by default)."""
         c = self.c
         At = self._at(mo, occ, keep)
         rho = torch.empty(mo.shape[0], c.ld, dtype=torch.float64, device=c.device)
-        K.orb_density_q8(st["phi_q8"], st["phi_e8"], At, rho, self.Q8_SLICES, bidx)
+        K.orb_density_q8(st["phi_q8"], st["phi_e8"], At, rho, self.Q8_SLICES, bidx, nuse)
         return rho
 
     def poisson(self, rho, save=False, geo=None, paired=False):
@@ -704,10 +708,14 @@ class Engine:
         if n:
             # orbital form sum_i f_i psi_i^2: non-negative in any precision, unlike the
             # D form, whose FP32 rounding can turn a tiny density negative
-            lp = "phi_h2" if "phi_h2" in st else "phi_lp"
-            r = self.orb_density(st[lp], mo if idx is None else mo[idx.long()],
-                                 occ if idx is None else occ[idx.long()], self._gix(st, na, idx),
-                                 keep)
+            mo_d = mo if idx is None else mo[idx.long()]
+            occ_d = occ if idx is None else occ[idx.long()]
+            if "phi_q8" in st and self.Q8_WARMUP_DIGITS:
+                r = self.orb_density_q8(st, mo_d, occ_d, self._gix(st, na, idx), keep,
+                                        self.Q8_WARMUP_DIGITS)
+            else:
+                lp = "phi_h2" if "phi_h2" in st else "phi_lp"
+                r = self.orb_density(st[lp], mo_d, occ_d, self._gix(st, na, idx), keep)
             self._scatter(rho, idx, r)
             self.stats["rho32"] += 1
         idx, n = self._subset(inc) if inc is not None else (None, 0)

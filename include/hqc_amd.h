@@ -134,12 +134,14 @@ int hqc_orb_density_h2(const unsigned* ph, const int* bidx, const double* At, in
  * in FP64 and sums psi^2 in FP64.  Error: the fixed phi rounding,
  * 2^-(7 nslice) of max_k |phi_k(r)| per value (a perturbation of the basis
  * that does not change between calls), plus ~2^-42 of max|At_i| max|phi(r)|
- * per AO from the coefficient digits.  nao_pad <= 128 (HQC_E_SIZE beyond). */
+ * per AO from the coefficient digits.  nao_pad <= 128 (HQC_E_SIZE beyond).
+ * nuse (3..nslice) of the nslice stored digits enter the density (fewer: a
+ * coarser fixed basis, 2^-(7 nuse) per value, and fewer bytes read). */
 int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, int nslice,
                      signed char* q, short* pe, void* stream);
 int hqc_orb_density_q8(const signed char* q, const short* pe, const int* bidx,
                        const double* At, int B, int nao_pad, int m_pad, int ld, int nslice,
-                       double* rho, void* stream);
+                       int nuse, double* rho, void* stream);
 
 /* Grid potential matrix V[b] = dV * phi diag(v[b]) phi^T (symmetric, full).
  * Replaces the 'pr,qr,r->pq' contraction (hqc/pbc/solver.py:693) and, through

@@ -234,6 +234,12 @@ def test_orb_density_q8(K, nao_pad, m, ns):
     K.orb_density_q8(q, pe, At, rho2, ns, bidx)
     emul2 = q8_density_digits(ph[::-1].copy(), A, ns)
     assert np.allclose(rho2.cpu().numpy(), emul2, rtol=1e-13, atol=1e-300)
+    # the first 3 of the stored digits are the 3-digit expansion
+    if ns > 3:
+        rho3 = torch.empty_like(rho)
+        K.orb_density_q8(q, pe, At, rho3, ns, nuse=3)
+        assert np.allclose(rho3.cpu().numpy(), q8_density_digits(ph, A, 3), rtol=1e-13,
+                           atol=1e-300)
 
 
 @pytest.mark.parametrize("g", [1, 2, 5, 7, 13, 17, 24, 25, 27, 31, 33, 48, 49, 51, 55, 57, 61,
