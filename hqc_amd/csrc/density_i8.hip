@@ -185,44 +185,6 @@ __global__ __launch_bounds__(64 * Q8_NW) void k_orb_density_q8(
     }
     __syncthreads();
     const int MT = MW >> 4;
-    auto tile = [&](const u32x4q (&Bf)[SP][KS], int er, int pt) {
-      double racc = 0.0;
-      for (int t = 0; t < MT; ++t) {
-        i32x4 acc[Q8_T + 1];
-#pragma unroll
-        for (int d = 0; d <= Q8_T; ++d) acc[d] = i32x4{0, 0, 0, 0};
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-          i32x4 a[Q8_SC];
-#pragma unroll
-          for (int k = 0; k < Q8_SC; ++k)
-            a[k] = *reinterpret_cast<const i32x4*>(As + k * PLANE + (16 * t + j) * STRA + 64 * ks +
-                                                   16 * kq);
-#pragma unroll
-          for (int l = 0; l < SP; ++l) {
-            const i32x4 bl = __builtin_bit_cast(i32x4, Bf[l][ks]);
-#pragma unroll
-            for (int k = 0; k + l <= Q8_T; ++k)
-              acc[k + l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[k], bl, acc[k + l], 0, 0, 0);
-          }
-        }
-        // lane holds orbitals 16 t + 4 kq + e (e = 0..3) at point j
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          double Q = (double)acc[Q8_T][e];
-#pragma unroll
-          for (int d = Q8_T - 1; d >= 0; --d) Q = fma(Q, 0.0078125, (double)acc[d][e]);
-          racc = fma(wo[16 * t + 4 * kq + e] * Q, Q, racc);
-        }
-      }
-      racc += __shfl_xor(racc, 16, 64);
-      racc += __shfl_xor(racc, 32, 64);
-      if (kq == 0) {
-        const double r = ldexp(racc, 2 * er - 24);
-        const long idx = (long)pt * 16 + j;
-        Rb[idx] = og == 0 ? r : Rb[idx] + r;
-      }
-    };
     // point tiles in pairs: each coefficient fragment read from LDS serves both
     // (one pair buffer; 3.74 ms against 3.98 for single tiles with the next
     // tile's loads in flight, while two pair buffers cost occupancy)
