@@ -58,6 +58,7 @@ __host__ __device__ constexpr int lds_pad() {
 // NTG row tiles the wave runs the full K loop over AOs: X tile = D(rows, :) *
 // phi(:, cols) with D read through L2 (D is symmetric so its row-major rows
 // give the A operand coalesced); the epilogue forms sum_p phi_p X_p per column.
+// Serves FP32 phi; the FP64 form runs on the vector ALUs (k_density_valu).
 // ---------------------------------------------------------------------------
 template <typename T, int NTG>
 __global__ __launch_bounds__(256) void k_density(const T* __restrict__ phi, const int* __restrict__ bidx,
@@ -124,9 +125,14 @@ static int launch_density(const T* phi, const int* bidx, const double* D, int B,
   return HQC_OK;
 }
 
+static int launch_density_valu(const double* phi, const int* bidx, const double* D, int B,
+                               int nao_pad, int ld, double* rho, double* X, void* stream);
+
 extern "C" int hqc_ao_density_f64(const double* phi, const int* bidx, const double* D, int B,
                                   int nao_pad, int ld, double* rho, double* X, void* stream) {
-  return launch_density<double>(phi, bidx, D, B, nao_pad, ld, rho, X, stream);
+  if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63)) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  return launch_density_valu(phi, bidx, D, B, nao_pad, ld, rho, X, stream);
 }
 
 extern "C" int hqc_ao_density_f32(const float* phi, const int* bidx, const double* D, int B,
@@ -370,6 +376,130 @@ __global__ __launch_bounds__(64 * NW) void k_orb_density_valu(const double* __re
     for (int v = 0; v < NW; ++v) s += red[v][t];
     if (col0 + t < ld) rho[(long)b * ld + col0 + t] = s;
   }
+}
+
+// FP64 D-form density with the AO-space product kept: X = D^T phi (D is
+// symmetric, so X_p = sum_q D_pq phi_q, the force adjoint's operand) and
+// rho = sum_p phi_p X_p, in the layout of k_orb_density_valu with D in place of
+// At: D[k][p] is wave-uniform and enters each FMA through a scalar load, wave w
+// of a pass owns the MG rows [MG (NW pass + w), +MG) of X (a last group that
+// would run past nao_pad is moved back to end at nao_pad and stores only the
+// rows no other group owns; MG <= nao_pad), and the waves of a
+// workgroup read the same phi rows (HBM once, L1 for the rest).  The epilogue
+// stores X (512-byte row segments) and adds phi_p X_p with phi re-read through
+// the cache; the per-wave sums meet in LDS in a fixed order.  Against the
+// 16x16x4 f64 MFMA form (k_density) at 256 geometries x 112 AOs x 49^3 points:
+// 22.6 instead of 32.6 ms with X, whose every K step waited on eight L2 loads.
+template <int MG, int NW>
+__global__ __launch_bounds__(64 * NW) void k_density_valu(const double* __restrict__ phi,
+                                                          const int* __restrict__ bidx,
+                                                          const double* __restrict__ D,
+                                                          int nao_pad, int ld,
+                                                          double* __restrict__ rho,
+                                                          double* __restrict__ X) {
+  constexpr int NC = 64 * VD_COLS;
+  __shared__ double red[NW][NC];
+  const int b = blockIdx.y;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const long col0 = (long)blockIdx.x * NC;
+  const double* P = phi + (long)(bidx ? bidx[b] : b) * nao_pad * ld + col0 + lane;
+  const double* Db = D + (long)b * nao_pad * nao_pad;
+  double* Xb = X ? X + (long)b * nao_pad * ld + col0 + lane : nullptr;
+  // a column block past ld (last workgroup only) re-reads block 0 instead, so
+  // every load stays inside its row; those columns are never stored
+  int off[VD_COLS];
+#pragma unroll
+  for (int j = 0; j < VD_COLS; ++j) off[j] = (col0 + 64 * j < ld) ? 64 * j : 0;
+  double rs[VD_COLS];
+#pragma unroll
+  for (int j = 0; j < VD_COLS; ++j) rs[j] = 0.0;
+  const int klast = nao_pad - 1;
+  for (int mg = 0; mg < nao_pad; mg += NW * MG) {
+    const int o0 = mg + w * MG;
+    if (o0 >= nao_pad) continue;                // wave-uniform
+    const int ob = min(o0, nao_pad - MG);       // rows ob .. ob + MG - 1
+    double acc[MG][VD_COLS];
+#pragma unroll
+    for (int i = 0; i < MG; ++i)
+#pragma unroll
+      for (int j = 0; j < VD_COLS; ++j) acc[i][j] = 0.0;
+    double pb[VD_PF][VD_COLS];
+#pragma unroll
+    for (int q = 0; q < VD_PF; ++q)
+#pragma unroll
+      for (int j = 0; j < VD_COLS; ++j) pb[q][j] = P[(long)q * ld + off[j]];
+    for (int k0 = 0; k0 < nao_pad; k0 += VD_PF) {
+#pragma unroll
+      for (int q = 0; q < VD_PF; ++q) {
+        const int k = k0 + q;
+        const int kp = min(k + VD_PF, klast);
+        const double* Dk = Db + (long)k * nao_pad + ob;
+        double a[MG];
+#pragma unroll
+        for (int i = 0; i < MG; ++i) a[i] = Dk[i];
+        double p[VD_COLS];
+#pragma unroll
+        for (int j = 0; j < VD_COLS; ++j) {
+          p[j] = pb[q][j];
+          pb[q][j] = P[(long)kp * ld + off[j]];
+        }
+#pragma unroll
+        for (int i = 0; i < MG; ++i)
+#pragma unroll
+          for (int j = 0; j < VD_COLS; ++j) acc[i][j] = fma(a[i], p[j], acc[i][j]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < MG; ++i) {
+      if (ob + i >= o0) {                       // wave-uniform
+        const long row = (long)(ob + i) * ld;
+#pragma unroll
+        for (int j = 0; j < VD_COLS; ++j) {
+          rs[j] = fma(acc[i][j], P[row + off[j]], rs[j]);
+          if (Xb && col0 + 64 * j < ld) Xb[row + 64 * j] = acc[i][j];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < VD_COLS; ++j) red[w][lane + 64 * j] = rs[j];
+  __syncthreads();
+  for (int t = threadIdx.x; t < NC; t += 64 * NW) {
+    double s = 0.0;
+#pragma unroll
+    for (int v = 0; v < NW; ++v) s += red[v][t];
+    if (col0 + t < ld) rho[(long)b * ld + col0 + t] = s;
+  }
+}
+
+static int launch_density_valu(const double* phi, const int* bidx, const double* D, int B,
+                               int nao_pad, int ld, double* rho, double* X, void* stream) {
+  // 32-row groups over at most 8 waves, balanced (112 AOs: 4 waves, one pass)
+  const int mgr = nao_pad >= 32 ? 32 : 16;
+  const int ng = (nao_pad + mgr - 1) / mgr;
+  const int passes = (ng + 7) / 8;
+  const int nw = (ng + passes - 1) / passes;
+  const dim3 grid((ld + 64 * VD_COLS - 1) / (64 * VD_COLS), B);
+  hipStream_t s = (hipStream_t)stream;
+#define DV_CASE(N)                                                                         \
+  case N:                                                                                  \
+    hipLaunchKernelGGL((k_density_valu<32, N>), grid, dim3(64 * N), 0, s, phi, bidx, D,    \
+                       nao_pad, ld, rho, X);                                               \
+    break;
+  if (mgr == 16) {                              // nao_pad = 16: one group
+    hipLaunchKernelGGL((k_density_valu<16, 1>), grid, dim3(64), 0, s, phi, bidx, D, nao_pad,
+                       ld, rho, X);
+    HQC_CHECK_LAUNCH();
+    return HQC_OK;
+  }
+  switch (nw) {
+    DV_CASE(1) DV_CASE(2) DV_CASE(3) DV_CASE(4) DV_CASE(5) DV_CASE(6) DV_CASE(7)
+    default: DV_CASE(8)
+  }
+#undef DV_CASE
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
 }
 
 template <typename T>

@@ -98,7 +98,9 @@ int hqc_ao_points(const double* xp, const double* xe, const double* kpt, int B, 
  * Replaces the 'pr,qr,pq->r' contraction of Exc_Vxc_integral
  * (hqc/pbc/solver.py:690) and the density used by hartree_rhoG (:627).
  *   rho[b,r] = sum_pq D[b,p,q] phi[b,p,r] phi[b,q,r]    (D symmetric)
- * FP64 MFMA (v_mfma_f64_16x16x4f64).  rho (B,ld); X (B,nao_pad,ld) or NULL. */
+ * FP64 FMAs on the vector ALUs with D as the scalar operand (X_p = sum_q
+ * D_qp phi_q, then rho = sum_p phi_p X_p).  rho (B,ld); X (B,nao_pad,ld) or
+ * NULL. */
 int hqc_ao_density_f64(const double* phi, const int* bidx, const double* D, int B,
                        int nao_pad, int ld, double* rho, double* X, void* stream);
 /* FP32 operands on v_mfma_f32_16x16x4f32 with FP32 accumulation over the AO
