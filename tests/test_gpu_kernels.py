@@ -256,6 +256,25 @@ def test_fft3d(K, g):
     assert torch.allclose(y / g ** 3, x, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("g", [43, 49, 51])
+def test_fft3d_plane_thread_counts_agree(K, g):
+    """Planes up to 51^2 run with 256 threads once there are >= 512 planes (12
+    grids here), with 1024 threads otherwise (2 grids): the same outputs bit for
+    bit, and both equal to torch's FFT."""
+    print(f"[fft3d plane threads] g={g}", flush=True)
+    torch.manual_seed(100 + g)
+    x = torch.randn(12, g, g, g, dtype=torch.complex128, device=DEV)
+    y = x.clone()
+    K.fft3d(y, -1)                                   # 12 g planes >= 512: 256 threads
+    y2 = x[:2].clone()
+    K.fft3d(y2, -1)                                  # 2 g planes < 512: 1024 threads
+    ref = torch.fft.fftn(x, dim=(1, 2, 3))
+    torch.cuda.synchronize()
+    assert 12 * g >= 512 and 2 * g < 512
+    assert torch.equal(y[:2], y2)
+    assert torch.allclose(y, ref, rtol=1e-12, atol=1e-11 * g ** 1.5)
+
+
 def test_gto_tables_and_values(K):
     s, xpb, L = _sys()
     print(f"[gto] g={s.g}", flush=True)
