@@ -36,7 +36,7 @@ _SIGS = {
     "hqc_orb_density_h2": [_P, _P, _P, _I, _I, _I, _I, _D, _P, _P],
     "hqc_ao_vmat_h2_workspace": [_I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
     "hqc_ao_vmat_h2": [_P, _P, _P, _P, _I, _I, _I, _D, _D, _P, _P, ctypes.c_size_t, _P],
-    "hqc_phi_slice_i8": [_P, _I, _I, _I, _I, _P, _P, _P],
+    "hqc_phi_slice_i8": [_P, _I, _I, _I, _I, _P, _P, _P, _P],
     "hqc_orb_density_q8": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
     "hqc_pair_pack": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
@@ -241,14 +241,20 @@ def phi_split_f16(phi, scale, out):
 Q8_MAX_NAO_PAD = 128
 
 
-def phi_slice_i8(phi, nslice, q, pe):
+def phi_slice_i8(phi, nslice, q, pe, amax=None):
     """q (G, ld*nslice*nao_pad) int8 digits and pe (G, ld) int16 exponents of phi
-    (G, nao_pad, ld) FP64, in the tile layout the int8 density kernel reads."""
+    (G, nao_pad, ld) FP64, in the tile layout the int8 density kernel reads;
+    amax (G,) FP64, if given, is zero-filled here and receives max |phi| per
+    geometry from the same pass."""
     G, nao_pad, ld = phi.shape
     assert q.dtype == torch.int8 and q.shape == (G, ld * nslice * nao_pad)
     assert pe.dtype == torch.int16 and pe.shape == (G, ld)
+    if amax is not None:
+        assert amax.dtype == F64 and amax.shape == (G,) and amax.is_contiguous()
+        amax.zero_()
     _call("hqc_phi_slice_i8", _ptr(phi, F64, "phi"), G, nao_pad, ld, nslice,
-          _ptr(q, torch.int8, "q"), _ptr(pe, torch.int16, "pe"), _stream())
+          _ptr(q, torch.int8, "q"), _ptr(pe, torch.int16, "pe"), _ptr(amax, F64, "amax"),
+          _stream())
 
 
 def orb_density_q8(q, pe, At, rho, nslice, bidx=None, nuse=None):

@@ -45,12 +45,16 @@ __device__ __forceinline__ int q8_exponent(double m) {
 // ---------------------------------------------------------------------------
 // phi [G][nao_pad][ld] (FP64) -> q (layout above) and pe [G][ld]
 // grid (ld/64, G), 256 threads: point p = 64 blockIdx.x + lane, AO blocks
-// mb = w, w + 4 of that point per thread
+// mb = w, w + 4 of that point per thread.  amax [G] (optional, zeroed by the
+// caller) receives max |phi| of each geometry: the workgroup's largest point
+// maximum enters by one 64-bit atomic max on its bit pattern (non-negative
+// doubles order as their bits), which saves the caller a separate pass over phi.
 // ---------------------------------------------------------------------------
 template <int SP>
 __global__ __launch_bounds__(256) void k_phi_slice_i8(const double* __restrict__ phi, int nao_pad,
                                                       int ld, signed char* __restrict__ q,
-                                                      short* __restrict__ pe) {
+                                                      short* __restrict__ pe,
+                                                      unsigned long long* __restrict__ amax) {
   __shared__ double red[4][64];
   const int NB = nao_pad >> 4;
   const int g = blockIdx.y;
@@ -72,7 +76,14 @@ __global__ __launch_bounds__(256) void k_phi_slice_i8(const double* __restrict__
   __syncthreads();
   m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
   const int e = q8_exponent(m);
-  if (w == 0) pe[(long)g * ld + p] = (short)e;
+  if (w == 0) {
+    pe[(long)g * ld + p] = (short)e;
+    if (amax) {
+      double mw = m;
+      for (int o = 32; o > 0; o >>= 1) mw = fmax(mw, __shfl_xor(mw, o, 64));
+      if (lane == 0) atomicMax(amax + g, (unsigned long long)__double_as_longlong(mw));
+    }
+  }
   signed char* Q = q + ((long)g * ld + (p & ~15L)) * SP * nao_pad + (p & 15) * 16;
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
@@ -267,17 +278,18 @@ static size_t q8_lds_bytes(int KS) {
 }
 
 extern "C" int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, int nslice,
-                                signed char* q, short* pe, void* stream) {
+                                signed char* q, short* pe, double* amax, void* stream) {
   if (G < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63) || !phi || !q || !pe)
     return HQC_E_ARG;
   if (nao_pad > 64 * Q8_MAXKS) return HQC_E_SIZE;
   if (G == 0) return HQC_OK;
   const dim3 grid(ld / 64, G);
   hipStream_t s = (hipStream_t)stream;
+  unsigned long long* am = reinterpret_cast<unsigned long long*>(amax);
   switch (nslice) {
-    case 3: hipLaunchKernelGGL(k_phi_slice_i8<3>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe); break;
-    case 4: hipLaunchKernelGGL(k_phi_slice_i8<4>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe); break;
-    case 5: hipLaunchKernelGGL(k_phi_slice_i8<5>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe); break;
+    case 3: hipLaunchKernelGGL(k_phi_slice_i8<3>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe, am); break;
+    case 4: hipLaunchKernelGGL(k_phi_slice_i8<4>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe, am); break;
+    case 5: hipLaunchKernelGGL(k_phi_slice_i8<5>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe, am); break;
     default: return HQC_E_ARG;
   }
   HQC_CHECK_LAUNCH();

@@ -520,20 +520,25 @@ class Engine:
             if need32:
                 st["phi_lp"] = phi32
                 phi_keys.append("phi_lp")
+            amax_g = None
+            if self.MIXED_Q8 and c.nao_pad <= K.Q8_MAX_NAO_PAD:
+                # int8 digits of phi for the accurate density (exact integer products);
+                # the same pass returns max |phi| per geometry for the FP16 scale
+                ns = self.Q8_SLICES
+                st["phi_q8"] = torch.empty(B, c.ld * ns * c.nao_pad, dtype=torch.int8, device=dev)
+                st["phi_e8"] = torch.empty(B, c.ld, dtype=torch.int16, device=dev)
+                amax_g = torch.empty(B, dtype=f64, device=dev)
+                K.phi_slice_i8(phi, ns, st["phi_q8"], st["phi_e8"], amax_g)
+                phi_keys += ["phi_q8", "phi_e8"]
             if use_h2:
                 # FP16 (hi, lo) pairs of s*phi, s = 2^k with |s phi| <= 2^14
-                amax = float(torch.linalg.vector_norm(phi, float("inf")).item())
+                am = amax_g.max() if amax_g is not None else \
+                    torch.linalg.vector_norm(phi, float("inf"))
+                amax = float(am.item())
                 self._phi_scale = 2.0 ** math.floor(math.log2(2.0 ** 14 / max(amax, 1e-300)))
                 st["phi_h2"] = torch.empty(B, c.nao_pad, c.ld, dtype=torch.int32, device=dev)
                 K.phi_split_f16(phi, self._phi_scale, st["phi_h2"])
                 phi_keys.append("phi_h2")
-            if self.MIXED_Q8 and c.nao_pad <= K.Q8_MAX_NAO_PAD:
-                # int8 digits of phi for the accurate density (exact integer products)
-                ns = self.Q8_SLICES
-                st["phi_q8"] = torch.empty(B, c.ld * ns * c.nao_pad, dtype=torch.int8, device=dev)
-                st["phi_e8"] = torch.empty(B, c.ld, dtype=torch.int16, device=dev)
-                K.phi_slice_i8(phi, ns, st["phi_q8"], st["phi_e8"])
-                phi_keys += ["phi_q8", "phi_e8"]
             if self.MIXED_RHO_INCREMENTS:
                 st["phisq"] = (phi[:, :c.nao, :c.N] ** 2).sum(1)         # sum_p phi_p(r)^2
                 st["Da"] = torch.zeros(B, c.nao, c.nao, dtype=f64, device=dev)

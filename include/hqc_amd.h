@@ -138,9 +138,11 @@ int hqc_orb_density_h2(const unsigned* ph, const int* bidx, const double* At, in
  * that does not change between calls), plus ~2^-42 of max|At_i| max|phi(r)|
  * per AO from the coefficient digits.  nao_pad <= 128 (HQC_E_SIZE beyond).
  * nuse (3..nslice) of the nslice stored digits enter the density (fewer: a
- * coarser fixed basis, 2^-(7 nuse) per value, and fewer bytes read). */
+ * coarser fixed basis, 2^-(7 nuse) per value, and fewer bytes read).
+ * amax (G) or NULL: if given (zero-filled by the caller), receives max |phi|
+ * over each geometry's nao_pad x ld values, exactly, from the same pass. */
 int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, int nslice,
-                     signed char* q, short* pe, void* stream);
+                     signed char* q, short* pe, double* amax, void* stream);
 int hqc_orb_density_q8(const signed char* q, const short* pe, const int* bidx,
                        const double* At, int B, int nao_pad, int m_pad, int ld, int nslice,
                        int nuse, double* rho, void* stream);

@@ -213,10 +213,12 @@ def test_orb_density_q8(K, nao_pad, m, ns):
     At[0, :, m // 2] = 0.0                              # an empty orbital column
     q = torch.empty(B, ld * ns * nao_pad, dtype=torch.int8, device=DEV)
     pe = torch.empty(B, ld, dtype=torch.int16, device=DEV)
-    K.phi_slice_i8(phi, ns, q, pe)
+    amax = torch.full((B,), float("nan"), dtype=torch.float64, device=DEV)
+    K.phi_slice_i8(phi, ns, q, pe, amax)
     rho = torch.full((B, ld), float("nan"), dtype=torch.float64, device=DEV)
     K.orb_density_q8(q, pe, At, rho, ns)
     torch.cuda.synchronize()
+    assert torch.equal(amax, phi.abs().amax(dim=(1, 2)))       # max |phi| per geometry
     assert torch.isfinite(rho).all()
     ph, A = phi.cpu().numpy(), At.cpu().numpy()
     emul = q8_density_digits(ph, A, ns)
