@@ -37,6 +37,7 @@ _SIGS = {
     "hqc_ao_vmat_h2_workspace": [_I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
     "hqc_ao_vmat_h2": [_P, _P, _P, _P, _I, _I, _I, _D, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_phi_slice_i8": [_P, _I, _I, _I, _I, _P, _P, _P, _P],
+    "hqc_mixed_v_prep": [_P, _P, _P, _I, _I, _P, _P, _P, _P],
     "hqc_orb_density_q8": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
     "hqc_pair_pack": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
@@ -275,6 +276,18 @@ def vmat_h2_workspace(B, nao_pad, ld):
     n = ctypes.c_size_t(0)
     _call("hqc_ao_vmat_h2_workspace", B, nao_pad, ld, ctypes.byref(n))
     return n.value
+
+
+def mixed_v_prep(v, va, direct, vv, vmax, dvmax):
+    """vmax = max_r |v|, dvmax = max_r |v - va| per row, vv = v where direct else
+    v - va: the mixed V step's operands in one pass over v and va (B, ld)."""
+    B, ld = v.shape
+    assert va.shape == (B, ld) and vv.shape == (B, ld)
+    assert direct.dtype == torch.bool and direct.shape == (B,) and direct.is_contiguous()
+    assert vmax.shape == (B,) and dvmax.shape == (B,)
+    _call("hqc_mixed_v_prep", _ptr(v, F64, "v"), _ptr(va, F64, "va"),
+          _ptr(direct.view(torch.uint8), torch.uint8, "direct"), B, ld, _ptr(vv, F64, "vv"),
+          _ptr(vmax, F64, "vmax"), _ptr(dvmax, F64, "dvmax"), _stream())
 
 
 def ao_vmat_h2(ph, v, scale, dV, V, work, bidx=None, vmax=None):

@@ -1017,6 +1017,67 @@ __global__ __launch_bounds__(256) void k_vmax(const double* __restrict__ v, int 
   if (threadIdx.x == 0) vm[b] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
+// Operand preparation of the mixed-precision V step, one workgroup per
+// geometry b: vmax[b] = max_r |v|, dvmax[b] = max_r |v - va|, and vv = v where
+// direct[b], v - va elsewhere.  One pass over v and va in place of separate
+// abs, max, subtract and select passes; the values are those of the separate
+// FP64 operations (a NaN propagates into the maxima).
+__device__ __forceinline__ double nanmax(double m, double a) { return (a > m || a != a) ? a : m; }
+
+__global__ __launch_bounds__(1024) void k_mixed_v_prep(const double* __restrict__ v,
+                                                       const double* __restrict__ va,
+                                                       const unsigned char* __restrict__ direct,
+                                                       int ld, double* __restrict__ vv,
+                                                       double* __restrict__ vmax,
+                                                       double* __restrict__ dvmax) {
+  __shared__ double red[2][16];
+  const int b = blockIdx.x;
+  const bool dir = direct[b] != 0;
+  const double2* v2 = reinterpret_cast<const double2*>(v + (long)b * ld);
+  const double2* a2 = reinterpret_cast<const double2*>(va + (long)b * ld);
+  double2* o2 = reinterpret_cast<double2*>(vv + (long)b * ld);
+  double m = 0.0, dm = 0.0;
+  for (int r = threadIdx.x; r < ld / 2; r += 1024) {
+    const double2 x = v2[r], y = a2[r];
+    const double d0 = x.x - y.x, d1 = x.y - y.y;
+    m = nanmax(nanmax(m, fabs(x.x)), fabs(x.y));
+    dm = nanmax(nanmax(dm, fabs(d0)), fabs(d1));
+    o2[r] = dir ? x : make_double2(d0, d1);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    m = nanmax(m, __shfl_xor(m, o, 64));
+    dm = nanmax(dm, __shfl_xor(dm, o, 64));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = m;
+    red[1][w] = dm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = red[0][0], c = red[1][0];
+    for (int i = 1; i < 16; ++i) {
+      a = nanmax(a, red[0][i]);
+      c = nanmax(c, red[1][i]);
+    }
+    vmax[b] = a;
+    dvmax[b] = c;
+  }
+}
+
+extern "C" int hqc_mixed_v_prep(const double* v, const double* va, const unsigned char* direct,
+                                int B, int ld, double* vv, double* vmax, double* dvmax,
+                                void* stream) {
+  if (B < 0 || ld <= 0 || (ld & 1)) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  if (!v || !va || !direct || !vv || !vmax || !dvmax) return HQC_E_ARG;
+  if (((uintptr_t)v | (uintptr_t)va | (uintptr_t)vv) & 15) return HQC_E_ARG;
+  hipLaunchKernelGGL(k_mixed_v_prep, dim3(B), dim3(1024), 0, (hipStream_t)stream, v, va, direct,
+                     ld, vv, vmax, dvmax);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}
+
 // power-of-two scale with |s v| <= 1 (1 for v = 0)
 __device__ __forceinline__ float vscale_of(double m) {
   return m > 0.0 ? (float)exp2(-ceil(log2(m))) : 1.0f;

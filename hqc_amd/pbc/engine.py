@@ -764,13 +764,15 @@ class Engine:
         c = self.c
         sel = lambda key: st[key][:na]
         err = sel("err")
-        st["vmax"][:na] = v.abs().amax(-1)
         direct = err >= self.MIXED_DIRECT_ERR
-        dv = v - sel("va")
+        # max |v|, max |v - v_a| and the operand v (direct) or v - v_a, in one pass
+        v = v.contiguous()
+        vv = torch.empty_like(v)
+        dvmax = torch.empty(na, dtype=torch.float64, device=c.device)
+        K.mixed_v_prep(v, sel("va"), direct.contiguous(), vv, st["vmax"][:na], dvmax)
         tol_e = self.MIXED_SAFETY * err / c.nao                 # per-element error budget
         lp = "phi_h2" if "phi_h2" in st else "phi_lp"
         eps = self.MIXED_EPS_SPLIT16 if lp == "phi_h2" else self.MIXED_EPS
-        dvmax = dv.abs().amax(-1)
         inc = ~direct & sel("va_set") & (eps * dvmax <= tol_e)
         low = direct | inc
         dVs = self._dv(st, na)
@@ -778,7 +780,6 @@ class Engine:
         (idx, n), high_sub, _ = self._split(low)
         if n:
             il = None if idx is None else idx.long()
-            vv = torch.where(direct[:, None], v, dv)
             vvm = torch.where(direct, st["vmax"][:na], dvmax)
             d = dVs if not torch.is_tensor(dVs) or il is None else dVs[il]
             V32 = self.vmat(st[lp], vv if il is None else vv[il], d, self._gix(st, na, idx),

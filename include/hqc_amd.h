@@ -172,6 +172,14 @@ int hqc_ao_vmat_h2_workspace(int B, int nao_pad, int ld, size_t* bytes);
 int hqc_ao_vmat_h2(const unsigned* ph, const int* bidx, const double* v, const double* vmax,
                    int B, int nao_pad, int ld, double phi_scale, double dV, double* V,
                    void* work, size_t work_bytes, void* stream);
+/* Operands of the mixed-precision V step (no counterpart in the reference,
+ * whose V is FP64 throughout): for each geometry b, vmax[b] = max_r |v[b,r]|,
+ * dvmax[b] = max_r |v[b,r] - va[b,r]|, and vv[b,:] = v[b,:] where direct[b]
+ * (one byte per geometry, nonzero = true), v[b,:] - va[b,:] elsewhere.  One pass
+ * over v and va; values as from the separate FP64 operations (NaN propagates
+ * into the maxima).  v, va, vv (B,ld), 16-byte aligned rows, ld even. */
+int hqc_mixed_v_prep(const double* v, const double* va, const unsigned char* direct, int B,
+                     int ld, double* vv, double* vmax, double* dvmax, void* stream);
 
 /* Batched in-place 3-D complex FFT of B grids g^3 (any g <= 128; mixed-radix
  * Stockham passes in LDS, one launch per axis).  sign = -1 forward,

@@ -126,6 +126,28 @@ def test_vmat_split16(K, nao_pad, g, B):
     assert ((V2 - Vref2).abs() <= 2.0 ** -19 * bound2 + 1e-300).all()
 
 
+def test_mixed_v_prep(K):
+    """The fused operand pass of the mixed V step equals the separate torch
+    operations bit for bit (max |v|, max |v - v_a|, select), NaN included."""
+    torch.manual_seed(9)
+    B, ld = 5, 4160
+    v = torch.randn(B, ld, dtype=torch.float64, device=DEV) * 10.0
+    va = v + 1e-6 * torch.randn(B, ld, dtype=torch.float64, device=DEV)
+    va[3] = 0.0
+    v[4, 77] = float("nan")
+    direct = torch.tensor([True, False, True, False, False], device=DEV)
+    vv = torch.empty_like(v)
+    vmax = torch.empty(B, dtype=torch.float64, device=DEV)
+    dvmax = torch.empty_like(vmax)
+    K.mixed_v_prep(v, va, direct, vv, vmax, dvmax)
+    dv = v - va
+    torch.cuda.synchronize()
+    assert torch.equal(vv[:4], torch.where(direct[:, None], v, dv)[:4])
+    assert torch.equal(vmax[:4], v.abs().amax(-1)[:4])
+    assert torch.equal(dvmax[:4], dv.abs().amax(-1)[:4])
+    assert torch.isnan(vmax[4]) and torch.isnan(dvmax[4])
+
+
 @pytest.mark.parametrize("nao_pad,m,dt", [(48, 20, torch.float64), (112, 60, torch.float64),
                                            (112, 60, torch.float32), (112, 100, torch.float32),
                                            (64, 140, torch.float64), (224, 100, torch.float64),
