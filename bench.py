@@ -332,7 +332,7 @@ def main():
         print(summarize_pmc(a.summarize_pmc))
         return
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU only (--gpus 1); "
+        print("bench.py measures one GPU only (--gpus 1); "
               f"--gpus {a.gpus} is not supported.", file=sys.stderr)
         sys.exit(2)
     import torch

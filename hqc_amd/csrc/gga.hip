@@ -1,6 +1,6 @@
 // GGA (PBE) exchange-correlation on the grid with spectral derivatives.
 //
-// New in this build: the reference accepts only LDA exchange
+// Not in the reference, which accepts only LDA exchange
 // (hqc/tools/excor.py:14-17).  The discrete functional is
 //   E_xc = dV sum_r e(rho_r, sigma_r),  sigma = |D rho|^2,
 // with D the FFT derivative on the odd grid (real, no Nyquist term), and its

@@ -245,7 +245,7 @@ int hqc_xc_blocks(int g);
 int hqc_xc_eval(const double* rho, const double* vh, int vh_paired, int B, int g, int ld,
                 int xc_id, double* vout, double* part, void* stream);
 
-/* GGA (PBE) on the grid; new in this build (the reference accepts only LDA
+/* GGA (PBE) on the grid; not in the reference (which accepts only LDA
  * exchange, hqc/tools/excor.py:14-17).  Spectral derivatives on the odd grid:
  *   hqc_grad_G: out (B,3,g^3) complex = scale * i G_d rhoG (inverse FFT ->
  *     grad rho with scale = 1/Omega when rhoG = dV FFT(rho));

@@ -381,7 +381,8 @@ def _ao_from_tables(sysm, P, grad_dir=None):
 
 def ao_gradients(sysm, xp_bohr, kpt=None):
     """grad phi (3, n_ao, N_g) = d phi / d r on the grid (complex at a k-point),
-    from the same 1-D tables with one extra power.  New in this build (forces)."""
+    from the same 1-D tables with one extra power (for the forces, which the
+    reference does not compute)."""
     P = gto_tables(sysm, xp_bohr, kpt, deriv=True)
     return np.array([_ao_from_tables(sysm, P, grad_dir=d) for d in range(3)])
 
@@ -633,7 +634,7 @@ def xc_eps_v(rho, xc="lda,vwn"):
 
 
 # --------------------------------------------------------------------------
-# GGA (PBE) exchange-correlation on the grid.  New in this build: the reference
+# GGA (PBE) exchange-correlation on the grid.  Not in the reference, which
 # accepts only LDA exchange (hqc/tools/excor.py:14-17).
 # --------------------------------------------------------------------------
 PBE_KAPPA = 0.804
@@ -1016,7 +1017,7 @@ def madelung(kappa, G):
 
 def ewald_pp(xp, L, rs, kappa=10, Gmax=15, forces=False):
     """V_pp in Ry (hqc/pbc/potential.py:103-130) for xp, L in rs units; with
-    forces=True also -dE/dR in Ha/Bohr (new in this build).
+    forces=True also -dE/dR in Ha/Bohr (not in the reference).
     Pinned by: simple-cubic and bcc Madelung energies, kappa independence
     (tests/test_oracle_ewald.py)."""
     xp = np.asarray(xp, float)
@@ -1069,7 +1070,7 @@ def potential_energy(x, kappa, G, L, rs):
 
 
 # --------------------------------------------------------------------------
-# Forces of the Mermin free energy A = E - sigma S_e + V_pp (new in this build)
+# Forces of the Mermin free energy A = E - sigma S_e + V_pp (not in the reference)
 # --------------------------------------------------------------------------
 def forces(solver, xp, kappa=10, Gmax=15):
     """Analytic -dA/dR (n, 3) in Ha/Bohr after solver.solve(xp[, kpt]) (Gamma, or a
