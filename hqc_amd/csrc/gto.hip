@@ -203,29 +203,31 @@ __global__ __launch_bounds__(256) void k_ao_values_k(const double2* __restrict__
   __syncthreads();
   const int A = blockIdx.y, b = blockIdx.z;
   const int N = g * g * g;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= N) return;
-  const int iz = r % g, iy = (r / g) % g, ix = r / (g * g);
   const long tab = (long)(b * n + A) * 3 * g * na * nl;
-  const double2* Px = P + tab + (long)(0 * g + ix) * na * nl;
-  const double2* Py = P + tab + (long)(1 * g + iy) * na * nl;
-  const double2* Pz = P + tab + (long)(2 * g + iz) * na * nl;
-  double* out_re = phi + ((long)b * 2 * nao_pad + (long)A * nloc) * ld + r;
-  double* out_im = out_re + (long)nao_pad * ld;
-  for (int o = 0; o < nloc; ++o) {
-    const int c = s_comp[o];
-    const int px = (c == 0), py = (c == 1), pz = (c == 2);
-    double2 acc = make_double2(0.0, 0.0);
-    for (int a = 0; a < na; ++a) {
-      const double w = s_coef[a * nloc + o];
-      if (w != 0.0) {
-        const double2 t = cmul(cmul(Px[a * nl + px], Py[a * nl + py]), Pz[a * nl + pz]);
-        acc.x += w * t.x;
-        acc.y += w * t.y;
+  for (int k = 0; k < AOV_PPT; ++k) {           // AOV_PPT points per thread, as k_ao_values
+    const int r = (blockIdx.x * AOV_PPT + k) * blockDim.x + threadIdx.x;
+    if (r >= N) break;
+    const int iz = r % g, iy = (r / g) % g, ix = r / (g * g);
+    const double2* Px = P + tab + (long)(0 * g + ix) * na * nl;
+    const double2* Py = P + tab + (long)(1 * g + iy) * na * nl;
+    const double2* Pz = P + tab + (long)(2 * g + iz) * na * nl;
+    double* out_re = phi + ((long)b * 2 * nao_pad + (long)A * nloc) * ld + r;
+    double* out_im = out_re + (long)nao_pad * ld;
+    for (int o = 0; o < nloc; ++o) {
+      const int c = s_comp[o];
+      const int px = (c == 0), py = (c == 1), pz = (c == 2);
+      double2 acc = make_double2(0.0, 0.0);
+      for (int a = 0; a < na; ++a) {
+        const double w = s_coef[a * nloc + o];
+        if (w != 0.0) {
+          const double2 t = cmul(cmul(Px[a * nl + px], Py[a * nl + py]), Pz[a * nl + pz]);
+          acc.x += w * t.x;
+          acc.y += w * t.y;
+        }
       }
+      out_re[(long)o * ld] = acc.x;
+      out_im[(long)o * ld] = acc.y;
     }
-    out_re[(long)o * ld] = acc.x;
-    out_im[(long)o * ld] = acc.y;
   }
 }
 
@@ -237,7 +239,7 @@ extern "C" int hqc_ao_values_k(const double* P, int B, int n, int g, int na, int
   if (nao_pad < n * nloc || ld < g * g * g) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
   const int N = g * g * g;
-  hipLaunchKernelGGL(k_ao_values_k, dim3(hqc_div_up(N, 256), n, B), dim3(256), 0,
+  hipLaunchKernelGGL(k_ao_values_k, dim3(hqc_div_up(N, 256 * AOV_PPT), n, B), dim3(256), 0,
                      (hipStream_t)stream, (const double2*)P, n, g, na, nl, coef, comp, nloc,
                      nao_pad, ld, phi);
   HQC_CHECK_LAUNCH();
