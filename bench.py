@@ -199,22 +199,72 @@ def plain_fp64(ngeom=4):
     return ngeom / (time.perf_counter() - t0)
 
 
+def cpu_model():
+    """Host CPU model name (from /proc/cpuinfo) and the thread count the oracle uses."""
+    name = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return name, threads
+
+
 def cpu_baseline(sample_geoms=1):
     """Oracle (NumPy FP64, FFT-Poisson restatement) on the host: energy + analytic
-    forces of `sample_geoms` config geometries."""
+    forces of `sample_geoms` config geometries.  Returns (baseline record, oracle
+    results of seed 0: E + V_pp [Ry], bands [Ha], F [Ha/Bohr])."""
     from oracle import cpu_ref as R
     xps, L = workload_xp(sample_geoms)
     t0 = time.perf_counter()
+    res = None
     for b in range(sample_geoms):
         s = R.Solver(N_ATOMS, L, RS, BASIS, grid_length=GRID_LENGTH, dft=True, smearing=True,
                      smearing_sigma=SIGMA, tol=TOL)
-        s.solve(xps[b])
-        R.forces(s, xps[b])
+        o = s.solve(xps[b])
+        F = R.forces(s, xps[b])
+        if b == 0:
+            res = dict(E=o[3] + R.ewald_pp(xps[b], L, RS), w=o[2] / R.RY, F=F,
+                       n_iter=s.last["n_iter"], converged=bool(o[8]))
     dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": sample_geoms / dt, "unit": "geometries/s", "cores": cores, "kind": "port",
-            "sample": f"{sample_geoms} geometry (seed 0), full SCF + forces, oracle FFT route, "
-                      f"{dt:.1f} s"}
+    model, threads = cpu_model()
+    return ({"value": sample_geoms / dt, "unit": "geometries/s", "cores": threads,
+             "cpu_model": model, "kind": "port",
+             "sample": f"{sample_geoms} geometry (seed 0) of the workload, full SCF + forces, "
+                       f"oracle FFT-Poisson route, {dt:.1f} s"}, res)
+
+
+LITERAL_N = 16
+
+
+def cpu_baseline_literal():
+    """The reference's own algorithm on the host: the oracle's literal route builds the
+    pair-density tensor rhoG (g^3 n_ao^2 doubles) and takes J from it every iteration
+    (hqc/pbc/solver.py:391-479, 616-629), energy only (the reference has no forces).
+    At 54 H (11 GB rhoG, ~40 iterations over it) one geometry takes ~10 minutes on a
+    few cores, so the timed sample is the same settings at 16 H (33^3); the 54 H time
+    is projected from the two scalings (rhoG build ~ n^2 g^3, each J ~ n_ao^2 g^3)."""
+    from oracle import cpu_ref as R
+    n = LITERAL_N
+    L = (4 / 3 * math.pi * n) ** (1 / 3)
+    xp = np.random.default_rng(0).uniform(0, L, (n, 3))
+    t0 = time.perf_counter()
+    s = R.Solver(n, L, RS, BASIS, grid_length=GRID_LENGTH, dft=True, smearing=True,
+                 smearing_sigma=SIGMA, tol=TOL, literal=True)
+    s.solve(xp)
+    dt = time.perf_counter() - t0
+    g16, g54 = s.sys.g, 49
+    scale = (N_ATOMS / n) ** 2 * (g54 / g16) ** 3
+    model, threads = cpu_model()
+    return {"value": 1.0 / dt, "unit": "geometries/s (energy only)", "cores": threads,
+            "cpu_model": model, "kind": "port",
+            "sample": f"literal rhoG route, {n} H gth-dzv rs 1.25 LDA Fermi 1e4 K, {g16}^3, "
+                      f"seed 0, {s.last['n_iter']} SCF iterations, {dt:.1f} s",
+            "projected_54H_s_per_geometry": dt * scale,
+            "projection": f"x (54/{n})^2 (49/{g16})^3 = {scale:.1f}"}
 
 
 def kernel_probe(name, B, reps=5):
@@ -318,6 +368,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--path", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-literal", action="store_true",
+                    help="skip the CPU row of the reference's literal rhoG route")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP64 batched and plain runs (accuracy and speedups)")
     ap.add_argument("--kernel-probe", default=None,
@@ -421,14 +473,21 @@ def main():
         # 78.6 TFLOP/s peak) for the orbital-form density
         "roofline": {"bound": dom["bound"], "kernel": dom["kernel"], "achieved": dom["achieved"],
                      "peak": dom["peak"], "unit": dom["unit"], "frac": dom["frac"],
-                     "traffic": dom["traffic"], "ms_per_launch": dom["ms_per_launch"],
+                     "traffic": dom["traffic"],
+                     "traffic_source": None if dom["traffic"] is None else
+                     "profiles/" + os.path.basename(TRAFFIC_FILE) + " (rocprofv3 --pmc "
+                     "FETCH_SIZE x2.0, WRITE_SIZE x1.0 from a 1 GiB copy; not measured in "
+                     "this run)",
+                     "ms_per_launch": dom["ms_per_launch"],
                      "flops_per_launch": dom["flops_per_launch"]},
         "kernels": cand,
     }
+    seed0 = {a.path: (E[0].item(), F[0].cpu().numpy(), w_main[0].cpu().numpy())}
     if not a.no_compare:
         other = "fp64" if a.path == "mixed" else "mixed"
         pes2, dt2, E2, F2 = run_path(other, x, 1, 1)
         w_a, w_b = w_main, pes2.solver.last["w1"]
+        seed0[other] = (E2[0].item(), F2[0].cpu().numpy(), w_b[0].cpu().numpy())
         out["compare"] = {
             "against": other, "batched_value": B / dt2,
             "max_dE_per_atom_Ha": (E - E2).abs().max().item() / RY / N_ATOMS,
@@ -444,7 +503,18 @@ def main():
         fp64_batched = out["value"] if a.path == "fp64" else B / dt2
         out["speedup_vs_fp64_batched"] = out["value"] / fp64_batched
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(1)
+        base, ref = cpu_baseline(1)
+        out["cpu_baseline"] = base
+        # both GPU paths' seed-0 results against the oracle (the accuracy bar of the
+        # metric is 1 uHa/atom, 1e-5 Ha/Bohr, 1e-6 Ha bands)
+        chk = {"seed": 0, "oracle_n_iter": ref["n_iter"], "oracle_converged": ref["converged"]}
+        for path, (Eg, Fg, wg) in seed0.items():
+            chk[path] = {"dE_per_atom_Ha": abs(Eg - ref["E"]) / RY / N_ATOMS,
+                         "dF_Ha_per_Bohr": float(np.abs(Fg - ref["F"]).max()),
+                         "dbands_Ha": float(np.abs(wg - ref["w"]).max())}
+        out["oracle_check"] = chk
+        if not a.no_literal:
+            out["cpu_baseline_literal"] = cpu_baseline_literal()
     print(json.dumps(out), flush=True)
 
 

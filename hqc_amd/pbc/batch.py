@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from .engine import RY, Cell, Engine
-from .forces import compute_forces
+from .forces import check_supported, compute_forces
 
 
 def uniform_gas_length(n):
@@ -36,6 +36,7 @@ class BatchPES:
         self.kappa, self.Gmax = kappa, Gmax
         self.engine_kw = engine_kw
         self._engines = {}
+        self.last_states = {}
 
     def _engine(self, n, L, rs, g):
         key = (n, g)
@@ -63,10 +64,12 @@ class BatchPES:
         Lb = Lrs * rs
         E, F = [None] * ng, [None] * ng
         conv, nit = [None] * ng, [None] * ng
+        self.last_states = {}
         for (n, g), idx in self.buckets(ns, Lb).items():
             for s0 in range(0, len(idx), self.max_batch):
                 part = idx[s0:s0 + self.max_batch]
                 eng = self._engine(n, Lrs[part[0]], rs[part[0]], g)
+                check_supported(eng)
                 xpb = torch.tensor(np.stack([np.asarray(xps[i], dtype=float) * rs[i]
                                              for i in part]), dtype=torch.float64,
                                    device=eng.c.device)
@@ -74,6 +77,11 @@ class BatchPES:
                 st = eng.solve(xpb, Lb=Lt, forces=True)
                 Fb = compute_forces(eng, st, self.kappa, self.Gmax)
                 Eb = st["E"] * RY + st["Epp"]
+                # the n_ao x n_ao results of the bucket (S and the slot order as the SCF
+                # left them; D, F, bands in the bucket's input order), not the grid arrays
+                self.last_states[(n, g, s0)] = {k: st[k] for k in
+                                                ("S", "D", "F", "w1", "order", "conv", "nit")}
+                self.last_states[(n, g, s0)]["index"] = list(part)
                 for j, i in enumerate(part):
                     E[i], F[i] = Eb[j], Fb[j]
                     conv[i], nit[i] = bool(st["conv"][j].item()), int(st["nit"][j].item())

@@ -1060,8 +1060,9 @@ class Engine:
             out[order] = t
             return out
 
+        F = st["Hcore"] + V2                                   # Fock matrix of the final D
         st.update({k: unperm(t) for k, t in dict(mo=mo, w1=w1, D=D, occ=occ, E=E, conv=conv,
                                                  nit=nit, Ki=Ki, Vep=Vep, Vee=E2,
-                                                 Se=Se).items()})
+                                                 Se=Se, F=F).items()})
         st["unperm"] = unperm
         return st

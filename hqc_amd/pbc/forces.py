@@ -21,6 +21,21 @@ from .engine import EXCHANGE_G0_CONST
 from .potential import ewald_pp
 
 
+def check_supported(engine, kpt=False):
+    """Raise before any device work where the force terms are not -dA/dR.
+
+    Gaussian smearing: the reference's occupations f = (1 + erf((mu - eps)/sigma))/2
+    (hqc/tools/smearing.py:52) together with the Fermi-Dirac entropy S_e
+    (hqc/pbc/solver.py:579-590) do not make A = E - sigma S_e stationary in the
+    occupations, so the Hellmann-Feynman/Pulay terms here miss an occupation
+    response and would not be the gradient of A."""
+    if engine.smearing and engine.method == 1:
+        raise NotImplementedError("forces with Gaussian smearing: E - sigma*S_e is not "
+                                  "stationary in the reference's Gaussian occupations")
+    if kpt and not engine.dft:
+        raise NotImplementedError("Hartree-Fock forces at a k-point are not implemented")
+
+
 def _exchange_adjoint(engine, st):
     """Zx = -Y - c0 (D S~ D) phi for HF, (B, nao_pad, ld), each geometry with its own
     cell constants (VG, dV, c0)."""
@@ -60,6 +75,7 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
     cell-length dependent kernels run once per distinct cell length."""
     if st.get("kpt") is not None:
         return compute_forces_k(engine, st, kappa, Gmax)
+    check_supported(engine)
     c = engine.c
     B, dev = st["B"], c.device
     f64 = torch.float64
@@ -126,8 +142,7 @@ def compute_forces_k(engine, st, kappa=10, Gmax=15):
     """-dA/dR at one k-point per batch entry (DFT).  Complex S, T, D, W enter as
     2 Re sum_q [conj(W_pq) dS_pq - conj(D_pq) dT_pq]; the grid adjoint uses the
     stacked rows (Re phi; Im phi) with X = M rows, M = [[Re D, Im D], [-Im D, Re D]]."""
-    if not engine.dft:
-        raise NotImplementedError("Hartree-Fock forces at a k-point are not implemented")
+    check_supported(engine, kpt=True)
     c = engine.c
     B, dev = st["B"], c.device
     f64, c128 = torch.float64, torch.complex128

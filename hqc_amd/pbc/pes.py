@@ -14,7 +14,7 @@ gamma=False every call takes kpt as well (pes_kpt, hqc/pbc/pes.py:88-97).
 import torch
 
 from .engine import RY
-from .forces import FreeEnergy, compute_forces
+from .forces import FreeEnergy, check_supported, compute_forces
 from .potential import ewald_pp
 from .solver import _as_batch, make_solver
 
@@ -42,18 +42,21 @@ class PES:
         return out[0] if self.mode == "default" else out
 
     def energy_and_forces(self, xp, kpt=None):
+        check_supported(self.solver.engine, kpt is not None)
         st, single, x = self.solver.run(xp, kpt, forces=True)
         F = compute_forces(self.solver.engine, st, self.kappa, self.Gmax)
         Etot = st["E"] * RY + st["Epp"]
         return (Etot[0], F[0]) if single else (Etot, F)
 
     def free_energy_and_forces(self, x, kpt=None):
+        check_supported(self.solver.engine, kpt is not None)
         st, _, x = self.solver.run(x, kpt, forces=True)
         F = compute_forces(self.solver.engine, st, self.kappa, self.Gmax)
         A = st["E"] * RY - RY * self.solver.engine.sigma * st["Se"] + st["Epp"]
         return A, F
 
     def free_energy(self, xp, kpt=None):
+        check_supported(self.solver.engine, kpt is not None)
         x, single = _as_batch(xp.detach() if torch.is_tensor(xp) else xp, self.solver.n,
                               self.solver.cell.device)
         if torch.is_tensor(xp) and xp.requires_grad:

@@ -42,8 +42,8 @@ def read_basis_sets(basis, element="H"):
     (hqc/basis/parse.py:32-175): split on '#BASIS SET', take the block whose
     first line starts with the element, line 3 is the number of sets, each set
     is a quantum-number line 'n lmin lmax nprim nfunc(l)...' followed by nprim
-    rows 'alpha c_1 ... c_k'.  Pinned by: tests/test_oracle_basis.py
-    (known H gth-dzvp structure, normalisation closed form).
+    rows 'alpha c_1 ... c_k'.  Pinned by: tests/test_oracle_basis_grid.py
+    (test_basis_structure: known H gth-dzvp structure; test_normalisation_closed_form).
     """
     path = basis if os.path.isfile(basis) else os.path.join(_BASIS_DIR, basis + ".dat")
     if not os.path.isfile(path):
@@ -85,7 +85,7 @@ def normalize_sets(sets):
     c_i * alpha_i^(l/2+3/4) * 2^(l+7/4) / pi^(1/4) / sqrt((2l+1)!!) * N,
     N^-2 = 2^(l+3/2) sum_ij c_i c_j (a_i+a_j)^(-l-3/2) (a_i a_j)^(l/2+3/4).
     Pinned by: closed-form self-overlap of every contracted radial function = 1
-    (tests/test_oracle_basis.py).
+    (tests/test_oracle_basis_grid.py test_normalisation_closed_form).
     """
     out = []
     for quant, rows in sets:
@@ -170,8 +170,8 @@ class System:
     +-floor(24/L) images for grid AO values (hqc/pbc/gto.py:47-49, rcut fixed at
     its default 24 there).  Grid points are stored in (x, y, z) order with z
     fastest; the reference's meshgrid 'xy' order only permutes summands.
-    Pinned by: tests/test_oracle_grid.py (odd-grid rule incl. half-even
-    rounding, VG closed form, lattice counts).
+    Pinned by: tests/test_oracle_basis_grid.py (test_grid_rule_half_even,
+    test_VG_closed_form, test_lattice_counts).
     """
 
     def __init__(self, n, L, rs, basis, rcut=24.0, grid_length=0.12):
@@ -271,8 +271,8 @@ def overlap_kinetic(sysm, xp_bohr, kpt=None, deriv=False):
     S_pq = sum_T e^{-i k.T} sum_ab c_ap c_bq I_ab(R_A - R_B + T).
     deriv=True also returns dS[p, q, c] = d S_pq / d R_{atom(p), c} holding
     R_{atom(q)} fixed (first-argument derivative), used for forces.
-    Pinned by: tests/test_oracle_integrals.py (isolated-Gaussian closed forms at
-    large L, grid-quadrature overlap/kinetic of the AO values, FD of p blocks).
+    Pinned by: tests/test_oracle_physics.py (test_isolated_atom_normalised,
+    test_overlap_kinetic_vs_grid_quadrature, test_overlap_derivatives_fd).
     """
     b = sysm.basis
     n, nloc = sysm.n, b.n_ao_atom
@@ -326,7 +326,8 @@ def gto_tables(sysm, xp_bohr, kpt=None, deriv=False):
 
     Restates eval_pbc_gaussian_power_x(_kpt) (hqc/pbc/gto.py:326-339, 371-386)
     vmapped over atoms and the 1-D grid (hqc/pbc/solver.py:209-226).
-    Pinned by: brute-force 3-D image sums (tests/test_oracle_grid.py).
+    Pinned by: brute-force 3-D image sums (tests/test_oracle_basis_grid.py
+    test_tables_bruteforce_images).
     """
     b = sysm.basis
     lmax = b.l_max + (2 if deriv else 0)
@@ -346,7 +347,8 @@ def ao_values(sysm, xp_bohr, kpt=None):
     Restates eval_pbc_gto_sph / eval_pbc_ao (hqc/pbc/gto.py:341-369, 388-420):
     phi_(A,o)(r) = sum_a coef[a,o] prod_d P[A,d,r_d,a,pow_d(o)].
     Pinned by: brute-force 3-D image sum at random points, periodicity
-    (tests/test_oracle_grid.py).
+    (tests/test_oracle_basis_grid.py test_ao_periodicity, tests/test_oracle_slater.py
+    test_points_brute_force_image_sum).
     """
     P = gto_tables(sysm, xp_bohr, kpt)
     return _ao_from_tables(sysm, P)
@@ -598,8 +600,10 @@ def xc_eps_v(rho, xc="lda,vwn"):
 
     eps restates hqc/tools/excor.py:8-12 (Slater), :76-85 (VWN), :87-112
     (HL71/BH72/GL77); v_xc is the reference's jax.grad (hqc/pbc/solver.py:673)
-    written in closed form.  Pinned by: FD of rho*eps, uniform-gas values
-    (tests/test_oracle_xc.py).
+    written in closed form.  Pinned by: FD of rho*eps, uniform-gas values and the
+    VWN high-density slope (tests/test_oracle_physics.py), the high- and
+    low-density limits of HL71/BH72/GL77, which fix both constants C and A
+    (tests/test_oracle_pins.py test_hl_bh_gl_limits).
     """
     xname, cname = xc.split(",")
     if xname != "lda":
@@ -749,7 +753,9 @@ def make_occupation(n, n_mo, smearing=False, smearing_method="fermi", smearing_s
     """Restates make_occupation_func (hqc/tools/smearing.py:4-173), including the
     bisection return rule of :102 and the clipped Fermi function of :40.
     Pinned by: sum f = n, sigma -> 0 gives aufbau, entropy closed form
-    (tests/test_oracle_smearing.py).
+    (tests/test_oracle_physics.py), and hand-computed bisection end points for
+    mu > 0, mu < 0, a bracket around 0 and loop == search_cycle
+    (tests/test_oracle_pins.py test_bisection_return_rule).
     """
     assert n_mo >= n // 2
     if not smearing:
@@ -850,7 +856,12 @@ def scf_diis(v, Hcore, dm0, hartree_fn, xc_fn, dm_fn, errvec_fn, tol, max_cycle,
     diis_space copies of F(dm0), a fixed-point warm-up of diis_start_cycle +
     diis_space iterations, then Pulay DIIS with B + 1e-12 (tr B/m + 1) I and the
     bordered solve; mo/w1 come from eigh of the extrapolated F; stop when
-    |dE| <= tol and ||e_last||_F <= 10 tol, or at max_cycle."""
+    |dE| <= tol and ||e_last||_F <= 10 tol, or at max_cycle.
+    Pinned by: the iteration count start + space + 1 of a non-interacting problem and
+    the converged flag at max_cycle, the warm-up = fixed-point iterate relation, and
+    the returned (mo, w1, E) of one step from the extrapolated F
+    (tests/test_oracle_pins.py); E equal to the fixed-point SCF's at convergence
+    (tests/test_oracle_physics.py)."""
     F0 = Hcore + hartree_fn(dm0) + xc_fn(dm0)[1]
     e0 = errvec_fn(dm0, F0)
     Fk = [F0.copy() for _ in range(diis_space)]
@@ -917,7 +928,7 @@ class Solver:
         self.diis_start_cycle, self.diis_damp = diis_start_cycle, diis_damp
         self.dft, self.xc = dft, xc
         self.gamma, self.literal = gamma, literal
-        self.smearing = smearing
+        self.smearing, self.smearing_method = smearing, smearing_method
         self.sigma = smearing_sigma if smearing else 0.0
         self.gga = dft and is_gga(xc)
         if dft and not self.gga:
@@ -1019,7 +1030,7 @@ def ewald_pp(xp, L, rs, kappa=10, Gmax=15, forces=False):
     """V_pp in Ry (hqc/pbc/potential.py:103-130) for xp, L in rs units; with
     forces=True also -dE/dR in Ha/Bohr (not in the reference).
     Pinned by: simple-cubic and bcc Madelung energies, kappa independence
-    (tests/test_oracle_ewald.py)."""
+    (tests/test_oracle_physics.py test_ewald_*)."""
     xp = np.asarray(xp, float)
     n = xp.shape[0]
     x = xp - L * np.floor(xp / L)
@@ -1080,8 +1091,25 @@ def forces(solver, xp, kappa=10, Gmax=15):
     +Tr[W dS] (W = C f eps C^T), grid adjoint dV sum_r Z_p grad phi_p with
     Z = dE/dphi (2 v_eff D phi for the local part, minus exchange pair terms),
     structure-factor derivative of v_loc, and the Ewald term.
-    Pinned by: central finite differences of A (tests/test_oracle_forces.py).
+    Pinned by: central finite differences of A, for aufbau occupations and Fermi
+    smearing only (tests/test_oracle_physics.py test_forces_match_finite_differences,
+    tests/test_oracle_gga.py test_gga_scf_forces_match_finite_differences and
+    test_kpoint_forces_match_finite_differences).
+
+    Gaussian smearing raises NotImplementedError: the reference's Gaussian
+    occupations f = (1 + erf((mu - eps)/sigma))/2 (hqc/tools/smearing.py:52) with the
+    Fermi-Dirac entropy S_e (hqc/pbc/solver.py:579-590) do not make A stationary in
+    the occupations, so these terms are not -dA/dR there
+    (tests/test_oracle_pins.py test_gauss_forces_are_not_the_gradient).
     """
+    if solver.smearing and solver.smearing_method == "gauss":
+        raise NotImplementedError("forces with Gaussian smearing: E - sigma S_e is not "
+                                  "stationary in the Gaussian occupations")
+    return _analytic_forces(solver, xp, kappa, Gmax)
+
+
+def _analytic_forces(solver, xp, kappa=10, Gmax=15):
+    """The force terms of `forces` without the smearing check."""
     s = solver.sys
     st = solver.last
     xpb = st["xpb"]

@@ -418,6 +418,23 @@ def test_occupations(K, method, search):
         assert abs(mu[i].item() - occ_fn.mu(w[i])) < 1e-11
 
 
+def test_bisection_return_rule_known_answers(K):
+    """The kernel's mu on the hand-worked spectra of the oracle's pin
+    (tests/test_oracle_pins.py test_bisection_return_rule): 4 electrons in 4
+    orbitals, sigma = 0.01, mu_shift = 0; expected mu for search_cycle 100 / 3 / 2."""
+    base = [0.0, 1.0, 2.0, 3.0]
+    skew = [0.0, 1.0, 2.0, 4.0]
+    cases = [(1.0, base, 100, 1.0), (-5.0, base, 100, -3.5), (-1.0, base, 100, 0.0),
+             (-2.0, base, 100, -0.5), (1.0, skew, 100, 2.0), (1.0, skew, 3, 4.25),
+             (-5.0, skew, 100, -3.5), (-5.0, skew, 3, -7.25), (-5.0, skew, 2, -6.5)]
+    for a, w, cyc, want in cases:
+        wt = torch.tensor([[a + x for x in w]], dtype=torch.float64, device=DEV)
+        occ = torch.empty_like(wt)
+        mu = torch.empty(1, dtype=torch.float64, device=DEV)
+        K.occupations(wt, 4, 0.01, 0, 0, cyc, 1e-7, 0.0, occ, mu)
+        assert mu.item() == want, (a, w, cyc, mu.item(), want)
+
+
 @pytest.mark.parametrize("xc", ["lda,vwn", "lda,", "lda,hl", "lda,bh", "lda,gl"])
 def test_xc_eval(K, xc):
     g = 9

@@ -43,11 +43,15 @@ def test_solver_parity(case):
     xp, L = _xp(n, 7)
     kw = dict(kw)
     kw.setdefault("tol", 1e-10)
-    ref = R.Solver(n, L, rs, basis, grid_length=gl, **kw).solve(xp)
+    rs_ = R.Solver(n, L, rs, basis, grid_length=gl, **kw)
+    ref = rs_.solve(xp)
     sol = make_solver(n, L, rs, basis, grid_length=gl, **kw)
     out = sol(xp)
     got = [o.cpu().numpy() if torch.is_tensor(o) else o for o in out]
     assert bool(got[8]) == bool(ref[8])
+    # the same SCF iteration count as the reference loop semantics (warm-up, DIIS)
+    assert int(sol.last["nit"][0]) == rs_.last["n_iter"], (int(sol.last["nit"][0]),
+                                                            rs_.last["n_iter"])
     assert abs(got[3] - ref[3]) < 1e-8, (got[3], ref[3])        # E [Ry]
     assert np.allclose(got[2], ref[2], atol=1e-7)                # bands [Ry]
     for k in (4, 5, 6):
@@ -306,3 +310,68 @@ def test_ovlp_cut_matches_when_nothing_is_cut():
     c = make_solver(n, L, rs, "gth-dzvp", ovlp_cut=0.2, **kw)(xp)
     assert torch.isfinite(c[3]) and bool(c[8])
     assert (c[2] > 1e3).sum().item() >= 1            # shifted directions (bands in Ry)
+
+
+@pytest.mark.parametrize("name,kw", [
+    ("hf", dict(dft=False)),
+    ("pbe", dict(dft=True, xc="pbe,pbe", smearing=True, smearing_sigma=0.0316681)),
+], ids=["hf", "pbe"])
+def test_mixed_precision_hf_and_gga(name, kw):
+    """precision='mixed' with Hartree-Fock (make_pes's default dft=False: the AO-value
+    arrays move with the SCF slots) and with PBE, on a batch of 5 geometries that
+    converge at different iterations, so the slots are compacted; against FP64 at the
+    accuracy bar (1 uHa/atom, 1e-5 Ha/Bohr, 1e-6 Ha bands)."""
+    from hqc_amd.pbc import make_pes
+    print(f"[mixed {name}]", flush=True)
+    n, rs = 16, 1.25
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    xps = np.random.default_rng(17).uniform(0, L, (5, n, 3))
+    kw = dict(kw, grid_length=0.2, tol=1e-8)
+    p64 = make_pes(n, L, rs, "gth-dzv", precision="fp64", **kw)
+    pmx = make_pes(n, L, rs, "gth-dzv", precision="mixed", **kw)
+    E64, F64 = p64.energy_and_forces(xps)
+    Emx, Fmx = pmx.energy_and_forces(xps)
+    st = pmx.solver.last
+    nit = st["nit"].cpu().numpy()
+    print(f"  iterations {nit.tolist()}, slot order {st['order'].tolist()}", flush=True)
+    assert bool(st["conv"].all())
+    assert len(set(nit.tolist())) > 1                         # compaction ran
+    assert (st["order"].cpu().numpy() != np.arange(5)).any()  # and permuted the slots
+    if name == "hf":
+        assert "phi_h2" in st["per"] and "gidx" not in st["per"]
+    assert (E64 - Emx).abs().max().item() / 2 / n <= 1e-6
+    assert (F64 - Fmx).abs().max().item() <= 1e-5
+    assert (p64.solver.last["w1"] - st["w1"]).abs().max().item() <= 1e-6
+    assert pmx.solver.engine.stats["v32"] > 0
+
+
+def test_free_energy_autograd_shapes():
+    """pes.free_energy as a torch.autograd.Function for (B, n, 3) and (n, 3) inputs:
+    the gradient is -2 rs F of energy_and_forces (A in Ry, xp in rs units)."""
+    from hqc_amd.pbc import make_pes
+    n, rs = 4, 1.5
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    xps = np.random.default_rng(9).uniform(0, L, (2, n, 3))
+    pes = make_pes(n, L, rs, "gth-dzv", grid_length=0.3, dft=True, smearing=True,
+                   smearing_sigma=0.1, tol=1e-11, search_tol=1e-13, search_cycle=200)
+    _, F = pes.energy_and_forces(xps)
+    x = torch.tensor(xps, dtype=torch.float64, device="cuda", requires_grad=True)
+    pes.free_energy(x).sum().backward()
+    assert (x.grad + 2 * rs * F).abs().max().item() < 1e-12
+    x1 = torch.tensor(xps[1], dtype=torch.float64, device="cuda", requires_grad=True)
+    A1 = pes.free_energy(x1)
+    assert A1.dim() == 0
+    (g1,) = torch.autograd.grad(2.5 * A1, x1)
+    assert (g1 + 2.5 * 2 * rs * F[1]).abs().max().item() < 1e-12
+
+
+def test_gauss_forces_refused():
+    from hqc_amd.pbc import make_pes
+    n, rs = 4, 1.5
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    pes = make_pes(n, L, rs, "gth-dzv", grid_length=0.3, dft=True, smearing=True,
+                   smearing_method="gauss", smearing_sigma=0.1)
+    xp = np.random.default_rng(9).uniform(0, L, (n, 3))
+    assert torch.isfinite(pes(xp))                      # energies are fine
+    with pytest.raises(NotImplementedError, match="Gaussian"):
+        pes.energy_and_forces(xp)

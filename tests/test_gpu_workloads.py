@@ -1,0 +1,274 @@
+"""GPU checks at the benchmark workload sizes (the configurations the throughput
+metric and the batched PES are quoted on), against the oracle where it finishes
+in seconds and through size-independent properties otherwise:
+
+* 54 H gth-dzv, rs 1.25, LDA (VWN), Fermi-Dirac at 10^4 K, 49^3 grid, tol 1e-8,
+  bisection mu search: FP64 and mixed precision against the oracle (energy,
+  bands, forces), seeds 0 and 1;
+* 54 H gth-dzv, rs 1.4, PBE, Fermi-Dirac at 5000 K, 2x2x2 Monkhorst-Pack mesh on
+  the 49^3 grid: the 8 k-points as one batch, k-point 0 against the oracle,
+  time-reversal partners equal;
+* the uneven Hartree-Fock PES batch (n in {16, 32, 54, 64}, rs in [1, 1.5],
+  gth-dzv, grid spacing 0.16 Bohr): electron count, commutator residual and
+  total force of every geometry, two n = 16 geometries against the oracle;
+* 128 H gth-dzvp Hartree-Fock (640 AOs, 65^3, full exchange): properties and a
+  central finite difference of one force component.
+Long oracle runs print their progress."""
+import math
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _sigma(T):
+    """sigma = 1/(2 beta), beta = 157888.088922572 / T (Hartree)."""
+    return 1.0 / (157888.088922572 / T) / 2.0
+
+
+def _xps(n, seeds):
+    L = (4 / 3 * math.pi * n) ** (1 / 3)
+    return np.stack([np.random.default_rng(s).uniform(0, L, (n, 3)) for s in seeds]), L
+
+
+def _log(msg, t0):
+    print(f"  {msg} ({time.perf_counter() - t0:.1f} s)", flush=True)
+
+
+CFG2 = dict(grid_length=0.16, dft=True, xc="lda,vwn", smearing=True, smearing_method="fermi",
+            smearing_sigma=_sigma(1e4), search_method="bisect", tol=1e-8)
+
+
+@pytest.fixture(scope="module")
+def cfg2_oracle():
+    """Oracle energy (E + V_pp, Ry), bands (Ha), forces (Ha/Bohr) and SCF iterations
+    of seeds 0 and 1 of the 54 H workload."""
+    n, rs = 54, 1.25
+    xps, L = _xps(n, (0, 1))
+    out = []
+    t0 = time.perf_counter()
+    for b in range(2):
+        s = R.Solver(n, L, rs, "gth-dzv", **CFG2)
+        o = s.solve(xps[b])
+        F = R.forces(s, xps[b])
+        out.append(dict(E=o[3] + R.ewald_pp(xps[b], L, rs), w=o[2] / R.RY, F=F,
+                        nit=s.last["n_iter"], conv=o[8]))
+        _log(f"oracle seed {b}: {s.last['n_iter']} iterations", t0)
+    return xps, L, out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_metric_workload_vs_oracle(cfg2_oracle, precision):
+    """At the reference's settings (bisection tolerance 1e-7) the electron count of the
+    mu search leaves ~1e-8 Ha of noise in E, so the iteration that first meets the
+    stopping rule (|dE| <= 1e-8 Ha) can move by one with rounding: FP64 within two
+    iterations of the oracle, E within 2 tol (4e-8 Ry), bands 1e-7 Ha, forces 1e-6
+    Ha/Bohr.  Mixed: the accuracy bar of the metric (1 uHa/atom, 1e-6 Ha bands,
+    1e-5 Ha/Bohr) against the oracle itself."""
+    from hqc_amd.pbc import make_pes
+    print(f"[54 H dzv LDA 49^3] {precision}", flush=True)
+    n, rs = 54, 1.25
+    xps, L, ref = cfg2_oracle
+    pes = make_pes(n, L, rs, "gth-dzv", precision=precision, **CFG2)
+    E, F = pes.energy_and_forces(xps)
+    st = pes.solver.last
+    assert pes.solver.cell.g == 49
+    for b in range(2):
+        r = ref[b]
+        dE = abs(E[b].item() - r["E"])
+        dw = np.abs(st["w1"][b].cpu().numpy() - r["w"]).max()
+        dF = np.abs(F[b].cpu().numpy() - r["F"]).max()
+        print(f"  seed {b}: |dE| {dE:.2e} Ry, |dbands| {dw:.2e} Ha, |dF| {dF:.2e} Ha/Bohr, "
+              f"iterations {int(st['nit'][b])} (oracle {r['nit']})", flush=True)
+        assert bool(st["conv"][b]) and r["conv"]
+        if precision == "fp64":
+            assert abs(int(st["nit"][b]) - r["nit"]) <= 2
+            assert dE < 4e-8 and dw < 1e-7 and dF < 1e-6
+        else:
+            assert dE / R.RY / n <= 1e-6 and dw <= 1e-6 and dF <= 1e-5
+
+
+def test_metric_workload_tight_mu_vs_oracle():
+    """The same workload with the mu search run to 1e-12 (no noise floor): the FP64
+    SCF takes the oracle's iteration count and E agrees to 1e-9 Ry (seed 1)."""
+    from hqc_amd.pbc import make_pes
+    print("[54 H dzv LDA 49^3, search_tol 1e-12]", flush=True)
+    n, rs = 54, 1.25
+    xps, L = _xps(n, (1,))
+    kw = dict(CFG2, search_tol=1e-12, search_cycle=200)
+    t0 = time.perf_counter()
+    s = R.Solver(n, L, rs, "gth-dzv", **kw)
+    o = s.solve(xps[0])
+    Fref = R.forces(s, xps[0])
+    _log(f"oracle: {s.last['n_iter']} iterations", t0)
+    pes = make_pes(n, L, rs, "gth-dzv", **kw)
+    E, F = pes.energy_and_forces(xps[0])
+    st = pes.solver.last
+    dE = abs(E.item() - (o[3] + R.ewald_pp(xps[0], L, rs)))
+    dw = np.abs(st["w1"][0].cpu().numpy() - o[2] / R.RY).max()
+    dF = np.abs(F.cpu().numpy() - Fref).max()
+    print(f"  |dE| {dE:.2e} Ry, |dbands| {dw:.2e} Ha, |dF| {dF:.2e} Ha/Bohr, iterations "
+          f"{int(st['nit'][0])} (oracle {s.last['n_iter']})", flush=True)
+    assert o[8] and bool(st["conv"][0])
+    assert int(st["nit"][0]) == s.last["n_iter"]
+    assert dE < 1e-9 and dw < 1e-9 and dF < 1e-8
+
+
+def test_free_energy_autograd_54H():
+    """The torch.autograd.Function path at the workload size: the gradient of
+    pes.free_energy equals -2 rs F of energy_and_forces, and a central finite
+    difference (1e-4 Bohr) of the free energy (tight SCF and mu tolerances so the
+    energy is smooth to ~1e-11)."""
+    from hqc_amd.pbc import make_pes
+    print("[autograd 54 H]", flush=True)
+    n, rs = 54, 1.25
+    xps, L = _xps(n, (0,))
+    kw = dict(CFG2, tol=1e-11, search_tol=1e-13, search_cycle=200, max_cycle=300)
+    pes = make_pes(n, L, rs, "gth-dzv", **kw)
+    x = torch.tensor(xps[0], dtype=torch.float64, device="cuda", requires_grad=True)
+    A = pes.free_energy(x)
+    (g,) = torch.autograd.grad(A, x)
+    _, F = pes.energy_and_forces(xps[0])
+    assert (g + 2 * rs * F).abs().max().item() < 1e-12
+    h = 1e-4 / rs
+    for (a, c) in ((0, 0), (17, 2)):
+        xq = xps[0].copy(); xq[a, c] += h
+        xm = xps[0].copy(); xm[a, c] -= h
+        Aq = pes.free_energy(torch.tensor(xq, device="cuda")).item()
+        Am = pes.free_energy(torch.tensor(xm, device="cuda")).item()
+        fd = (Aq - Am) / (2 * h)
+        print(f"  atom {a} dir {c}: autograd {g[a, c].item():.9f} FD {fd:.9f} Ry/rs", flush=True)
+        assert abs(g[a, c].item() - fd) < 2e-6 * 2 * rs
+
+
+CFG5 = dict(grid_length=0.178, dft=True, xc="pbe,pbe", smearing=True, smearing_method="fermi",
+            smearing_sigma=_sigma(5000.0), search_method="bisect", tol=1e-8,
+            search_tol=1e-12, search_cycle=200, max_cycle=300)
+
+
+def test_kpoint_mesh_workload_vs_oracle():
+    """54 H rs 1.4 PBE at 5000 K on the 2x2x2 Monkhorst-Pack mesh (49^3): the 8
+    k-points of one geometry as one batch; k-point 0 against the oracle (E, bands,
+    forces) and every (k, -k) pair equal (time reversal).  The mu search runs to
+    1e-12 and up to 300 cycles so that every k-point converges (with the reference's
+    defaults some stop at max_cycle, where the result depends on rounding)."""
+    from hqc_amd.pbc import make_pes
+    print("[54 H PBE 2x2x2 k-mesh 49^3]", flush=True)
+    n, rs = 54, 1.4
+    xps, L = _xps(n, (0,))
+    Lb = L * rs
+    mp = np.array([[i, j, k] for i in (-0.25, 0.25) for j in (-0.25, 0.25) for k in (-0.25, 0.25)])
+    kpts = mp * 2 * np.pi / Lb
+    x8 = np.repeat(xps[:1], 8, 0)
+    t0 = time.perf_counter()
+    pes = make_pes(n, L, rs, "gth-dzv", gamma=False, **CFG5)
+    E, F = pes.energy_and_forces(x8, kpts)
+    st = pes.solver.last
+    _log(f"GPU 8 k-points, iterations {st['nit'].tolist()}", t0)
+    assert pes.solver.cell.g == 49
+    assert bool(st["conv"].all())
+    for a in range(4):                                  # k and -k are entries a and 7 - a
+        assert abs(E[a].item() - E[7 - a].item()) < 1e-8
+        assert (F[a] - F[7 - a]).abs().max().item() < 1e-6
+    s = R.Solver(n, L, rs, "gth-dzv", gamma=False, **CFG5)
+    o = s.solve(xps[0], kpts[0])
+    Fref = R.forces(s, xps[0])
+    _log(f"oracle k-point 0: {s.last['n_iter']} iterations", t0)
+    Eref = o[3] + R.ewald_pp(xps[0], L, rs)
+    assert o[8]
+    dE = abs(E[0].item() - Eref)
+    dw = np.abs(st["w1"][0].cpu().numpy() - o[2] / R.RY).max()
+    dF = np.abs(F[0].cpu().numpy() - Fref).max()
+    print(f"  k 0: |dE| {dE:.2e} Ry, |dbands| {dw:.2e} Ha, |dF| {dF:.2e} Ha/Bohr", flush=True)
+    assert dE < 1e-8 and dw < 1e-7 and dF < 1e-6
+
+
+def _hf_batch(ng, seed=5):
+    rng = np.random.default_rng(seed)
+    ns = rng.choice([16, 32, 54, 64], ng)
+    rss = rng.uniform(1.0, 1.5, ng)
+    xps = [rng.uniform(0, (4 / 3 * np.pi * n) ** (1 / 3), (n, 3)) for n in ns]
+    return ns, rss, xps
+
+
+def _residual_checks(st, tol, nelec):
+    """Electron count Tr(DS) and commutator ||SDF - FDS|| of every geometry of a solve."""
+    S, D, Fk = st["S"], st["D"], st["F"]
+    order = st["order"]
+    S = S[torch.argsort(order)]          # S is kept in slot order, D and F in input order
+    tr = torch.einsum("bpq,bqp->b", D, S)
+    e = S @ D @ Fk - Fk @ D @ S
+    return (tr - nelec).abs().max().item(), torch.linalg.norm(e, dim=(-2, -1)).max().item()
+
+
+def test_uneven_hf_batch():
+    """The batched PES over uneven systems (restricted HF at Gamma, gth-dzv, grid
+    spacing 0.16 Bohr): every geometry converged with Tr(DS) = n, ||SDF - FDS|| <=
+    10 tol and a small total force (the grid breaks translation invariance only
+    slightly); the first two n = 16 geometries against the oracle."""
+    from hqc_amd.pbc import make_pes_batch
+    print("[uneven HF batch, 16 geometries]", flush=True)
+    ns, rss, xps = _hf_batch(16)
+    tol = 1e-8
+    t0 = time.perf_counter()
+    pes = make_pes_batch("gth-dzv", grid_length=0.16, dft=False, tol=tol)
+    E, F, info = pes.energy_and_forces(xps, rss)
+    _log(f"GPU batch: buckets {info['buckets']}", t0)
+    assert all(info["converged"])
+    for (n, g, _), st in pes.last_states.items():
+        dn, res = _residual_checks(st, tol, n)
+        assert dn < 1e-9 and res <= 10 * tol, (n, g, dn, res)
+    sumF = max(F[i].sum(0).abs().max().item() for i in range(len(xps)))
+    print(f"  max |sum_A F_A| {sumF:.2e} Ha/Bohr", flush=True)
+    assert sumF < 1e-3
+    done = 0
+    for i, n in enumerate(ns):
+        if n != 16 or done == 2:
+            continue
+        L = (4 / 3 * np.pi * n) ** (1 / 3)
+        s = R.Solver(n, L, rss[i], "gth-dzv", grid_length=0.16, dft=False, tol=tol)
+        o = s.solve(xps[i])
+        Eref = o[3] + R.ewald_pp(xps[i], L, rss[i])
+        Fref = R.forces(s, xps[i])
+        _log(f"oracle geometry {i} (n 16, rs {rss[i]:.3f})", t0)
+        assert abs(E[i].item() - Eref) < 1e-7
+        assert np.abs(F[i].cpu().numpy() - Fref).max() < 1e-6
+        done += 1
+    assert done == 2
+
+
+def test_128H_dzvp_hf():
+    """128 H gth-dzvp Hartree-Fock (640 AOs, 65^3, full exchange).  The overlap has a
+    negative eigenvalue (-7e-6) at this density, so canonical orthogonalisation
+    without a cut (the reference's) would take its inverse square root; ovlp_cut drops
+    that direction.  Checks: convergence, Tr(DS) = n, ||SDF - FDS|| <= 10 tol,
+    total force, and one force component against a central finite difference."""
+    from hqc_amd.pbc import make_pes
+    print("[128 H dzvp HF 65^3]", flush=True)
+    n, rs = 128, 1.25
+    xps, L = _xps(n, (0,))
+    tol = 1e-8
+    kw = dict(grid_length=0.16, dft=False, tol=tol, ovlp_cut=1e-4)
+    t0 = time.perf_counter()
+    pes = make_pes(n, L, rs, "gth-dzvp", **kw)
+    E, F = pes.energy_and_forces(xps[0])
+    st = pes.solver.last
+    _log(f"GPU SCF + forces: {int(st['nit'][0])} iterations", t0)
+    assert pes.solver.cell.g == 65 and pes.solver.cell.nao == 640
+    assert bool(st["conv"][0])
+    dn, res = _residual_checks(st, tol, n)
+    assert dn < 1e-8 and res <= 10 * tol, (dn, res)
+    print(f"  |sum F| {F.sum(0).abs().max().item():.2e} Ha/Bohr", flush=True)
+    assert F.sum(0).abs().max().item() < 1e-3
+    h = 1e-4 / rs
+    a, c = 5, 1
+    xq = xps[0].copy(); xq[a, c] += h
+    xm = xps[0].copy(); xm[a, c] -= h
+    fd = -(pes(xq).item() - pes(xm).item()) / (2 * 1e-4) / R.RY
+    _log(f"FD atom {a} dir {c}: {fd:.7f} vs {F[a, c].item():.7f} Ha/Bohr", t0)
+    assert abs(fd - F[a, c].item()) < 1e-4

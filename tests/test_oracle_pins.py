@@ -1,0 +1,200 @@
+"""Pins of the oracle functions that sum rules and self-consistency leave open:
+the bisection return rule (hqc/tools/smearing.py:97-102), the HL71/BH72/GL77
+correlation constants (hqc/tools/excor.py:87-112), the DIIS iteration
+semantics (hqc/pbc/scf.py:90-242) and the choice to refuse forces with
+Gaussian smearing.  Each known answer below is worked out by hand or fixed by
+the mathematics, so a plausible transcription mistake fails one of them."""
+import math
+import types
+
+import numpy as np
+import pytest
+
+from oracle import cpu_ref as R
+
+
+# ------------------------------------------------------------ bisection rule
+def _mu(w, n, search_cycle, sigma=0.01, tol=1e-7):
+    occ = R.make_occupation(n, len(w), True, "fermi", sigma, "bisect", search_cycle, tol)
+    return occ.mu(np.asarray(w, float))
+
+
+def test_bisection_return_rule():
+    """Spectra chosen so every midpoint's electron count is exact in FP64.
+
+    w = a + (0, 1, 2, 3), 4 electrons in 4 orbitals: mu_shift = -sigma ln(2*4/4 - 1)
+    = 0, so the bracket is [a, a+3]; with sigma = 0.01 every |w - mid|/sigma >= 50 is
+    clipped (f = 2 or 2/(e^50 + 1), absorbed next to 4), so N(a+1.5) - 4 == 0 exactly:
+    mu_hi <- a+1.5, y_hi = 0 <= tol and the loop ends after one step with
+    (mu_lo, mu_hi) = (a, a+1.5).  The reference returns
+    (mu_lo > -tol) mu_lo + (mu_hi < tol) mu_hi [+ mid if loop == max_cycle]:
+    the true mu a+1.5 only when both ends are negative, the LOWER end when both
+    are positive, 0 when they straddle 0, and the sum of both when mu_hi = 0."""
+    base = np.array([0.0, 1.0, 2.0, 3.0])
+    assert _mu(1.0 + base, 4, 100) == 1.0          # mu_lo = 1, mu_hi = 2.5 > 0: lower end
+    assert _mu(-5.0 + base, 4, 100) == -3.5        # both negative: upper end = a + 1.5
+    assert _mu(-1.0 + base, 4, 100) == 0.0         # -1 < 0 < 0.5: neither term
+    assert _mu(-1.5 + base, 4, 100) == 0.0         # mu_hi = 0.0 < tol: adds 0
+    assert _mu(-2.0 + base, 4, 100) == -0.5        # (-2, -0.5): upper end
+
+    # w = a + (0, 1, 2, 4): three steps.  mid a+2: f = (2, 2, 1, 0) -> N = +1, hi <- a+2;
+    # mid a+1: f = (2, 1, 0, 0) -> N = -1, lo <- a+1; mid a+1.5: N = 0, hi <- a+1.5,
+    # loop = 3.  With search_cycle = 3 the midpoint (lo+hi)/2 = a+1.25 is added.
+    w = np.array([0.0, 1.0, 2.0, 4.0])
+    assert _mu(1.0 + w, 4, 100) == 2.0             # lower end a+1
+    assert _mu(1.0 + w, 4, 3) == 2.0 + 2.25        # lower end + midpoint at max_cycle
+    assert _mu(-5.0 + w, 4, 100) == -3.5           # upper end a+1.5
+    assert _mu(-5.0 + w, 4, 3) == -3.5 - 3.75      # upper end + midpoint
+    # a bracket cut short: after 2 steps (lo, hi) = (a+1, a+2), mid a+1.5
+    assert _mu(-5.0 + w, 4, 2) == -3.0 - 3.5
+
+
+# ------------------------------------------------- HL71 / BH72 / GL77 limits
+# published (C, A): Hedin-Lundqvist J. Phys. C 4, 2064 (1971); von Barth-Hedin
+# J. Phys. C 5, 1629 (1972); Gunnarsson-Lundqvist PRB 13, 4274 (1976)
+HL_BH_GL = {"hl71": (0.045, 21.0), "bh72": (0.0504, 30.0), "gl77": (0.0666, 11.4)}
+
+
+def _ec(name, rs):
+    rs = np.asarray(rs, float)
+    rho = 3.0 / (4.0 * np.pi * rs ** 3)
+    return R.xc_eps_v(rho, "lda," + name)[0] - R.xc_eps_v(rho, "lda,")[0]
+
+
+@pytest.mark.parametrize("name", sorted(HL_BH_GL))
+def test_hl_bh_gl_limits(name):
+    """eps_c = -(C/2) F(rs/A), F(x) = (1+x^3) ln(1+1/x) + x/2 - x^2 - 1/3.
+    High density (x -> 0): F = -ln x - 1/3 + 3x/2 - 3x^2/2 + O(x^3 ln x), so
+    eps_c = (C/2) ln x + C/6 - 3Cx/4 and v_c = eps_c - (rs/3) d eps_c/d rs
+    = eps_c - C/6 + Cx/4.  Low density (x -> oo), from the series of ln(1 + 1/x):
+    F = 3/(4x) - 3/(10x^2) + 1/(6x^3) - 3/(28x^4) + O(x^-5), so rs eps_c -> -3CA/8.
+    The two limits fix C and A separately."""
+    C, A = HL_BH_GL[name]
+    for x in (1e-6, 2e-6):
+        ec = _ec(name, [x * A])[0]
+        assert abs(ec - (C / 2 * math.log(x) + C / 6 - 0.75 * C * x)) < 1e-10
+        rho = np.array([3.0 / (4.0 * np.pi * (x * A) ** 3)])
+        vc = R.xc_eps_v(rho, "lda," + name)[1][0] - R.xc_eps_v(rho, "lda,")[1][0]
+        assert abs(vc - (ec - C / 6 + C * x / 4)) < 1e-10
+    for x in (50.0, 100.0):
+        F = 3 / (4 * x) - 3 / (10 * x ** 2) + 1 / (6 * x ** 3) - 3 / (28 * x ** 4)
+        assert abs(_ec(name, [x * A])[0] / (-C / 2 * F) - 1) < 1e-7
+
+
+# --------------------------------------------------------------- DIIS semantics
+def _noninteracting(nmo=6, nelec=4, seed=0):
+    rng = np.random.default_rng(seed)
+    H = rng.normal(size=(nmo, nmo))
+    H = H + H.T
+    occ = R.make_occupation(nelec, nmo)
+    dm_fn = lambda mo, w1: (mo * occ(w1)) @ mo.T
+    w, c = np.linalg.eigh(H)
+    zero = np.zeros_like(H)
+    return dict(v=np.eye(nmo), Hcore=H, dm0=dm_fn(c, w), hartree_fn=lambda d: zero,
+                xc_fn=lambda d: (0.0, zero), dm_fn=dm_fn,
+                errvec_fn=lambda d, F: d @ F - F @ d), w
+
+
+@pytest.mark.parametrize("start,space", [(1, 8), (2, 3), (0, 4)])
+def test_diis_iteration_count_without_interaction(start, space):
+    """With F independent of D the first Roothaan step is exact.  scf.py still runs the
+    fixed-point warm-up of exactly diis_start_cycle + diis_space iterations
+    (:168-172), then resets E_old = E - 1 (:173), so one DIIS iteration follows and
+    the loop stops at start + space + 1; converged = (loop != max_cycle) (:240), so
+    max_cycle = start + space + 1 reports False although nothing changed."""
+    a, w = _noninteracting()
+    nexp = start + space + 1
+    mo, w1, E, conv, nit = R.scf_diis(tol=1e-10, max_cycle=100, diis_space=space,
+                                      diis_start_cycle=start, **a)
+    assert nit == nexp and conv
+    assert np.allclose(w1, w, atol=1e-12)
+    assert abs(E - 2 * w[:2].sum()) < 1e-12          # E = Tr[H D] for 4 electrons
+    _, _, _, conv2, nit2 = R.scf_diis(tol=1e-10, max_cycle=nexp, diis_space=space,
+                                      diis_start_cycle=start, **a)
+    assert nit2 == nexp and not conv2
+    _, _, _, conv3, nit3 = R.scf_diis(tol=1e-10, max_cycle=nexp - 1, diis_space=space,
+                                      diis_start_cycle=start, **a)
+    assert nit3 == nexp - 1 and not conv3             # stopped inside the warm-up
+
+
+def test_diis_warmup_is_the_fixed_point_iteration():
+    """The warm-up iterations are Roothaan steps from the newest F: stopped at
+    max_cycle = start + space, DIIS returns E of D_9 (the density after 9 fixed-point
+    steps from the core guess), which fixed_point_scf (scf.py:54-85, E from the
+    INCOMING density) reports after 10 iterations."""
+    n, rs = 4, 1.5
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    xp = np.random.default_rng(1).uniform(0, L, (n, 3))
+    kw = dict(grid_length=0.35, dft=True, tol=0.0)
+    a = R.Solver(n, L, rs, "gth-dzv", diis=True, max_cycle=9, **kw).solve(xp)
+    b = R.Solver(n, L, rs, "gth-dzv", diis=False, max_cycle=10, **kw).solve(xp)
+    c = R.Solver(n, L, rs, "gth-dzv", diis=False, max_cycle=9, **kw).solve(xp)
+    assert abs(a[3] - b[3]) < 1e-12
+    assert abs(a[3] - c[3]) > 1e-8                    # one step fewer differs
+    assert not a[8] and not b[8]
+
+
+def test_diis_returns_the_extrapolated_step():
+    """After the first DIIS iteration (max_cycle = start + space + 1) the returned
+    mo, bands and E belong to one Roothaan step from the extrapolated F (:191-209):
+    E is the energy of the density built from the returned mo, and the bands are
+    not those of F(D) (which a restatement returning eigh(F_new) would give)."""
+    n, rs = 4, 1.5
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    xp = np.random.default_rng(4).uniform(0, L, (n, 3))
+    s = R.Solver(n, L, rs, "gth-dzv", grid_length=0.35, dft=True, tol=0.0, max_cycle=10)
+    mo, dm, bands, E = s.solve(xp)[:4]
+    st = s.last
+    sysm = s.sys
+    phi = st["phi"]
+    rho = R.grid_density(phi, dm)
+    vh, _ = R.poisson(sysm, rho)
+    eps, vxc = R.xc_eps_v(rho, "lda,vwn")
+    J = R.grid_matrix(sysm, phi, vh)
+    E_d = 0.5 * np.einsum("pq,qp", 2 * st["Hcore"] + J, dm) + np.sum(rho * eps) * sysm.dV
+    assert abs(E / R.RY - E_d) < 1e-12
+    F = st["Hcore"] + J + R.grid_matrix(sysm, phi, vxc)
+    w, u = np.linalg.eigh(st["S"])
+    v = u @ np.diag(w ** -0.5)
+    w_new = np.linalg.eigvalsh(v.T @ F @ v)
+    assert np.abs(w_new - bands / R.RY).max() > 1e-7
+
+
+# ------------------------------------------------------------ Gaussian smearing
+def test_gauss_forces_are_not_the_gradient():
+    """The force terms make -dA/dR only where A = E - sigma S_e is stationary in the
+    occupations (Fermi-Dirac with its own entropy).  The reference's Gaussian
+    occupations (hqc/tools/smearing.py:52) with the Fermi-Dirac entropy are not, so
+    the oracle and the product refuse forces there; this shows the mismatch."""
+    n, rs = 3, 1.5
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    xp = np.random.default_rng(8).uniform(0, L, (n, 3))
+    kw = dict(grid_length=0.35, dft=True, tol=1e-12, max_cycle=400, smearing=True,
+              smearing_sigma=0.1, search_tol=1e-14, search_cycle=200)
+    h = 1e-4
+    err = {}
+    for method in ("fermi", "gauss"):
+        sv = R.Solver(n, L, rs, "gth-dzv", smearing_method=method, **kw)
+        sv.solve(xp)
+        F = R._analytic_forces(sv, xp)
+        if method == "gauss":
+            with pytest.raises(NotImplementedError):
+                R.forces(sv, xp)
+        xq = xp.copy(); xq[0, 0] += h / rs
+        xm = xp.copy(); xm[0, 0] -= h / rs
+        fd = -(R.free_energy(sv, xq) - R.free_energy(sv, xm)) / (2 * h)
+        err[method] = abs(F[0, 0] - fd)
+    assert err["fermi"] < 1e-7
+    assert err["gauss"] > 1e-6
+
+
+def test_product_refuses_gauss_forces():
+    """The product checks before any device work (no GPU needed here)."""
+    from hqc_amd.pbc.forces import check_supported
+    eng = types.SimpleNamespace(smearing=True, method=1, dft=True)
+    with pytest.raises(NotImplementedError, match="Gaussian"):
+        check_supported(eng)
+    check_supported(types.SimpleNamespace(smearing=True, method=0, dft=True))
+    check_supported(types.SimpleNamespace(smearing=False, method=1, dft=False))
+    with pytest.raises(NotImplementedError, match="k-point"):
+        check_supported(types.SimpleNamespace(smearing=False, method=0, dft=False), kpt=True)
