@@ -424,6 +424,7 @@ class Engine:
             keep = w > self.ovlp_cut
             st["v"] = u * torch.where(keep, w.clamp(min=self.ovlp_cut).rsqrt(), 0.0).unsqueeze(-2)
             st["lindep"] = (~keep).to(torch.float64) * self.LINDEP_SHIFT
+            st["ovlp_w"], st["ovlp_u"] = w, u           # for the forces of the cut basis
         else:
             st["v"] = u * w.rsqrt().unsqueeze(-2)
 
@@ -513,7 +514,7 @@ class Engine:
             per += ["dS", "dT"]
         self._orthogonalizer(st, S)
         if "lindep" in st:
-            per.append("lindep")
+            per += ["lindep", "ovlp_w", "ovlp_u"]
         st["phi"] = phi
         dVs = self._dv(st, B)
         if mixed:
@@ -1051,7 +1052,8 @@ class Engine:
         Ki = _tr(st["T"], D).real
         Vep = _tr(st["V"], D).real
         Se = self.entropy(occ)
-        slot = dict(mo=mo, w1=w1, D=D, occ=occ, E=E)
+        F = st["Hcore"] + V2                                   # Fock matrix of the final D
+        slot = dict(mo=mo, w1=w1, D=D, occ=occ, E=E, F=F)
         st["slot"] = slot
         st["order"] = order
 
@@ -1060,7 +1062,6 @@ class Engine:
             out[order] = t
             return out
 
-        F = st["Hcore"] + V2                                   # Fock matrix of the final D
         st.update({k: unperm(t) for k, t in dict(mo=mo, w1=w1, D=D, occ=occ, E=E, conv=conv,
                                                  nit=nit, Ki=Ki, Vep=Vep, Vee=E2,
                                                  Se=Se, F=F).items()})

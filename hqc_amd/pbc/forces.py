@@ -36,6 +36,29 @@ def check_supported(engine, kpt=False):
         raise NotImplementedError("Hartree-Fock forces at a k-point are not implemented")
 
 
+def _cut_basis_term(st, D, F):
+    """Pulay-type term of a basis with overlap directions cut (ovlp_cut > 0).
+
+    The SCF is variational in the subspace of the overlap eigenvectors u_k with
+    s_k > ovlp_cut, V = U_k s_k^-1/2.  When a nucleus moves, that subspace turns:
+    first order perturbation gives u_c^H du_k = u_c^H dS u_k / (s_k - s_c) for a cut
+    direction c, so dE/dR gains 2 Re Tr[V^H F U_c B D'] with B = U_c^H dV and
+    V D' = D S V.  Written as Re Tr[X dS/dR], X = U N U^H with
+        N_kc = 2 (U^H S D F U)_kc / (s_k (s_k - s_c))   (k kept, c cut; else 0),
+    it enters like the energy-weighted density: W_eff = W - (X + X^H)/2.  Without a
+    cut the term is absent (the full space does not turn)."""
+    w, U = st["ovlp_w"], st["ovlp_u"]
+    cut = st["lindep"] > 0
+    S = st["S"]
+    G = U.mH @ S @ D @ F @ U                                     # rows k, columns c
+    wk, wc = w.unsqueeze(-1), w.unsqueeze(-2)
+    mask = (~cut).unsqueeze(-1) & cut.unsqueeze(-2)
+    den = torch.where(mask, wk * (wk - wc), torch.ones_like(wk * wc))
+    N = torch.where(mask, 2.0 * G / den, torch.zeros_like(G))
+    X = U @ N @ U.mH
+    return 0.5 * (X + X.mH)
+
+
 def _exchange_adjoint(engine, st):
     """Zx = -Y - c0 (D S~ D) phi for HF, (B, nao_pad, ld), each geometry with its own
     cell constants (VG, dV, c0)."""
@@ -100,6 +123,8 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
             put(dS, idx, dSg)
             put(dT, idx, dTg)
     W = (mo * (occ * w1).unsqueeze(-2)) @ mo.transpose(-1, -2)
+    if "lindep" in st:
+        W = W - _cut_basis_term(st, D, sl["F"])
     atom = c.atom_of_ao
     other = (atom[:, None] != atom[None, :]).to(f64)
     t = 2.0 * ((W * other).unsqueeze(-1) * dS - (D * other).unsqueeze(-1) * dT).sum(2)
@@ -158,6 +183,8 @@ def compute_forces_k(engine, st, kappa=10, Gmax=15):
         K.overlap_kinetic_k(st["xpb"], st["kpt"], c.lattice, c.alpha, c.coef, c.comp, S, T,
                             dS, dT)
     W = (mo * (occ * w1).unsqueeze(-2)) @ mo.mH
+    if "lindep" in st:
+        W = W - _cut_basis_term(st, D, sl["F"])
     atom = c.atom_of_ao
     other = (atom[:, None] != atom[None, :]).to(f64)
     t = 2.0 * ((W * other).conj().unsqueeze(-1) * dS

@@ -122,7 +122,7 @@ class KEngine(Engine):
         if forces:
             per += ["dS", "dT"]
         if "lindep" in st:
-            per.append("lindep")
+            per += ["lindep", "ovlp_w", "ovlp_u"]
         cb = torch.empty(B, c.g, c.g, c.g, dtype=c128, device=dev)
         K.vloc_G(st["xpb"], c.g, c.L, c.VG, cb)
         K.fft3d(cb, +1)

@@ -375,3 +375,38 @@ def test_gauss_forces_refused():
     assert torch.isfinite(pes(xp))                      # energies are fine
     with pytest.raises(NotImplementedError, match="Gaussian"):
         pes.energy_and_forces(xp)
+
+
+@pytest.mark.parametrize("name,kw,kpt", [
+    ("dft", dict(dft=True, smearing=True, smearing_sigma=0.05), None),
+    ("hf", dict(dft=False), None),
+    ("dft_k", dict(dft=True, smearing=True, smearing_sigma=0.05), [0.3, -0.2, 0.45]),
+], ids=["dft", "hf", "dft_k"])
+def test_ovlp_cut_forces_fd(name, kw, kpt):
+    """Forces of a basis with overlap directions cut (ovlp_cut = 5e-3 removes the 3
+    overlap eigenvalues 1.6e-4, 5.7e-4, 1.3e-3 of this 4 H dzvp cell; the next is
+    9.2e-3): the turning of the kept subspace adds a
+    Pulay-type term; the analytic forces equal a central finite difference
+    (1e-4 Bohr) of the free energy."""
+    from hqc_amd.pbc import make_pes
+    n, rs = 4, 1.5
+    xp, L = _xp(n, 3)
+    print(f"[ovlp_cut forces] {name}", flush=True)
+    # tol 1e-10: the shift given to the cut directions (1e4 Ha) sets the eigensolver's
+    # relative tolerance scale, so residuals below ~1e-10 are not reached
+    kw = dict(kw, grid_length=0.3, tol=1e-10, search_tol=1e-13, search_cycle=200,
+              max_cycle=300, ovlp_cut=5e-3)
+    pes = make_pes(n, L, rs, "gth-dzvp", gamma=kpt is None, **kw)
+    k = None if kpt is None else np.array(kpt)
+    A, F = pes.free_energy_and_forces(torch.tensor(xp[None], device="cuda"), k)
+    st = pes.solver.last
+    assert int((st["lindep"] > 0).sum()) >= 1 and bool(st["conv"].all())
+    h = 1e-4 / rs
+    for a, c in ((0, 0), (2, 1)):
+        xq = xp.copy(); xq[a, c] += h
+        xm = xp.copy(); xm[a, c] -= h
+        Aq = pes.free_energy_and_forces(torch.tensor(xq[None], device="cuda"), k)[0].item()
+        Am = pes.free_energy_and_forces(torch.tensor(xm[None], device="cuda"), k)[0].item()
+        fd = -(Aq - Am) / (2 * 1e-4) / 2
+        print(f"  atom {a} dir {c}: analytic {F[0, a, c].item():.9f} FD {fd:.9f}", flush=True)
+        assert abs(F[0, a, c].item() - fd) < 2e-6
