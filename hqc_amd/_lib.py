@@ -32,6 +32,8 @@ _SIGS = {
     "hqc_ao_vmat_workspace": [_I, _I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
     "hqc_ao_vmat_f64": [_P, _P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_ao_vmat_f32": [_P, _P, _P, _I, _I, _I, _D, _P, _P, ctypes.c_size_t, _P],
+    "hqc_ao_vmat_scratch_bytes": [],
+    "hqc_ao_vmat_f64_noalloc": [_P, _P, _P, _I, _I, _I, _D, _P, _P],
     "hqc_phi_split_f16": [_P, ctypes.c_long, _D, _P, _P],
     "hqc_orb_density_h2": [_P, _P, _P, _I, _I, _I, _I, _D, _P, _P],
     "hqc_ao_vmat_h2_workspace": [_I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
@@ -73,7 +75,7 @@ _SIGS = {
 }
 
 
-_RESTYPES = {"hqc_eigh_lds_bytes": ctypes.c_size_t}
+_RESTYPES = {"hqc_eigh_lds_bytes": ctypes.c_size_t, "hqc_ao_vmat_scratch_bytes": ctypes.c_size_t}
 
 
 def lib():
@@ -229,6 +231,23 @@ def ao_vmat(phi, v, dV, V, work, bidx=None):
     _call(name, _ptr(phi, None, "phi"), _ptr(bi, I32, "bidx"), _ptr(v, F64, "v"), B, nao_pad, ld,
           float(dV), _ptr(V, F64, "V"), _ptr(work, None, "work"),
           work.numel() * work.element_size(), _stream())
+
+
+def ao_vmat_noalloc(phi, v, dV, V, bidx=None):
+    """hqc_ao_vmat_f64 without a workspace (the library's own device array holds the
+    partial tiles): for callers that cannot allocate.  Raises on sizes beyond it."""
+    nao_pad, ld = phi.shape[1], phi.shape[2]
+    B = v.shape[0]
+    assert v.shape == (B, ld) and V.shape == (B, nao_pad, nao_pad)
+    bi = _bidx(phi, bidx, B)
+    rc = lib().hqc_ao_vmat_f64_noalloc(_ptr(phi, F64, "phi"), _ptr(bi, I32, "bidx"),
+                                       _ptr(v, F64, "v"), B, nao_pad, ld, float(dV),
+                                       _ptr(V, F64, "V"), _stream())
+    if rc == 3:
+        raise ValueError(f"hqc_ao_vmat_f64_noalloc: {B} geometries of {nao_pad} AOs need more "
+                         f"than the {lib().hqc_ao_vmat_scratch_bytes() >> 20} MiB scratch array")
+    if rc != 0:
+        raise RuntimeError(f"hqc_ao_vmat_f64_noalloc failed: {_ERRORS.get(rc, rc)}")
 
 
 def phi_split_f16(phi, scale, out):

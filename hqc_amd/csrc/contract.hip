@@ -11,6 +11,8 @@
 //           f64: C[i][j] in lane j + 16*(i%4), element i/4.
 #include "common.h"
 
+#include <mutex>
+
 __device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -901,13 +903,17 @@ extern "C" int hqc_ao_vmat_workspace(int B, int nao_pad, int ld, int prec, size_
   return HQC_OK;
 }
 
+// max_parts > 0 caps the grid chunks per geometry (each one a partial tile set
+// in `work`); 0 keeps vmat_shape's choice.
 template <typename T>
 static int launch_vmat(const T* phi, const int* bidx, const double* v, int B, int nao_pad, int ld, double dV,
-                       double* V, void* work, size_t work_bytes, int prec, void* stream) {
+                       double* V, void* work, size_t work_bytes, int prec, void* stream,
+                       int max_parts = 0) {
   if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63)) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
   int KC, ntiles, ngroups, nwg;
   vmat_shape(B, nao_pad, ld, prec, KC, ntiles, ngroups, nwg);
+  if (max_parts > 0 && nwg > max_parts) nwg = max_parts;
   const size_t need = (size_t)B * ntiles * nwg * 256 * sizeof(double);
   if (!work || work_bytes < need) return HQC_E_ARG;
   hipStream_t s = (hipStream_t)stream;
@@ -963,6 +969,63 @@ extern "C" int hqc_ao_vmat_f32(const float* phi, const int* bidx, const double* 
                                int nao_pad, int ld, double dV, double* V, void* work,
                                size_t work_bytes, void* stream) {
   return launch_vmat<float>(phi, bidx, v, B, nao_pad, ld, dV, V, work, work_bytes, 32, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form of hqc_ao_vmat_f64, for callers that cannot allocate
+// device memory (code under HIP graph capture, serving loops).  The partial
+// tiles that the caller's workspace holds in hqc_ao_vmat_f64 live here in a
+// file-scope device array of HQC_VMAT_SCRATCH_BYTES.  Host code never passes
+// the array's name to a kernel (on the host it is only a stand-in for the
+// symbol): its device address is looked up once per device with that device
+// current, and that address is what the kernels receive.
+//
+// Every partial tile slot the reduction reads is written by the product
+// kernel of the same call (an empty chunk range writes zeros), so nothing is
+// cleared between calls.  Calls that overlap on one device (two streams, two
+// host threads) share the array and race.
+// ---------------------------------------------------------------------------
+#define HQC_VMAT_SCRATCH_BYTES ((size_t)32 << 20)
+#define HQC_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[HQC_VMAT_SCRATCH_BYTES / sizeof(double)];
+
+static std::once_flag g_scratch_once[HQC_MAX_DEVICES];
+static double* g_scratch_ptr[HQC_MAX_DEVICES];
+static hipError_t g_scratch_err[HQC_MAX_DEVICES];
+
+// Device address of `scratch` on the current device (looked up on its first use there).
+static int scratch_address(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return HQC_E_HIP;
+  if (dev < 0 || dev >= HQC_MAX_DEVICES) return HQC_E_SIZE;
+  std::call_once(g_scratch_once[dev], [dev] {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[dev] = e;
+    g_scratch_ptr[dev] = e == hipSuccess ? static_cast<double*>(p) : nullptr;
+  });
+  if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev]) return HQC_E_HIP;
+  *out = g_scratch_ptr[dev];
+  return HQC_OK;
+}
+
+extern "C" size_t hqc_ao_vmat_scratch_bytes(void) { return HQC_VMAT_SCRATCH_BYTES; }
+
+extern "C" int hqc_ao_vmat_f64_noalloc(const double* phi, const int* bidx, const double* v, int B,
+                                       int nao_pad, int ld, double dV, double* V, void* stream) {
+  if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63)) return HQC_E_ARG;
+  if (B == 0) return HQC_OK;
+  int KC, ntiles, ngroups, nwg;
+  vmat_shape(B, nao_pad, ld, 64, KC, ntiles, ngroups, nwg);
+  // one partial tile set (every upper-triangle tile of every geometry) per grid chunk
+  const size_t per_part = (size_t)B * ntiles * 256 * sizeof(double);
+  if (per_part > HQC_VMAT_SCRATCH_BYTES) return HQC_E_SIZE;
+  const size_t cap = HQC_VMAT_SCRATCH_BYTES / per_part;
+  double* part = nullptr;
+  const int rc = scratch_address(&part);
+  if (rc != HQC_OK) return rc;
+  return launch_vmat<double>(phi, bidx, v, B, nao_pad, ld, dV, V, part, HQC_VMAT_SCRATCH_BYTES, 64,
+                             stream, cap < (size_t)nwg ? (int)cap : nwg);
 }
 
 // ---------------------------------------------------------------------------

@@ -159,6 +159,21 @@ int hqc_ao_vmat_f64(const double* phi, const int* bidx, const double* v, int B, 
  * block, FP64 accumulation across blocks; V written in FP64. */
 int hqc_ao_vmat_f32(const float* phi, const int* bidx, const double* v, int B, int nao_pad,
                     int ld, double dV, double* V, void* work, size_t work_bytes, void* stream);
+/* Allocation-free form of hqc_ao_vmat_f64 (no counterpart in the reference):
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Same arguments and result without the
+ * workspace; the partial tiles live in a device array of the library of
+ * hqc_ao_vmat_scratch_bytes() (32 MiB) per device, held until the process
+ * ends.  Results equal hqc_ao_vmat_f64's bit for bit while
+ * B * n_tiles * n_chunks * 2 KiB fits in it (n_tiles = (nao_pad/16)(nao_pad/16+1)/2,
+ * n_chunks as hqc_ao_vmat_workspace chooses); beyond that fewer grid chunks per
+ * geometry are taken (the same sum in another order); HQC_E_SIZE when even one
+ * chunk per geometry does not fit (B * n_tiles * 2 KiB > 32 MiB).  Calls that
+ * overlap on one device (several streams or host threads) share the array and
+ * race; calls in sequence on one stream do not. */
+size_t hqc_ao_vmat_scratch_bytes(void);
+int hqc_ao_vmat_f64_noalloc(const double* phi, const int* bidx, const double* v, int B,
+                            int nao_pad, int ld, double dV, double* V, void* stream);
 /* Split-precision form of the same contraction (nao_pad <= 128).
  * hqc_phi_split_f16: out[i] = (hi, lo) FP16 pair (hi in the low half) with
  *   scale*phi[i] = hi + lo to 22 significant bits; scale = 2^k, |scale phi| <= 2^14.
