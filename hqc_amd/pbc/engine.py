@@ -173,6 +173,7 @@ class Engine:
     # the warm-up density from the first 3 digits (2^-21 per value, a fixed basis
     # with no residual floor; 0: the split-FP16 form)
     Q8_WARMUP_DIGITS = 3
+    mat_dtype = torch.float64        # n_ao x n_ao matrices (complex128 at a k-point)
 
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
@@ -777,7 +778,7 @@ class Engine:
         inc = ~direct & sel("va_set") & (eps * dvmax <= tol_e)
         low = direct | inc
         dVs = self._dv(st, na)
-        V2 = torch.empty(na, c.nao, c.nao, dtype=torch.float64, device=c.device)
+        V2 = torch.empty(na, c.nao, c.nao, dtype=self.mat_dtype, device=c.device)
         (idx, n), high_sub, _ = self._split(low)
         if n:
             il = None if idx is None else idx.long()

@@ -32,8 +32,6 @@ def check_supported(engine, kpt=False):
     if engine.smearing and engine.method == 1:
         raise NotImplementedError("forces with Gaussian smearing: E - sigma*S_e is not "
                                   "stationary in the reference's Gaussian occupations")
-    if kpt and not engine.dft:
-        raise NotImplementedError("Hartree-Fock forces at a k-point are not implemented")
 
 
 def _cut_basis_term(st, D, F):
@@ -89,6 +87,42 @@ def _exchange_adjoint(engine, st):
         D = st["slot"]["D"][b]
         M = D @ st["Sgrid"][b] @ D
         Zx[b, :c.nao, :c.N] = -Y - cst.c0 * (M @ phi[b])
+    return Zx
+
+
+def _exchange_adjoint_k(engine, st):
+    """Zx for HF at one k-point per entry, stacked (B, 2 nao_pad, ld): with the pair
+    densities n_ji = psi_j psi_i^* and u_ji = dV N IFFT[VG FFT(n_ji)],
+        dE_x = -dV Re sum_r conj(dphi_p) Y_p,   Y_p = sum_j f_j conj(C_pj) G_j + c0 (M^T phi)_p,
+        G_j = sum_i f_i psi_i u_ji,  M = D S~ D,  S~ = dV phi^H phi,
+    and Zx = -(Re Y; Im Y) pairs with the derivatives of (Re phi; Im phi).  At Gamma
+    it reduces to _exchange_adjoint's Y."""
+    c = engine.c
+    B, n, p, N, g = st["B"], c.nao, c.nao_pad, c.N, c.g
+    dev = c.device
+    mo, occ, D = st["slot"]["mo"], st["slot"]["occ"], st["slot"]["D"]
+    phi = st["phi"]
+    VG = c.VG.reshape(1, g, g, g)
+    c0 = 4 * math.pi / c.L * EXCHANGE_G0_CONST
+    Zx = torch.zeros(B, 2 * p, c.ld, dtype=torch.float64, device=dev)
+    for b in range(B):
+        phic = torch.complex(phi[b, :n, :N], phi[b, p:p + n, :N])
+        keep = torch.nonzero(occ[b] > 1e-14).flatten()
+        C = mo[b][:, keep]
+        f = occ[b][keep].to(torch.complex128)
+        psi = C.transpose(0, 1) @ phic                      # (m, N)
+        G = torch.zeros_like(psi)
+        for i in range(psi.shape[0]):
+            pair = (psi * psi[i].conj()).reshape(-1, g, g, g).contiguous()
+            K.fft3d(pair, -1)
+            pair *= VG
+            K.fft3d(pair, +1)
+            G += f[i] * psi[i] * pair.reshape(-1, N) * c.dV
+        Y = C.conj() @ (f[:, None] * G)
+        M = D[b] @ st["Sgrid"][b] @ D[b]
+        Y += c0 * (M.transpose(0, 1) @ phic)
+        Zx[b, :n, :N] = -Y.real
+        Zx[b, p:p + n, :N] = -Y.imag
     return Zx
 
 
@@ -164,7 +198,7 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
 
 
 def compute_forces_k(engine, st, kappa=10, Gmax=15):
-    """-dA/dR at one k-point per batch entry (DFT).  Complex S, T, D, W enter as
+    """-dA/dR at one k-point per batch entry (DFT or HF).  Complex S, T, D, W enter as
     2 Re sum_q [conj(W_pq) dS_pq - conj(D_pq) dT_pq]; the grid adjoint uses the
     stacked rows (Re phi; Im phi) with X = M rows, M = [[Re D, Im D], [-Im D, Re D]]."""
     check_supported(engine, kpt=True)
@@ -199,7 +233,8 @@ def compute_forces_k(engine, st, kappa=10, Gmax=15):
     P = torch.empty(B, c.n, 3, c.g, c.na, 3, dtype=c128, device=dev)
     K.gto_tables_k(st["xpb"], st["kpt"], c.g, c.L, c.tmax_img, c.alpha, 3, P)
     fp = torch.empty(B, c.n, K.force_blocks(c.g), 3, dtype=f64, device=dev)
-    K.grid_forces_k(P, c.alpha, c.coef, c.comp, c.nao_pad, c.ld, X, veff, None, c.dV, fp)
+    Zx = None if engine.dft else _exchange_adjoint_k(engine, st)
+    K.grid_forces_k(P, c.alpha, c.coef, c.comp, c.nao_pad, c.ld, X, veff, Zx, c.dV, fp)
     F += fp.sum(2)
     dE = torch.empty(B, c.n, 3, dtype=f64, device=dev)
     K.vloc_force(st["xpb"], c.g, c.L, c.VG, rg, dE)

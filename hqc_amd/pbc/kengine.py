@@ -16,8 +16,10 @@ MFMA contraction:
                              (Re C; -Im C) and (Im C; Re C).
 The SCF (DIIS with the reference's complex symmetric B, no conjugate) is the
 Gamma engine's; eigh is torch's complex Hermitian solver applied to the
-Hermitian part of v^H F v, as jnp.linalg.eigh does (symmetrize_input).  The
-k-point path runs in FP64.
+Hermitian part of v^H F v, as jnp.linalg.eigh does (symmetrize_input).
+precision='mixed' takes the Gamma engine's defect correction with FP32 stacked rows
+(complex64 grid work): V increments and the warm-up density on FP32 MFMA, FP64
+anchors, accurate density, exchange and the complex128 eigensolve.
 """
 import math
 
@@ -30,11 +32,16 @@ from .engine import EXCHANGE_G0_CONST, Engine, _round_up
 class KEngine(Engine):
     """Batched SCF at k-points (one k per batch entry), complex Hermitian."""
 
+    # the mixed path's low-precision operands are FP32 stacked rows (complex64 grid
+    # work): 2 nao_pad rows exceed the split-FP16 and int8 kernels' 128, so V
+    # increments and the warm-up density run on FP32 MFMA, the accurate density in FP64
+    MIXED_SPLIT16 = False
+    MIXED_Q8 = False
+    mat_dtype = torch.complex128
+
     def __init__(self, cell, **kw):
-        if kw.get("precision", "fp64") != "fp64":
-            raise ValueError("the k-point path runs in FP64 only (precision='fp64')")
         super().__init__(cell, **kw)
-        self._kwork = None
+        self._kwork = {}
 
     # ------------------------------------------------------------- contractions
     def vmat_dv(self, phi, v, dV, bidx=None, vmax=None):
@@ -43,12 +50,14 @@ class KEngine(Engine):
         c = self.c
         B = v.shape[0]
         n2 = 2 * c.nao_pad
-        need = K.vmat_workspace(B, n2, c.ld, 64)
-        if self._kwork is None or self._kwork.numel() * 8 < need:
-            self._kwork = torch.empty(max((need + 7) // 8, 1), dtype=torch.float64,
-                                      device=c.device)
+        prec = 64 if phi.dtype == torch.float64 else 32
+        need = K.vmat_workspace(B, n2, c.ld, prec)
+        work = self._kwork.get(prec)
+        if work is None or work.numel() * 8 < need:
+            work = torch.empty(max((need + 7) // 8, 1), dtype=torch.float64, device=c.device)
+            self._kwork[prec] = work
         M = torch.empty(B, n2, n2, dtype=torch.float64, device=c.device)
-        K.ao_vmat(phi, v.contiguous(), dV, M, self._kwork, bidx)
+        K.ao_vmat(phi, v.contiguous(), dV, M, work, bidx)
         n, p = c.nao, c.nao_pad
         rr, ii = M[:, :n, :n], M[:, p:p + n, p:p + n]
         ri, ir = M[:, :n, p:p + n], M[:, p:p + n, :n]
@@ -119,6 +128,17 @@ class KEngine(Engine):
         st["gidx"] = torch.arange(B, dtype=torch.int32, device=dev)
         per = ["xpb", "kpt", "Lb", "dVb", "S", "T", "v", "vloc", "V", "Hcore"]
         per.append("gidx" if self.dft else "phi")
+        if self.precision == "mixed":
+            # FP32 stacked rows (complex64 AO values) and the FP64 anchors of V
+            st["phi_lp"] = phi.float()
+            st["va"] = torch.zeros(B, c.ld, dtype=f64, device=dev)
+            st["Va"] = torch.zeros(B, c.nao, c.nao, dtype=c128, device=dev)
+            st["vmax"] = torch.zeros(B, dtype=f64, device=dev)
+            st["va_set"] = torch.zeros(B, dtype=torch.bool, device=dev)
+            st["err"] = torch.full((B,), float("inf"), dtype=f64, device=dev)
+            per += ["va", "Va", "vmax", "va_set", "err"]
+            if not self.dft:
+                per.append("phi_lp")
         if forces:
             per += ["dS", "dT"]
         if "lindep" in st:

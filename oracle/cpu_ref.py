@@ -1084,8 +1084,8 @@ def potential_energy(x, kappa, G, L, rs):
 # Forces of the Mermin free energy A = E - sigma S_e + V_pp (not in the reference)
 # --------------------------------------------------------------------------
 def forces(solver, xp, kappa=10, Gmax=15):
-    """Analytic -dA/dR (n, 3) in Ha/Bohr after solver.solve(xp[, kpt]) (Gamma, or a
-    k-point for DFT; complex matrices enter as 2 Re Tr[conj(W) dS] etc.).
+    """Analytic -dA/dR (n, 3) in Ha/Bohr after solver.solve(xp[, kpt]) (Gamma or one
+    k-point, DFT or HF; complex matrices enter as 2 Re Tr[conj(W) dS] etc.).
 
     A = E_elec - sigma S_e + E_pp (Hartree).  Terms: kinetic -Tr[dT D], Pulay
     +Tr[W dS] (W = C f eps C^T), grid adjoint dV sum_r Z_p grad phi_p with
@@ -1093,6 +1093,7 @@ def forces(solver, xp, kappa=10, Gmax=15):
     structure-factor derivative of v_loc, and the Ewald term.
     Pinned by: central finite differences of A, for aufbau occupations and Fermi
     smearing only (tests/test_oracle_physics.py test_forces_match_finite_differences,
+    tests/test_oracle_pins.py test_hf_kpoint_forces_match_finite_differences,
     tests/test_oracle_gga.py test_gga_scf_forces_match_finite_differences and
     test_kpoint_forces_match_finite_differences).
 
@@ -1116,8 +1117,6 @@ def _analytic_forces(solver, xp, kappa=10, Gmax=15):
     n, nloc = s.n, s.basis.n_ao_atom
     dm, occ, w1, mo, phi = st["dm"], st["occ"], st["w1"], st["mo"], st["phi"]
     k = st.get("kpt")
-    if k is not None and not solver.dft:
-        raise NotImplementedError("Hartree-Fock forces at a k-point")
     _, _, dS, dT = overlap_kinetic(s, xpb, k, deriv=True)
     W = (mo * (occ * w1)) @ np.conj(mo).T
     atom = np.repeat(np.arange(n), nloc)
@@ -1136,21 +1135,27 @@ def _analytic_forces(solver, xp, kappa=10, Gmax=15):
     X = dm @ np.conj(phi)
     Z = 2 * veff[None, :] * X
     if not solver.dft:
+        # exchange: E_x = -1/4 sum_ij f_i f_j dV sum_r conj(n_ji) u_ji, n_ji = psi_j psi_i^*,
+        # u = dV N IFFT[VG FFT(n)], plus -1/4 c0 Tr[S~ D S~ D], S~ = dV conj(phi) phi^T;
+        # dE = -dV Re sum conj(dphi_p) Y_p, Y_p = sum_j f_j conj(C_pj) G_j,
+        # G_j = sum_i f_i psi_i u_ji (and -c0 (M^T phi)_p, M = D S~ D, for G = 0)
         g = s.g
         c0 = 4 * np.pi / s.L * EXCHANGE_G0_CONST
         keep = occ > 0
         C, f = mo[:, keep], occ[keep]
-        psi = C.T @ phi
-        Y = np.zeros_like(phi)
+        psi = C.T @ phi                                   # psi_j = sum_p C_pj phi_p
+        Gm = np.zeros_like(psi)
         for i in range(len(f)):
-            pair = psi * psi[i][None, :]
+            pair = psi * np.conj(psi[i])[None, :]
             u = np.fft.ifftn(np.fft.fftn(pair.reshape(-1, g, g, g), axes=(1, 2, 3))
-                             * s.VG[None], axes=(1, 2, 3)).real.reshape(len(f), -1) * s.dV * s.N
-            Gi = np.einsum("j,jr,jr->r", f, psi, u)
-            Y += f[i] * np.outer(C[:, i], Gi)
-        St = phi @ phi.T * s.dV
+                             * s.VG[None], axes=(1, 2, 3)).reshape(len(f), -1) * s.dV * s.N
+            if k is None:
+                u = u.real
+            Gm += f[i] * psi[i][None, :] * u
+        Y = np.conj(C) @ (f[:, None] * Gm)
+        St = np.conj(phi) @ phi.T * s.dV
         M = dm @ St @ dm
-        Z = Z - Y - c0 * (M @ phi)
+        Z = Z - np.conj(Y) - c0 * (M @ np.conj(phi))
     gphi = ao_gradients(s, xpb, k)
     for c in range(3):
         np.add.at(F[:, c], atom, s.dV * np.einsum("pr,pr->p", Z, gphi[c]).real)

@@ -228,15 +228,17 @@ def test_kpoint_solver_parity(case):
         assert np.allclose(out[1][b].cpu().numpy(), ref[1], atol=1e-6)
 
 
-@pytest.mark.parametrize("xc", ["lda,vwn", "pbe,pbe"])
+@pytest.mark.parametrize("xc", ["lda,vwn", "pbe,pbe", "hf"])
 def test_kpoint_forces_vs_oracle(xc):
     from hqc_amd.pbc import make_pes
     n, rs = 4, 1.5
     xp, L = _xp(n, 6)
     kpt = np.array([0.3, -0.2, 0.45])
     print(f"[k forces] {xc}", flush=True)
-    kw = dict(dft=True, xc=xc, smearing=True, smearing_sigma=0.1, tol=1e-11, search_tol=1e-13,
+    kw = dict(dft=xc != "hf", smearing=True, smearing_sigma=0.1, tol=1e-11, search_tol=1e-13,
               search_cycle=200, max_cycle=300)
+    if xc != "hf":
+        kw["xc"] = xc
     s = R.Solver(n, L, rs, "gth-dzv", grid_length=0.3, gamma=False, **kw)
     s.solve(xp, kpt)
     Fref = R.forces(s, xp)
@@ -410,3 +412,38 @@ def test_ovlp_cut_forces_fd(name, kw, kpt):
         fd = -(Aq - Am) / (2 * 1e-4) / 2
         print(f"  atom {a} dir {c}: analytic {F[0, a, c].item():.9f} FD {fd:.9f}", flush=True)
         assert abs(F[0, a, c].item() - fd) < 2e-6
+
+
+@pytest.mark.parametrize("name,kw", [
+    ("lda", dict(dft=True, smearing=True, smearing_sigma=0.0316681)),
+    ("pbe", dict(dft=True, xc="pbe,pbe", smearing=True, smearing_sigma=0.0316681)),
+    # HF at sigma 0.0317 oscillates to max_cycle in FP64 as well for some entries
+    ("hf", dict(dft=False, smearing=True, smearing_sigma=0.1, max_cycle=300)),
+], ids=["lda", "pbe", "hf"])
+def test_mixed_kpoints_match_fp64(name, kw):
+    """precision='mixed' at k-points (FP32 stacked rows: V increments and the warm-up
+    density on FP32 MFMA, FP64 anchors) on 2 geometries x 2 k-points against FP64 at
+    the accuracy bar (1 uHa/atom, 1e-5 Ha/Bohr, 1e-6 Ha bands)."""
+    from hqc_amd.pbc import make_pes
+    print(f"[mixed k] {name}", flush=True)
+    n, rs = 16, 1.25
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    Lb = L * rs
+    x2 = np.random.default_rng(23).uniform(0, L, (2, n, 3))
+    xps = np.repeat(x2, 2, 0)
+    kpts = np.array([[0.25, 0.25, 0.25], [0.25, -0.25, 0.25]] * 2) * 2 * np.pi / Lb
+    kw = dict(kw, grid_length=0.2, tol=1e-8, gamma=False)
+    p64 = make_pes(n, L, rs, "gth-dzv", precision="fp64", **kw)
+    pmx = make_pes(n, L, rs, "gth-dzv", precision="mixed", **kw)
+    E64, F64 = p64.energy_and_forces(xps, kpts)
+    Emx, Fmx = pmx.energy_and_forces(xps, kpts)
+    st = pmx.solver.last
+    dE = (E64 - Emx).abs().max().item() / 2 / n
+    dF = (F64 - Fmx).abs().max().item()
+    dw = (p64.solver.last["w1"] - st["w1"]).abs().max().item()
+    print(f"  dE {dE:.1e} Ha/atom, dF {dF:.1e} Ha/Bohr, dbands {dw:.1e} Ha, "
+          f"stats {pmx.solver.engine.stats}, iterations {st['nit'].tolist()} "
+          f"(FP64 {p64.solver.last['nit'].tolist()})", flush=True)
+    assert bool(st["conv"].all()) and bool(p64.solver.last["conv"].all())
+    assert dE <= 1e-6 and dF <= 1e-5 and dw <= 1e-6
+    assert pmx.solver.engine.stats["v32"] > 0 and pmx.solver.engine.stats["rho32"] > 0

@@ -187,6 +187,19 @@ def test_kpoint_mesh_workload_vs_oracle():
     dF = np.abs(F[0].cpu().numpy() - Fref).max()
     print(f"  k 0: |dE| {dE:.2e} Ry, |dbands| {dw:.2e} Ha, |dF| {dF:.2e} Ha/Bohr", flush=True)
     assert dE < 1e-8 and dw < 1e-7 and dF < 1e-6
+    # the mixed path (FP32 stacked rows, complex64 grid work) on the same mesh against
+    # the oracle at the accuracy bar of the metric
+    pmx = make_pes(n, L, rs, "gth-dzv", gamma=False, precision="mixed", **CFG5)
+    Emx, Fmx = pmx.energy_and_forces(x8, kpts)
+    smx = pmx.solver.last
+    _log(f"GPU mixed 8 k-points, iterations {smx['nit'].tolist()}", t0)
+    assert bool(smx["conv"].all())
+    dE = abs(Emx[0].item() - Eref) / R.RY / n
+    dw = np.abs(smx["w1"][0].cpu().numpy() - o[2] / R.RY).max()
+    dF = np.abs(Fmx[0].cpu().numpy() - Fref).max()
+    print(f"  mixed k 0 vs oracle: |dE| {dE:.2e} Ha/atom, |dbands| {dw:.2e} Ha, "
+          f"|dF| {dF:.2e} Ha/Bohr", flush=True)
+    assert dE <= 1e-6 and dw <= 1e-6 and dF <= 1e-5
 
 
 def _hf_batch(ng, seed=5):

@@ -196,5 +196,28 @@ def test_product_refuses_gauss_forces():
         check_supported(eng)
     check_supported(types.SimpleNamespace(smearing=True, method=0, dft=True))
     check_supported(types.SimpleNamespace(smearing=False, method=1, dft=False))
-    with pytest.raises(NotImplementedError, match="k-point"):
-        check_supported(types.SimpleNamespace(smearing=False, method=0, dft=False), kpt=True)
+    check_supported(types.SimpleNamespace(smearing=False, method=0, dft=False), kpt=True)
+
+
+# --------------------------------------------------------------- HF at a k-point
+@pytest.mark.parametrize("basis", ["gth-szv", "gth-dzv"])
+def test_hf_kpoint_forces_match_finite_differences(basis):
+    """Exchange part of the grid adjoint for complex Bloch orbitals (pair densities
+    psi_j psi_i^* and the G = 0 term with S~ = dV conj(phi) phi^T) against a central
+    finite difference of the free energy at a general k-point; the Gamma-point
+    restriction of the same code is pinned by test_forces_match_finite_differences."""
+    n, rs = 3, 1.5
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    xp = np.random.default_rng(8).uniform(0, L, (n, 3))
+    kpt = np.array([0.31, -0.17, 0.45])
+    sv = R.Solver(n, L, rs, basis, grid_length=0.35, dft=False, tol=1e-12, max_cycle=400,
+                  smearing=True, smearing_sigma=0.1, search_tol=1e-14, search_cycle=200,
+                  gamma=False)
+    sv.solve(xp, kpt)
+    F = R.forces(sv, xp)
+    h = 1e-4
+    for a, c in ((0, 0), (1, 2)):
+        xq = xp.copy(); xq[a, c] += h / rs
+        xm = xp.copy(); xm[a, c] -= h / rs
+        fd = -(R.free_energy(sv, xq, kpt=kpt) - R.free_energy(sv, xm, kpt=kpt)) / (2 * h)
+        assert abs(F[a, c] - fd) < 1e-7, (a, c, F[a, c], fd)
