@@ -360,6 +360,94 @@ def summarize_pmc(d):
     return "\n".join(lines)
 
 
+# ---------------------------------------------------------------------------
+# Other workloads (non-default; --workload): the uneven Hartree-Fock PES batch, one
+# 128 H gth-dzvp Hartree-Fock cell and the 54 H PBE 2x2x2 k-mesh.  Each prints one
+# JSON line of the same shape with its own metric; `value` is again whole-job
+# throughput with the positions already on the GPU.
+# ---------------------------------------------------------------------------
+def hf_batch_workload(B, seed=0):
+    """B geometries, atom counts uniform from {16, 32, 54, 64}, rs uniform in [1, 1.5]
+    Bohr, protons uniform in the uniform-gas cell (rs units), from default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.choice([16, 32, 54, 64], B)
+    rss = rng.uniform(1.0, 1.5, B)
+    xps = [rng.uniform(0, (4 / 3 * math.pi * n) ** (1 / 3), (n, 3)) for n in ns]
+    return ns, rss, xps
+
+
+def run_workload(a):
+    import torch
+    from hqc_amd.pbc import make_pes, make_pes_batch
+    out = {"n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "data": "synthetic: uniform-random protons"}
+    if a.workload == "hf-batch":
+        ns, rss, xps = hf_batch_workload(a.batch)
+        pes = make_pes_batch("gth-dzv", grid_length=GRID_LENGTH, dft=False, tol=TOL,
+                             precision=a.path, workers=a.workers)
+        run = lambda: pes.energy_and_forces(xps, rss)
+        units = len(xps)
+        out.update(metric="energy+force geometries/sec, uneven HF PES batch (gth-dzv)",
+                   unit="geometries/s",
+                   config={"workload": f"{units} geometries, n in {{16,32,54,64}}, rs in [1,1.5], "
+                                       "restricted HF at Gamma, gth-dzv, grid spacing 0.16 Bohr, "
+                                       f"tol {TOL}, energy+forces", "path": a.path,
+                           "workers": a.workers})
+    elif a.workload == "hf-128":
+        n, rs = 128, 1.25
+        L = (4 / 3 * math.pi * n) ** (1 / 3)
+        xp = torch.tensor(np.random.default_rng(1).uniform(0, L, (1, n, 3)), device="cuda")
+        pes = make_pes(n, L, rs, "gth-dzvp", grid_length=GRID_LENGTH, dft=False, tol=TOL,
+                       ovlp_cut=1e-4, max_cycle=200, precision=a.path)
+        run = lambda: pes.energy_and_forces(xp)
+        units = 1
+        out.update(metric="energy+force geometries/sec, 128 H gth-dzvp HF (640 AOs, 65^3)",
+                   unit="geometries/s",
+                   config={"workload": "128 H rs 1.25 gth-dzvp restricted HF, 65^3, full "
+                                       "exchange, ovlp_cut 1e-4, seed 1, energy+forces",
+                           "path": a.path})
+    elif a.workload == "kmesh":
+        n, rs = 54, 1.4
+        L = (4 / 3 * math.pi * n) ** (1 / 3)
+        xp = np.random.default_rng(0).uniform(0, L, (n, 3))
+        mp = np.array([[i, j, k] for i in (-0.25, 0.25) for j in (-0.25, 0.25)
+                       for k in (-0.25, 0.25)])
+        kpts = torch.tensor(mp * 2 * np.pi / (L * rs), device="cuda")
+        x8 = torch.tensor(np.repeat(xp[None], 8, 0), device="cuda")
+        pes = make_pes(n, L, rs, "gth-dzv", grid_length=0.178, dft=True, xc="pbe,pbe",
+                       smearing=True, smearing_sigma=1.0 / (157888.088922572 / 5000.0) / 2.0,
+                       tol=TOL, search_tol=1e-12, search_cycle=200, max_cycle=300,
+                       gamma=False, precision=a.path)
+        run = lambda: pes.energy_and_forces(x8, kpts)
+        units = 8
+        out.update(metric="energy+force k-points/sec, 54 H PBE 2x2x2 Monkhorst-Pack mesh",
+                   unit="k-points/s",
+                   config={"workload": "54 H rs 1.4 gth-dzv PBE Fermi 5000 K, 2x2x2 MP mesh "
+                                       "of one geometry, 49^3, energy+forces", "path": a.path})
+    else:
+        raise SystemExit(f"unknown workload {a.workload}")
+    for _ in range(a.warmup):
+        run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res = run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(a.steps, 1)
+    if a.workload == "hf-batch":
+        info = res[2]
+        out["config"]["converged"] = int(sum(info["converged"]))
+        out["config"]["n_iter_mean"] = float(np.mean(info["n_iter"]))
+        out["config"]["buckets"] = len(info["buckets"])
+    else:
+        st = pes.solver.last
+        out["config"]["n_iter"] = st["nit"].tolist()
+        out["config"]["converged"] = bool(st["conv"].all())
+    out.update(value=units / dt, ms_per_step=dt * 1e3,
+               dtype="f64" if a.path == "fp64" else "f64+f32")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -374,6 +462,12 @@ def main():
                     help="skip the FP64 batched and plain runs (accuracy and speedups)")
     ap.add_argument("--kernel-probe", default=None,
                     help="vmat_f64|vmat_f32|vmat_h2|orb_density|copy: launch that kernel only")
+    ap.add_argument("--workload", default="metric",
+                    choices=["metric", "hf-batch", "hf-128", "kmesh"],
+                    help="metric (default): the 54 H LDA batch; the others are the uneven "
+                         "HF PES batch (--batch geometries), 128 H dzvp HF and the PBE k-mesh")
+    ap.add_argument("--workers", type=int, default=1,
+                    help="hf-batch: (n, grid) buckets solved concurrently on their own streams")
     ap.add_argument("--summarize-trace", default=None, metavar="DIR")
     ap.add_argument("--summarize-pmc", default=None, metavar="DIR")
     a = ap.parse_args()
@@ -390,6 +484,9 @@ def main():
     import torch
     from hqc_amd.pbc.engine import RY
 
+    if a.workload != "metric":
+        run_workload(a)
+        return
     B = a.batch
     if a.kernel_probe:
         kernel_probe(a.kernel_probe, B)

@@ -42,6 +42,9 @@ _SIGS = {
     "hqc_mixed_v_prep": [_P, _P, _P, _I, _I, _P, _P, _P, _P],
     "hqc_orb_density_q8": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
+    "hqc_fft3d_c2c_f32": [_P, _I, _I, _I, _P],
+    "hqc_pair_pack_c64": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
+    "hqc_pair_split_c64": [_P, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_pack": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_split_f64": [_P, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_split_f32": [_P, _I, _I, _I, _I, _I, _P, _P],
@@ -323,10 +326,11 @@ def ao_vmat_h2(ph, v, scale, dV, V, work, bidx=None, vmax=None):
 
 
 def fft3d(c, sign):
-    """In-place 3-D FFT of a complex128 tensor (B, g, g, g)."""
+    """In-place 3-D FFT of a complex128 or complex64 tensor (B, g, g, g)."""
     B, g = c.shape[0], c.shape[1]
-    assert c.dtype == torch.complex128 and c.shape[1:] == (g, g, g)
-    _call("hqc_fft3d_c2c_f64", _ptr(c, None, "c"), B, g, int(sign), _stream())
+    assert c.dtype in (torch.complex128, torch.complex64) and c.shape[1:] == (g, g, g)
+    name = "hqc_fft3d_c2c_f64" if c.dtype == torch.complex128 else "hqc_fft3d_c2c_f32"
+    _call(name, _ptr(c, None, "c"), B, g, int(sign), _stream())
 
 
 def pair_pack(phi, psi, table, nao, g, z):
@@ -337,7 +341,7 @@ def pair_pack(phi, psi, table, nao, g, z):
     mp = table.shape[0]
     assert not table.is_cuda and table.dtype == torch.int32 and table.shape == (mp, 3)
     assert psi.shape == (B, nmo, ld)
-    assert z.dtype == torch.complex128 and z.shape == (mp, nao, g ** 3)
+    assert z.dtype in (torch.complex128, torch.complex64) and z.shape == (mp, nao, g ** 3)
     if mp == 0:
         return
     if not (0 <= int(table[:, 0].min()) and int(table[:, 0].max()) < B
@@ -345,7 +349,8 @@ def pair_pack(phi, psi, table, nao, g, z):
             and -1 <= int(table[:, 2].min()) and int(table[:, 2].max()) < nmo):
         raise ValueError("pair table entry out of range")
     td = table.to(phi.device)
-    _call("hqc_pair_pack", _ptr(phi, F64, "phi"), _ptr(psi, F64, "psi"),
+    name = "hqc_pair_pack" if z.dtype == torch.complex128 else "hqc_pair_pack_c64"
+    _call(name, _ptr(phi, F64, "phi"), _ptr(psi, F64, "psi"),
           _ptr(td, torch.int32, "table"), mp, nao, nao_pad, nmo, g, ld, _ptr(z, None, "z"),
           _stream())
 
@@ -355,7 +360,11 @@ def pair_split(z, nao_pad, ld, M):
     mp, nao, N = z.shape
     g = round(N ** (1 / 3))
     assert g ** 3 == N and M.shape == (2 * mp, nao_pad, ld)
-    name = "hqc_pair_split_f64" if M.dtype == F64 else "hqc_pair_split_f32"
+    if z.dtype == torch.complex64:
+        assert M.dtype == F32
+        name = "hqc_pair_split_c64"
+    else:
+        name = "hqc_pair_split_f64" if M.dtype == F64 else "hqc_pair_split_f32"
     _call(name, _ptr(z, None, "z"), mp, nao, nao_pad, g, ld, _ptr(M, None, "M"), _stream())
 
 

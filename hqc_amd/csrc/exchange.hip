@@ -21,11 +21,19 @@
 
 // z[m, p, r] = phi_p(r) (psi_{i_m}(r) + i psi_{j_m}(r)); psi rows are already
 // weighted by sqrt(f) dV.  table[m] = (geometry, i, j), j < 0 for none.
+// Z is double2, or float2 for the FP32 pair transforms (one rounding of the FP64
+// product).
+__device__ __forceinline__ void store_c(double2* z, double a, double b) { *z = make_double2(a, b); }
+__device__ __forceinline__ void store_c(float2* z, double a, double b) {
+  *z = make_float2((float)a, (float)b);
+}
+
+template <typename Z>
 __global__ __launch_bounds__(256) void k_pair_pack(const double* __restrict__ phi,
                                                    const double* __restrict__ psi,
                                                    const int* __restrict__ table, int nao,
                                                    int nao_pad, int nmo, int N, long ld,
-                                                   double2* __restrict__ z) {
+                                                   Z* __restrict__ z) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= N) return;
   const int p = blockIdx.y, m = blockIdx.z;
@@ -34,11 +42,11 @@ __global__ __launch_bounds__(256) void k_pair_pack(const double* __restrict__ ph
   const double* ps = psi + (long)b * nmo * ld;
   const double a = ps[(long)i * ld + r];
   const double c = j >= 0 ? ps[(long)j * ld + r] : 0.0;
-  z[((long)m * nao + p) * N + r] = make_double2(f * a, f * c);
+  store_c(z + ((long)m * nao + p) * N + r, f * a, f * c);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_pair_split(const double2* __restrict__ z, int nao,
+template <typename T, typename Z>
+__global__ __launch_bounds__(256) void k_pair_split(const Z* __restrict__ z, int nao,
                                                     int nao_pad, int g, int ld,
                                                     T* __restrict__ M) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
@@ -57,50 +65,66 @@ __global__ __launch_bounds__(256) void k_pair_split(const double2* __restrict__ 
   const int s1 = c / (g * g) - h, s2 = (c / g) % g - h, s3 = c % g - h;
   const int k = ((s1 + g) % g * g + (s2 + g) % g) * g + (s3 + g) % g;
   const int kn = ((g - s1) % g * g + (g - s2) % g) * g + (g - s3) % g;
-  const double2 zp = z[((long)m * nao + p) * N + k];
-  const double2 zn = z[((long)m * nao + p) * N + kn];
+  const Z zp = z[((long)m * nao + p) * N + k];
+  const Z zn = z[((long)m * nao + p) * N + kn];
   if (col < nh) {
-    Ma[col] = (T)(0.5 * (zp.x + zn.x));     // Re A_a
-    Mb[col] = (T)(0.5 * (zp.y + zn.y));     // Re A_b
+    Ma[col] = (T)(0.5 * ((double)zp.x + (double)zn.x));     // Re A_a
+    Mb[col] = (T)(0.5 * ((double)zp.y + (double)zn.y));     // Re A_b
   } else {
-    Ma[col] = (T)(0.5 * (zp.y - zn.y));     // Im A_a
-    Mb[col] = (T)(0.5 * (zn.x - zp.x));     // Im A_b
+    Ma[col] = (T)(0.5 * ((double)zp.y - (double)zn.y));     // Im A_a
+    Mb[col] = (T)(0.5 * ((double)zn.x - (double)zp.x));     // Im A_b
   }
+}
+
+template <typename Z>
+static int launch_pack(const double* phi, const double* psi, const int* table, int mp, int nao,
+                       int nao_pad, int nmo, int g, int ld, Z* z, void* stream) {
+  if (mp < 0 || nao <= 0 || nao > nao_pad || nmo <= 0 || g <= 0) return HQC_E_ARG;
+  const long N = (long)g * g * g;
+  if (ld < N || mp > 65535 || nao > 65535) return HQC_E_SIZE;
+  if (mp == 0) return HQC_OK;
+  hipLaunchKernelGGL(k_pair_pack<Z>, dim3(hqc_div_up(N, 256), nao, mp), dim3(256), 0,
+                     (hipStream_t)stream, phi, psi, table, nao, nao_pad, nmo, (int)N, ld, z);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
 }
 
 extern "C" int hqc_pair_pack(const double* phi, const double* psi, const int* table, int mp,
                              int nao, int nao_pad, int nmo, int g, int ld, double* z,
                              void* stream) {
-  if (mp < 0 || nao <= 0 || nao > nao_pad || nmo <= 0 || g <= 0) return HQC_E_ARG;
-  const long N = (long)g * g * g;
-  if (ld < N || mp > 65535 || nao > 65535) return HQC_E_SIZE;
-  if (mp == 0) return HQC_OK;
-  hipLaunchKernelGGL(k_pair_pack, dim3(hqc_div_up(N, 256), nao, mp), dim3(256), 0,
-                     (hipStream_t)stream, phi, psi, table, nao, nao_pad, nmo, (int)N, ld,
-                     (double2*)z);
-  HQC_CHECK_LAUNCH();
-  return HQC_OK;
+  return launch_pack<double2>(phi, psi, table, mp, nao, nao_pad, nmo, g, ld, (double2*)z, stream);
 }
 
-template <typename T>
-static int launch_split(const double* z, int mp, int nao, int nao_pad, int g, int ld, T* M,
+extern "C" int hqc_pair_pack_c64(const double* phi, const double* psi, const int* table, int mp,
+                                 int nao, int nao_pad, int nmo, int g, int ld, float* z,
+                                 void* stream) {
+  return launch_pack<float2>(phi, psi, table, mp, nao, nao_pad, nmo, g, ld, (float2*)z, stream);
+}
+
+template <typename T, typename Z>
+static int launch_split(const Z* z, int mp, int nao, int nao_pad, int g, int ld, T* M,
                         void* stream) {
   if (mp < 0 || nao <= 0 || nao > nao_pad || g <= 0 || (g & 1) == 0) return HQC_E_ARG;
   const long N = (long)g * g * g;
   if (ld < N || mp > 65535 || nao_pad > 65535) return HQC_E_SIZE;
   if (mp == 0) return HQC_OK;
-  hipLaunchKernelGGL(k_pair_split<T>, dim3(hqc_div_up(ld, 256), nao_pad, mp), dim3(256), 0,
-                     (hipStream_t)stream, (const double2*)z, nao, nao_pad, g, ld, M);
+  hipLaunchKernelGGL((k_pair_split<T, Z>), dim3(hqc_div_up(ld, 256), nao_pad, mp), dim3(256), 0,
+                     (hipStream_t)stream, z, nao, nao_pad, g, ld, M);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }
 
 extern "C" int hqc_pair_split_f64(const double* z, int mp, int nao, int nao_pad, int g, int ld,
                                   double* M, void* stream) {
-  return launch_split<double>(z, mp, nao, nao_pad, g, ld, M, stream);
+  return launch_split<double>((const double2*)z, mp, nao, nao_pad, g, ld, M, stream);
 }
 
 extern "C" int hqc_pair_split_f32(const double* z, int mp, int nao, int nao_pad, int g, int ld,
                                   float* M, void* stream) {
-  return launch_split<float>(z, mp, nao, nao_pad, g, ld, M, stream);
+  return launch_split<float>((const double2*)z, mp, nao, nao_pad, g, ld, M, stream);
+}
+
+extern "C" int hqc_pair_split_c64(const float* z, int mp, int nao, int nao_pad, int g, int ld,
+                                  float* M, void* stream) {
+  return launch_split<float>((const float2*)z, mp, nao, nao_pad, g, ld, M, stream);
 }

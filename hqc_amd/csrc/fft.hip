@@ -22,18 +22,36 @@ struct FftPlan {
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C> __device__ __forceinline__ C czero();
+template <> __device__ __forceinline__ double2 czero<double2>() { return make_double2(0.0, 0.0); }
+template <> __device__ __forceinline__ float2 czero<float2>() { return make_float2(0.f, 0.f); }
+// twiddle exp(sign 2 pi i m / g), formed in FP64 and rounded once for FP32
+template <typename C> __device__ __forceinline__ C twiddle(int m, int g, int sign);
+template <> __device__ __forceinline__ double2 twiddle<double2>(int m, int g, int sign) {
+  double s, c;
+  sincospi((double)(2 * m) / (double)g, &s, &c);
+  return make_double2(c, sign < 0 ? -s : s);
+}
+template <> __device__ __forceinline__ float2 twiddle<float2>(int m, int g, int sign) {
+  double s, c;
+  sincospi((double)(2 * m) / (double)g, &s, &c);
+  return make_float2((float)c, (float)(sign < 0 ? -s : s));
+}
 
 // Stockham pass with compile-time radix R (see the header comment).  Element
 // (position p, line) of the lpb lines lives at p*ps + line*ls in LDS.
-template <int R>
-__device__ void pass_r(const double2* __restrict__ in, double2* __restrict__ out,
-                       const double2* __restrict__ tw, int N, int Ns, int lpb, int ps, int ls) {
+template <int R, typename C>
+__device__ void pass_r(const C* __restrict__ in, C* __restrict__ out,
+                       const C* __restrict__ tw, int N, int Ns, int lpb, int ps, int ls) {
   const int NR = N / R;
   const int step = N / (Ns * R);
   for (int it = threadIdx.x; it < NR * lpb; it += blockDim.x) {
     const int line = it % lpb, j = it / lpb;
     const int k = j % Ns;
-    double2 v[R];
+    C v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = in[(j + r * NR) * ps + line * ls];
 #pragma unroll
@@ -41,10 +59,10 @@ __device__ void pass_r(const double2* __restrict__ in, double2* __restrict__ out
     const int idx = (j / Ns) * Ns * R + k;
 #pragma unroll
     for (int kk = 0; kk < R; ++kk) {
-      double2 acc = v[0];
+      C acc = v[0];
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const double2 wr = tw[((r * kk) % R) * NR];
+        const C wr = tw[((r * kk) % R) * NR];
         acc.x += v[r].x * wr.x - v[r].y * wr.y;
         acc.y += v[r].x * wr.y + v[r].y * wr.x;
       }
@@ -54,8 +72,9 @@ __device__ void pass_r(const double2* __restrict__ in, double2* __restrict__ out
 }
 
 // Generic radix: one work item per output element.
-__device__ void pass_generic(const double2* __restrict__ in, double2* __restrict__ out,
-                             const double2* __restrict__ tw, int N, int Ns, int R, int lpb,
+template <typename C>
+__device__ void pass_generic(const C* __restrict__ in, C* __restrict__ out,
+                             const C* __restrict__ tw, int N, int Ns, int R, int lpb,
                              int ps, int ls) {
   const int NR = N / R;
   const int step = N / (Ns * R);
@@ -64,11 +83,11 @@ __device__ void pass_generic(const double2* __restrict__ in, double2* __restrict
     const int q = it / lpb;  // q = j*R + kk
     const int j = q / R, kk = q % R;
     const int k = j % Ns;
-    double2 acc = make_double2(0.0, 0.0);
+    C acc = czero<C>();
     for (int r = 0; r < R; ++r) {
-      double2 x = in[(j + r * NR) * ps + line * ls];
+      C x = in[(j + r * NR) * ps + line * ls];
       if (r) x = cmul(x, tw[k * r * step]);
-      const double2 w = tw[((r * kk) % R) * NR];
+      const C w = tw[((r * kk) % R) * NR];
       acc.x += x.x * w.x - x.y * w.y;
       acc.y += x.x * w.y + x.y * w.x;
     }
@@ -83,23 +102,24 @@ __device__ void pass_generic(const double2* __restrict__ in, double2* __restrict
 // q after input r is (r * q) mod N, stepped by q with one conditional subtract.
 #define FFT_PRIME_PR 8
 
-__device__ void pass_prime(const double2* __restrict__ in, double2* __restrict__ out,
-                           const double2* __restrict__ tw, int N, int lpb, int ps, int ls) {
+template <typename C>
+__device__ void pass_prime(const C* __restrict__ in, C* __restrict__ out,
+                           const C* __restrict__ tw, int N, int lpb, int ps, int ls) {
   const int nblk = (N + FFT_PRIME_PR - 1) / FFT_PRIME_PR;
   for (int it = threadIdx.x; it < nblk * lpb; it += blockDim.x) {
     const int line = it % lpb, q0 = (it / lpb) * FFT_PRIME_PR;
-    double2 acc[FFT_PRIME_PR];
+    C acc[FFT_PRIME_PR];
     int idx[FFT_PRIME_PR];
 #pragma unroll
     for (int j = 0; j < FFT_PRIME_PR; ++j) {
-      acc[j] = make_double2(0.0, 0.0);
+      acc[j] = czero<C>();
       idx[j] = 0;
     }
     for (int r = 0; r < N; ++r) {
-      const double2 x = in[r * ps + line * ls];
+      const C x = in[r * ps + line * ls];
 #pragma unroll
       for (int j = 0; j < FFT_PRIME_PR; ++j) {
-        const double2 w = tw[idx[j]];
+        const C w = tw[idx[j]];
         acc[j].x += x.x * w.x - x.y * w.y;
         acc[j].y += x.x * w.y + x.y * w.x;
         // (r + 1) q mod N for q = q0 + j; outputs past N (discarded) keep q = 0
@@ -116,8 +136,9 @@ __device__ void pass_prime(const double2* __restrict__ in, double2* __restrict__
 
 // All Stockham passes of one transform of length N over lpb lines; returns the
 // buffer that holds the result (in or out).
-__device__ double2* run_passes(double2* in, double2* out, const double2* __restrict__ tw,
-                               const FftPlan& plan, int N, int lpb, int ps, int ls) {
+template <typename C>
+__device__ C* run_passes(C* in, C* out, const C* __restrict__ tw, const FftPlan& plan, int N,
+                         int lpb, int ps, int ls) {
   if (plan.nst == 1 && plan.rad[0] > 13) {
     pass_prime(in, out, tw, N, lpb, ps, ls);
     __syncthreads();
@@ -137,7 +158,7 @@ __device__ double2* run_passes(double2* in, double2* out, const double2* __restr
       default: pass_generic(in, out, tw, N, Ns, R, lpb, ps, ls); break;
     }
     __syncthreads();
-    double2* t = in;
+    C* t = in;
     in = out;
     out = t;
     Ns *= R;
@@ -145,21 +166,17 @@ __device__ double2* run_passes(double2* in, double2* out, const double2* __restr
   return in;
 }
 
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(double2* __restrict__ data, int g,
-                                                          int axis, int lpb, FftPlan plan,
-                                                          int sign) {
-  extern __shared__ double2 sm[];
-  double2* bufA = sm;
-  double2* bufB = sm + g * lpb;
-  double2* tw = bufB + g * lpb;
+template <typename C>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(C* __restrict__ data, int g, int axis,
+                                                          int lpb, FftPlan plan, int sign) {
+  extern __shared__ unsigned char sm_raw[];
+  C* bufA = reinterpret_cast<C*>(sm_raw);
+  C* bufB = bufA + g * lpb;
+  C* tw = bufB + g * lpb;
   const long gg = (long)g * g;
-  double2* base = data + (long)blockIdx.y * gg * g;
+  C* base = data + (long)blockIdx.y * gg * g;
   const int line0 = blockIdx.x * lpb;
-  for (int m = threadIdx.x; m < g; m += blockDim.x) {
-    double s, c;
-    sincospi((double)(2 * m) / (double)g, &s, &c);
-    tw[m] = make_double2(c, sign < 0 ? -s : s);
-  }
+  for (int m = threadIdx.x; m < g; m += blockDim.x) tw[m] = twiddle<C>(m, g, sign);
   const long stride = axis == 2 ? 1 : (axis == 1 ? g : gg);
   for (int e = threadIdx.x; e < g * lpb; e += blockDim.x) {
     int p, line;
@@ -171,7 +188,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(double2* __restrict__ 
       p = e / lpb;
     }
     const long l = line0 + line;
-    double2 val = make_double2(0.0, 0.0);
+    C val = czero<C>();
     if (l < gg) {
       const long lb = axis == 2 ? l * g : (axis == 1 ? (l / g) * gg + (l % g) : l);
       val = base[lb + p * stride];
@@ -179,7 +196,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(double2* __restrict__ 
     bufA[p * lpb + line] = val;
   }
   __syncthreads();
-  double2* in = run_passes(bufA, bufB, tw, plan, g, lpb, lpb, 1);
+  C* in = run_passes(bufA, bufB, tw, plan, g, lpb, lpb, 1);
   for (int e = threadIdx.x; e < g * lpb; e += blockDim.x) {
     int p, line;
     if (axis == 2) {
@@ -209,24 +226,20 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(double2* __restrict__ 
 // 65^3 and a few planes per CU run slower at 256).
 #define FFT_PLANE_MAX_LDS (156 * 1024)
 
-template <int NT>
-__global__ __launch_bounds__(NT) void k_fft_plane(double2* __restrict__ data, int g, FftPlan plan,
+template <int NT, typename C>
+__global__ __launch_bounds__(NT) void k_fft_plane(C* __restrict__ data, int g, FftPlan plan,
                                                   int sign) {
-  extern __shared__ double2 sm[];
+  extern __shared__ unsigned char sm_raw[];
   const int gg = g * g;
-  double2* bufA = sm;
-  double2* bufB = sm + gg;
-  double2* tw = bufB + gg;
-  double2* plane = data + ((long)blockIdx.y * g + blockIdx.x) * gg;
-  for (int m = threadIdx.x; m < g; m += blockDim.x) {
-    double s, c;
-    sincospi((double)(2 * m) / (double)g, &s, &c);
-    tw[m] = make_double2(c, sign < 0 ? -s : s);
-  }
+  C* bufA = reinterpret_cast<C*>(sm_raw);
+  C* bufB = bufA + gg;
+  C* tw = bufB + gg;
+  C* plane = data + ((long)blockIdx.y * g + blockIdx.x) * gg;
+  for (int m = threadIdx.x; m < g; m += blockDim.x) tw[m] = twiddle<C>(m, g, sign);
   for (int e = threadIdx.x; e < gg; e += blockDim.x) bufA[e] = plane[e];
   __syncthreads();
-  double2* r = run_passes(bufA, bufB, tw, plan, g, g, 1, g);          // along z
-  double2* o = (r == bufA) ? bufB : bufA;
+  C* r = run_passes(bufA, bufB, tw, plan, g, g, 1, g);                // along z
+  C* o = (r == bufA) ? bufB : bufA;
   r = run_passes(r, o, tw, plan, g, g, g, 1);                          // along y
   for (int e = threadIdx.x; e < gg; e += blockDim.x) plane[e] = r[e];
 }
@@ -252,28 +265,29 @@ static int make_plan(int g, FftPlan& plan) {
   return HQC_OK;
 }
 
-extern "C" int hqc_fft3d_c2c_f64(double* data, int B, int g, int sign, void* stream) {
+template <typename C>
+static int fft3d(C* data, int B, int g, int sign, void* stream) {
   if (B < 0 || g <= 0 || (sign != 1 && sign != -1)) return HQC_E_ARG;
   if (g > FFT_MAX_G) return HQC_E_SIZE;
   if (B == 0 || g == 1) return HQC_OK;
   FftPlan plan;
   if (make_plan(g, plan) != HQC_OK) return HQC_E_SIZE;
   const long lines = (long)g * g;
-  const size_t plane_lds = (size_t)(2 * g * g + g) * sizeof(double2);
+  const size_t plane_lds = (size_t)(2 * g * g + g) * sizeof(C);
   int first_axis = 2;
   if (plane_lds <= FFT_PLANE_MAX_LDS) {
     const bool small = g <= 51 && (long)g * B >= 512;
-    const void* kf = small ? (const void*)k_fft_plane<256> : (const void*)k_fft_plane<1024>;
+    const void* kf = small ? (const void*)k_fft_plane<256, C> : (const void*)k_fft_plane<1024, C>;
     if (plane_lds > 64 * 1024 &&
         hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds) !=
             hipSuccess)
       (void)hipGetLastError();
     if (small)
-      hipLaunchKernelGGL(k_fft_plane<256>, dim3(g, B), dim3(256), plane_lds, (hipStream_t)stream,
-                         (double2*)data, g, plan, sign);
+      hipLaunchKernelGGL((k_fft_plane<256, C>), dim3(g, B), dim3(256), plane_lds,
+                         (hipStream_t)stream, data, g, plan, sign);
     else
-      hipLaunchKernelGGL(k_fft_plane<1024>, dim3(g, B), dim3(1024), plane_lds,
-                         (hipStream_t)stream, (double2*)data, g, plan, sign);
+      hipLaunchKernelGGL((k_fft_plane<1024, C>), dim3(g, B), dim3(1024), plane_lds,
+                         (hipStream_t)stream, data, g, plan, sign);
     HQC_CHECK_LAUNCH();
     first_axis = 0;
   }
@@ -281,13 +295,21 @@ extern "C" int hqc_fft3d_c2c_f64(double* data, int B, int g, int sign, void* str
   for (int axis = first_axis; axis >= 0; --axis) {
     // lines per block: 32 on the strided x axis (longer coalesced segments), 16
     // otherwise, and never more than 64 KB of LDS
-    const int fit = (int)((64 * 1024 / sizeof(double2) - g) / (2 * g));
+    const int fit = (int)((64 * 1024 / sizeof(C) - g) / (2 * g));
     const int lpb = max(1, min(axis == 0 ? 32 : 16, fit));
-    const size_t lds = (size_t)(2 * g * lpb + g) * sizeof(double2);
+    const size_t lds = (size_t)(2 * g * lpb + g) * sizeof(C);
     dim3 grid(hqc_div_up(lines, lpb), B);
-    hipLaunchKernelGGL(k_fft_axis, grid, dim3(FFT_THREADS), lds, (hipStream_t)stream,
-                       (double2*)data, g, axis, lpb, plan, sign);
+    hipLaunchKernelGGL((k_fft_axis<C>), grid, dim3(FFT_THREADS), lds, (hipStream_t)stream, data,
+                       g, axis, lpb, plan, sign);
     HQC_CHECK_LAUNCH();
   }
   return HQC_OK;
+}
+
+extern "C" int hqc_fft3d_c2c_f64(double* data, int B, int g, int sign, void* stream) {
+  return fft3d<double2>(reinterpret_cast<double2*>(data), B, g, sign, stream);
+}
+
+extern "C" int hqc_fft3d_c2c_f32(float* data, int B, int g, int sign, void* stream) {
+  return fft3d<float2>(reinterpret_cast<float2*>(data), B, g, sign, stream);
 }

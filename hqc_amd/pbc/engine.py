@@ -174,6 +174,14 @@ class Engine:
     # with no residual floor; 0: the split-FP16 form)
     Q8_WARMUP_DIGITS = 3
     mat_dtype = torch.float64        # n_ao x n_ao matrices (complex128 at a k-point)
+    # Hartree-Fock in the mixed mode: the exchange pair densities, their transforms
+    # (complex64) and the contraction in FP32 while the DIIS residual is >= this, in
+    # FP64 after.  Restricted HF of random hydrogen cells often oscillates near the
+    # stopping rule; 24 geometries of the uneven HF batch converged 13 (FP64), 14
+    # (switch at 1e-2, 14% less time) and 10 (switch at 1e-4), so the FP32 exchange
+    # is kept to the early iterations
+    MIXED_X32 = True
+    MIXED_X_DIRECT_ERR = 1e-2
 
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
@@ -224,7 +232,8 @@ class Engine:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
         # kernel launches: FP64 / FP32 densities and potential matrices
-        self.stats = {"rho64": 0, "rhoq8": 0, "rho32": 0, "v64": 0, "v32": 0}
+        self.stats = {"rho64": 0, "rhoq8": 0, "rho32": 0, "v64": 0, "v32": 0, "x64": 0,
+                      "x32": 0}
         self.sweeps = torch.zeros((), dtype=torch.long, device=cell.device)
         self.eigh_calls = 0
         self.active = []
@@ -602,7 +611,7 @@ class Engine:
     # ----------------------------------------------------------- two-electron
     EXCHANGE_CHUNK_BYTES = 4 << 30      # pair grids + row blocks per launch group
 
-    def exchange(self, st, na, mo, occ):
+    def exchange(self, st, na, mo, occ, low=None):
         """K (B, nao, nao) for geometries [0, na): equals sum_rs D_rs (eris+eris0)[p,q,s,r]
         (hqc/pbc/solver.py:430-431, 604-614).
 
@@ -611,7 +620,9 @@ class Engine:
         Two occupied orbitals share one complex FFT (hqc_pair_pack); the split kernel
         returns the [Re | Im] blocks on half the G-set, and the sum over orbitals of
         M diag(2 VG) M^T is the AO x grid MFMA contraction, batched over all pairs
-        of all geometries."""
+        of all geometries.  low (host list of bools over [0, na), or None): those
+        geometries take the pair densities, their transforms and the contraction in
+        FP32 (complex64 grids, FP32 MFMA), the others in FP64."""
         c = self.c
         dev = c.device
         keep = (occ > 1e-14).cpu()
@@ -629,23 +640,36 @@ class Engine:
             wgt = occ[:, :nmax].clamp_min(0).sqrt() * dVs * keep[:na, :nmax].to(dev)
             psi = (mo[:, :, :nmax] * wgt.unsqueeze(-2)).transpose(-1, -2).contiguous() \
                 @ st["phi"][:na, :c.nao, :]                                  # (na, nmax, ld)
-            table = torch.tensor(rows, dtype=torch.int32)
-            per_pair = c.nao * c.N * 16 + 2 * c.nao_pad * (c.ld + c.nao_pad + 1) * 8
-            chunk = max(1, min(len(rows), int(self.EXCHANGE_CHUNK_BYTES // per_pair)))
             VGh = st["VGh"]
             phi = st["phi"][:na]
-            for s0 in range(0, len(rows), chunk):
-                t = table[s0:s0 + chunk]
-                mp = t.shape[0]
-                z = torch.empty(mp, c.nao, c.N, dtype=torch.complex128, device=dev)
-                K.pair_pack(phi, psi, t, c.nao, c.g, z)
-                K.fft3d(z.view(mp * c.nao, c.g, c.g, c.g), -1)
-                M = torch.empty(2 * mp, c.nao_pad, c.ld, dtype=torch.float64, device=dev)
-                K.pair_split(z, c.nao_pad, c.ld, M)
-                del z
-                gid = t[:, 0].long().repeat_interleave(2).to(dev)
-                part = self.vmat_dv(M, VGh[gid], 1.0)
-                Kmat.index_add_(0, gid, part)
+            for f32 in (False, True):
+                sel = [r for r in rows if bool(low[r[0]]) == f32] if low is not None else \
+                    (rows if not f32 else [])
+                if not sel:
+                    continue
+                table = torch.tensor(sel, dtype=torch.int32)
+                zb, mb = (8, 4) if f32 else (16, 8)
+                per_pair = c.nao * c.N * zb + 2 * c.nao_pad * (c.ld + c.nao_pad + 1) * mb
+                chunk = max(1, min(len(sel), int(self.EXCHANGE_CHUNK_BYTES // per_pair)))
+                for s0 in range(0, len(sel), chunk):
+                    t = table[s0:s0 + chunk]
+                    mp = t.shape[0]
+                    z = torch.empty(mp, c.nao, c.N, device=dev,
+                                    dtype=torch.complex64 if f32 else torch.complex128)
+                    K.pair_pack(phi, psi, t, c.nao, c.g, z)
+                    K.fft3d(z.view(mp * c.nao, c.g, c.g, c.g), -1)
+                    M = torch.empty(2 * mp, c.nao_pad, c.ld, device=dev,
+                                    dtype=torch.float32 if f32 else torch.float64)
+                    K.pair_split(z, c.nao_pad, c.ld, M)
+                    del z
+                    gid = t[:, 0].long().repeat_interleave(2).to(dev)
+                    part = self.vmat_dv(M, VGh[gid], 1.0)
+                    # rows of one geometry are contiguous: a fixed-order sum per geometry
+                    # (index_add_ would accumulate with atomics, in no fixed order)
+                    geo, cnt = torch.unique_consecutive(t[:, 0], return_counts=True)
+                    sums = torch.stack([q.sum(0) for q in torch.split(part, (2 * cnt).tolist())])
+                    Kmat[geo.long().to(dev)] += sums
+                self.stats["x32" if f32 else "x64"] += 1
         D = (mo * occ.unsqueeze(-2)) @ mo.transpose(-1, -2).contiguous()
         Sg = st["Sgrid"][:na]
         return Kmat + st["c0b"][:na, None, None] * (Sg @ D @ Sg)
@@ -822,7 +846,10 @@ class Engine:
         Exc, EH = E[:, 0] * dVs, 0.5 * E[:, 1] * dVs
         if self.dft:
             return V2, EH + Exc
-        Vx = -0.5 * self.exchange(st, na, mo, occ)
+        low = None
+        if mixed and self.MIXED_X32:
+            low = (st["err"][:na] >= self.MIXED_X_DIRECT_ERR).tolist()
+        Vx = -0.5 * self.exchange(st, na, mo, occ, low)
         Ex = 0.5 * _tr(Vx, D).real
         return V2 + Vx, EH + Ex
 

@@ -57,36 +57,69 @@ def _cut_basis_term(st, D, F):
     return 0.5 * (X + X.mH)
 
 
+ADJOINT_BLOCK_BYTES = 4 << 30      # pair potentials of one block of orbital columns
+
+
 def _exchange_adjoint(engine, st):
     """Zx = -Y - c0 (D S~ D) phi for HF, (B, nao_pad, ld), each geometry with its own
-    cell constants (VG, dV, c0)."""
+    cell constants (VG, dV, c0):
+        Y_p = sum_j f_j C_pj G_j,   G_j = sum_i f_i psi_i u_ij,
+        u_ij = dV N IFFT[VG FFT(psi_i psi_j)]  (real and symmetric in i, j).
+    Only the pairs i <= j are transformed, two real pair densities per complex grid
+    (VG is real and even, so the two potentials come back as Re and Im); a pair adds
+    f_i psi_i u_ij to G_j and, off the diagonal, f_j psi_j u_ij to G_i.  The columns
+    j run in blocks whose pair potentials fit ADJOINT_BLOCK_BYTES; every sum is a
+    fixed-order reduction."""
     c = engine.c
-    B = st["B"]
-    phi = st["phi"][:, :c.nao, :c.N]
+    B, N, g = st["B"], c.N, c.g
+    dev = c.device
+    phi = st["phi"][:, :c.nao, :N]
     mo, occ = st["slot"]["mo"], st["slot"]["occ"]
-    Zx = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float64, device=c.device)
-    g = c.g
+    Zx = torch.zeros(B, c.nao_pad, c.ld, dtype=torch.float64, device=dev)
     Lh = st["Lb"].cpu().tolist()
+    keep_h = (occ > 1e-14).cpu()
     for b in range(B):
         cst = c.consts(Lh[b])
         VG = cst.VG.reshape(1, g, g, g)
-        keep = torch.nonzero(occ[b] > 1e-14).flatten()
-        C = mo[b][:, keep]
-        f = occ[b][keep]
-        psi = C.transpose(0, 1) @ phi[b]                       # (nocc, N)
-        Y = torch.zeros(c.nao, c.N, dtype=torch.float64, device=c.device)
-        for ii in range(len(f)):
-            pair = (psi * psi[ii]).to(torch.complex128).reshape(-1, g, g, g).contiguous()
-            pair *= cst.dV
-            K.fft3d(pair, -1)
-            pair *= VG
-            K.fft3d(pair, +1)
-            u = pair.real.reshape(len(f), c.N)
-            Gi = (f[:, None] * psi * u).sum(0)                 # sum_j f_j psi_j u_ij
-            Y += f[ii] * torch.outer(C[:, ii], Gi)
+        keep = torch.nonzero(keep_h[b]).flatten()
+        m = keep.numel()
+        C = mo[b][:, keep.to(dev)]
+        f = occ[b][keep.to(dev)]
+        psi = C.transpose(0, 1) @ phi[b]                       # (m, N)
+        fpsi = f[:, None] * psi
+        G = torch.zeros(m, N, dtype=torch.float64, device=dev)
+        jb = max(1, min(m, int(ADJOINT_BLOCK_BYTES // (3 * 8 * m * N))))
+        for j0 in range(0, m, jb):
+            j1 = min(m, j0 + jb)
+            ii, jj = [], []
+            for j in range(j0, j1):
+                ii += list(range(j + 1))
+                jj += [j] * (j + 1)
+            P = len(ii)
+            it = torch.tensor(ii, device=dev)
+            jt = torch.tensor(jj, device=dev)
+            n = psi[it] * psi[jt] * cst.dV                     # (P, N) pair densities
+            if P % 2:
+                n = torch.cat([n, torch.zeros_like(n[:1])])
+            z = torch.complex(n[0::2], n[1::2]).reshape(-1, g, g, g)
+            del n
+            K.fft3d(z, -1)
+            z *= VG
+            K.fft3d(z, +1)
+            u = torch.stack([z.real, z.imag], 1).reshape(-1, N)[:P]
+            del z
+            ub = torch.zeros(m, j1 - j0, N, dtype=torch.float64, device=dev)
+            ub[it, jt - j0] = u                                # u_ij for i <= j, j in block
+            del u
+            G[j0:j1] += (fpsi[:, None, :] * ub).sum(0)          # G_j += f_i psi_i u_ij
+            d = torch.arange(j0, j1, device=dev)
+            ub[d, d - j0] = 0.0
+            G += (fpsi[None, j0:j1, :] * ub).sum(1)             # G_i += f_j psi_j u_ij, j > i
+            del ub
+        Y = C @ (f[:, None] * G)
         D = st["slot"]["D"][b]
         M = D @ st["Sgrid"][b] @ D
-        Zx[b, :c.nao, :c.N] = -Y - cst.c0 * (M @ phi[b])
+        Zx[b, :c.nao, :N] = -Y - cst.c0 * (M @ phi[b])
     return Zx
 
 
@@ -162,8 +195,8 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
     atom = c.atom_of_ao
     other = (atom[:, None] != atom[None, :]).to(f64)
     t = 2.0 * ((W * other).unsqueeze(-1) * dS - (D * other).unsqueeze(-1) * dT).sum(2)
-    F = torch.zeros(B, c.n, 3, dtype=f64, device=dev)
-    F.index_add_(1, atom, t)
+    # AOs are atom-major: a fixed-order sum per atom (index_add_ would sum with atomics)
+    F = t.reshape(B, c.n, c.nloc, 3).sum(2)
     # grid adjoint (FP64 phi regardless of the SCF precision)
     # the D-form kernel writes every row and grid column of X and rho
     X = torch.empty(B, c.nao_pad, c.ld, dtype=f64, device=dev)
@@ -223,8 +256,8 @@ def compute_forces_k(engine, st, kappa=10, Gmax=15):
     other = (atom[:, None] != atom[None, :]).to(f64)
     t = 2.0 * ((W * other).conj().unsqueeze(-1) * dS
                - (D * other).conj().unsqueeze(-1) * dT).sum(2).real
-    F = torch.zeros(B, c.n, 3, dtype=f64, device=dev)
-    F.index_add_(1, atom, t)
+    # AOs are atom-major: a fixed-order sum per atom (index_add_ would sum with atomics)
+    F = t.reshape(B, c.n, c.nloc, 3).sum(2)
     X = torch.empty(B, 2 * c.nao_pad, c.ld, dtype=f64, device=dev)
     rho = torch.empty(B, c.ld, dtype=f64, device=dev)
     K.ao_density(st["phi"], engine.stacked_dm(D), rho, X, bidx=st["gidx"])

@@ -37,6 +37,7 @@ class KEngine(Engine):
     # increments and the warm-up density run on FP32 MFMA, the accurate density in FP64
     MIXED_SPLIT16 = False
     MIXED_Q8 = False
+    MIXED_X32 = False              # the complex pair densities stay FP64
     mat_dtype = torch.complex128
 
     def __init__(self, cell, **kw):
@@ -173,7 +174,7 @@ class KEngine(Engine):
         return w, v @ C, C
 
     # ---------------------------------------------------------------- exchange
-    def exchange(self, st, na, mo, occ):
+    def exchange(self, st, na, mo, occ, low=None):
         """K_pq = sum_i f_i sum_G VG conj(A_pi) A_qi + c0 (S~ D S~)_pq with
         A_pi = dV FFT[phi_p conj(psi_i)] (hqc/pbc/solver.py:521-528, 604-614): complex
         pair densities, one FFT each, contracted as stacked real rows."""

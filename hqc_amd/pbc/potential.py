@@ -65,9 +65,12 @@ def ewald_pp(xp, L, rs, kappa=10, Gmax=15, forces=False):
     Lb = L * rs
     dsr = -(er / (d * d) + 2.0 * kappa / math.sqrt(math.pi) * torch.exp(-(kappa * d) ** 2) / d)
     grad = (dsr / d).unsqueeze(-1) * rij / Lb ** 2                 # d(pair)/d r_ij
-    F = -grec / Lb ** 2
-    F.index_add_(1, i, -grad)
-    F.index_add_(1, j, grad)
+    # pair terms into an antisymmetric (n, n) array, then a fixed-order row sum
+    # (index_add_ would accumulate with atomics, in no fixed order)
+    Mp = torch.zeros(xp.shape[0], n, n, 3, dtype=grad.dtype, device=dev)
+    Mp[:, i, j] = -grad
+    Mp[:, j, i] = grad
+    F = -grec / Lb ** 2 + Mp.sum(2)
     return E, F
 
 

@@ -201,6 +201,10 @@ int hqc_mixed_v_prep(const double* v, const double* va, const unsigned char* dir
  * +1 inverse, unnormalised.  Replaces jnp.fft.fft (hqc/pbc/solver.py:412)
  * in the FFT-Poisson restatement. */
 int hqc_fft3d_c2c_f64(double* data, int B, int g, int sign, void* stream);
+/* The same transform of complex64 grids (interleaved floats; twiddles formed in
+ * FP64 and rounded once): the FP32 pair-density transforms of the exchange
+ * (hqc/pbc/solver.py:412, 430-431 in the reference, FP64 there). */
+int hqc_fft3d_c2c_f32(float* data, int B, int g, int sign, void* stream);
 
 /* Hartree-Fock exchange pair densities, two orbitals per complex grid:
  *   z[m, p, r] = phi[b, p, r] * (psi[b, i, r] + i psi[b, j, r]),
@@ -210,6 +214,9 @@ int hqc_fft3d_c2c_f64(double* data, int B, int g, int sign, void* stream);
  * eris (hqc/pbc/solver.py:412-431) on the exchange route K = sum_rs D_rs eris. */
 int hqc_pair_pack(const double* phi, const double* psi, const int* table, int mp, int nao,
                   int nao_pad, int nmo, int g, int ld, double* z, void* stream);
+/* _c64: z as complex64 (each FP64 product rounded once), for hqc_fft3d_c2c_f32. */
+int hqc_pair_pack_c64(const double* phi, const double* psi, const int* table, int mp, int nao,
+                      int nao_pad, int nmo, int g, int ld, float* z, void* stream);
 
 /* Split each transformed pair grid Z = FFT(a + i b) into the real row blocks
  * M[2m] = [Re A_a(H) | Im A_a(H)] and M[2m+1] = [Re A_b(H) | Im A_b(H)] over the
@@ -221,6 +228,9 @@ int hqc_pair_pack(const double* phi, const double* psi, const int* table, int mp
 int hqc_pair_split_f64(const double* z, int mp, int nao, int nao_pad, int g, int ld, double* M,
                        void* stream);
 int hqc_pair_split_f32(const double* z, int mp, int nao, int nao_pad, int g, int ld, float* M,
+                       void* stream);
+/* _c64: from complex64 transforms (hqc_pair_pack_c64 + hqc_fft3d_c2c_f32) to FP32 M. */
+int hqc_pair_split_c64(const float* z, int mp, int nao, int nao_pad, int g, int ld, float* M,
                        void* stream);
 
 /* Local (electron-proton) potential in G space, FFT order:

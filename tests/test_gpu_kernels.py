@@ -280,6 +280,24 @@ def test_fft3d(K, g):
     assert torch.allclose(y / g ** 3, x, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("g", [7, 25, 49, 61, 65, 71, 97, 128])
+def test_fft3d_f32(K, g):
+    """complex64 transforms (plane and line kernels, prime lengths on the direct pass)
+    against torch's complex128 FFT of the same inputs: FP32 rounding only, ~1e-7 of the
+    typical output g^1.5."""
+    print(f"[fft3d f32] g={g}", flush=True)
+    torch.manual_seed(g)
+    x = torch.randn(3, g, g, g, dtype=torch.complex64, device=DEV)
+    y = x.clone()
+    K.fft3d(y, -1)
+    ref = torch.fft.fftn(x.to(torch.complex128), dim=(1, 2, 3))
+    err = (y.to(torch.complex128) - ref).abs().max().item()
+    assert err < 2e-6 * g ** 1.5, err
+    K.fft3d(y, +1)
+    back = (y.to(torch.complex128) / g ** 3 - x.to(torch.complex128)).abs().max().item()
+    assert back < 2e-6, back
+
+
 @pytest.mark.parametrize("g", [43, 49, 51])
 def test_fft3d_plane_thread_counts_agree(K, g):
     """Planes up to 51^2 run with 256 threads once there are >= 512 planes (12

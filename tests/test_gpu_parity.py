@@ -341,6 +341,8 @@ def test_mixed_precision_hf_and_gga(name, kw):
     assert (st["order"].cpu().numpy() != np.arange(5)).any()  # and permuted the slots
     if name == "hf":
         assert "phi_h2" in st["per"] and "gidx" not in st["per"]
+        # FP32 pair transforms while the residual is large, FP64 after
+        assert pmx.solver.engine.stats["x32"] > 0 and pmx.solver.engine.stats["x64"] > 0
     assert (E64 - Emx).abs().max().item() / 2 / n <= 1e-6
     assert (F64 - Fmx).abs().max().item() <= 1e-5
     assert (p64.solver.last["w1"] - st["w1"]).abs().max().item() <= 1e-6
@@ -447,3 +449,22 @@ def test_mixed_kpoints_match_fp64(name, kw):
     assert bool(st["conv"].all()) and bool(p64.solver.last["conv"].all())
     assert dE <= 1e-6 and dF <= 1e-5 and dw <= 1e-6
     assert pmx.solver.engine.stats["v32"] > 0 and pmx.solver.engine.stats["rho32"] > 0
+
+
+def test_concurrent_buckets_match_sequential():
+    """make_pes_batch(workers=3): the (n, grid) buckets of an uneven batch solved
+    concurrently on their own streams give the same numbers as one after another."""
+    from hqc_amd.pbc import make_pes_batch
+    rng = np.random.default_rng(41)
+    specs = [(4, 1.30), (4, 1.35), (3, 1.30), (4, 1.50), (6, 1.32), (3, 1.45), (6, 1.2)]
+    xps = [rng.uniform(0, (4 / 3 * np.pi * n) ** (1 / 3), (n, 3)) for n, _ in specs]
+    rss = [r for _, r in specs]
+    for kw in (dict(dft=True, smearing=True, smearing_sigma=0.1), dict(dft=False)):
+        print(f"[concurrent buckets] {kw}", flush=True)
+        kw = dict(kw, grid_length=0.3, tol=1e-10)
+        E1, F1, i1 = make_pes_batch("gth-dzv", **kw).energy_and_forces(xps, rss)
+        E3, F3, i3 = make_pes_batch("gth-dzv", workers=3, **kw).energy_and_forces(xps, rss)
+        assert len(i1["buckets"]) >= 4 and i1["n_iter"] == i3["n_iter"]
+        for i in range(len(xps)):
+            assert E1[i].item() == E3[i].item()
+            assert torch.equal(F1[i], F3[i])
