@@ -51,12 +51,16 @@ def workload_xp(B, seed0=0):
                      for b in range(B)]), L
 
 
+TUNING = {}       # path -> autotuned launch parameters (--autotune), else library defaults
+
+
 def make(precision):
     from hqc_amd.pbc import make_pes
     L = (4 / 3 * math.pi * N_ATOMS) ** (1 / 3)
     return make_pes(N_ATOMS, L, RS, BASIS, grid_length=GRID_LENGTH, dft=True, xc="lda,vwn",
                     smearing=True, smearing_method="fermi", smearing_sigma=SIGMA,
-                    search_method="bisect", tol=TOL, precision=precision)
+                    search_method="bisect", tol=TOL, precision=precision,
+                    tuning=TUNING.get(precision))
 
 
 def vmat_kind(pes):
@@ -466,6 +470,9 @@ def main():
                     choices=["metric", "hf-batch", "hf-128", "kmesh"],
                     help="metric (default): the 54 H LDA batch; the others are the uneven "
                          "HF PES batch (--batch geometries), 128 H dzvp HF and the PBE k-mesh")
+    ap.add_argument("--autotune", action="store_true",
+                    help="sweep the kernels' launch parameters at the workload shape first "
+                         "(untimed) and run both paths with the choice")
     ap.add_argument("--workers", type=int, default=1,
                     help="hf-batch: (n, grid) buckets solved concurrently on their own streams")
     ap.add_argument("--summarize-trace", default=None, metavar="DIR")
@@ -493,6 +500,16 @@ def main():
         return
     xps, _ = workload_xp(B)
     x = torch.tensor(xps, dtype=torch.float64, device="cuda")
+    tuned = None
+    if a.autotune:
+        # untimed: sweep the launch parameters at the workload shape for both paths
+        from hqc_amd import tune
+        tuned = {}
+        for path in ("mixed", "fp64"):
+            rec = tune.autotune(make(path).solver.engine, B)
+            TUNING[path] = rec["choice"]
+            tuned[path] = rec["choice"]
+        torch.cuda.empty_cache()
     pes, dt, E, F = run_path(a.path, x, a.steps, a.warmup)
     st = pes.solver.last
     eng = pes.solver.engine
@@ -578,6 +595,7 @@ def main():
                      "ms_per_launch": dom["ms_per_launch"],
                      "flops_per_launch": dom["flops_per_launch"]},
         "kernels": cand,
+        "tuning": tuned if tuned is not None else "library defaults (no --autotune)",
     }
     seed0 = {a.path: (E[0].item(), F[0].cpu().numpy(), w_main[0].cpu().numpy())}
     if not a.no_compare:

@@ -22,6 +22,8 @@ _D = ctypes.c_double
 
 _SIGS = {
     "hqc_version": [],
+    "hqc_set_tuning": [_I, _I],
+    "hqc_get_tuning": [_I],
     "hqc_gto_tables": [_P, _I, _I, _I, _D, _I, _P, _I, _I, _P, _P],
     "hqc_ao_values_f64": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
     "hqc_ao_values_f32": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
@@ -126,6 +128,19 @@ def _stream():
 F64 = torch.float64
 F32 = torch.float32
 I32 = torch.int32
+
+
+TUNE_KEYS = {"vmat_blocks": 0, "vh_blocks": 1, "vmat_parts": 2, "vmat_tpw": 3, "q8_blocks": 4,
+             "q8_min_tiles": 5, "fft_plane_threads": 6, "vbig_blocks": 7}
+
+
+def set_tuning(name, value):
+    """Launch parameter `name` (TUNE_KEYS) of the calling host thread; 0 = default."""
+    _call("hqc_set_tuning", TUNE_KEYS[name], int(value))
+
+
+def get_tuning(name):
+    return lib().hqc_get_tuning(TUNE_KEYS[name])
 
 
 def gto_tables(xp, g, L, tmax, alpha, nl, out):

@@ -14,6 +14,10 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
     if (e_ != hipSuccess) return HQC_E_HIP;                  \
   } while (0)
 
+// Tuned launch parameter `key` of the calling host thread, or dflt when unset
+// (tune.hip; keys HQC_TUNE_* in hqc_amd.h).
+int hqc_tuned(int key, int dflt);
+
 // Thread-block sizes used throughout (one 64-lane wavefront = 64 threads).
 #define HQC_WAVE 64
 

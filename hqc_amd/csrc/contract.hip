@@ -861,13 +861,17 @@ __global__ void k_vmat_reduce(const double* __restrict__ part, int nao_pad, int 
 
 static bool vmat_pipelined(int nao_pad) { return nao_pad <= 16 * VMAT_PIPE_MAXV; }
 
-// Tiles per wave of the pipelined kernel: the smallest count that covers all
-// tiles with 4 waves (capped at VMAT_TPW; more tiles use several groups).
+// Tiles per wave of the pipelined kernel: with g groups of 4 waves x VMAT_TPW
+// tiles needed to cover all tiles, the smallest count that covers them with g
+// groups, so the groups are balanced (112 AOs: 28 tiles, 1 group of 7 per wave;
+// 128 AOs: 36 tiles, 2 groups of 5 instead of 8 + 1, which the autotuner measured
+// at 0.49 against 0.61 ms for 4 geometries of 61^3 with 6 per wave).
 static int vmat_pipe_tpw(int nao_pad) {
   const int NT = nao_pad >> 4;
   const int nt = NT * (NT + 1) / 2;
-  const int t = (nt + 3) / 4;
-  return t < VMAT_TPW ? t : VMAT_TPW;
+  const int groups = (nt + 4 * VMAT_TPW - 1) / (4 * VMAT_TPW);
+  const int t = (nt + 4 * groups - 1) / (4 * groups);
+  return hqc_tuned(HQC_TUNE_VMAT_TPW, t < VMAT_TPW ? t : VMAT_TPW);
 }
 
 static void vmat_shape(int B, int nao_pad, int ld, int prec, int& KC, int& ntiles, int& ngroups,
@@ -879,17 +883,19 @@ static void vmat_shape(int B, int nao_pad, int ld, int prec, int& KC, int& ntile
     KC = prec == 64 ? 32 : 64;                 // 16 vectors of 16 B per row (16: 64 pairs)
     const int tpw = vmat_pipe_tpw(nao_pad);
     ngroups = (ntiles + 4 * tpw - 1) / (4 * tpw);
-    target_blocks = prec == 16 ? VH_TARGET_BLOCKS : 2048;
+    target_blocks = prec == 16 ? hqc_tuned(HQC_TUNE_VH_BLOCKS, VH_TARGET_BLOCKS)
+                               : hqc_tuned(HQC_TUNE_VMAT_BLOCKS, 2048);
   } else {
     KC = prec == 32 ? 32 : 16;                 // 8 vectors of 16 B per row
     const int NB = (nao_pad + VBIG - 1) / VBIG;
     ngroups = NB * (NB + 1) / 2;
-    target_blocks = VMAT_TARGET_BLOCKS;
+    target_blocks = hqc_tuned(HQC_TUNE_VBIG_BLOCKS, VMAT_TARGET_BLOCKS);
   }
   const int nchunks = ld / KC;
   long target = target_blocks / ((long)(B > 0 ? B : 1) * ngroups);
   if (target < 1) target = 1;
-  if (target > VMAT_MAX_PARTS) target = VMAT_MAX_PARTS;
+  const int max_parts = hqc_tuned(HQC_TUNE_VMAT_PARTS, VMAT_MAX_PARTS);
+  if (target > max_parts) target = max_parts;
   if (target > nchunks) target = nchunks;
   nwg = (int)target;
 }

@@ -308,8 +308,9 @@ extern "C" int hqc_orb_density_q8(const signed char* q, const short* pe, const i
   const int KS = nao_pad > 64 ? 2 : 1;
   const size_t lds = q8_lds_bytes(KS);
   const int npt = ld >> 4;
-  long nwg = Q8_TARGET_BLOCKS / B;
-  const long maxwg = (npt + Q8_NW * Q8_MIN_TILES - 1) / (Q8_NW * Q8_MIN_TILES);
+  long nwg = hqc_tuned(HQC_TUNE_Q8_BLOCKS, Q8_TARGET_BLOCKS) / B;
+  const long min_tiles = hqc_tuned(HQC_TUNE_Q8_MIN_TILES, Q8_MIN_TILES);
+  const long maxwg = (npt + Q8_NW * min_tiles - 1) / (Q8_NW * min_tiles);
   if (nwg > maxwg) nwg = maxwg;
   if (nwg < 1) nwg = 1;
   const dim3 grid((unsigned)nwg, B);

@@ -276,7 +276,8 @@ static int fft3d(C* data, int B, int g, int sign, void* stream) {
   const size_t plane_lds = (size_t)(2 * g * g + g) * sizeof(C);
   int first_axis = 2;
   if (plane_lds <= FFT_PLANE_MAX_LDS) {
-    const bool small = g <= 51 && (long)g * B >= 512;
+    const int nt = hqc_tuned(HQC_TUNE_FFT_PLANE_THREADS, 0);
+    const bool small = nt ? nt == 256 : (g <= 51 && (long)g * B >= 512);
     const void* kf = small ? (const void*)k_fft_plane<256, C> : (const void*)k_fft_plane<1024, C>;
     if (plane_lds > 64 * 1024 &&
         hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds) !=

@@ -154,7 +154,8 @@ def make_pes_batch(basis, rcut=24, tol=1e-7, max_cycle=100, grid_length=0.12, di
                    diis_space=8, diis_start_cycle=1, diis_damp=0.0, dft=False, xc="lda,vwn",
                    smearing=False, smearing_method="fermi", smearing_sigma=0.0,
                    search_method="bisect", search_cycle=100, search_tol=1e-7, Gmax=15,
-                   kappa=10, precision="fp64", max_batch=64, ovlp_cut=0.0, workers=1):
+                   kappa=10, precision="fp64", max_batch=64, ovlp_cut=0.0, workers=1,
+                   tuning=None):
     """make_pes's options (hqc/pbc/pes.py:10-31) for a list of systems of any size;
     workers > 1 solves that many (n, grid) buckets concurrently on their own streams."""
     return BatchPES(basis, rcut=rcut, grid_length=grid_length, max_batch=max_batch,
@@ -163,4 +164,4 @@ def make_pes_batch(basis, rcut=24, tol=1e-7, max_cycle=100, grid_length=0.12, di
                     search_method=search_method, search_cycle=search_cycle,
                     search_tol=search_tol, tol=tol, max_cycle=max_cycle, diis=diis,
                     diis_space=diis_space, diis_start_cycle=diis_start_cycle,
-                    diis_damp=diis_damp, precision=precision, ovlp_cut=ovlp_cut)
+                    diis_damp=diis_damp, precision=precision, ovlp_cut=ovlp_cut, tuning=tuning)

@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from .. import _lib as K
+from .. import tune
 from ..basis import HBasis
 
 RY = 2.0
@@ -186,8 +187,11 @@ class Engine:
     def __init__(self, cell, dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100, search_tol=1e-7,
                  tol=1e-7, max_cycle=100, diis=True, diis_space=8, diis_start_cycle=1,
-                 diis_damp=0.0, precision="fp64", ovlp_cut=0.0):
+                 diis_damp=0.0, precision="fp64", ovlp_cut=0.0, tuning=None):
         self.c = cell
+        # launch parameters from the autotuner (hqc_amd.tune), set on the host thread
+        # at the start of every solve; None keeps the library defaults
+        self.tuning = dict(tuning) if tuning else None
         self.ovlp_cut = float(ovlp_cut)
         self.dft = dft
         self.gga = False
@@ -1072,6 +1076,7 @@ class Engine:
         Per-geometry device state in st stays in slot order (st['order'][i] is
         the geometry held by slot i); the user-facing results mo, D, w1, E, conv,
         nit, Ki, Vep, Vee, Se, occ are returned in input order."""
+        tune.apply(self.tuning)
         st = self.setup(xpb, kpt, Lb, forces)
         mo, w1, E, conv, nit, order = self.scf(st)
         B = st["B"]

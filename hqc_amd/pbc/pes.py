@@ -69,7 +69,7 @@ def make_pes(n, L, rs, basis, rcut=24, tol=1e-7, max_cycle=100, grid_length=0.12
              diis_space=8, diis_start_cycle=1, diis_damp=0.0, use_jit=True, dft=False,
              xc="lda,vwn", smearing=False, smearing_method="fermi", smearing_sigma=0.0,
              search_method="bisect", search_cycle=100, search_tol=1e-7, gamma=True, Gmax=15,
-             kappa=10, mode="default", precision="fp64", ovlp_cut=0.0):
+             kappa=10, mode="default", precision="fp64", ovlp_cut=0.0, tuning=None):
     if mode not in ("default", "dev"):
         raise ValueError(f"Invalid mode: {mode}.")
     solver = make_solver(n, L, rs, basis, rcut=rcut, tol=tol, max_cycle=max_cycle,
@@ -79,5 +79,5 @@ def make_pes(n, L, rs, basis, rcut=24, tol=1e-7, max_cycle=100, grid_length=0.12
                          smearing_method=smearing_method, smearing_sigma=smearing_sigma,
                          search_method=search_method, search_cycle=search_cycle,
                          search_tol=search_tol, gamma=gamma, precision=precision,
-                         ovlp_cut=ovlp_cut)
+                         ovlp_cut=ovlp_cut, tuning=tuning)
     return PES(solver, kappa, Gmax, mode)

@@ -36,7 +36,8 @@ class GPWSolver:
                  diis=True, diis_space=8, diis_start_cycle=1, diis_damp=0.0, use_jit=True,
                  dft=False, xc="lda,vwn", smearing=False, smearing_method="fermi",
                  smearing_sigma=0.0, search_method="bisect", search_cycle=100,
-                 search_tol=1e-7, gamma=True, precision="fp64", device="cuda", ovlp_cut=0.0):
+                 search_tol=1e-7, gamma=True, precision="fp64", device="cuda", ovlp_cut=0.0,
+                 tuning=None):
         self.n, self.L, self.rs = n, L, rs
         self.gamma = gamma
         self.cell = Cell(n, L, rs, basis, rcut=rcut, grid_length=grid_length, device=device)
@@ -45,7 +46,8 @@ class GPWSolver:
                              search_method=search_method, search_cycle=search_cycle,
                              search_tol=search_tol, tol=tol, max_cycle=max_cycle, diis=diis,
                              diis_space=diis_space, diis_start_cycle=diis_start_cycle,
-                             diis_damp=diis_damp, precision=precision, ovlp_cut=ovlp_cut)
+                             diis_damp=diis_damp, precision=precision, ovlp_cut=ovlp_cut,
+                             tuning=tuning)
         self.last = None
 
     def _kpts(self, kpt, B):
@@ -86,7 +88,7 @@ def make_solver(n, L, rs, basis, rcut=24, tol=1e-7, max_cycle=100, grid_length=0
                 diis_space=8, diis_start_cycle=1, diis_damp=0.0, use_jit=True, dft=False,
                 xc="lda,vwn", smearing=False, smearing_method="fermi", smearing_sigma=0.0,
                 search_method="bisect", search_cycle=100, search_tol=1e-7, gamma=True,
-                precision="fp64", ovlp_cut=0.0):
+                precision="fp64", ovlp_cut=0.0, tuning=None):
     """hqc/pbc/solver.py:14 signature (search_method default 'bisect' there)."""
     return GPWSolver(n, L, rs, basis, rcut=rcut, tol=tol, max_cycle=max_cycle,
                      grid_length=grid_length, diis=diis, diis_space=diis_space,
@@ -94,4 +96,4 @@ def make_solver(n, L, rs, basis, rcut=24, tol=1e-7, max_cycle=100, grid_length=0
                      dft=dft, xc=xc, smearing=smearing, smearing_method=smearing_method,
                      smearing_sigma=smearing_sigma, search_method=search_method,
                      search_cycle=search_cycle, search_tol=search_tol, gamma=gamma,
-                     precision=precision, ovlp_cut=ovlp_cut)
+                     precision=precision, ovlp_cut=ovlp_cut, tuning=tuning)

@@ -41,6 +41,26 @@ extern "C" {
 /* Library version (major*100 + minor). */
 int hqc_version(void);
 
+/* Launch parameters of the calling host thread (no counterpart in the
+ * reference, whose XLA programs are tuned by its compiler): set by the project's
+ * autotuner (hqc_amd/tune.py) for a workload shape.  0 = built-in default.
+ * They change how the work is split, not what is computed (partial sums of the
+ * potential matrix are then added in another grouping).  Thread-local, so host
+ * threads that drive different streams keep their own values.
+ * hqc_set_tuning returns HQC_E_ARG for an unknown key or a value out of range;
+ * hqc_get_tuning returns the value (-1 for an unknown key). */
+#define HQC_TUNE_VMAT_BLOCKS 0       /* workgroups aimed at, FP64/FP32 V, n_ao <= 128 (2048) */
+#define HQC_TUNE_VH_BLOCKS 1         /* the same for the split-FP16 V (2048) */
+#define HQC_TUNE_VMAT_PARTS 2        /* grid chunks per geometry at most (64) */
+#define HQC_TUNE_VMAT_TPW 3          /* upper-triangle tiles per wave, 1..8 (smallest that covers) */
+#define HQC_TUNE_Q8_BLOCKS 4         /* int8-digit density: workgroups aimed at (2048) */
+#define HQC_TUNE_Q8_MIN_TILES 5      /* int8-digit density: point tiles per wave at least (8) */
+#define HQC_TUNE_FFT_PLANE_THREADS 6 /* 256 or 1024 threads per y-z plane (by size) */
+#define HQC_TUNE_VBIG_BLOCKS 7       /* workgroups aimed at, V for n_ao > 128 (1024) */
+#define HQC_TUNE_COUNT 8
+int hqc_set_tuning(int key, int value);
+int hqc_get_tuning(int key);
+
 /* 1-D periodic Gaussian-power tables.
  * Replaces eval_pbc_gaussian_power_x (hqc/pbc/gto.py:326-339) vmapped over
  * atoms and the 1-D grid (hqc/pbc/solver.py:211-214).

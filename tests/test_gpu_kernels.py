@@ -692,3 +692,32 @@ def test_geometry_index_lists(K, dt):
     K.ao_vmat(phi, v, 0.7, V1, work, bidx=idx)
     K.ao_vmat(sub, v, 0.7, V2, work)
     assert torch.equal(V1, V2)
+
+
+def test_autotune_changes_split_not_results(K):
+    """The autotuner at a small shape returns one setting per tunable kernel; the
+    potential matrix under any tile split equals the default one to rounding (partial
+    sums grouped differently), and the table is restored afterwards."""
+    from hqc_amd import tune
+    from hqc_amd.pbc import make_pes
+    n = 16
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    eng = make_pes(n, L, 1.25, "gth-dzv", grid_length=0.3, dft=True,
+                   precision="mixed").solver.engine
+    rec = tune.autotune(eng, 3, reps=1)
+    assert {"vh_blocks", "vmat_parts", "vmat_tpw", "q8_blocks", "q8_min_tiles",
+            "fft_plane_threads"} <= set(rec["choice"])
+    assert all(K.get_tuning(k) == 0 for k in K.TUNE_KEYS)
+    c = eng.c
+    torch.manual_seed(4)
+    phi = torch.zeros(3, c.nao_pad, c.ld, dtype=torch.float64, device=DEV)
+    phi[:, :c.nao, :c.N] = torch.rand(3, c.nao, c.N, dtype=torch.float64, device=DEV)
+    v = torch.zeros(3, c.ld, dtype=torch.float64, device=DEV)
+    v[:, :c.N] = torch.rand(3, c.N, dtype=torch.float64, device=DEV)
+    V0 = eng.vmat(phi, v)
+    for tpw, parts in ((1, 2), (3, 64), (8, 5)):
+        tune.apply({"vmat_tpw": tpw, "vmat_parts": parts})
+        V1 = eng.vmat(phi, v)
+        torch.cuda.synchronize()
+        assert (V1 - V0).abs().max().item() <= 1e-13 * V0.abs().max().item()
+    tune.apply({})

@@ -65,3 +65,23 @@ def test_batch_buckets_by_atom_count_and_grid():
     Lb = [uniform_gas_length(n) * r for n, r in zip(ns, rs)]
     b = pes.buckets(ns, Lb)
     assert b == {(3, 11): [0, 1, 4], (2, 9): [2], (3, 13): [3]}
+
+
+def test_tuning_table_is_per_thread_and_checked(libpath):
+    """The launch-parameter table (host only): values are kept per host thread,
+    bad keys and out-of-range values are refused, 0 restores the default."""
+    import threading
+    from hqc_amd import _lib
+    _lib.set_tuning("vmat_tpw", 6)
+    seen = []
+    t = threading.Thread(target=lambda: seen.append(_lib.get_tuning("vmat_tpw")))
+    t.start()
+    t.join()
+    assert _lib.get_tuning("vmat_tpw") == 6 and seen == [0]
+    with pytest.raises(RuntimeError):
+        _lib.set_tuning("vmat_tpw", 9)
+    with pytest.raises(RuntimeError):
+        _lib.set_tuning("fft_plane_threads", 512)
+    assert _lib.lib().hqc_set_tuning(99, 1) == 1 and _lib.lib().hqc_get_tuning(99) == -1
+    _lib.set_tuning("vmat_tpw", 0)
+    assert _lib.get_tuning("vmat_tpw") == 0
