@@ -66,6 +66,7 @@ _SIGS = {
     "hqc_eigh_lds_bytes": [_I],
     "hqc_eigh_jacobi": [_P, _I, _I, _P, _I, _D, _P, _P, _P, _P],
     "hqc_overlap_kinetic": [_P, _I, _I, _P, _I, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P],
+    "hqc_overlap_kinetic_s": [_P, _I, _I, _P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P],
     "hqc_ewald_pairs": [_P, _I, _I, _P, _P, _I, _D, _D, _P, _P],
     "hqc_ewald_recip": [_P, _I, _I, _P, _P, _I, _P, _P, _P, _P],
     "hqc_force_blocks": [_I],
@@ -519,9 +520,16 @@ def eigh_jacobi(A, max_sweeps=30, tol=1e-14, sweeps=None, pairs=None):
     return w, V
 
 
-def overlap_kinetic(xp, lattice, alpha, coef, comp, S, T, dS=None, dT=None):
+def overlap_kinetic(xp, lattice, alpha, coef, comp, S, T, dS=None, dT=None, s_only=False):
+    """S, T (and centre derivatives) by lattice sums; s_only: the basis has s functions
+    only (the caller's knowledge of comp), which takes the specialised kernel."""
     B, n, _ = xp.shape
     na, nloc = coef.shape
+    if s_only and na == 4 and nloc <= 2:
+        _call("hqc_overlap_kinetic_s", _ptr(xp, F64, "xp"), B, n, _ptr(lattice, F64, "lattice"),
+              lattice.shape[0], _ptr(alpha, F64), na, _ptr(coef, F64), nloc, _ptr(S, F64, "S"),
+              _ptr(T, F64, "T"), _ptr(dS, F64, "dS"), _ptr(dT, F64, "dT"), _stream())
+        return
     _call("hqc_overlap_kinetic", _ptr(xp, F64, "xp"), B, n, _ptr(lattice, F64, "lattice"),
           lattice.shape[0], _ptr(alpha, F64), na, _ptr(coef, F64), _ptr(comp, I32), nloc,
           _ptr(S, F64, "S"), _ptr(T, F64, "T"), _ptr(dS, F64, "dS"), _ptr(dT, F64, "dT"),

@@ -214,6 +214,117 @@ __global__ __launch_bounds__(256) void k_overlap_kinetic(
   }
 }
 
+// s-only bases (every comp = -1: gth-szv, gth-dzv) at Gamma: one thread per
+// unordered atom pair (b, A <= Bat) computes every contracted pair (o, q) of the
+// two atoms, so the 4 x 4 x nT lattice exponentials are formed once per atom pair
+// instead of once per (o, q); sizes are compile-time, so the Hermite factors stay
+// in registers.  Every entry is summed in the order of k_overlap_kinetic (lattice
+// terms per primitive pair, then the primitive pairs); the two kernels agree to
+// rounding (here the compiler fuses some products into their sums).
+template <int NA, int NLOC, bool DERIV>
+__global__ __launch_bounds__(256) void k_overlap_kinetic_s(
+    const double* __restrict__ xp, long total, int n, const double* __restrict__ lat, int nT,
+    const double* __restrict__ alpha, const double* __restrict__ coef, double* __restrict__ S,
+    double* __restrict__ Tm, double* __restrict__ dS, double* __restrict__ dT) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const long npair = (long)n * (n + 1) / 2;
+  const int pr = (int)(idx % npair);
+  const int b = (int)(idx / npair);
+  int A = (int)((2.0 * n + 1.0 - sqrt((2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * pr)) * 0.5);
+  if (A < 0) A = 0;
+  if (A > n - 1) A = n - 1;
+  while (A > 0 && A * n - A * (A - 1) / 2 > pr) --A;
+  while (A < n - 1 && (A + 1) * n - (A + 1) * A / 2 <= pr) ++A;
+  const int Bat = A + (pr - (A * n - A * (A - 1) / 2));
+  const double* xA = xp + ((long)b * n + A) * 3;
+  const double* xB = xp + ((long)b * n + Bat) * 3;
+  const double d0 = xA[0] - xB[0], d1 = xA[1] - xB[1], d2 = xA[2] - xB[2];
+  constexpr int NM = 2 + (DERIV ? 1 : 0);
+  double s[NLOC][NLOC], t[NLOC][NLOC], ds[NLOC][NLOC][3], dt[NLOC][NLOC][3];
+#pragma unroll
+  for (int o = 0; o < NLOC; ++o)
+#pragma unroll
+    for (int q = 0; q < NLOC; ++q) {
+      s[o][q] = 0.0;
+      t[o][q] = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ds[o][q][c] = dt[o][q][c] = 0.0;
+    }
+  for (int ia = 0; ia < NA; ++ia) {
+    for (int ib = 0; ib < NA; ++ib) {
+      const double a = alpha[ia], bb = alpha[ib];
+      const double mu = a * bb / (a + bb);
+      double ss = 0.0, tt = 0.0, dss[3] = {0.0, 0.0, 0.0}, dtt[3] = {0.0, 0.0, 0.0};
+      for (int it = 0; it < nT; ++it) {
+        const double* T3 = lat + 3 * it;
+        const double R[3] = {d0 + T3[0], d1 + T3[1], d2 + T3[2]};
+        const double mr2 = mu * (R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+        if (mr2 > INT_EXP_CUT) continue;
+        const double E = exp(-mr2);
+        double h[3][NM + 1];
+        hermite(mu, R[0], h[0], NM);
+        hermite(mu, R[1], h[1], NM);
+        hermite(mu, R[2], h[2], NM);
+        ss += E;
+        tt += E * ((h[0][2] + h[1][2]) + h[2][2]);
+        if (DERIV) {
+          dss[0] += E * h[0][1];
+          dss[1] += E * h[1][1];
+          dss[2] += E * h[2][1];
+          dtt[0] += E * ((h[0][3] + h[0][1] * h[1][2]) + h[0][1] * h[2][2]);
+          dtt[1] += E * ((h[0][2] * h[1][1] + h[1][3]) + h[1][1] * h[2][2]);
+          dtt[2] += E * ((h[0][2] * h[2][1] + h[1][2] * h[2][1]) + h[2][3]);
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < NLOC; ++o) {
+        const double wa = coef[ia * NLOC + o];
+        if (wa == 0.0) continue;
+#pragma unroll
+        for (int q = 0; q < NLOC; ++q) {
+          const double wb = coef[ib * NLOC + q];
+          if (wb == 0.0) continue;
+          const double fac = wa * wb * M_PI * sqrt(M_PI) * pow(a + bb, -1.5);
+          s[o][q] += fac * ss;
+          t[o][q] += -0.5 * fac * tt;
+          if (DERIV) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              ds[o][q][c] += fac * dss[c];
+              dt[o][q][c] += -0.5 * fac * dtt[c];
+            }
+          }
+        }
+      }
+    }
+  }
+  const long nao = (long)n * NLOC;
+#pragma unroll
+  for (int o = 0; o < NLOC; ++o)
+#pragma unroll
+    for (int q = 0; q < NLOC; ++q) {
+      if (Bat == A && q < o) continue;
+      const long e = ((long)b * nao + (long)A * NLOC + o) * nao + (long)Bat * NLOC + q;
+      const long em = ((long)b * nao + (long)Bat * NLOC + q) * nao + (long)A * NLOC + o;
+      S[e] = s[o][q];
+      Tm[e] = t[o][q];
+      if (DERIV)
+        for (int c = 0; c < 3; ++c) {
+          dS[e * 3 + c] = ds[o][q][c];
+          dT[e * 3 + c] = dt[o][q][c];
+        }
+      if (em == e) continue;
+      S[em] = s[o][q];
+      Tm[em] = t[o][q];
+      if (DERIV)
+        for (int c = 0; c < 3; ++c) {
+          dS[em * 3 + c] = -ds[o][q][c];
+          dT[em * 3 + c] = -dt[o][q][c];
+        }
+    }
+}
+
 template <bool CPLX>
 static int launch_ovlp(const double* xp, const double* kpt, int B, int n, const double* lattice,
                        int nT, const double* alpha, int na, const double* coef, const int* comp,
@@ -245,4 +356,27 @@ extern "C" int hqc_overlap_kinetic_k(const double* xp, const double* kpt, int B,
                                      double* T, double* dS, double* dT, void* stream) {
   return launch_ovlp<true>(xp, kpt, B, n, lattice, nT, alpha, na, coef, comp, nloc, S, T, dS,
                            dT, stream);
+}
+
+extern "C" int hqc_overlap_kinetic_s(const double* xp, int B, int n, const double* lattice, int nT,
+                                     const double* alpha, int na, const double* coef, int nloc,
+                                     double* S, double* T, double* dS, double* dT, void* stream) {
+  if (B < 0 || n <= 0 || nT <= 0 || na <= 0 || nloc <= 0) return HQC_E_ARG;
+  if ((dS == nullptr) != (dT == nullptr)) return HQC_E_ARG;
+  if (na != 4 || nloc > 2) return HQC_E_SIZE;
+  if (B == 0) return HQC_OK;
+  const long pairs = (long)B * (n * (n + 1) / 2);
+  const dim3 grid(hqc_div_up(pairs, 256));
+  const bool der = dS != nullptr;
+  hipStream_t st = (hipStream_t)stream;
+#define OVLP_S(NL, D)                                                                           \
+  hipLaunchKernelGGL((k_overlap_kinetic_s<4, NL, D>), grid, dim3(256), 0, st, xp, pairs, n, lattice, \
+                     nT, alpha, coef, S, T, dS, dT)
+  if (nloc == 1 && !der) OVLP_S(1, false);
+  else if (nloc == 1) OVLP_S(1, true);
+  else if (!der) OVLP_S(2, false);
+  else OVLP_S(2, true);
+#undef OVLP_S
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
 }

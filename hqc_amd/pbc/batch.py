@@ -100,7 +100,8 @@ class BatchPES:
                 Eb = st["E"] * RY + st["Epp"]
                 # the n_ao x n_ao results of the chunk (S and the slot order as the SCF
                 # left them; D, F, bands in the chunk's input order), not the grid arrays
-                light = {k: st[k] for k in ("S", "D", "F", "w1", "order", "conv", "nit")}
+                light = {k: st[k] for k in ("S", "D", "F", "w1", "order", "conv", "nit")
+                         if k in st}
                 light["index"] = list(part)
                 out.append((s0, part, Eb, Fb, st["conv"].tolist(), st["nit"].tolist(), light))
             if concurrent:

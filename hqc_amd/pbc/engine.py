@@ -501,7 +501,8 @@ class Engine:
                 dSg = dS if idx is None else torch.empty(nb, c.nao, c.nao, 3, dtype=f64,
                                                          device=dev)
                 dTg = dT if idx is None else torch.empty_like(dSg)
-            K.overlap_kinetic(x, cst.lattice, c.alpha, c.coef, c.comp, Sg, Tg, dSg, dTg)
+            K.overlap_kinetic(x, cst.lattice, c.alpha, c.coef, c.comp, Sg, Tg, dSg, dTg,
+                              s_only=c.basis.l_max == 0)
             self._put(S, idx, Sg)
             self._put(T, idx, Tg)
             if forces:
@@ -830,8 +831,9 @@ class Engine:
             self.stats["v64"] += 1
         return V2
 
-    def fock2e(self, st, na, D, mo, occ, full=False):
-        """(J + V_xc or J - K/2, E_H + E_xc or E_H + E_x) for geometries [0, na)."""
+    def fock2e(self, st, na, D, mo, occ, full=False, need_v=True):
+        """(J + V_xc or J - K/2, E_H + E_xc or E_H + E_x) for geometries [0, na).
+        need_v=False (FP64 DFT only): the energies alone, V is None."""
         c = self.c
         mixed = self.precision == "mixed" and not full
         dVs = self._dv(st, na)
@@ -841,8 +843,10 @@ class Engine:
             rho = self.orb_density(st["phi"], mo, occ, gi)
             self.stats["rho64"] += 1
             v, E, _ = self.xc_potential(rho, geo=geo)
-            V2 = self.vmat(st["phi"], v, dVs, gi)
-            self.stats["v64"] += 1
+            V2 = None
+            if need_v or not self.dft:
+                V2 = self.vmat(st["phi"], v, dVs, gi)
+                self.stats["v64"] += 1
         else:
             rho = self._mixed_density(st, na, D, mo, occ)
             v, E, _ = self.xc_potential(rho, geo=geo)
@@ -1081,11 +1085,14 @@ class Engine:
         mo, w1, E, conv, nit, order = self.scf(st)
         B = st["B"]
         D, occ = self.density_matrix(mo, w1)                              # solver.py:758-765
-        V2, E2 = self.fock2e(st, B, D, mo, occ, full=True)
+        # the Fock matrix of the final density is needed for HF (exchange enters the
+        # residual checks) and for the forces of a cut basis; DFT energies need rho only
+        need_v = not self.dft or "lindep" in st
+        V2, E2 = self.fock2e(st, B, D, mo, occ, full=True, need_v=need_v)
         Ki = _tr(st["T"], D).real
         Vep = _tr(st["V"], D).real
         Se = self.entropy(occ)
-        F = st["Hcore"] + V2                                   # Fock matrix of the final D
+        F = st["Hcore"] + V2 if V2 is not None else None       # Fock matrix of the final D
         slot = dict(mo=mo, w1=w1, D=D, occ=occ, E=E, F=F)
         st["slot"] = slot
         st["order"] = order
@@ -1097,6 +1104,6 @@ class Engine:
 
         st.update({k: unperm(t) for k, t in dict(mo=mo, w1=w1, D=D, occ=occ, E=E, conv=conv,
                                                  nit=nit, Ki=Ki, Vep=Vep, Vee=E2,
-                                                 Se=Se, F=F).items()})
+                                                 Se=Se, F=F).items() if t is not None})
         st["unperm"] = unperm
         return st

@@ -186,7 +186,8 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
             dSg = dS if idx is None else torch.empty(nb, c.nao, c.nao, 3, dtype=f64,
                                                      device=dev)
             dTg = dT if idx is None else torch.empty_like(dSg)
-            K.overlap_kinetic(x, cst.lattice, c.alpha, c.coef, c.comp, S, T, dSg, dTg)
+            K.overlap_kinetic(x, cst.lattice, c.alpha, c.coef, c.comp, S, T, dSg, dTg,
+                              s_only=c.basis.l_max == 0)
             put(dS, idx, dSg)
             put(dT, idx, dTg)
     W = (mo * (occ * w1).unsqueeze(-2)) @ mo.transpose(-1, -2)

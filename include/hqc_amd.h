@@ -345,6 +345,14 @@ int hqc_overlap_kinetic(const double* xp, int B, int n, const double* lattice, i
                         int nloc, double* S, double* T, double* dS, double* dT,
                         void* stream);
 
+/* The same for bases of s functions only (comp all -1, e.g. gth-szv, gth-dzv) with
+ * na = 4 primitives and nloc <= 2 functions per atom (HQC_E_SIZE otherwise): one
+ * thread per atom pair forms each lattice exponential once for all its contracted
+ * pairs; every entry is summed in hqc_overlap_kinetic's order (equal to rounding). */
+int hqc_overlap_kinetic_s(const double* xp, int B, int n, const double* lattice, int nT,
+                          const double* alpha, int na, const double* coef, int nloc, double* S,
+                          double* T, double* dS, double* dT, void* stream);
+
 /* k-point overlap and kinetic matrices, eval_overlap_kinetic_kpt
  * (hqc/pbc/solver.py:257-305): every lattice term carries exp(-i k.T)
  * (:270-271).  kpt (B,3) 1/Bohr; S,T (B,n_ao,n_ao) complex128 interleaved;

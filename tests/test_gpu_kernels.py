@@ -721,3 +721,32 @@ def test_autotune_changes_split_not_results(K):
         torch.cuda.synchronize()
         assert (V1 - V0).abs().max().item() <= 1e-13 * V0.abs().max().item()
     tune.apply({})
+
+
+@pytest.mark.parametrize("basis", ["gth-szv", "gth-dzv"])
+def test_overlap_kinetic_s_only_matches_generic(K, basis):
+    """The s-only lattice-sum kernel (one thread per atom pair) against the generic
+    one: the same S, T and centre derivatives to rounding (the same sums; the compiler
+    fuses some products into the accumulation there, one rounding fewer)."""
+    from hqc_amd.pbc.engine import Cell
+    n = 9
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    cell = Cell(n, L, 1.3, basis, grid_length=0.3)
+    xp = torch.tensor(np.random.default_rng(2).uniform(0, L * 1.3, (5, n, 3)), device=DEV)
+    out = []
+    for s_only in (False, True):
+        S = torch.empty(5, cell.nao, cell.nao, dtype=torch.float64, device=DEV)
+        T = torch.empty_like(S)
+        dS = torch.empty(5, cell.nao, cell.nao, 3, dtype=torch.float64, device=DEV)
+        dT = torch.empty_like(dS)
+        K.overlap_kinetic(xp, cell.lattice, cell.alpha, cell.coef, cell.comp, S, T, dS, dT,
+                          s_only=s_only)
+        S2 = torch.empty_like(S)
+        T2 = torch.empty_like(S)
+        K.overlap_kinetic(xp, cell.lattice, cell.alpha, cell.coef, cell.comp, S2, T2,
+                          s_only=s_only)
+        out.append((S, T, dS, dT, S2, T2))
+    torch.cuda.synchronize()
+    for a, b in zip(*out):
+        d = (a - b).abs().max().item()
+        assert d <= 1e-15 * max(a.abs().max().item(), 1.0), d
