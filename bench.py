@@ -387,8 +387,13 @@ def run_workload(a):
            "scaling": "weak", "vs_baseline": None, "data": "synthetic: uniform-random protons"}
     if a.workload == "hf-batch":
         ns, rss, xps = hf_batch_workload(a.batch)
+        t_start = time.perf_counter()
+
+        def progress(n, g, k):
+            print(f"bucket n={n} g={g}: {k} geometries done "
+                  f"({time.perf_counter() - t_start:.1f} s)", file=sys.stderr, flush=True)
         pes = make_pes_batch("gth-dzv", grid_length=GRID_LENGTH, dft=False, tol=TOL,
-                             precision=a.path, workers=a.workers)
+                             precision=a.path, workers=a.workers, progress=progress)
         run = lambda: pes.energy_and_forces(xps, rss)
         units = len(xps)
         out.update(metric="energy+force geometries/sec, uneven HF PES batch (gth-dzv)",

@@ -42,11 +42,12 @@ class BatchPES:
     on the same data; only the interleaving differs)."""
 
     def __init__(self, basis, rcut=24, grid_length=0.12, max_batch=64, device="cuda",
-                 kappa=10, Gmax=15, workers=1, **engine_kw):
+                 kappa=10, Gmax=15, workers=1, progress=None, **engine_kw):
         self.basis, self.rcut, self.grid_length = basis, rcut, grid_length
         self.max_batch, self.device = int(max_batch), device
         self.kappa, self.Gmax = kappa, Gmax
         self.workers = max(1, int(workers))
+        self.progress = progress      # called with (n, g, geometries) after each bucket
         self.engine_kw = engine_kw
         self._engines = {}
         self.last_states = {}
@@ -106,6 +107,8 @@ class BatchPES:
                 out.append((s0, part, Eb, Fb, st["conv"].tolist(), st["nit"].tolist(), light))
             if concurrent:
                 stream.synchronize()
+        if self.progress is not None:
+            self.progress(n, g, len(idx))
         return out
 
     def energy_and_forces(self, xps, rs, L=None):
@@ -156,11 +159,12 @@ def make_pes_batch(basis, rcut=24, tol=1e-7, max_cycle=100, grid_length=0.12, di
                    smearing=False, smearing_method="fermi", smearing_sigma=0.0,
                    search_method="bisect", search_cycle=100, search_tol=1e-7, Gmax=15,
                    kappa=10, precision="fp64", max_batch=64, ovlp_cut=0.0, workers=1,
-                   tuning=None):
+                   tuning=None, progress=None):
     """make_pes's options (hqc/pbc/pes.py:10-31) for a list of systems of any size;
     workers > 1 solves that many (n, grid) buckets concurrently on their own streams."""
     return BatchPES(basis, rcut=rcut, grid_length=grid_length, max_batch=max_batch,
-                    kappa=kappa, Gmax=Gmax, workers=workers, dft=dft, xc=xc, smearing=smearing,
+                    kappa=kappa, Gmax=Gmax, workers=workers, progress=progress, dft=dft, xc=xc,
+                    smearing=smearing,
                     smearing_method=smearing_method, smearing_sigma=smearing_sigma,
                     search_method=search_method, search_cycle=search_cycle,
                     search_tol=search_tol, tol=tol, max_cycle=max_cycle, diis=diis,
