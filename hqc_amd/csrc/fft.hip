@@ -1,29 +1,75 @@
 // Batched 3-D complex FFT for the odd (and even) grids of the GPW solver.
 //
-// Up to g = 69 the y and z axes are done together on whole planes in LDS
-// (k_fft_plane), then the x axis by lines; larger grids go axis by axis.  A
-// line block stages `LPB` lines of length g in LDS
-// (position-major, line-minor so that strided axes load coalesced), runs the
-// mixed-radix Stockham passes between two LDS buffers, and writes the lines
-// back in place.  Radix butterflies are specialised for 2,3,4,5,7,11,13; any
-// other prime factor uses a direct per-output DFT pass.  Twiddles come from a
-// length-g table exp(sign 2 pi i m/g) built in LDS with sincospi.
+// When a whole (y, z) plane fits in LDS (g <= 69 with two buffers, g <= 98 in
+// place, FP64) the y and z axes are done together on planes (k_fft_plane), then
+// the x axis by lines; larger grids go axis by axis.  A line block stages `lpb`
+// lines of length g in LDS (position-major, line-minor so that strided axes load
+// coalesced) and runs mixed-radix Stockham passes, either between two LDS
+// buffers or in place (a thread reads the inputs of its work items into
+// registers, the block synchronises, and the outputs are written over them).
+//
+// Arithmetic.  Every output is formed by the same sequence of explicit FMAs,
+// adds and multiplies whichever schedule (buffers, threads, items per thread)
+// runs it, so all schedules give the same bits:
+//  * radix 2 and 4: adds only (the constants are +-1 and +-i);
+//  * odd radix R <= 13: the twiddled inputs v_r are paired,
+//        a_r = v_r + v_(R-r),  b_r = v_r - v_(R-r),  r = 1 .. (R-1)/2,
+//        X_0 = v_0 + sum_r a_r,
+//        X_k = T1 + i s T2,  X_(R-k) = T1 - i s T2,   k = 1 .. (R-1)/2,
+//        T1 = v_0 + sum_r a_r cos(2 pi r k / R),  T2 = sum_r b_r sin(2 pi r k / R),
+//    s = the transform's sign, with the cosines and sines as compile-time
+//    constants: a quarter of the multiplies of the direct R-point sum and no
+//    table reads;
+//  * any other factor (17, 19, 23, ..., and a prime length > 13 as the only
+//    stage): the same pairing in two sub-passes, first (a_r, b_r) over the
+//    twiddled inputs in place, then FFT_PAIRS output pairs per work item with
+//    cos and s*sin read from the twiddle table.
+// Inter-stage twiddles come from a length-g table exp(s 2 pi i m/g) built in
+// LDS with sincospi.
 #include "common.h"
 
 #define FFT_MAX_G 128
 #define FFT_MAX_STAGES 12
 #define FFT_THREADS 256
+#define FFT_PAIRS 4  // output pairs (X_k, X_(R-k)) per work item, generic factor
 
 struct FftPlan {
   int nst;
   int rad[FFT_MAX_STAGES];
 };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+__device__ __forceinline__ double sfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float sfma(float a, float b, float c) { return fmaf(a, b, c); }
+
+// complex product with explicit FMAs (no contraction left to the compiler)
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+  C r;
+  r.x = sfma(a.x, b.x, -(a.y * b.y));
+  r.y = sfma(a.x, b.y, a.y * b.x);
+  return r;
 }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) {
+  C r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) {
+  C r;
+  r.x = a.x - b.x;
+  r.y = a.y - b.y;
+  return r;
+}
+// t1 + i t2 and t1 - i t2
+template <typename C>
+__device__ __forceinline__ void pm_i(C t1, C t2, C& p, C& m) {
+  p.x = t1.x - t2.y;
+  p.y = t1.y + t2.x;
+  m.x = t1.x + t2.y;
+  m.y = t1.y - t2.x;
 }
 template <typename C> __device__ __forceinline__ C czero();
 template <> __device__ __forceinline__ double2 czero<double2>() { return make_double2(0.0, 0.0); }
@@ -41,11 +87,131 @@ template <> __device__ __forceinline__ float2 twiddle<float2>(int m, int g, int 
   return make_float2((float)c, (float)(sign < 0 ? -s : s));
 }
 
-// Stockham pass with compile-time radix R (see the header comment).  Element
-// (position p, line) of the lpb lines lives at p*ps + line*ls in LDS.
+// cos and sin of 2 pi m / R for m = 1 .. (R-1)/2, rounded to FP64 from
+// extended-precision values
+__host__ __device__ constexpr double bf_cos(int R, int m) {
+  switch (R * 16 + m) {
+    case 3 * 16 + 1: return -0.5;
+    case 5 * 16 + 1: return 0.30901699437494745;
+    case 5 * 16 + 2: return -0.8090169943749475;
+    case 7 * 16 + 1: return 0.6234898018587335;
+    case 7 * 16 + 2: return -0.2225209339563144;
+    case 7 * 16 + 3: return -0.9009688679024191;
+    case 11 * 16 + 1: return 0.8412535328311812;
+    case 11 * 16 + 2: return 0.41541501300188644;
+    case 11 * 16 + 3: return -0.14231483827328514;
+    case 11 * 16 + 4: return -0.6548607339452851;
+    case 11 * 16 + 5: return -0.9594929736144974;
+    case 13 * 16 + 1: return 0.8854560256532099;
+    case 13 * 16 + 2: return 0.5680647467311558;
+    case 13 * 16 + 3: return 0.12053668025532305;
+    case 13 * 16 + 4: return -0.3546048870425356;
+    case 13 * 16 + 5: return -0.7485107481711011;
+    case 13 * 16 + 6: return -0.970941817426052;
+  }
+  return 0.0;
+}
+__host__ __device__ constexpr double bf_sin(int R, int m) {
+  switch (R * 16 + m) {
+    case 3 * 16 + 1: return 0.8660254037844386;
+    case 5 * 16 + 1: return 0.9510565162951535;
+    case 5 * 16 + 2: return 0.5877852522924731;
+    case 7 * 16 + 1: return 0.7818314824680298;
+    case 7 * 16 + 2: return 0.9749279121818236;
+    case 7 * 16 + 3: return 0.4338837391175581;
+    case 11 * 16 + 1: return 0.5406408174555976;
+    case 11 * 16 + 2: return 0.9096319953545183;
+    case 11 * 16 + 3: return 0.9898214418809327;
+    case 11 * 16 + 4: return 0.7557495743542583;
+    case 11 * 16 + 5: return 0.28173255684142967;
+    case 13 * 16 + 1: return 0.46472317204376856;
+    case 13 * 16 + 2: return 0.8229838658936564;
+    case 13 * 16 + 3: return 0.992708874098054;
+    case 13 * 16 + 4: return 0.9350162426854148;
+    case 13 * 16 + 5: return 0.6631226582407952;
+    case 13 * 16 + 6: return 0.23931566428755777;
+  }
+  return 0.0;
+}
+// the same for any m = 1 .. R-1 (cos symmetric, sin antisymmetric about R/2)
+__host__ __device__ constexpr double bf_c(int R, int m) {
+  return 2 * m < R ? bf_cos(R, m) : bf_cos(R, R - m);
+}
+__host__ __device__ constexpr double bf_s(int R, int m) {
+  return 2 * m < R ? bf_sin(R, m) : -bf_sin(R, R - m);
+}
+
+// In-register R-point DFT X_k = sum_r v_r exp(sign 2 pi i r k / R), outputs in
+// natural order over the inputs.
 template <int R, typename C>
-__device__ void pass_r(const C* __restrict__ in, C* __restrict__ out,
-                       const C* __restrict__ tw, int N, int Ns, int lpb, int ps, int ls) {
+__device__ __forceinline__ void bfly(C (&v)[R], int sign) {
+  typedef decltype(v[0].x) T;
+  if constexpr (R == 2) {
+    const C a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    const C s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    C p, m;
+    pm_i(d02, d13, p, m);
+    v[1] = sign > 0 ? p : m;
+    v[3] = sign > 0 ? m : p;
+  } else {
+    constexpr int H = (R - 1) / 2;
+    C a[H], b[H];
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+      a[r - 1] = cadd(v[r], v[R - r]);
+      b[r - 1] = csub(v[r], v[R - r]);
+    }
+    const C x0 = v[0];
+    C s = x0;
+#pragma unroll
+    for (int r = 0; r < H; ++r) s = cadd(s, a[r]);
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+      C t1 = x0, t2 = czero<C>();
+#pragma unroll
+      for (int r = 1; r <= H; ++r) {
+        const T c = (T)bf_c(R, (r * k) % R), sn = (T)bf_s(R, (r * k) % R);
+        t1.x = sfma(a[r - 1].x, c, t1.x);
+        t1.y = sfma(a[r - 1].y, c, t1.y);
+        t2.x = sfma(b[r - 1].x, sn, t2.x);
+        t2.y = sfma(b[r - 1].y, sn, t2.y);
+      }
+      C p, m;
+      pm_i(t1, t2, p, m);
+      v[k] = sign > 0 ? p : m;
+      v[R - k] = sign > 0 ? m : p;
+    }
+    v[0] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Passes.  Element (position p, line) of the lpb lines lives at p*ps + line*ls.
+// Stage with radix R after Ns (the product of the earlier radices): work item
+// (line, j), j < N/R, reads positions j + r N/R, multiplies input r by
+// tw[k r N/(Ns R)] (k = j mod Ns; all ones when Ns = 1, skipped: exact), and
+// writes output kk to (j/Ns) Ns R + k + kk Ns.
+// ---------------------------------------------------------------------------
+template <int R, typename C>
+__device__ __forceinline__ void radix_item(C (&v)[R], const C* __restrict__ tw, int Ns, int k,
+                                           int step, int sign) {
+  if (Ns > 1) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[k * r * step]);
+  }
+  bfly<R>(v, sign);
+}
+
+template <int R, typename C>
+__device__ void pass_radix(const C* __restrict__ in, C* __restrict__ out,
+                           const C* __restrict__ tw, int N, int Ns, int lpb, int ps, int ls,
+                           int sign) {
   const int NR = N / R;
   const int step = N / (Ns * R);
   for (int it = threadIdx.x; it < NR * lpb; it += blockDim.x) {
@@ -54,107 +220,190 @@ __device__ void pass_r(const C* __restrict__ in, C* __restrict__ out,
     C v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = in[(j + r * NR) * ps + line * ls];
-#pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[k * r * step]);
+    radix_item<R>(v, tw, Ns, k, step, sign);
     const int idx = (j / Ns) * Ns * R + k;
 #pragma unroll
-    for (int kk = 0; kk < R; ++kk) {
-      C acc = v[0];
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        const C wr = tw[((r * kk) % R) * NR];
-        acc.x += v[r].x * wr.x - v[r].y * wr.y;
-        acc.y += v[r].x * wr.y + v[r].y * wr.x;
-      }
-      out[(idx + kk * Ns) * ps + line * ls] = acc;
-    }
+    for (int kk = 0; kk < R; ++kk) out[(idx + kk * Ns) * ps + line * ls] = v[kk];
   }
 }
 
-// Generic radix: one work item per output element.
-template <typename C>
-__device__ void pass_generic(const C* __restrict__ in, C* __restrict__ out,
-                             const C* __restrict__ tw, int N, int Ns, int R, int lpb,
-                             int ps, int ls) {
+// In-place form: at most ip_items(R, MI) work items per thread, all inputs read
+// before the block synchronises (the host checks that the items fit).
+__host__ __device__ constexpr int ip_items(int R, int MI) { return R >= 11 ? 1 : MI; }
+
+template <int R, int MI, typename C>
+__device__ void pass_radix_ip(C* __restrict__ buf, const C* __restrict__ tw, int N, int Ns,
+                              int lpb, int ps, int ls, int sign) {
+  constexpr int M = ip_items(R, MI);
   const int NR = N / R;
   const int step = N / (Ns * R);
-  for (int it = threadIdx.x; it < N * lpb; it += blockDim.x) {
-    const int line = it % lpb;
-    const int q = it / lpb;  // q = j*R + kk
-    const int j = q / R, kk = q % R;
-    const int k = j % Ns;
-    C acc = czero<C>();
-    for (int r = 0; r < R; ++r) {
-      C x = in[(j + r * NR) * ps + line * ls];
-      if (r) x = cmul(x, tw[k * r * step]);
-      const C w = tw[((r * kk) % R) * NR];
-      acc.x += x.x * w.x - x.y * w.y;
-      acc.y += x.x * w.y + x.y * w.x;
+  const int total = NR * lpb;
+  C v[M][R];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int it = threadIdx.x + m * blockDim.x;
+    if (it < total) {
+      const int line = it % lpb, j = it / lpb;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[m][r] = buf[(j + r * NR) * ps + line * ls];
     }
-    const int idx = (j / Ns) * Ns * R + k;
-    out[(idx + kk * Ns) * ps + line * ls] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int it = threadIdx.x + m * blockDim.x;
+    if (it < total) {
+      const int line = it % lpb, j = it / lpb;
+      const int k = j % Ns;
+      radix_item<R>(v[m], tw, Ns, k, step, sign);
+      const int idx = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int kk = 0; kk < R; ++kk) buf[(idx + kk * Ns) * ps + line * ls] = v[m][kk];
+    }
+  }
+  __syncthreads();
+}
+
+// Generic odd factor R, sub-pass 1 (in place, one thread per pair): twiddle
+// inputs r and R-r of every group and store (a_r, b_r) over them.
+template <typename C>
+__device__ void generic_pairs(C* __restrict__ buf, const C* __restrict__ tw, int N, int Ns, int R,
+                              int lpb, int ps, int ls) {
+  const int NR = N / R, H = (R - 1) / 2;
+  const int step = N / (Ns * R);
+  for (int it = threadIdx.x; it < NR * lpb * H; it += blockDim.x) {
+    const int line = it % lpb, q = it / lpb;
+    const int j = q % NR, r = 1 + q / NR;
+    const int k = j % Ns;
+    C* px = buf + (j + r * NR) * ps + line * ls;
+    C* py = buf + (j + (R - r) * NR) * ps + line * ls;
+    C x = *px, y = *py;
+    if (Ns > 1) {
+      x = cmul(x, tw[k * r * step]);
+      y = cmul(y, tw[k * (R - r) * step]);
+    }
+    *px = cadd(x, y);
+    *py = csub(x, y);
   }
 }
 
-// A whole transform of prime length N > 13 in one pass (direct DFT, the only
-// stage).  Work item = (line, block of PR outputs): the inputs of the line are
-// read once each and accumulated into PR outputs; the twiddle index of output
-// q after input r is (r * q) mod N, stepped by q with one conditional subtract.
-#define FFT_PRIME_PR 8
+// Sub-pass 2, the sums of one work item (line, j, block of FFT_PAIRS pairs):
+// t1 = v_0 + sum_r a_r cos, t2 = sum_r b_r s sin for each kk of the block (kk
+// past (R-1)/2 marked 0, computed on index 0 and not stored), s0 = X_0.
+template <typename C>
+__device__ __forceinline__ void generic_sums(const C* __restrict__ in, const C* __restrict__ tw,
+                                             int R, int NR, int lpb, int ps, int ls, int it,
+                                             C (&t1)[FFT_PAIRS], C (&t2)[FFT_PAIRS], C& s0,
+                                             int (&kk)[FFT_PAIRS]) {
+  const int H = (R - 1) / 2;
+  const int line = it % lpb, q = it / lpb;
+  const int j = q % NR, blk = q / NR;
+  const C* base = in + j * ps + line * ls;
+  const C v0 = base[0];
+  int m[FFT_PAIRS];
+#pragma unroll
+  for (int p = 0; p < FFT_PAIRS; ++p) {
+    kk[p] = 1 + blk * FFT_PAIRS + p;
+    if (kk[p] > H) kk[p] = 0;
+    m[p] = 0;
+    t1[p] = v0;
+    t2[p] = czero<C>();
+  }
+  s0 = v0;
+  const int rs = NR * ps;
+  for (int r = 1; r <= H; ++r) {
+    const C a = base[r * rs], b = base[(R - r) * rs];
+    s0 = cadd(s0, a);
+#pragma unroll
+    for (int p = 0; p < FFT_PAIRS; ++p) {
+      m[p] += kk[p];
+      if (m[p] >= R) m[p] -= R;
+      const C w = tw[m[p] * NR];
+      t1[p].x = sfma(a.x, w.x, t1[p].x);
+      t1[p].y = sfma(a.y, w.x, t1[p].y);
+      t2[p].x = sfma(b.x, w.y, t2[p].x);
+      t2[p].y = sfma(b.y, w.y, t2[p].y);
+    }
+  }
+}
 
 template <typename C>
-__device__ void pass_prime(const C* __restrict__ in, C* __restrict__ out,
-                           const C* __restrict__ tw, int N, int lpb, int ps, int ls) {
-  const int nblk = (N + FFT_PRIME_PR - 1) / FFT_PRIME_PR;
-  for (int it = threadIdx.x; it < nblk * lpb; it += blockDim.x) {
-    const int line = it % lpb, q0 = (it / lpb) * FFT_PRIME_PR;
-    C acc[FFT_PRIME_PR];
-    int idx[FFT_PRIME_PR];
+__device__ __forceinline__ void generic_store(C* __restrict__ out, int R, int NR, int Ns, int lpb,
+                                              int ps, int ls, int it, const C (&t1)[FFT_PAIRS],
+                                              const C (&t2)[FFT_PAIRS], C s0,
+                                              const int (&kk)[FFT_PAIRS]) {
+  const int line = it % lpb, q = it / lpb;
+  const int j = q % NR, blk = q / NR;
+  const int k = j % Ns;
+  const int idx = (j / Ns) * Ns * R + k;
+  C* o = out + line * ls;
 #pragma unroll
-    for (int j = 0; j < FFT_PRIME_PR; ++j) {
-      acc[j] = czero<C>();
-      idx[j] = 0;
+  for (int p = 0; p < FFT_PAIRS; ++p) {
+    if (kk[p]) {
+      C xp, xm;
+      pm_i(t1[p], t2[p], xp, xm);
+      o[(idx + kk[p] * Ns) * ps] = xp;
+      o[(idx + (R - kk[p]) * Ns) * ps] = xm;
     }
-    for (int r = 0; r < N; ++r) {
-      const C x = in[r * ps + line * ls];
-#pragma unroll
-      for (int j = 0; j < FFT_PRIME_PR; ++j) {
-        const C w = tw[idx[j]];
-        acc[j].x += x.x * w.x - x.y * w.y;
-        acc[j].y += x.x * w.y + x.y * w.x;
-        // (r + 1) q mod N for q = q0 + j; outputs past N (discarded) keep q = 0
-        const int q = q0 + j < N ? q0 + j : 0;
-        const int t = idx[j] + q;
-        idx[j] = t >= N ? t - N : t;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < FFT_PRIME_PR; ++j)
-      if (q0 + j < N) out[(q0 + j) * ps + line * ls] = acc[j];
+  }
+  if (blk == 0) o[idx * ps] = s0;
+}
+
+template <typename C>
+__device__ void pass_generic(C* __restrict__ in, C* __restrict__ out, const C* __restrict__ tw,
+                             int N, int Ns, int R, int lpb, int ps, int ls) {
+  generic_pairs(in, tw, N, Ns, R, lpb, ps, ls);
+  __syncthreads();
+  const int NR = N / R;
+  const int nblk = ((R - 1) / 2 + FFT_PAIRS - 1) / FFT_PAIRS;
+  for (int it = threadIdx.x; it < NR * lpb * nblk; it += blockDim.x) {
+    C t1[FFT_PAIRS], t2[FFT_PAIRS], s0;
+    int kk[FFT_PAIRS];
+    generic_sums(in, tw, R, NR, lpb, ps, ls, it, t1, t2, s0, kk);
+    generic_store(out, R, NR, Ns, lpb, ps, ls, it, t1, t2, s0, kk);
   }
 }
 
-// All Stockham passes of one transform of length N over lpb lines; returns the
-// buffer that holds the result (in or out).
+template <int MI, typename C>
+__device__ void pass_generic_ip(C* __restrict__ buf, const C* __restrict__ tw, int N, int Ns,
+                                int R, int lpb, int ps, int ls) {
+  generic_pairs(buf, tw, N, Ns, R, lpb, ps, ls);
+  __syncthreads();
+  const int NR = N / R;
+  const int nblk = ((R - 1) / 2 + FFT_PAIRS - 1) / FFT_PAIRS;
+  const int total = NR * lpb * nblk;
+  C t1[MI][FFT_PAIRS], t2[MI][FFT_PAIRS], s0[MI];
+  int kk[MI][FFT_PAIRS];
+#pragma unroll
+  for (int m = 0; m < MI; ++m) {
+    const int it = threadIdx.x + m * blockDim.x;
+    if (it < total) generic_sums(buf, tw, R, NR, lpb, ps, ls, it, t1[m], t2[m], s0[m], kk[m]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < MI; ++m) {
+    const int it = threadIdx.x + m * blockDim.x;
+    if (it < total) generic_store(buf, R, NR, Ns, lpb, ps, ls, it, t1[m], t2[m], s0[m], kk[m]);
+  }
+  __syncthreads();
+}
+
+// All passes of one transform of length N over lpb lines between two buffers;
+// returns the buffer that holds the result (in or out).  `in` is overwritten.
 template <typename C>
 __device__ C* run_passes(C* in, C* out, const C* __restrict__ tw, const FftPlan& plan, int N,
-                         int lpb, int ps, int ls) {
-  if (plan.nst == 1 && plan.rad[0] > 13) {
-    pass_prime(in, out, tw, N, lpb, ps, ls);
-    __syncthreads();
-    return out;
-  }
+                         int lpb, int ps, int ls, int sign) {
   int Ns = 1;
   for (int s = 0; s < plan.nst; ++s) {
     const int R = plan.rad[s];
     switch (R) {
-      case 2: pass_r<2>(in, out, tw, N, Ns, lpb, ps, ls); break;
-      case 3: pass_r<3>(in, out, tw, N, Ns, lpb, ps, ls); break;
-      case 4: pass_r<4>(in, out, tw, N, Ns, lpb, ps, ls); break;
-      case 5: pass_r<5>(in, out, tw, N, Ns, lpb, ps, ls); break;
-      case 7: pass_r<7>(in, out, tw, N, Ns, lpb, ps, ls); break;
-      case 11: pass_r<11>(in, out, tw, N, Ns, lpb, ps, ls); break;
-      case 13: pass_r<13>(in, out, tw, N, Ns, lpb, ps, ls); break;
+      case 2: pass_radix<2>(in, out, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 3: pass_radix<3>(in, out, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 4: pass_radix<4>(in, out, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 5: pass_radix<5>(in, out, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 7: pass_radix<7>(in, out, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 11: pass_radix<11>(in, out, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 13: pass_radix<13>(in, out, tw, N, Ns, lpb, ps, ls, sign); break;
       default: pass_generic(in, out, tw, N, Ns, R, lpb, ps, ls); break;
     }
     __syncthreads();
@@ -166,13 +415,49 @@ __device__ C* run_passes(C* in, C* out, const C* __restrict__ tw, const FftPlan&
   return in;
 }
 
-template <typename C>
+template <int MI, typename C>
+__device__ void run_passes_ip(C* buf, const C* __restrict__ tw, const FftPlan& plan, int N,
+                              int lpb, int ps, int ls, int sign) {
+  int Ns = 1;
+  for (int s = 0; s < plan.nst; ++s) {
+    const int R = plan.rad[s];
+    switch (R) {
+      case 2: pass_radix_ip<2, MI>(buf, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 3: pass_radix_ip<3, MI>(buf, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 4: pass_radix_ip<4, MI>(buf, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 5: pass_radix_ip<5, MI>(buf, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 7: pass_radix_ip<7, MI>(buf, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 11: pass_radix_ip<11, MI>(buf, tw, N, Ns, lpb, ps, ls, sign); break;
+      case 13: pass_radix_ip<13, MI>(buf, tw, N, Ns, lpb, ps, ls, sign); break;
+      default: pass_generic_ip<MI>(buf, tw, N, Ns, R, lpb, ps, ls); break;
+    }
+    Ns *= R;
+  }
+}
+
+static bool fft_radix_known(int R) { return R <= 5 || R == 7 || R == 11 || R == 13; }
+
+// Whether every pass of a length-N transform over lpb lines fits the in-place
+// passes with at most MI work items per thread and nt threads.
+static bool fft_inplace_fits(const FftPlan& plan, int N, int lpb, int nt, int MI) {
+  for (int s = 0; s < plan.nst; ++s) {
+    const int R = plan.rad[s];
+    const long items = fft_radix_known(R)
+                           ? (long)(N / R) * lpb
+                           : (long)(N / R) * lpb * (((R - 1) / 2 + FFT_PAIRS - 1) / FFT_PAIRS);
+    const int per = fft_radix_known(R) ? ip_items(R, MI) : MI;
+    if (items > (long)per * nt) return false;
+  }
+  return true;
+}
+
+template <typename C, int MI>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(C* __restrict__ data, int g, int axis,
                                                           int lpb, FftPlan plan, int sign) {
   extern __shared__ unsigned char sm_raw[];
   C* bufA = reinterpret_cast<C*>(sm_raw);
   C* bufB = bufA + g * lpb;
-  C* tw = bufB + g * lpb;
+  C* tw = MI ? bufB : bufB + g * lpb;
   const long gg = (long)g * g;
   C* base = data + (long)blockIdx.y * gg * g;
   const int line0 = blockIdx.x * lpb;
@@ -196,7 +481,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(C* __restrict__ data, 
     bufA[p * lpb + line] = val;
   }
   __syncthreads();
-  C* in = run_passes(bufA, bufB, tw, plan, g, lpb, lpb, 1);
+  C* in = bufA;
+  if (MI)
+    run_passes_ip<MI ? MI : 1>(bufA, tw, plan, g, lpb, lpb, 1, sign);
+  else
+    in = run_passes(bufA, bufB, tw, plan, g, lpb, lpb, 1, sign);
   for (int e = threadIdx.x; e < g * lpb; e += blockDim.x) {
     int p, line;
     if (axis == 2) {
@@ -219,28 +508,32 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(C* __restrict__ data, 
 // (lines y: element (z, y) at y*g + z) and then along y (lines z: element
 // (y, z) at y*g + z) in LDS, and writes the plane back.  Together with the
 // strided x pass this reads and writes the grid twice instead of three times.
-// NT threads: 1024, or 256 for planes up to 51^2 when there are at least two
-// planes per CU (the radix passes of a 49^2 plane have 343 to 1225 work items;
-// 128 grids of 49^3: whole transform 0.452 -> 0.393 ms, the same results bit
-// for bit, since each output is formed the same way whichever thread forms it;
-// 65^3 and a few planes per CU run slower at 256).
+// NT threads (256, 512 or 1024): each output is formed the same way whichever
+// thread forms it, so the thread count and the in-place form change the
+// schedule only.
 #define FFT_PLANE_MAX_LDS (156 * 1024)
 
-template <int NT, typename C>
+template <int NT, typename C, int MI>
 __global__ __launch_bounds__(NT) void k_fft_plane(C* __restrict__ data, int g, FftPlan plan,
                                                   int sign) {
   extern __shared__ unsigned char sm_raw[];
   const int gg = g * g;
   C* bufA = reinterpret_cast<C*>(sm_raw);
   C* bufB = bufA + gg;
-  C* tw = bufB + gg;
+  C* tw = MI ? bufB : bufB + gg;
   C* plane = data + ((long)blockIdx.y * g + blockIdx.x) * gg;
   for (int m = threadIdx.x; m < g; m += blockDim.x) tw[m] = twiddle<C>(m, g, sign);
   for (int e = threadIdx.x; e < gg; e += blockDim.x) bufA[e] = plane[e];
   __syncthreads();
-  C* r = run_passes(bufA, bufB, tw, plan, g, g, 1, g);                // along z
+  if (MI) {
+    run_passes_ip<MI ? MI : 1>(bufA, tw, plan, g, g, 1, g, sign);      // along z
+    run_passes_ip<MI ? MI : 1>(bufA, tw, plan, g, g, g, 1, sign);      // along y
+    for (int e = threadIdx.x; e < gg; e += blockDim.x) plane[e] = bufA[e];
+    return;
+  }
+  C* r = run_passes(bufA, bufB, tw, plan, g, g, 1, g, sign);          // along z
   C* o = (r == bufA) ? bufB : bufA;
-  r = run_passes(r, o, tw, plan, g, g, g, 1);                          // along y
+  r = run_passes(r, o, tw, plan, g, g, g, 1, sign);                    // along y
   for (int e = threadIdx.x; e < gg; e += blockDim.x) plane[e] = r[e];
 }
 
@@ -265,6 +558,40 @@ static int make_plan(int g, FftPlan& plan) {
   return HQC_OK;
 }
 
+template <int NT, typename C>
+static const void* plane_kernel(int mi) {
+  return mi == 0 ? (const void*)k_fft_plane<NT, C, 0>
+                 : (mi == 1 ? (const void*)k_fft_plane<NT, C, 1> : (const void*)k_fft_plane<NT, C, 2>);
+}
+
+template <int NT, typename C>
+static void launch_plane(int mi, dim3 grid, size_t lds, hipStream_t st, C* data, int g,
+                         const FftPlan& plan, int sign) {
+  if (mi == 0)
+    hipLaunchKernelGGL((k_fft_plane<NT, C, 0>), grid, dim3(NT), lds, st, data, g, plan, sign);
+  else if (mi == 1)
+    hipLaunchKernelGGL((k_fft_plane<NT, C, 1>), grid, dim3(NT), lds, st, data, g, plan, sign);
+  else
+    hipLaunchKernelGGL((k_fft_plane<NT, C, 2>), grid, dim3(NT), lds, st, data, g, plan, sign);
+}
+
+// Work items per thread of the in-place passes for a transform over lpb lines
+// with nt threads (0: the two-buffer passes).  Tuned values when they fit;
+// by default in place only in FP64 and only where two buffers would hold more
+// than 40 KB of LDS (fewer than four workgroups per CU), with 2 items per thread
+// where they fit, else 1.  Measured on 48-256 grids: in place 49^3 0.30 -> 0.24
+// ms, 61^3 0.48 -> 0.34 ms (FP64); two buffers faster at 29^3 (no LDS limit,
+// fewer registers) and in FP32 (same occupancy either way, fewer barriers).
+static int inplace_items(const FftPlan& plan, int g, int lpb, int nt, size_t csize) {
+  if (hqc_tuned(HQC_TUNE_FFT_TWO_BUFFER, 0)) return 0;
+  const int want = hqc_tuned(HQC_TUNE_FFT_IP_ITEMS, 0);
+  if (want) return fft_inplace_fits(plan, g, lpb, nt, want) ? want : 0;
+  if (csize < 16 || (size_t)2 * g * lpb * csize <= 40 * 1024) return 0;
+  for (int mi = 2; mi >= 1; --mi)
+    if (fft_inplace_fits(plan, g, lpb, nt, mi)) return mi;
+  return 0;
+}
+
 template <typename C>
 static int fft3d(C* data, int B, int g, int sign, void* stream) {
   if (B < 0 || g <= 0 || (sign != 1 && sign != -1)) return HQC_E_ARG;
@@ -272,36 +599,59 @@ static int fft3d(C* data, int B, int g, int sign, void* stream) {
   if (B == 0 || g == 1) return HQC_OK;
   FftPlan plan;
   if (make_plan(g, plan) != HQC_OK) return HQC_E_SIZE;
+  hipStream_t st = (hipStream_t)stream;
   const long lines = (long)g * g;
-  const size_t plane_lds = (size_t)(2 * g * g + g) * sizeof(C);
+  // threads per plane: with enough planes to fill the GPU, 256 (several planes
+  // per CU) up to 61^2 in FP64 and 51^2 in FP32, above that 1024 (FP64) or 512
+  // (FP32); 1024 for few planes
+  int nt = hqc_tuned(HQC_TUNE_FFT_PLANE_THREADS, 0);
+  if (!nt) {
+    const bool f64 = sizeof(C) == 16;
+    if ((long)g * B < 512)
+      nt = 1024;
+    else if (g <= (f64 ? 61 : 51))
+      nt = 256;
+    else
+      nt = f64 ? 1024 : 512;
+  }
+  const int mi = inplace_items(plan, g, g, nt, sizeof(C));
+  const size_t plane_lds = (size_t)((mi ? 1 : 2) * g * g + g) * sizeof(C);
   int first_axis = 2;
   if (plane_lds <= FFT_PLANE_MAX_LDS) {
-    const int nt = hqc_tuned(HQC_TUNE_FFT_PLANE_THREADS, 0);
-    const bool small = nt ? nt == 256 : (g <= 51 && (long)g * B >= 512);
-    const void* kf = small ? (const void*)k_fft_plane<256, C> : (const void*)k_fft_plane<1024, C>;
+    const void* kf = nt == 256 ? plane_kernel<256, C>(mi)
+                               : (nt == 512 ? plane_kernel<512, C>(mi) : plane_kernel<1024, C>(mi));
     if (plane_lds > 64 * 1024 &&
         hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds) !=
             hipSuccess)
       (void)hipGetLastError();
-    if (small)
-      hipLaunchKernelGGL((k_fft_plane<256, C>), dim3(g, B), dim3(256), plane_lds,
-                         (hipStream_t)stream, data, g, plan, sign);
+    const dim3 grid(g, B);
+    if (nt == 256)
+      launch_plane<256>(mi, grid, plane_lds, st, data, g, plan, sign);
+    else if (nt == 512)
+      launch_plane<512>(mi, grid, plane_lds, st, data, g, plan, sign);
     else
-      hipLaunchKernelGGL((k_fft_plane<1024, C>), dim3(g, B), dim3(1024), plane_lds,
-                         (hipStream_t)stream, data, g, plan, sign);
+      launch_plane<1024>(mi, grid, plane_lds, st, data, g, plan, sign);
     HQC_CHECK_LAUNCH();
     first_axis = 0;
   }
   // remaining axes line by line; more lines per block on the strided x axis
   for (int axis = first_axis; axis >= 0; --axis) {
     // lines per block: 32 on the strided x axis (longer coalesced segments), 16
-    // otherwise, and never more than 64 KB of LDS
+    // otherwise, and never more than 64 KB of LDS for two buffers
     const int fit = (int)((64 * 1024 / sizeof(C) - g) / (2 * g));
     const int lpb = max(1, min(axis == 0 ? 32 : 16, fit));
-    const size_t lds = (size_t)(2 * g * lpb + g) * sizeof(C);
-    dim3 grid(hqc_div_up(lines, lpb), B);
-    hipLaunchKernelGGL((k_fft_axis<C>), grid, dim3(FFT_THREADS), lds, (hipStream_t)stream, data,
-                       g, axis, lpb, plan, sign);
+    const int ami = inplace_items(plan, g, lpb, FFT_THREADS, sizeof(C));
+    const size_t lds = (size_t)((ami ? 1 : 2) * g * lpb + g) * sizeof(C);
+    const dim3 grid(hqc_div_up(lines, lpb), B);
+    if (ami == 0)
+      hipLaunchKernelGGL((k_fft_axis<C, 0>), grid, dim3(FFT_THREADS), lds, st, data, g, axis,
+                         lpb, plan, sign);
+    else if (ami == 1)
+      hipLaunchKernelGGL((k_fft_axis<C, 1>), grid, dim3(FFT_THREADS), lds, st, data, g, axis,
+                         lpb, plan, sign);
+    else
+      hipLaunchKernelGGL((k_fft_axis<C, 2>), grid, dim3(FFT_THREADS), lds, st, data, g, axis,
+                         lpb, plan, sign);
     HQC_CHECK_LAUNCH();
   }
   return HQC_OK;

@@ -14,7 +14,10 @@ int hqc_tuned(int key, int dflt) {
 extern "C" int hqc_set_tuning(int key, int value) {
   if (key < 0 || key >= HQC_TUNE_COUNT || value < 0) return HQC_E_ARG;
   if (key == HQC_TUNE_VMAT_TPW && value > 8) return HQC_E_ARG;
-  if (key == HQC_TUNE_FFT_PLANE_THREADS && value != 0 && value != 256 && value != 1024)
+  if (key == HQC_TUNE_FFT_PLANE_THREADS && value != 0 && value != 256 && value != 512 &&
+      value != 1024)
+    return HQC_E_ARG;
+  if ((key == HQC_TUNE_FFT_TWO_BUFFER && value > 1) || (key == HQC_TUNE_FFT_IP_ITEMS && value > 2))
     return HQC_E_ARG;
   g_tune[key] = value;
   return HQC_OK;

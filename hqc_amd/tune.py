@@ -9,7 +9,8 @@ stream the kernels run on, and keeps the fastest:
   workgroups aimed at per launch, grid chunks per geometry at most, upper-triangle
   tiles per wave;
 * int8-digit orbital density: workgroups aimed at, point tiles per wave at least;
-* 3-D FFT: threads per y-z plane.
+* 3-D FFT: threads per y-z plane, in-place or two-buffer LDS passes (the same
+  outputs bit for bit either way).
 
 The choice changes how work is split across workgroups, not what is computed
 (the potential matrix's partial sums are then added in another grouping).  It is
@@ -125,7 +126,8 @@ def autotune(engine, B, reps=5, seed=0):
     # FFTs of the Poisson solve (two geometries per complex grid)
     cb = torch.randn((B + 1) // 2, c.g, c.g, c.g, dtype=torch.complex128, generator=None,
                      device=dev)
-    t = _sweep({"fft_plane_threads": [256, 1024]}, lambda: K.fft3d(cb, -1), reps)
+    t = _sweep({"fft_plane_threads": [256, 512, 1024], "fft_two_buffer": [0, 1]},
+               lambda: K.fft3d(cb, -1), reps)
     rec["timings"]["fft3d"] = t
     rec["choice"].update(json.loads(min(t, key=t.get)))
     return rec

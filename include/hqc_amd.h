@@ -55,9 +55,13 @@ int hqc_version(void);
 #define HQC_TUNE_VMAT_TPW 3          /* upper-triangle tiles per wave, 1..8 (smallest that covers) */
 #define HQC_TUNE_Q8_BLOCKS 4         /* int8-digit density: workgroups aimed at (2048) */
 #define HQC_TUNE_Q8_MIN_TILES 5      /* int8-digit density: point tiles per wave at least (8) */
-#define HQC_TUNE_FFT_PLANE_THREADS 6 /* 256 or 1024 threads per y-z plane (by size) */
+#define HQC_TUNE_FFT_PLANE_THREADS 6 /* 256, 512 or 1024 threads per y-z plane (by size) */
 #define HQC_TUNE_VBIG_BLOCKS 7       /* workgroups aimed at, V for n_ao > 128 (1024) */
-#define HQC_TUNE_COUNT 8
+#define HQC_TUNE_FFT_TWO_BUFFER 8    /* 1: 3-D FFT passes between two LDS buffers instead of
+                                        in place (the same outputs; kept for comparison) */
+#define HQC_TUNE_FFT_IP_ITEMS 9      /* 1 or 2: work items per thread of the in-place FFT
+                                        passes, when they fit (the fewest that fit) */
+#define HQC_TUNE_COUNT 10
 int hqc_set_tuning(int key, int value);
 int hqc_get_tuning(int key);
 

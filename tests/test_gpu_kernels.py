@@ -317,6 +317,36 @@ def test_fft3d_plane_thread_counts_agree(K, g):
     assert torch.allclose(y, ref, rtol=1e-12, atol=1e-11 * g ** 1.5)
 
 
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+@pytest.mark.parametrize("g", [26, 34, 45, 49, 57, 61, 64, 65, 97])
+def test_fft3d_schedules_agree(K, g, dtype):
+    """In-place LDS passes (1 or 2 work items per thread; 256, 512 or 1024 threads
+    per plane) and the two-buffer passes form every output with the same
+    operations: all schedules give the same bits.  Sizes cover radix 2/3/4/5/7/13,
+    the generic radix (34 = 2*17, 57 = 3*19), the direct prime pass (61, 97), and
+    grids whose in-place planes fit where two-buffer planes do not (97)."""
+    from hqc_amd import tune
+    print(f"[fft3d schedules] g={g} {dtype}", flush=True)
+    torch.manual_seed(300 + g)
+    x = torch.randn(3, g, g, g, dtype=dtype, device=DEV)
+    try:
+        tune.apply({"fft_two_buffer": 1})
+        ref = x.clone()
+        K.fft3d(ref, -1)
+        for nt in (0, 256, 512, 1024):
+            for mi in (0, 1, 2):
+                tune.apply({"fft_plane_threads": nt, "fft_ip_items": mi})
+                y = x.clone()
+                K.fft3d(y, -1)
+                torch.cuda.synchronize()
+                assert torch.equal(y, ref), (nt, mi)
+    finally:
+        tune.apply({})
+    exact = torch.fft.fftn(x.to(torch.complex128), dim=(1, 2, 3))
+    tol = 1e-11 if dtype == torch.complex128 else 2e-6
+    assert (ref.to(torch.complex128) - exact).abs().max().item() < tol * g ** 1.5
+
+
 def test_gto_tables_and_values(K):
     s, xpb, L = _sys()
     print(f"[gto] g={s.g}", flush=True)

@@ -81,7 +81,9 @@ def test_tuning_table_is_per_thread_and_checked(libpath):
     with pytest.raises(RuntimeError):
         _lib.set_tuning("vmat_tpw", 9)
     with pytest.raises(RuntimeError):
-        _lib.set_tuning("fft_plane_threads", 512)
+        _lib.set_tuning("fft_plane_threads", 384)
+    with pytest.raises(RuntimeError):
+        _lib.set_tuning("fft_two_buffer", 2)
     assert _lib.lib().hqc_set_tuning(99, 1) == 1 and _lib.lib().hqc_get_tuning(99) == -1
     _lib.set_tuning("vmat_tpw", 0)
     assert _lib.get_tuning("vmat_tpw") == 0
