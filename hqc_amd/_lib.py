@@ -78,10 +78,15 @@ _SIGS = {
     "hqc_ao_points": [_P, _P, _P, _I, _I, _I, _D, _I, _P, _I, _I, _P, _P, _I, _P, _P],
     "hqc_grid_forces": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P, _D, _P, _P],
     "hqc_vloc_force": [_P, _I, _I, _I, _D, _P, _P, _P, _P],
+    "hqc_scf_grid": [_D, _D],
+    "hqc_scf_create": [_P, _P],
+    "hqc_scf_run": [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "hqc_scf_destroy": [_P],
 }
 
 
-_RESTYPES = {"hqc_eigh_lds_bytes": ctypes.c_size_t, "hqc_ao_vmat_scratch_bytes": ctypes.c_size_t}
+_RESTYPES = {"hqc_eigh_lds_bytes": ctypes.c_size_t, "hqc_ao_vmat_scratch_bytes": ctypes.c_size_t,
+             "hqc_scf_destroy": None}
 
 
 def lib():
@@ -595,3 +600,74 @@ def vloc_force(xp, g, L, VG, rhoG, dE):
     B, n, _ = xp.shape
     _call("hqc_vloc_force", _ptr(xp, F64, "xp"), B, n, g, float(L), _ptr(VG, F64, "VG"),
           _ptr(rhoG, None, "rhoG"), _ptr(dE, F64, "dE"), _stream())
+
+
+# --------------------------------------------------------------------------- SCF driver
+class ScfCell(ctypes.Structure):
+    """hqc_scf_cell of include/hqc_amd.h."""
+    _fields_ = [("n", _I), ("L", _D), ("g", _I), ("na", _I), ("nloc", _I), ("nl", _I),
+                ("alpha", ctypes.POINTER(_D)), ("coef", ctypes.POINTER(_D)),
+                ("comp", ctypes.POINTER(_I)), ("rcut", _D), ("xc_id", _I), ("smearing", _I),
+                ("smearing_method", _I), ("sigma", _D), ("search", _I), ("search_cycle", _I),
+                ("search_tol", _D), ("tol", _D), ("max_cycle", _I), ("diis_space", _I),
+                ("diis_start_cycle", _I), ("diis_damp", _D)]
+
+
+def scf_grid(L, grid_length):
+    """The reference's odd grid for a cell of L Bohr (hqc_scf_grid, host only)."""
+    return lib().hqc_scf_grid(float(L), float(grid_length))
+
+
+class ScfDriver:
+    """The C SCF driver (hqc_scf_create / run / destroy) through ctypes: the binding
+    a host without Python makes.  Basis arrays alpha (na), coef (na, nloc), comp
+    (nloc) as hqc_amd.basis gives them; L in Bohr."""
+
+    def __init__(self, n, L, g, alpha, coef, comp, nl, rcut=24.0, xc_id=1, smearing=False,
+                 smearing_method="fermi", sigma=0.0, search="bisect", search_cycle=100,
+                 search_tol=1e-7, tol=1e-7, max_cycle=100, diis_space=8, diis_start_cycle=1,
+                 diis_damp=0.0):
+        import numpy as np
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        c = np.ascontiguousarray(coef, dtype=np.float64)
+        p = np.ascontiguousarray(comp, dtype=np.int32)
+        self.n, self.nao = int(n), int(n) * c.shape[1]
+        cell = ScfCell(int(n), float(L), int(g), a.shape[0], c.shape[1], int(nl),
+                       a.ctypes.data_as(ctypes.POINTER(_D)), c.ctypes.data_as(ctypes.POINTER(_D)),
+                       p.ctypes.data_as(ctypes.POINTER(_I)), float(rcut), int(xc_id),
+                       int(bool(smearing)), 0 if smearing_method == "fermi" else 1, float(sigma),
+                       0 if search == "bisect" else 1, int(search_cycle), float(search_tol),
+                       float(tol), int(max_cycle), int(diis_space), int(diis_start_cycle),
+                       float(diis_damp))
+        self._ctx = ctypes.c_void_p()
+        self.max_cycle = int(max_cycle)
+        _call("hqc_scf_create", ctypes.byref(cell), ctypes.byref(self._ctx))
+
+    def run(self, xp):
+        """xp (B, n, 3) Bohr on the GPU -> dict of make_solver's outputs (mo, dm,
+        bands [Ry], E, Ki, Vep, Vee [Ry], Se, converged) plus n_iter."""
+        B = xp.shape[0]
+        dev = xp.device
+        f64 = torch.float64
+        out = {k: torch.empty(B, self.nao, self.nao, dtype=f64, device=dev) for k in ("mo", "dm")}
+        out["bands"] = torch.empty(B, self.nao, dtype=f64, device=dev)
+        for k in ("E", "Ki", "Vep", "Vee", "Se"):
+            out[k] = torch.empty(B, dtype=f64, device=dev)
+        for k in ("converged", "n_iter"):
+            out[k] = torch.empty(B, dtype=torch.int32, device=dev)
+        _call("hqc_scf_run", self._ctx, _ptr(xp.contiguous(), F64, "xp"), B,
+              *[_ptr(out[k]) for k in ("mo", "dm", "bands", "E", "Ki", "Vep", "Vee", "Se",
+                                       "converged", "n_iter")], _stream())
+        out["converged"] = out["converged"].bool()
+        return out
+
+    def close(self):
+        if self._ctx:
+            lib().hqc_scf_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

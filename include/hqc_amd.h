@@ -406,6 +406,54 @@ int hqc_grid_forces_k(const double* P, int B, int n, int g, int na, int nl, cons
 int hqc_vloc_force(const double* xp, int B, int n, int g, double L, const double* VG,
                    const double* rhoG, double* dE, void* stream);
 
+/* SCF driver: the batched Gamma-point restricted Kohn-Sham SCF of
+ * make_solver(..., dft=True) (hqc/pbc/solver.py:705-767, the DIIS loop of
+ * hqc/pbc/scf.py:93-242), run from C on the entry points above and rocBLAS, for
+ * hosts without Python (the reference has no FFI: its plugin surface is the
+ * Python factory, which hqc_amd.pbc.make_solver mirrors).  Scope: FP64, LDA
+ * functionals (xc_id 1..5 of hqc_xc_eval), n * nloc <= 110, DIIS on, one cell
+ * length per context.  The cell's arrays are host memory, copied at create.
+ *   g: grid points per axis, the reference's odd grid (hqc_scf_grid);
+ *   alpha (na), coef (na, nloc), comp (nloc): the normalised basis with the
+ *     Y_lm factors folded in, as hqc_ao_values takes them;
+ *   rcut: lattice-sum cut-off of S and T in Bohr (24 in the reference);
+ *   smearing 0: aufbau occupations (smearing.py:171); 1: smearing_method
+ *     0 fermi / 1 gauss at sigma (Hartree), search 0 bisect / 1 newton;
+ *   tol, max_cycle, diis_space, diis_start_cycle, diis_damp as make_solver's.
+ * hqc_scf_run: xp (B, n, 3) Bohr on the device (the reference's xp * rs);
+ * device outputs, any of them NULL to skip: mo (B, n_ao, n_ao), dm (B, n_ao,
+ * n_ao), bands (B, n_ao) Ry, E, Ki, Vep, Vee (B) Ry, Se (B), converged and
+ * n_iter (B) int32 -- make_solver's 9-tuple plus the iteration counts.  The
+ * call allocates its work memory and synchronises the stream before it returns.
+ * hqc_scf_grid: round(L / grid_length / 2) * 2 + 1 with ties to even
+ * (solver.py:159). */
+typedef struct hqc_scf_cell {
+  int n;
+  double L;
+  int g;
+  int na, nloc, nl;
+  const double* alpha;
+  const double* coef;
+  const int* comp;
+  double rcut;
+  int xc_id;
+  int smearing, smearing_method;
+  double sigma;
+  int search, search_cycle;
+  double search_tol;
+  double tol;
+  int max_cycle;
+  int diis_space, diis_start_cycle;
+  double diis_damp;
+} hqc_scf_cell;
+typedef struct hqc_scf_ctx hqc_scf_ctx;
+int hqc_scf_grid(double L, double grid_length);
+int hqc_scf_create(const hqc_scf_cell* cell, hqc_scf_ctx** ctx);
+int hqc_scf_run(hqc_scf_ctx* ctx, const double* xp, int B, double* mo, double* dm,
+                double* bands, double* E, double* Ki, double* Vep, double* Vee, double* Se,
+                int* converged, int* n_iter, void* stream);
+void hqc_scf_destroy(hqc_scf_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     text = open(os.path.join(ROOT, "include", "hqc_amd.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(?:int|size_t)\s+(hqc_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(?:int|size_t|void)\s+(hqc_\w+)\s*\(", text)))
 
 
 @pytest.fixture(scope="module")
@@ -87,3 +87,23 @@ def test_tuning_table_is_per_thread_and_checked(libpath):
     assert _lib.lib().hqc_set_tuning(99, 1) == 1 and _lib.lib().hqc_get_tuning(99) == -1
     _lib.set_tuning("vmat_tpw", 0)
     assert _lib.get_tuning("vmat_tpw") == 0
+
+
+def test_scf_grid_rounds_like_the_reference(libpath):
+    """hqc_scf_grid is solver.py:159's odd grid, Python's round (ties to even)."""
+    from hqc_amd import _lib
+    for L, gl in [(7.7, 0.16), (4.03 * 1.25, 0.16), (10.0, 0.3), (0.32 * 5, 0.32),
+                  (0.32 * 7, 0.32), (9.6, 0.12)]:
+        assert _lib.scf_grid(L, gl) == round(L / gl / 2) * 2 + 1, (L, gl)
+
+
+def test_scf_driver_rejects_what_it_does_not_cover(libpath):
+    """Host-side argument checks of hqc_scf_create (no device call is reached):
+    Hartree-Fock (xc_id 0) and n_ao > 110 are refused with the library's codes."""
+    from hqc_amd import _lib
+    from hqc_amd.basis import HBasis
+    b = HBasis("gth-dzv")
+    with pytest.raises(RuntimeError, match="bad argument"):
+        _lib.ScfDriver(8, 7.7, 25, b.alpha, b.coef, b.comp, 1, xc_id=0)
+    with pytest.raises(RuntimeError, match="size not supported"):
+        _lib.ScfDriver(64, 12.0, 25, b.alpha, b.coef, b.comp, 1, xc_id=1)
