@@ -780,3 +780,47 @@ def test_overlap_kinetic_s_only_matches_generic(K, basis):
     for a, b in zip(*out):
         d = (a - b).abs().max().item()
         assert d <= 1e-15 * max(a.abs().max().item(), 1.0), d
+
+
+MIXED_V_SHAPES = [
+    # n, rs, basis, grid_length: n_ao_pad and grid spread around the workloads
+    (8, 1.25, "gth-szv", 0.3),       # 16 AOs, 25^3
+    (16, 1.25, "gth-dzv", 0.16),     # 32 AOs, 33^3
+    (54, 1.25, "gth-dzv", 0.16),     # 112 AOs, 49^3 (the benchmark)
+    (64, 1.5, "gth-dzv", 0.16),      # 128 AOs, 61^3 (the HF batch's largest)
+    (25, 1.25, "gth-dzvp", 0.16),    # 125 -> 128 AOs with p functions, 41^3
+    (128, 1.25, "gth-dzvp", 0.16),   # 640 AOs, 65^3: FP32 MFMA, not split-FP16
+]
+
+
+@pytest.mark.parametrize("shape", MIXED_V_SHAPES, ids=[f"{s[0]}H-{s[2]}" for s in MIXED_V_SHAPES])
+def test_mixed_v_error_margin(K, shape):
+    """The mixed path takes V(dv) in low precision while eps * max|dv| stays within its
+    error budget (engine.py _mixed_vmat), so eps must bound the per-element error of
+    the low-precision V relative to max|v|.  On the real AO values of each shape and
+    both a smooth potential (v_H + v_xc of the superposed-atom density) and a random
+    one, the measured max|V_low - V_64| / max|v| stays at least 3x below the engine's
+    eps (MIXED_EPS_SPLIT16 for the FP16 split kernel, MIXED_EPS for FP32)."""
+    from hqc_amd.pbc.engine import Cell, Engine
+    n, rs, basis, gl = shape
+    print(f"[mixed V margin] {n} H {basis} gl {gl}", flush=True)
+    L = (4 / 3 * math.pi * n) ** (1 / 3)
+    cell = Cell(n, L, rs, basis, grid_length=gl, device=DEV)
+    eng = Engine(cell, dft=True, precision="mixed")
+    xp = torch.tensor(np.random.default_rng(n).uniform(0, L, (1, n, 3)) * rs, device=DEV)
+    st = eng.setup(xp)
+    lp = "phi_h2" if "phi_h2" in st else "phi_lp"
+    eps = eng.MIXED_EPS_SPLIT16 if lp == "phi_h2" else eng.MIXED_EPS
+    # smooth: the potential of the sum of the AO densities; random: uniform in [-1, 1]
+    rho = (st["phi"][:, :cell.nao, :] ** 2).sum(1)
+    v_s, _, _ = eng.xc_potential(rho)
+    gen = torch.Generator(device="cpu").manual_seed(n)
+    v_r = torch.zeros(1, cell.ld, dtype=torch.float64)
+    v_r[:, :cell.N] = torch.rand(1, cell.N, generator=gen, dtype=torch.float64) * 2 - 1
+    for name, v in (("smooth", v_s), ("random", v_r.to(DEV))):
+        vmax = v.abs().amax(1)
+        V64 = eng.vmat(st["phi"], v)
+        Vlo = eng.vmat(st[lp], v, vmax=vmax)
+        ratio = ((Vlo - V64).abs().max() / vmax.max()).item()
+        print(f"  {name}: {lp} max|dV|/max|v| = {ratio:.2e} (eps {eps:.0e})", flush=True)
+        assert ratio * 3 <= eps, (name, ratio, eps)
