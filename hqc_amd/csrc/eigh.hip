@@ -65,7 +65,11 @@ __device__ __forceinline__ double2 jacobi_cs(double app, double aqq, double apq)
   return make_double2(c, t * c);
 }
 
-template <bool TABLE>
+// FAST: the round's pairs are written to LDS once (with the packed-row offsets of
+// both indices) by the threads that form the rotations, so the block and V
+// updates read them instead of recomputing positions and triangle offsets; the
+// arithmetic on A and V is the same, so are the results.
+template <bool TABLE, bool FAST>
 __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __restrict__ Ain, int n,
                                                      int npad, const int* __restrict__ pairs,
                                                      int max_sweeps, double tol,
@@ -81,6 +85,7 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
   const int cso = (na + n * n + 1) & ~1;          // 16-byte aligned
   double2* cs = reinterpret_cast<double2*>(sm + cso);   // np: (c, s) per pair
   double* red = sm + cso + 2 * np;                // EIG_THREADS/64 partial sums
+  int4* prs = reinterpret_cast<int4*>(red + EIG_THREADS / 64);   // np: (p, q, off_p, off_q)
   __shared__ double s_fro2;
   __shared__ int s_done, s_rot;
   const int nblk = np * (np + 1) / 2;
@@ -144,6 +149,8 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
       if (tid < np) {
         int p, q;
         round_pair<TABLE>(prow, tid, r, npad, p, q);
+        // off_i + j is the packed index of (i, j), i <= j
+        if (FAST) prs[tid] = make_int4(p, q, pk(p, 0, n), q < n ? pk(q, 0, n) : 0);
         double2 rot = make_double2(1.0, 0.0);
         if (q < n) {
           const double apq = Ap[pk(p, q, n)];
@@ -166,22 +173,53 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
         const double2 rI = cs[I], rJ = cs[J];
         if (rI.y == 0.0 && rJ.y == 0.0) continue;   // both rotations skipped
         int pI, qI, pJ, qJ;
-        round_pair<TABLE>(prow, I, r, npad, pI, qI);
-        round_pair<TABLE>(prow, J, r, npad, pJ, qJ);
+        int oPI = 0, oQI = 0, oPJ = 0, oQJ = 0;
+        if (FAST) {
+          const int4 a = prs[I], c = prs[J];
+          pI = a.x; qI = a.y; oPI = a.z; oQI = a.w;
+          pJ = c.x; qJ = c.y; oPJ = c.z; oQJ = c.w;
+        } else {
+          round_pair<TABLE>(prow, I, r, npad, pI, qI);
+          round_pair<TABLE>(prow, J, r, npad, pJ, qJ);
+        }
         const bool vqI = qI < n, vqJ = qJ < n;
         const double cI = rI.x, sI = rI.y, cJ = rJ.x, sJ = rJ.y;
+        // packed index of (i, j) from the row offsets of both
+        auto ix = [&](int i, int oi, int j, int oj) { return i <= j ? oi + j : oj + i; };
         // rows {pI,qI}, cols {pJ,qJ}; M' = R_I^T M R_J (for I == J the
         // symmetric diagonal block, whose off-diagonal comes out ~0)
-        const double m00 = getA(Ap, pI, pJ, n);
-        const double m01 = vqJ ? getA(Ap, pI, qJ, n) : 0.0;
-        const double m10 = vqI ? getA(Ap, qI, pJ, n) : 0.0;
-        const double m11 = (vqI && vqJ) ? getA(Ap, qI, qJ, n) : 0.0;
+        double m00, m01, m10, m11;
+        if (FAST) {
+          m00 = Ap[ix(pI, oPI, pJ, oPJ)];
+          m01 = vqJ ? Ap[ix(pI, oPI, qJ, oQJ)] : 0.0;
+          m10 = vqI ? Ap[ix(qI, oQI, pJ, oPJ)] : 0.0;
+          m11 = (vqI && vqJ) ? Ap[ix(qI, oQI, qJ, oQJ)] : 0.0;
+        } else {
+          m00 = getA(Ap, pI, pJ, n);
+          m01 = vqJ ? getA(Ap, pI, qJ, n) : 0.0;
+          m10 = vqI ? getA(Ap, qI, pJ, n) : 0.0;
+          m11 = (vqI && vqJ) ? getA(Ap, qI, qJ, n) : 0.0;
+        }
         // rows: [x; y] -> [c x - s y; s x + c y]
         const double r00 = cI * m00 - sI * m10, r01 = cI * m01 - sI * m11;
         const double r10 = sI * m00 + cI * m10, r11 = sI * m01 + cI * m11;
         // cols: [u v] -> [c u - s v, s u + c v]
         const double n00 = cJ * r00 - sJ * r01, n01 = sJ * r00 + cJ * r01;
         const double n10 = cJ * r10 - sJ * r11, n11 = sJ * r10 + cJ * r11;
+        if (FAST) {
+          if (I == J) {
+            if (!vqI) continue;
+            Ap[oPI + pI] = n00;
+            Ap[oQI + qI] = n11;
+            Ap[ix(pI, oPI, qI, oQI)] = 0.5 * (n01 + n10);
+            continue;
+          }
+          Ap[ix(pI, oPI, pJ, oPJ)] = n00;
+          if (vqJ) Ap[ix(pI, oPI, qJ, oQJ)] = n01;
+          if (vqI) Ap[ix(qI, oQI, pJ, oPJ)] = n10;
+          if (vqI && vqJ) Ap[ix(qI, oQI, qJ, oQJ)] = n11;
+          continue;
+        }
         auto put = [&](int i, int j, double val) {
           if (i <= j) Ap[pk(i, j, n)] = val;
           else Ap[pk(j, i, n)] = val;
@@ -204,7 +242,13 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
           const double2 rJ = cs[J];
           if (rJ.y == 0.0) continue;
           int pJ, qJ;
-          round_pair<TABLE>(prow, J, r, npad, pJ, qJ);
+          if (FAST) {
+            const int4 c = prs[J];
+            pJ = c.x;
+            qJ = c.y;
+          } else {
+            round_pair<TABLE>(prow, J, r, npad, pJ, qJ);
+          }
           if (qJ >= n) continue;
           const double x = V[pJ * n + vrow], y = V[qJ * n + vrow];
           V[pJ * n + vrow] = rJ.x * x - rJ.y * y;
@@ -232,7 +276,7 @@ extern "C" size_t hqc_eigh_lds_bytes(int n) {
   const int npad = n + (n & 1);
   const int np = npad / 2;
   const int cso = (n * (n + 1) / 2 + n * n + 1) & ~1;
-  return (size_t)(cso + 2 * np + EIG_THREADS / 64) * sizeof(double);
+  return (size_t)(cso + 2 * np + EIG_THREADS / 64) * sizeof(double) + (size_t)np * sizeof(int4);
 }
 
 extern "C" int hqc_eigh_jacobi(const double* A, int B, int n, const int* pairs, int max_sweeps,
@@ -242,7 +286,11 @@ extern "C" int hqc_eigh_jacobi(const double* A, int B, int n, const int* pairs, 
   if (B == 0) return HQC_OK;
   const int npad = n + (n & 1);
   const size_t lds = hqc_eigh_lds_bytes(n);
-  const void* kern = pairs ? (const void*)k_jacobi_eigh<true> : (const void*)k_jacobi_eigh<false>;
+  const bool fast = hqc_tuned(HQC_TUNE_EIGH_RECOMPUTE, 0) == 0;
+  const void* kern = pairs ? (fast ? (const void*)k_jacobi_eigh<true, true>
+                                   : (const void*)k_jacobi_eigh<true, false>)
+                           : (fast ? (const void*)k_jacobi_eigh<false, true>
+                                   : (const void*)k_jacobi_eigh<false, false>);
   if (lds > 64 * 1024) {
     // raise the dynamic-LDS limit where the runtime asks for it; a runtime that
     // does not support the attribute leaves the launch check below to decide
@@ -250,12 +298,19 @@ extern "C" int hqc_eigh_jacobi(const double* A, int B, int n, const int* pairs, 
         hipSuccess)
       (void)hipGetLastError();
   }
-  if (pairs)
-    hipLaunchKernelGGL(k_jacobi_eigh<true>, dim3(B), dim3(EIG_THREADS), lds, (hipStream_t)stream, A,
-                       n, npad, pairs, max_sweeps, tol, w, V, sweeps);
+  hipStream_t st = (hipStream_t)stream;
+#define EIG_LAUNCH(T_, F_)                                                                     \
+  hipLaunchKernelGGL((k_jacobi_eigh<T_, F_>), dim3(B), dim3(EIG_THREADS), lds, st, A, n, npad, \
+                     pairs, max_sweeps, tol, w, V, sweeps)
+  if (pairs && fast)
+    EIG_LAUNCH(true, true);
+  else if (pairs)
+    EIG_LAUNCH(true, false);
+  else if (fast)
+    EIG_LAUNCH(false, true);
   else
-    hipLaunchKernelGGL(k_jacobi_eigh<false>, dim3(B), dim3(EIG_THREADS), lds, (hipStream_t)stream,
-                       A, n, npad, pairs, max_sweeps, tol, w, V, sweeps);
+    EIG_LAUNCH(false, false);
+#undef EIG_LAUNCH
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }

@@ -17,7 +17,8 @@ extern "C" int hqc_set_tuning(int key, int value) {
   if (key == HQC_TUNE_FFT_PLANE_THREADS && value != 0 && value != 256 && value != 512 &&
       value != 1024)
     return HQC_E_ARG;
-  if ((key == HQC_TUNE_FFT_TWO_BUFFER && value > 1) || (key == HQC_TUNE_FFT_IP_ITEMS && value > 2))
+  if ((key == HQC_TUNE_FFT_TWO_BUFFER && value > 1) || (key == HQC_TUNE_FFT_IP_ITEMS && value > 2) ||
+      (key == HQC_TUNE_EIGH_RECOMPUTE && value > 1))
     return HQC_E_ARG;
   g_tune[key] = value;
   return HQC_OK;

@@ -61,7 +61,10 @@ int hqc_version(void);
                                         in place (the same outputs; kept for comparison) */
 #define HQC_TUNE_FFT_IP_ITEMS 9      /* 1 or 2: work items per thread of the in-place FFT
                                         passes, when they fit (the fewest that fit) */
-#define HQC_TUNE_COUNT 10
+#define HQC_TUNE_EIGH_RECOMPUTE 10   /* 1: Jacobi recomputes pair positions and packed offsets
+                                        per use instead of reading the round's table (the
+                                        same results; for comparison) */
+#define HQC_TUNE_COUNT 11
 int hqc_set_tuning(int key, int value);
 int hqc_get_tuning(int key);
 
