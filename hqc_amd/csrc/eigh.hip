@@ -128,6 +128,8 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
   const int vrow = vstep > 0 ? tid % n : n;
   const int vj0 = vstep > 0 ? tid / n : 0;
   const bool vrow_ok = vstep > 0 && tid < vstep * n;
+  const int vG = max(1, (int)blockDim.x / np);      // threads per pair (FAST)
+  const int vJ = tid / vG, vr0 = tid % vG;
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
     // off-diagonal norm
@@ -236,8 +238,21 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_eigh(const double* __res
         if (vqI) put(qI, pJ, n10);
         if (vqI && vqJ) put(qI, qJ, n11);
       }
-      // V <- V J: thread owns row vrow and every vstep-th pair from vj0
-      if (vrow_ok) {
+      // V <- V J.  FAST: thread owns pair vJ and rows vr0, vr0 + vG, ... (one read
+      // of its rotation and positions); else row vrow and every vstep-th pair from vj0
+      if (FAST) {
+        if (vJ < np) {
+          const double2 rJ = cs[vJ];
+          const int4 c = prs[vJ];
+          if (rJ.y != 0.0 && c.y < n) {
+            for (int row = vr0; row < n; row += vG) {
+              const double x = V[c.x * n + row], y = V[c.y * n + row];
+              V[c.x * n + row] = rJ.x * x - rJ.y * y;
+              V[c.y * n + row] = rJ.y * x + rJ.x * y;
+            }
+          }
+        }
+      } else if (vrow_ok) {
         for (int J = vj0; J < np; J += vstep) {
           const double2 rJ = cs[J];
           if (rJ.y == 0.0) continue;
