@@ -17,6 +17,8 @@
 // H is the first half of the grid in centred order: column c < (N-1)/2 is the
 // point with signed frequency s = (c / g^2, (c / g) % g, c % g) - g/2; its
 // partner -s is column N-1-c, and the centre (N-1)/2 is G = 0.
+#include <algorithm>
+
 #include "common.h"
 
 // z[m, p, r] = phi_p(r) (psi_{i_m}(r) + i psi_{j_m}(r)); psi rows are already
@@ -45,35 +47,40 @@ __global__ __launch_bounds__(256) void k_pair_pack(const double* __restrict__ ph
   store_c(z + ((long)m * nao + p) * N + r, f * a, f * c);
 }
 
+// One thread per column pair (c, c + nh): Z(G) and Z(-G) are read once for the
+// four outputs (Re and Im of A_a and A_b); columns past 2 nh and rows past n_ao
+// are written as zero.
 template <typename T, typename Z>
 __global__ __launch_bounds__(256) void k_pair_split(const Z* __restrict__ z, int nao,
                                                     int nao_pad, int g, int ld,
                                                     T* __restrict__ M) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ld) return;
-  const int p = blockIdx.y, m = blockIdx.z;
   const int N = g * g * g, nh = (N - 1) / 2;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y, m = blockIdx.z;
+  if (col >= max(nh, ld - 2 * nh)) return;        // threads per row
   T* Ma = M + ((long)(2 * m) * nao_pad + p) * ld;
   T* Mb = M + ((long)(2 * m + 1) * nao_pad + p) * ld;
-  if (p >= nao || col >= 2 * nh) {
-    Ma[col] = (T)0;
-    Mb[col] = (T)0;
+  // zero tail: columns [2 nh, ld)
+  if (2 * nh + col < ld) {
+    Ma[2 * nh + col] = (T)0;
+    Mb[2 * nh + col] = (T)0;
+  }
+  if (col >= nh) return;
+  if (p >= nao) {
+    Ma[col] = Ma[col + nh] = (T)0;
+    Mb[col] = Mb[col + nh] = (T)0;
     return;
   }
-  const int c = col < nh ? col : col - nh;
   const int h = g / 2;
-  const int s1 = c / (g * g) - h, s2 = (c / g) % g - h, s3 = c % g - h;
+  const int s1 = col / (g * g) - h, s2 = (col / g) % g - h, s3 = col % g - h;
   const int k = ((s1 + g) % g * g + (s2 + g) % g) * g + (s3 + g) % g;
   const int kn = ((g - s1) % g * g + (g - s2) % g) * g + (g - s3) % g;
   const Z zp = z[((long)m * nao + p) * N + k];
   const Z zn = z[((long)m * nao + p) * N + kn];
-  if (col < nh) {
-    Ma[col] = (T)(0.5 * ((double)zp.x + (double)zn.x));     // Re A_a
-    Mb[col] = (T)(0.5 * ((double)zp.y + (double)zn.y));     // Re A_b
-  } else {
-    Ma[col] = (T)(0.5 * ((double)zp.y - (double)zn.y));     // Im A_a
-    Mb[col] = (T)(0.5 * ((double)zn.x - (double)zp.x));     // Im A_b
-  }
+  Ma[col] = (T)(0.5 * ((double)zp.x + (double)zn.x));         // Re A_a
+  Mb[col] = (T)(0.5 * ((double)zp.y + (double)zn.y));         // Re A_b
+  Ma[col + nh] = (T)(0.5 * ((double)zp.y - (double)zn.y));    // Im A_a
+  Mb[col + nh] = (T)(0.5 * ((double)zn.x - (double)zp.x));    // Im A_b
 }
 
 template <typename Z>
@@ -108,8 +115,9 @@ static int launch_split(const Z* z, int mp, int nao, int nao_pad, int g, int ld,
   const long N = (long)g * g * g;
   if (ld < N || mp > 65535 || nao_pad > 65535) return HQC_E_SIZE;
   if (mp == 0) return HQC_OK;
-  hipLaunchKernelGGL((k_pair_split<T, Z>), dim3(hqc_div_up(ld, 256), nao_pad, mp), dim3(256), 0,
-                     (hipStream_t)stream, z, nao, nao_pad, g, ld, M);
+  const long nthr = std::max((N - 1) / 2, ld - (N - 1));     // threads per row
+  hipLaunchKernelGGL((k_pair_split<T, Z>), dim3(hqc_div_up(nthr, 256), nao_pad, mp), dim3(256),
+                     0, (hipStream_t)stream, z, nao, nao_pad, g, ld, M);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }

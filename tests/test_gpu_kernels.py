@@ -824,3 +824,37 @@ def test_mixed_v_error_margin(K, shape):
         ratio = ((Vlo - V64).abs().max() / vmax.max()).item()
         print(f"  {name}: {lp} max|dV|/max|v| = {ratio:.2e} (eps {eps:.0e})", flush=True)
         assert ratio * 3 <= eps, (name, ratio, eps)
+
+
+@pytest.mark.parametrize("g", [1, 3, 5, 9, 25])
+@pytest.mark.parametrize("dt", [torch.complex128, torch.complex64])
+def test_pair_split(K, g, dt):
+    """Half-spectrum split of transformed pair grids Z = FFT(a + i b) into the rows
+    [Re A_a(H) | Im A_a(H)], [Re A_b(H) | Im A_b(H)] with A_a(G) = (Z(G) + conj Z(-G))/2,
+    A_b(G) = (Z(G) - conj Z(-G))/(2i) over the centred half G-set H: every element
+    of M written (NaN-filled beforehand), padding rows and the columns past N - 1
+    zero, including grids whose zero tail is wider than the half-set (g = 1, 3)."""
+    print(f"[pair split] g={g} {dt}", flush=True)
+    torch.manual_seed(g)
+    mp, nao, nao_pad = 3, 5, 16
+    N = g ** 3
+    ld = (N + 63) // 64 * 64
+    z = torch.randn(mp, nao, N, dtype=dt, device=DEV)
+    rt = torch.float64 if dt == torch.complex128 else torch.float32
+    M = torch.full((2 * mp, nao_pad, ld), float("nan"), dtype=rt, device=DEV)
+    K.pair_split(z, nao_pad, ld, M)
+    nh, h = (N - 1) // 2, g // 2
+    c = torch.arange(nh, device=DEV)
+    s1, s2, s3 = c // (g * g) - h, (c // g) % g - h, c % g - h
+    k = ((s1 % g) * g + s2 % g) * g + s3 % g
+    kn = (((-s1) % g) * g + (-s2) % g) * g + (-s3) % g
+    zd = z.to(torch.complex128)
+    zp, zn = zd[..., k], zd[..., kn]
+    ref = torch.zeros(2 * mp, nao_pad, ld, dtype=torch.float64, device=DEV)
+    ref[0::2, :nao, :nh] = 0.5 * (zp.real + zn.real)
+    ref[0::2, :nao, nh:2 * nh] = 0.5 * (zp.imag - zn.imag)
+    ref[1::2, :nao, :nh] = 0.5 * (zp.imag + zn.imag)
+    ref[1::2, :nao, nh:2 * nh] = 0.5 * (zn.real - zp.real)
+    torch.cuda.synchronize()
+    assert not torch.isnan(M).any()
+    assert torch.equal(M, ref.to(rt))
