@@ -575,18 +575,25 @@ static void launch_plane(int mi, dim3 grid, size_t lds, hipStream_t st, C* data,
     hipLaunchKernelGGL((k_fft_plane<NT, C, 2>), grid, dim3(NT), lds, st, data, g, plan, sign);
 }
 
+// Whether the in-place passes are wanted by default: in FP64 only, and only where
+// two buffers would keep a CU below 16 waves (more than 40 KB of LDS per 256
+// threads).  Measured on 21-304 grids: in place 49^3 0.30 -> 0.24 ms, 61^3 0.48 ->
+// 0.34 ms at 256 threads, 55^3 0.161 -> 0.144 ms at 512; two buffers faster where
+// LDS does not limit (29^3 at 256, 45^3 at 512: fewer registers and barriers) and
+// in FP32 (the same occupancy either way).
+static bool inplace_wanted(int g, int lpb, int nt, size_t csize) {
+  return csize >= 16 && (size_t)2 * g * lpb * csize > (size_t)40 * 1024 * (nt / 256);
+}
+
 // Work items per thread of the in-place passes for a transform over lpb lines
-// with nt threads (0: the two-buffer passes).  Tuned values when they fit;
-// by default in place only in FP64 and only where two buffers would hold more
-// than 40 KB of LDS (fewer than four workgroups per CU), with 2 items per thread
-// where they fit, else 1.  Measured on 48-256 grids: in place 49^3 0.30 -> 0.24
-// ms, 61^3 0.48 -> 0.34 ms (FP64); two buffers faster at 29^3 (no LDS limit,
-// fewer registers) and in FP32 (same occupancy either way, fewer barriers).
+// with nt threads (0: the two-buffer passes): tuned values when they fit, else by
+// default 2 items per thread where they fit, else 1, where the in-place form is
+// wanted.
 static int inplace_items(const FftPlan& plan, int g, int lpb, int nt, size_t csize) {
   if (hqc_tuned(HQC_TUNE_FFT_TWO_BUFFER, 0)) return 0;
   const int want = hqc_tuned(HQC_TUNE_FFT_IP_ITEMS, 0);
   if (want) return fft_inplace_fits(plan, g, lpb, nt, want) ? want : 0;
-  if (csize < 16 || (size_t)2 * g * lpb * csize <= 40 * 1024) return 0;
+  if (!inplace_wanted(g, lpb, nt, csize)) return 0;
   for (int mi = 2; mi >= 1; --mi)
     if (fft_inplace_fits(plan, g, lpb, nt, mi)) return mi;
   return 0;
@@ -602,17 +609,25 @@ static int fft3d(C* data, int B, int g, int sign, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   const long lines = (long)g * g;
   // threads per plane: with enough planes to fill the GPU, 256 (several planes
-  // per CU) up to 61^2 in FP64 and 51^2 in FP32, above that 1024 (FP64) or 512
-  // (FP32); 1024 for few planes
+  // per CU) up to 61^2 in FP64 and 49^2 in FP32, 512 above; 1024 for few planes
   int nt = hqc_tuned(HQC_TUNE_FFT_PLANE_THREADS, 0);
   if (!nt) {
     const bool f64 = sizeof(C) == 16;
-    if ((long)g * B < 512)
+    if ((long)g * B < 512) {
       nt = 1024;
-    else if (g <= (f64 ? 61 : 51))
+    } else if (g <= (f64 ? 61 : 49)) {
       nt = 256;
-    else
-      nt = f64 ? 1024 : 512;
+      // FP64 planes that want the in-place form but whose passes do not fit 256
+      // threads (45^2 .. 57^2: radix-3/5 passes of 600-1100 items) take 512
+      // threads, in place or with two buffers as inplace_items decides there
+      // (45, 51, 55, 57: 0.155 -> 0.140, 0.225 -> 0.197, 0.179 -> 0.144, 0.276 ->
+      // 0.234 ms per 32-65 grids)
+      if (f64 && inplace_wanted(g, g, 256, sizeof(C)) && !fft_inplace_fits(plan, g, g, 256, 1) &&
+          !fft_inplace_fits(plan, g, g, 256, 2))
+        nt = 512;
+    } else {
+      nt = 512;
+    }
   }
   const int mi = inplace_items(plan, g, g, nt, sizeof(C));
   const size_t plane_lds = (size_t)((mi ? 1 : 2) * g * g + g) * sizeof(C);
