@@ -81,6 +81,7 @@ _SIGS = {
     "hqc_scf_grid": [_D, _D],
     "hqc_scf_create": [_P, _P],
     "hqc_scf_run": [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "hqc_scf_forces": [_P, _P, _I, _D, _I, _P, _P, _P, _P, _P],
     "hqc_scf_destroy": [_P],
 }
 
@@ -660,6 +661,19 @@ class ScfDriver:
                                        "converged", "n_iter")], _stream())
         out["converged"] = out["converged"].bool()
         return out
+
+    def forces(self, xp, kappa=10.0, Gmax=15):
+        """xp (B, n, 3) Bohr on the GPU -> (E_tot (B) Ry, F (B, n, 3) Ha/Bohr, converged,
+        n_iter): make_pes(...).energy_and_forces from the C driver."""
+        B, n = xp.shape[0], xp.shape[1]
+        dev = xp.device
+        E = torch.empty(B, dtype=F64, device=dev)
+        F = torch.empty(B, n, 3, dtype=F64, device=dev)
+        conv = torch.empty(B, dtype=torch.int32, device=dev)
+        nit = torch.empty(B, dtype=torch.int32, device=dev)
+        _call("hqc_scf_forces", self._ctx, _ptr(xp.contiguous(), F64, "xp"), B, float(kappa),
+              int(Gmax), _ptr(E), _ptr(F), _ptr(conv), _ptr(nit), _stream())
+        return E, F, conv.bool(), nit
 
     def close(self):
         if self._ctx:

@@ -226,6 +226,119 @@ __global__ void k_scf_scatter(unsigned* __restrict__ base, const unsigned* __res
   }
 }
 
+// Dp[b] = D[b] zero-padded to npad x npad
+__global__ void k_scf_pad(const double* __restrict__ D, double* __restrict__ Dp, int n, int npad,
+                          long total) {
+  const long pp = (long)npad * npad;
+  SCF_LOOP(total) {
+    const long b = idx / pp, r = idx % pp;
+    const int i = (int)(r / npad), j = (int)(r % npad);
+    Dp[idx] = (i < n && j < n) ? D[(b * n + i) * n + j] : 0.0;
+  }
+}
+
+__global__ void k_scf_add(const double* __restrict__ a, const double* __restrict__ b,
+                          double* __restrict__ out, long total) {
+  SCF_LOOP(total) out[idx] = a[idx] + b[idx];
+}
+
+// Pulay forces from the S/T centre derivatives (forces.py compute_forces):
+// F[b,A,c] = sum_{p in A} 2 sum_{q: atom(q) != A} (W_pq dS_pqc - D_pq dT_pqc)
+__global__ void k_scf_pulay(const double* __restrict__ W, const double* __restrict__ D,
+                            const double* __restrict__ dS, const double* __restrict__ dT, int n,
+                            int nloc, int B, double* __restrict__ F) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B * n * 3) return;
+  const int c = idx % 3, A = (idx / 3) % n, b = idx / (3 * n);
+  const int nao = n * nloc;
+  const long nn = (long)nao * nao;
+  double f = 0.0;
+  for (int o = 0; o < nloc; ++o) {
+    const int p = A * nloc + o;
+    double t = 0.0;
+    for (int q = 0; q < nao; ++q) {
+      if (q / nloc == A) continue;
+      const long pq = b * nn + (long)p * nao + q;
+      t += W[pq] * dS[pq * 3 + c] - D[pq] * dT[pq * 3 + c];
+    }
+    f += 2.0 * t;
+  }
+  F[idx] = f;
+}
+
+// fractional coordinates of xp wrapped into the cell (potential.py:38-40)
+__global__ void k_scf_frac(const double* __restrict__ xp, double L, double* __restrict__ fr,
+                           long total) {
+  SCF_LOOP(total) {
+    const double x = xp[idx] - L * floor(xp[idx] / L);
+    fr[idx] = x / L;
+  }
+}
+
+// Ewald proton-proton energy (Ry) and forces (Ha/Bohr) of one geometry per block
+// (potential.py:35-67 as hqc_amd/pbc/potential.py ewald_pp, Bohr units): the
+// real-space pair terms erfc(kappa d)/d over minimum-image fractional
+// separations, the reciprocal sums from hqc_ewald_recip, the Madelung term.
+__global__ __launch_bounds__(256) void k_scf_ewald(const double* __restrict__ fr, int n, double L,
+                                                   double kappa, const double* __restrict__ Erec,
+                                                   const double* __restrict__ grec, double mad,
+                                                   double* __restrict__ Epp,
+                                                   double* __restrict__ Fpp) {
+  __shared__ double red[4];
+  const int b = blockIdx.x;
+  const double* x = fr + (long)b * n * 3;
+  const double L2 = L * L, c2 = 2.0 * kappa / sqrt(M_PI);
+  double ps = 0.0;
+  for (int k = threadIdx.x; k < n * n; k += 256) {
+    const int i = k / n, j = k % n;
+    if (j <= i) continue;
+    double r[3], d2 = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      r[c] = x[i * 3 + c] - x[j * 3 + c];
+      r[c] -= rint(r[c]);
+      d2 += r[c] * r[c];
+    }
+    const double d = sqrt(d2);
+    ps += erfc(kappa * d) / d;
+  }
+  ps = block_sum(ps, red);
+  const int npair = n * (n - 1) / 2;
+  if (threadIdx.x == 0)
+    Epp[b] = 2.0 * (ps + Erec[b] + npair * (-M_PI / (kappa * kappa))) / L + n * mad / L;
+  // forces: atom i sums its pair gradients over j in order
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const int a = i < j ? i : j, bj = i < j ? j : i;
+      double r[3], d2 = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        r[c] = x[a * 3 + c] - x[bj * 3 + c];
+        r[c] -= rint(r[c]);
+        d2 += r[c] * r[c];
+      }
+      const double d = sqrt(d2), er = erfc(kappa * d);
+      const double dsr = -(er / (d * d) + c2 * exp(-(kappa * d) * (kappa * d)) / d);
+      const double s = i < j ? -1.0 : 1.0;          // F_i gets -grad_ij (i < j), +grad_ji
+      for (int c = 0; c < 3; ++c) f[c] += s * (dsr / d) * r[c] / L2;
+    }
+    for (int c = 0; c < 3; ++c) Fpp[((long)b * n + i) * 3 + c] = -grec[((long)b * n + i) * 3 + c] / L2 + f[c];
+  }
+}
+
+// F = F_pulay + (sum_blocks fp - dE + F_pp)
+__global__ void k_scf_ftotal(const double* __restrict__ Fp, const double* __restrict__ fp,
+                             const double* __restrict__ dE, const double* __restrict__ Fpp,
+                             int n, int nblk, int B, double* __restrict__ F) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B * n * 3) return;
+  const int c = idx % 3;
+  const long bA = idx / 3;
+  double s = 0.0;
+  for (int k = 0; k < nblk; ++k) s += fp[(bA * nblk + k) * 3 + c];
+  F[idx] = Fp[idx] + (s - dE[idx] + Fpp[idx]);
+}
+
 static dim3 grid_of(long total) {
   long b = (total + 255) / 256;
   return dim3((unsigned)std::min(b, 65536L));
@@ -600,10 +713,22 @@ static void pulay_solve(const double* Bm, int m, std::vector<double>& ck) {
 
 }  // namespace
 
-extern "C" int hqc_scf_run(hqc_scf_ctx* x, const double* xp, int B, double* mo_out,
-                           double* dm_out, double* bands, double* E_out, double* Ki_out,
-                           double* Vep_out, double* Vee_out, double* Se_out, int* conv_out,
-                           int* nit_out, void* stream) {
+namespace {
+struct ForceOut {
+  double kappa;
+  int Gmax;
+  double* Etot;
+  double* F;
+};
+}  // namespace
+
+static int scf_forces(Run& R, Buf& buf, const double* xp, const double* Ein_ry,
+                      const ForceOut& fo);
+
+static int scf_core(hqc_scf_ctx* x, const double* xp, int B, double* mo_out, double* dm_out,
+                    double* bands, double* E_out, double* Ki_out, double* Vep_out,
+                    double* Vee_out, double* Se_out, int* conv_out, int* nit_out, void* stream,
+                    const ForceOut* fo) {
   if (!x || B < 0 || !xp) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
   const hqc_scf_cell& c = x->c;
@@ -871,6 +996,155 @@ extern "C" int hqc_scf_run(hqc_scf_ctx* x, const double* xp, int B, double* mo_o
     return HQC_OK;
   };
   if ((r = unperm(R.mo, mo_out)) || (r = unperm(R.D, dm_out))) return r;
+  if (fo) {
+    // the force pass works in input order: mo and w1 of each geometry back there
+    if ((r = unperm(R.mo, R.t1))) return r;
+    hipLaunchKernelGGL(k_scf_scatter, grid_of((long)B * R.nao * 2), dim3(256), 0, st,
+                       (unsigned*)R.ws, (const unsigned*)R.w1, R.order, (long)R.nao * 2,
+                       (long)B * R.nao * 2);
+    HQC_CHECK_LAUNCH();
+    if (hipMemcpyAsync(R.mo, R.t1, (size_t)B * nn * 8, hipMemcpyDeviceToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(R.w1, R.ws, (size_t)B * R.nao * 8, hipMemcpyDeviceToDevice, st) !=
+            hipSuccess)
+      return HQC_E_HIP;
+    if ((r = scf_forces(R, buf, xp, oE.data(), *fo))) return r;
+  }
   if (hipStreamSynchronize(st) != hipSuccess) return HQC_E_HIP;
   return HQC_OK;
+}
+
+// Forces of the converged states in input order (forces.py compute_forces, DFT
+// at Gamma): Pulay terms from the S/T centre derivatives with W = mo (f e) mo^T,
+// the grid adjoint of the AO contractions (X = D phi, v_eff = v_H + v_xc + v_loc),
+// the structure-factor derivative of the local potential, and Ewald.
+static int scf_forces(Run& R, Buf& buf, const double* xp, const double* Ein_ry,
+                      const ForceOut& fo) {
+  hqc_scf_ctx* x = R.x;
+  const hqc_scf_cell& c = x->c;
+  hipStream_t st = R.st;
+  const int B = R.B, n = c.n, nao = R.nao, np = R.np;
+  const long nn = R.nn, ld = R.ld;
+  const int nlf = std::max(3, c.nl + 1);           // l_max + 2 powers, at least 3
+  const int nblkF = hqc_force_blocks(c.g);
+  // Ewald G vectors and weights (potential.py:6-33)
+  std::vector<double> Gv, gk;
+  double mad = 0.0;
+  for (int i = -fo.Gmax; i <= fo.Gmax; ++i)
+    for (int j = -fo.Gmax; j <= fo.Gmax; ++j)
+      for (int k = -fo.Gmax; k <= fo.Gmax; ++k) {
+        const int G2 = i * i + j * j + k * k;
+        if (G2 == 0 || G2 > fo.Gmax * fo.Gmax) continue;
+        Gv.push_back(i);
+        Gv.push_back(j);
+        Gv.push_back(k);
+        const double w =
+            std::exp(-M_PI * M_PI * G2 / (fo.kappa * fo.kappa)) / (M_PI * G2);
+        gk.push_back(w);
+        mad += w;
+      }
+  mad += -M_PI / (fo.kappa * fo.kappa) - 2.0 * fo.kappa / std::sqrt(M_PI);
+  const int nG = (int)gk.size();
+  double* X = buf.get<double>((size_t)B * np * ld);
+  double* Dp = buf.get<double>((size_t)B * np * np);
+  double* dS = buf.get<double>((size_t)B * nn * 3);
+  double* dT = buf.get<double>((size_t)B * nn * 3);
+  double* P3 = buf.get<double>((size_t)B * n * 3 * c.g * c.na * nlf);
+  double* fp = buf.get<double>((size_t)B * n * nblkF * 3);
+  double* Fpul = buf.get<double>((size_t)B * n * 3);
+  double* dE = buf.get<double>((size_t)B * n * 3);
+  double* Fpp = buf.get<double>((size_t)B * n * 3);
+  double* grec = buf.get<double>((size_t)B * n * 3);
+  double* fr = buf.get<double>((size_t)B * n * 3);
+  double* Erec = buf.get<double>((size_t)B);
+  double* Epp = buf.get<double>((size_t)B);
+  double* work = buf.get<double>((size_t)B * nG * 2);
+  double* rg = buf.get<double>((size_t)2 * B * R.N);
+  double* dG = nullptr;
+  double* dgk = nullptr;
+  if (to_device(Gv.data(), Gv.size(), &dG) || to_device(gk.data(), gk.size(), &dgk)) {
+    (void)hipFree(dG);
+    (void)hipFree(dgk);
+    return HQC_E_HIP;
+  }
+  buf.all.push_back(dG);
+  buf.all.push_back(dgk);
+  for (void* p : buf.all)
+    if (!p) return HQC_E_HIP;
+  int r;
+  // D and W = mo diag(f e) mo^T of the final orbitals (input order, phi by identity)
+  hipLaunchKernelGGL(k_scf_iota, dim3((B + 255) / 256), dim3(256), 0, st, R.gidx, B);
+  HQC_CHECK_LAUNCH();
+  if ((r = R.density_matrix(B))) return r;
+  hipLaunchKernelGGL(k_scf_scale_cols, grid_of(B * nn), dim3(256), 0, st, R.mo, R.occ, R.t1, nao,
+                     (long)B * nn, 1);
+  hipLaunchKernelGGL(k_scf_scale_cols, grid_of(B * nn), dim3(256), 0, st, R.t1, R.w1, R.t2, nao,
+                     (long)B * nn, 1);
+  HQC_CHECK_LAUNCH();
+  if ((r = R.gemm(false, true, R.t2, nn, R.mo, nn, R.A, B))) return r;       // W
+  // S/T centre derivatives
+  r = x->s_only ? hqc_overlap_kinetic_s(xp, B, n, x->d_lattice, x->nT, x->d_alpha, c.na,
+                                        x->d_coef, c.nloc, R.S, R.T, dS, dT, st)
+                : hqc_overlap_kinetic(xp, B, n, x->d_lattice, x->nT, x->d_alpha, c.na,
+                                      x->d_coef, x->d_comp, c.nloc, R.S, R.T, dS, dT, st);
+  if (r) return r;
+  hipLaunchKernelGGL(k_scf_pulay, dim3((B * n * 3 + 255) / 256), dim3(256), 0, st, R.A, R.D, dS,
+                     dT, n, c.nloc, B, Fpul);
+  hipLaunchKernelGGL(k_scf_pad, grid_of((long)B * np * np), dim3(256), 0, st, R.D, Dp, nao, np,
+                     (long)B * np * np);
+  HQC_CHECK_LAUNCH();
+  // grid adjoint: rho, X = D phi, v_H with rho~(G) kept, v_xc, v_eff
+  if ((r = hqc_ao_density_f64(R.phi, nullptr, Dp, B, np, ld, R.rho, X, st)) ||
+      (r = hqc_real_to_complex(R.rho, B, c.g, ld, x->dV, R.cb, st)) ||
+      (r = hqc_fft3d_c2c_f64(R.cb, B, c.g, -1, st)) ||
+      (r = hqc_poisson_mul(R.cb, x->d_VG, B, c.g, 0, rg, st)) ||
+      (r = hqc_fft3d_c2c_f64(R.cb, B, c.g, +1, st)) ||
+      (r = hqc_xc_eval(R.rho, R.cb, 0, B, c.g, ld, c.xc_id, R.v, R.part, st)))
+    return r;
+  hipLaunchKernelGGL(k_scf_add, grid_of(B * ld), dim3(256), 0, st, R.v, R.vl, R.v, (long)B * ld);
+  HQC_CHECK_LAUNCH();
+  if ((r = hqc_gto_tables(xp, B, n, c.g, c.L, x->tmax_img, x->d_alpha, c.na, nlf, P3, st)) ||
+      (r = hqc_grid_forces(P3, B, n, c.g, c.na, nlf, x->d_alpha, x->d_coef, x->d_comp, c.nloc,
+                           np, (int)ld, X, R.v, nullptr, x->dV, fp, st)) ||
+      (r = hqc_vloc_force(xp, B, n, c.g, c.L, x->d_VG, rg, dE, st)))
+    return r;
+  // Ewald in Bohr units
+  hipLaunchKernelGGL(k_scf_frac, grid_of((long)B * n * 3), dim3(256), 0, st, xp, c.L, fr,
+                     (long)B * n * 3);
+  HQC_CHECK_LAUNCH();
+  if ((r = hqc_ewald_recip(fr, B, n, dG, dgk, nG, Erec, grec, work, st))) return r;
+  hipLaunchKernelGGL(k_scf_ewald, dim3(B), dim3(256), 0, st, fr, n, c.L, fo.kappa, Erec, grec,
+                     mad, Epp, Fpp);
+  hipLaunchKernelGGL(k_scf_ftotal, dim3((B * n * 3 + 255) / 256), dim3(256), 0, st, Fpul, fp, dE,
+                     Fpp, n, nblkF, B, fo.F);
+  HQC_CHECK_LAUNCH();
+  // E_tot = E + V_pp (Ry)
+  std::vector<double> hEpp((size_t)B);
+  if (hipMemcpyAsync(hEpp.data(), Epp, B * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return HQC_E_HIP;
+  for (int i = 0; i < B; ++i) hEpp[i] += Ein_ry[i];
+  if (fo.Etot && hipMemcpyAsync(fo.Etot, hEpp.data(), B * 8, hipMemcpyHostToDevice, st) !=
+                     hipSuccess)
+    return HQC_E_HIP;
+  return hipStreamSynchronize(st) == hipSuccess ? HQC_OK : HQC_E_HIP;
+}
+
+extern "C" int hqc_scf_run(hqc_scf_ctx* x, const double* xp, int B, double* mo_out,
+                           double* dm_out, double* bands, double* E_out, double* Ki_out,
+                           double* Vep_out, double* Vee_out, double* Se_out, int* conv_out,
+                           int* nit_out, void* stream) {
+  return scf_core(x, xp, B, mo_out, dm_out, bands, E_out, Ki_out, Vep_out, Vee_out, Se_out,
+                  conv_out, nit_out, stream, nullptr);
+}
+
+extern "C" int hqc_scf_forces(hqc_scf_ctx* x, const double* xp, int B, double kappa, int Gmax,
+                              double* E_tot, double* F, int* conv_out, int* nit_out,
+                              void* stream) {
+  if (!x || !F || !(kappa > 0.0) || Gmax < 1) return HQC_E_ARG;
+  // the forces are -dA/dR only for Fermi occupations (forces.py check_supported)
+  if (x->c.smearing && x->c.smearing_method != 0) return HQC_E_ARG;
+  const ForceOut fo{kappa, Gmax, E_tot, F};
+  return scf_core(x, xp, B, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                  conv_out, nit_out, stream, &fo);
 }

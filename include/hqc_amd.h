@@ -385,8 +385,9 @@ int hqc_ewald_recip(const double* frac, int B, int n, const double* Gv, const do
 /* Forces: grid adjoint of the AO contractions (new; the reference has none).
  *   F[b,A,c] = dV * sum_r sum_{p in A} Z_p(r) d phi_p(r)/d r_c,
  *   Z = 2 v(r) X_p(r) + Zx_p(r)  (Zx may be NULL)
- * grad phi is recomputed from tables P built with nl = 3 (s) or 4 (p), i.e.
- * l_max + 2 powers; alpha (na) are the primitive exponents.
+ * grad phi is recomputed from tables P built with nl = l_max + 2 powers and at
+ * least 3 (nl = 3 for s and p bases; HQC_E_ARG below 3); alpha (na) are the
+ * primitive exponents.
  * part (B, n, nblk, 3), nblk = hqc_force_blocks(g). */
 int hqc_force_blocks(int g);
 int hqc_grid_forces(const double* P, int B, int n, int g, int na, int nl, const double* alpha,
@@ -455,6 +456,15 @@ int hqc_scf_create(const hqc_scf_cell* cell, hqc_scf_ctx** ctx);
 int hqc_scf_run(hqc_scf_ctx* ctx, const double* xp, int B, double* mo, double* dm,
                 double* bands, double* E, double* Ki, double* Vep, double* Vee, double* Se,
                 int* converged, int* n_iter, void* stream);
+/* hqc_scf_forces: the same SCF, then the energy and forces of make_pes(...,
+ * dft=True).energy_and_forces (hqc/pbc/pes.py:10-154 with the forces this package
+ * adds): E_tot (B) Ry = E + V_pp (the Ewald sum of potential.py:35-67 with kappa,
+ * Gmax), F (B, n, 3) Ha/Bohr = -dA/dR, A = E - sigma S + V_pp (Pulay, grid
+ * adjoint, local potential and Ewald terms); converged, n_iter as above.  F is
+ * required; Gaussian smearing is refused (HQC_E_ARG: its forces are not the
+ * gradient of A). */
+int hqc_scf_forces(hqc_scf_ctx* ctx, const double* xp, int B, double kappa, int Gmax,
+                   double* E_tot, double* F, int* converged, int* n_iter, void* stream);
 void hqc_scf_destroy(hqc_scf_ctx* ctx);
 
 #ifdef __cplusplus

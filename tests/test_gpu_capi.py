@@ -113,3 +113,46 @@ def test_c_driver_headline_workload():
     assert torch.allclose(out["E"], ref[3], atol=4e-8 * n, rtol=0), (out["E"], ref[3])
     assert torch.allclose(out["bands"], ref[2], atol=1e-6, rtol=0)
     assert torch.allclose(out["dm"], ref[1], atol=1e-5, rtol=0)
+
+
+FORCE_CASES = [
+    ("dzv_fermi_n8", 8, 1.25, "gth-dzv", 0.3, 3,
+     dict(smearing=True, smearing_sigma=0.02, tol=1e-9, search_tol=1e-12, search_cycle=200)),
+    ("szv_aufbau_n8", 8, 1.25, "gth-szv", 0.3, 2, dict(tol=1e-9)),
+    ("dzv_54H_49", 54, 1.25, "gth-dzv", 0.16, 3,
+     dict(smearing=True, smearing_sigma=1.0 / (157888.088922572 / 1e4) / 2.0, tol=1e-8,
+          search_tol=1e-12, search_cycle=200)),
+]
+
+
+@pytest.mark.parametrize("case", FORCE_CASES, ids=[c[0] for c in FORCE_CASES])
+def test_c_driver_forces_match_pes(case):
+    """hqc_scf_forces against make_pes(...).energy_and_forces on the same geometries:
+    total energy (with V_pp) and forces (Pulay, grid adjoint, v_loc, Ewald)."""
+    from hqc_amd.pbc import make_pes
+    name, n, rs, basis, gl, B, kw = case
+    print(f"[C SCF forces] {name}", flush=True)
+    xp, L = _geoms(n, B, 3)
+    pes = make_pes(n, L, rs, basis, grid_length=gl, dft=True, **kw)
+    E_ref, F_ref = pes.energy_and_forces(xp)
+    drv = _driver(pes.solver, rs, kw)
+    E, F, conv, nit = drv.forces(torch.tensor(xp * rs, device=DEV))
+    drv.close()
+    print("  dE", (E - E_ref).abs().max().item(), "dF", (F - F_ref).abs().max().item(),
+          "n_iter", nit.tolist(), pes.solver.last["nit"].tolist(), flush=True)
+    assert conv.tolist() == pes.solver.last["conv"].bool().tolist()
+    assert torch.allclose(E, E_ref, atol=1e-8 * n, rtol=0), (E, E_ref)
+    assert torch.allclose(F, F_ref, atol=1e-7, rtol=0)
+
+
+def test_c_driver_forces_refuse_gauss():
+    from hqc_amd import _lib as K
+    from hqc_amd.pbc import make_solver
+    n, rs = 4, 1.5
+    L = (4 / 3 * np.pi * n) ** (1 / 3)
+    kw = dict(smearing=True, smearing_method="gauss", smearing_sigma=0.05, tol=1e-8)
+    sol = make_solver(n, L, rs, "gth-dzv", grid_length=0.3, dft=True, **kw)
+    drv = _driver(sol, rs, kw)
+    with pytest.raises(RuntimeError, match="bad argument"):
+        drv.forces(torch.zeros(1, n, 3, dtype=torch.float64, device=DEV))
+    drv.close()
