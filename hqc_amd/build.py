@@ -30,15 +30,23 @@ def sources():
 
 def build(verbose=False):
     """Compile every kernel source into one shared library; returns its path."""
-    objs = []
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    cmds, objs = [], []
     for src in sources():
         obj = os.path.join(HERE, "_build", os.path.basename(src) + ".o")
-        cmd = [HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-c", src, "-o", obj]
+        cmds.append([HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-c", src, "-o", obj])
+        objs.append(obj)
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    # one hipcc per source, a few at a time (each takes seconds to a minute)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(compile_one, cmds))
     tmp = LIB + ".tmp"
     # rocBLAS for the SCF driver's batched GEMMs (scf.hip)
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + \
