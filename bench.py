@@ -173,6 +173,24 @@ def measured_traffic(kernel_prefix, B):
     return None
 
 
+MFMA_FILE = os.path.join(ROOT, "profiles", f"pmc_mfma_54H_dzv_b{TRAFFIC_BATCH}.txt")
+
+
+def measured_mfma_util(kernel_prefix, B):
+    """rocprofv3 MfmaUtil (%) of a kernel at this workload with batch TRAFFIC_BATCH,
+    from the table kept with the profiles (None for other batch sizes or when absent)."""
+    if B != TRAFFIC_BATCH or not os.path.isfile(MFMA_FILE):
+        return None
+    for line in open(MFMA_FILE):
+        parts = line.split()
+        if len(parts) >= 4 and not line.startswith("#") and \
+                line.split(None, 1)[1].startswith(kernel_prefix):
+            for i, tok in enumerate(parts):      # the first number after the kernel name
+                if tok.endswith(">"):
+                    return float(parts[i + 1])
+    return None
+
+
 def run_path(path, x, steps, warmup):
     """(pes, seconds per call of energy_and_forces(x), E, F)."""
     import torch
@@ -548,7 +566,8 @@ def main():
         vk = {"kernel": vname + "> (dV phi diag(v) phi^T, symmetric: n(n+1)N flops)",
               "bound": "mfma", "unit": "TFLOP/s", "peak": MI355X_PEAK[prec_v]}
     vk.update({"ms_per_launch": ms_v, "flops_per_launch": fl_v,
-               "calls_per_step": n_vmat / nsteps, "traffic": measured_traffic(vname, B)})
+               "calls_per_step": n_vmat / nsteps, "traffic": measured_traffic(vname, B),
+               "mfma_util_pct": measured_mfma_util(vname, B)})
     if q8:
         # int8 digits (4 bytes per phi value) streamed once per launch: HBM roof; the
         # int8 MFMA work of the kept digit pairs is reported beside it
@@ -557,7 +576,8 @@ def main():
               "bytes_per_launch": by_d, "peak": MI355X_HBM_GBS,
               "int8_mfma_tops": ops_d / (ms_d * 1e-3) / 1e12,
               "int8_mfma_frac": ops_d / (ms_d * 1e-3) / 1e12 / MI355X_PEAK["i8"],
-              "traffic": measured_traffic("k_orb_density_q8", B)}
+              "traffic": measured_traffic("k_orb_density_q8", B),
+              "mfma_util_pct": measured_mfma_util("k_orb_density_q8", B)}
     else:
         dk = {"kernel": "k_orb_density_valu (sum_i f_i (C^T phi)_i^2, FP64 vector FMA)",
               "bound": "mfma", "unit": "TFLOP/s", "peak": MI355X_PEAK["f64"],
@@ -597,6 +617,10 @@ def main():
                      "profiles/" + os.path.basename(TRAFFIC_FILE) + " (rocprofv3 --pmc "
                      "FETCH_SIZE x2.0, WRITE_SIZE x1.0 from a 1 GiB copy; not measured in "
                      "this run)",
+                     "mfma_util_pct": dom.get("mfma_util_pct"),
+                     "mfma_util_source": None if dom.get("mfma_util_pct") is None else
+                     "profiles/" + os.path.basename(MFMA_FILE) + " (rocprofv3 --pmc MfmaUtil "
+                     "of the kernel probe; not measured in this run)",
                      "ms_per_launch": dom["ms_per_launch"],
                      "flops_per_launch": dom["flops_per_launch"]},
         "kernels": cand,
