@@ -337,12 +337,14 @@ def kernel_probe(name, B, reps=5):
         rho = torch.zeros(B, c.ld, dtype=torch.float64, device="cuda")
         for _ in range(reps):
             K.orb_density_q8(q, pe, At, rho, ns)
-    elif name in ("exchange", "exchange_c64"):
+    elif name in ("exchange", "exchange_c64", "exchange_fused", "exchange_fused_c64"):
         # the Hartree-Fock exchange pass of one geometry: 16 orbital pairs packed
         # against the AO values, 3-D transformed (complex128, or complex64 for the
-        # mixed path's early iterations) and split into the real contraction rows
+        # mixed path's early iterations) and split into the real contraction rows;
+        # _fused: packed on the transform's first pass, as the solver does
         mp = 16
-        c64 = name == "exchange_c64"
+        c64 = name.endswith("_c64")
+        fused = "fused" in name
         phi = torch.rand(1, c.nao_pad, c.ld, dtype=torch.float64, device="cuda")
         psi = torch.rand(1, 2 * mp, c.ld, dtype=torch.float64, device="cuda")
         table = torch.tensor([[0, 2 * k, 2 * k + 1] for k in range(mp)], dtype=torch.int32)
@@ -351,8 +353,11 @@ def kernel_probe(name, B, reps=5):
         M = torch.empty(2 * mp, c.nao_pad, c.ld, device="cuda",
                         dtype=torch.float32 if c64 else torch.float64)
         for _ in range(reps):
-            K.pair_pack(phi, psi, table, c.nao, c.g, z)
-            K.fft3d(z.view(mp * c.nao, c.g, c.g, c.g), -1)
+            if fused:
+                K.pair_pack(phi, psi, table, c.nao, c.g, z, fft_sign=-1)
+            else:
+                K.pair_pack(phi, psi, table, c.nao, c.g, z)
+                K.fft3d(z.view(mp * c.nao, c.g, c.g, c.g), -1)
             K.pair_split(z, c.nao_pad, c.ld, M)
     else:
         raise SystemExit(f"unknown kernel {name}")
@@ -506,7 +511,8 @@ def main():
                     help="skip the FP64 batched and plain runs (accuracy and speedups)")
     ap.add_argument("--kernel-probe", default=None,
                     help="vmat_f64|vmat_f32|vmat_h2|orb_density|orb_density_q8|exchange|"
-                         "exchange_c64|copy: launch that kernel (or the exchange pass) only")
+                         "exchange_c64|exchange_fused|exchange_fused_c64|copy: launch that "
+                         "kernel (or the exchange pass) only")
     ap.add_argument("--workload", default="metric",
                     choices=["metric", "hf-batch", "hf-128", "kmesh"],
                     help="metric (default): the 54 H LDA batch; the others are the uneven "

@@ -48,6 +48,8 @@ _SIGS = {
     "hqc_pair_pack_c64": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_split_c64": [_P, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_pack": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
+    "hqc_pair_pack_fft": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _I, _P],
+    "hqc_pair_pack_fft_c64": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _I, _P],
     "hqc_pair_split_f64": [_P, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_split_f32": [_P, _I, _I, _I, _I, _I, _P, _P],
     "hqc_vloc_G": [_P, _I, _I, _I, _D, _P, _P, _P],
@@ -356,9 +358,18 @@ def fft3d(c, sign):
     _call(name, _ptr(c, None, "c"), B, g, int(sign), _stream())
 
 
-def pair_pack(phi, psi, table, nao, g, z):
+# hqc_pair_pack_fft covers grids whose y-z plane fits the LDS with two buffers (g <= 69
+# in FP64, whatever schedule is tuned) and at most this
+# many (pair, AO) grids per call
+PAIR_FFT_MAX_G = 69
+PAIR_FFT_MAX_GRIDS = 65535
+
+
+def pair_pack(phi, psi, table, nao, g, z, fft_sign=None):
     """z (mp, nao, g^3) complex128 <- phi[b,p] (psi[b,i] + i psi[b,j]) per row (b, i, j)
-    of the host int32 table (validated here, then copied to the device)."""
+    of the host int32 table (validated here, then copied to the device).  fft_sign
+    (+1 or -1): z is also 3-D transformed with that sign, the pair densities formed
+    on the transform's first pass (hqc_pair_pack_fft)."""
     B, nao_pad, ld = phi.shape
     nmo = psi.shape[1]
     mp = table.shape[0]
@@ -373,9 +384,15 @@ def pair_pack(phi, psi, table, nao, g, z):
         raise ValueError("pair table entry out of range")
     td = table.to(phi.device)
     name = "hqc_pair_pack" if z.dtype == torch.complex128 else "hqc_pair_pack_c64"
-    _call(name, _ptr(phi, F64, "phi"), _ptr(psi, F64, "psi"),
-          _ptr(td, torch.int32, "table"), mp, nao, nao_pad, nmo, g, ld, _ptr(z, None, "z"),
-          _stream())
+    if fft_sign is None:
+        _call(name, _ptr(phi, F64, "phi"), _ptr(psi, F64, "psi"),
+              _ptr(td, torch.int32, "table"), mp, nao, nao_pad, nmo, g, ld, _ptr(z, None, "z"),
+              _stream())
+    else:
+        assert fft_sign in (1, -1)
+        _call(name.replace("_pack", "_pack_fft"), _ptr(phi, F64, "phi"), _ptr(psi, F64, "psi"),
+              _ptr(td, torch.int32, "table"), mp, nao, nao_pad, nmo, g, ld, _ptr(z, None, "z"),
+              int(fft_sign), _stream())
 
 
 def pair_split(z, nao_pad, ld, M):

@@ -513,9 +513,27 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(C* __restrict__ data, 
 // schedule only.
 #define FFT_PLANE_MAX_LDS (156 * 1024)
 
-template <int NT, typename C, int MI>
+// The exchange's pair densities formed on the plane pass's loads: grid m * nao + p
+// of the batch is phi_p (psi_i + i psi_j) of table row m = (geometry, i, j) (j < 0:
+// none), the product hqc_pair_pack stores (an FP64 multiply, rounded once to FP32
+// for complex64), so the pack's write of the grids and this pass's read of them
+// are saved.  Unused (null) for a plain transform.
+struct PairSrc {
+  const double* phi;   // (geometries, nao_pad, ld) AO values
+  const double* psi;   // (geometries, nmo, ld) weighted orbitals
+  const int* table;    // (mp, 3)
+  int nao, nao_pad, nmo;
+  long ld;
+};
+
+__device__ __forceinline__ double2 pair_c(double a, double b, double2*) { return make_double2(a, b); }
+__device__ __forceinline__ float2 pair_c(double a, double b, float2*) {
+  return make_float2((float)a, (float)b);
+}
+
+template <int NT, typename C, int MI, bool PACK>
 __global__ __launch_bounds__(NT) void k_fft_plane(C* __restrict__ data, int g, FftPlan plan,
-                                                  int sign) {
+                                                  int sign, PairSrc src) {
   extern __shared__ unsigned char sm_raw[];
   const int gg = g * g;
   C* bufA = reinterpret_cast<C*>(sm_raw);
@@ -523,7 +541,21 @@ __global__ __launch_bounds__(NT) void k_fft_plane(C* __restrict__ data, int g, F
   C* tw = MI ? bufB : bufB + gg;
   C* plane = data + ((long)blockIdx.y * g + blockIdx.x) * gg;
   for (int m = threadIdx.x; m < g; m += blockDim.x) tw[m] = twiddle<C>(m, g, sign);
-  for (int e = threadIdx.x; e < gg; e += blockDim.x) bufA[e] = plane[e];
+  if (PACK) {
+    const int m = blockIdx.y / src.nao, p = blockIdx.y - m * src.nao;
+    const int b = src.table[3 * m], i = src.table[3 * m + 1], j = src.table[3 * m + 2];
+    const long off = (long)blockIdx.x * gg;
+    const double* F = src.phi + ((long)b * src.nao_pad + p) * src.ld + off;
+    const double* A = src.psi + ((long)b * src.nmo + i) * src.ld + off;
+    const double* B = src.psi + ((long)b * src.nmo + (j >= 0 ? j : i)) * src.ld + off;
+    for (int e = threadIdx.x; e < gg; e += blockDim.x) {
+      const double f = F[e];
+      const double c = j >= 0 ? B[e] : 0.0;
+      bufA[e] = pair_c(f * A[e], f * c, (C*)nullptr);
+    }
+  } else {
+    for (int e = threadIdx.x; e < gg; e += blockDim.x) bufA[e] = plane[e];
+  }
   __syncthreads();
   if (MI) {
     run_passes_ip<MI ? MI : 1>(bufA, tw, plan, g, g, 1, g, sign);      // along z
@@ -558,21 +590,22 @@ static int make_plan(int g, FftPlan& plan) {
   return HQC_OK;
 }
 
-template <int NT, typename C>
+template <int NT, typename C, bool PACK>
 static const void* plane_kernel(int mi) {
-  return mi == 0 ? (const void*)k_fft_plane<NT, C, 0>
-                 : (mi == 1 ? (const void*)k_fft_plane<NT, C, 1> : (const void*)k_fft_plane<NT, C, 2>);
+  return mi == 0 ? (const void*)k_fft_plane<NT, C, 0, PACK>
+                 : (mi == 1 ? (const void*)k_fft_plane<NT, C, 1, PACK>
+                            : (const void*)k_fft_plane<NT, C, 2, PACK>);
 }
 
-template <int NT, typename C>
+template <int NT, typename C, bool PACK>
 static void launch_plane(int mi, dim3 grid, size_t lds, hipStream_t st, C* data, int g,
-                         const FftPlan& plan, int sign) {
+                         const FftPlan& plan, int sign, const PairSrc& src) {
   if (mi == 0)
-    hipLaunchKernelGGL((k_fft_plane<NT, C, 0>), grid, dim3(NT), lds, st, data, g, plan, sign);
+    hipLaunchKernelGGL((k_fft_plane<NT, C, 0, PACK>), grid, dim3(NT), lds, st, data, g, plan, sign, src);
   else if (mi == 1)
-    hipLaunchKernelGGL((k_fft_plane<NT, C, 1>), grid, dim3(NT), lds, st, data, g, plan, sign);
+    hipLaunchKernelGGL((k_fft_plane<NT, C, 1, PACK>), grid, dim3(NT), lds, st, data, g, plan, sign, src);
   else
-    hipLaunchKernelGGL((k_fft_plane<NT, C, 2>), grid, dim3(NT), lds, st, data, g, plan, sign);
+    hipLaunchKernelGGL((k_fft_plane<NT, C, 2, PACK>), grid, dim3(NT), lds, st, data, g, plan, sign, src);
 }
 
 // Whether the in-place passes are wanted by default: in FP64 only, and only where
@@ -599,11 +632,14 @@ static int inplace_items(const FftPlan& plan, int g, int lpb, int nt, size_t csi
   return 0;
 }
 
+// pack (non-null): the batch's grids are formed from pack's pair source on the plane
+// pass (which must then be taken: HQC_E_SIZE where the plane does not fit LDS)
 template <typename C>
-static int fft3d(C* data, int B, int g, int sign, void* stream) {
+static int fft3d(C* data, int B, int g, int sign, void* stream, const PairSrc* pack = nullptr) {
   if (B < 0 || g <= 0 || (sign != 1 && sign != -1)) return HQC_E_ARG;
   if (g > FFT_MAX_G) return HQC_E_SIZE;
-  if (B == 0 || g == 1) return HQC_OK;
+  if (B == 0) return HQC_OK;
+  if (g == 1 && !pack) return HQC_OK;
   FftPlan plan;
   if (make_plan(g, plan) != HQC_OK) return HQC_E_SIZE;
   hipStream_t st = (hipStream_t)stream;
@@ -632,23 +668,38 @@ static int fft3d(C* data, int B, int g, int sign, void* stream) {
   const int mi = inplace_items(plan, g, g, nt, sizeof(C));
   const size_t plane_lds = (size_t)((mi ? 1 : 2) * g * g + g) * sizeof(C);
   int first_axis = 2;
+  if (pack && (plane_lds > FFT_PLANE_MAX_LDS || B > 65535)) return HQC_E_SIZE;
   if (plane_lds <= FFT_PLANE_MAX_LDS) {
-    const void* kf = nt == 256 ? plane_kernel<256, C>(mi)
-                               : (nt == 512 ? plane_kernel<512, C>(mi) : plane_kernel<1024, C>(mi));
+    const PairSrc src = pack ? *pack : PairSrc{nullptr, nullptr, nullptr, 0, 0, 0, 0};
+    const void* kf =
+        pack ? (nt == 256 ? plane_kernel<256, C, true>(mi)
+                          : (nt == 512 ? plane_kernel<512, C, true>(mi) : plane_kernel<1024, C, true>(mi)))
+             : (nt == 256 ? plane_kernel<256, C, false>(mi)
+                          : (nt == 512 ? plane_kernel<512, C, false>(mi) : plane_kernel<1024, C, false>(mi)));
     if (plane_lds > 64 * 1024 &&
         hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds) !=
             hipSuccess)
       (void)hipGetLastError();
     const dim3 grid(g, B);
-    if (nt == 256)
-      launch_plane<256>(mi, grid, plane_lds, st, data, g, plan, sign);
-    else if (nt == 512)
-      launch_plane<512>(mi, grid, plane_lds, st, data, g, plan, sign);
-    else
-      launch_plane<1024>(mi, grid, plane_lds, st, data, g, plan, sign);
+    if (pack) {
+      if (nt == 256)
+        launch_plane<256, C, true>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+      else if (nt == 512)
+        launch_plane<512, C, true>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+      else
+        launch_plane<1024, C, true>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+    } else {
+      if (nt == 256)
+        launch_plane<256, C, false>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+      else if (nt == 512)
+        launch_plane<512, C, false>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+      else
+        launch_plane<1024, C, false>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+    }
     HQC_CHECK_LAUNCH();
     first_axis = 0;
   }
+  if (g == 1) return HQC_OK;
   // remaining axes line by line; more lines per block on the strided x axis
   for (int axis = first_axis; axis >= 0; --axis) {
     // lines per block: 32 on the strided x axis (longer coalesced segments), 16
@@ -678,4 +729,29 @@ extern "C" int hqc_fft3d_c2c_f64(double* data, int B, int g, int sign, void* str
 
 extern "C" int hqc_fft3d_c2c_f32(float* data, int B, int g, int sign, void* stream) {
   return fft3d<float2>(reinterpret_cast<float2*>(data), B, g, sign, stream);
+}
+
+template <typename C>
+static int pair_pack_fft(const double* phi, const double* psi, const int* table, int mp, int nao,
+                         int nao_pad, int nmo, int g, int ld, C* z, int sign, void* stream) {
+  if (mp < 0 || nao <= 0 || nao > nao_pad || nmo <= 0 || g <= 0) return HQC_E_ARG;
+  if ((long)ld < (long)g * g * g || (long)mp * nao > 65535) return HQC_E_SIZE;
+  if (mp == 0) return HQC_OK;
+  if (!phi || !psi || !table || !z) return HQC_E_ARG;
+  const PairSrc src{phi, psi, table, nao, nao_pad, nmo, (long)ld};
+  return fft3d<C>(z, mp * nao, g, sign, stream, &src);
+}
+
+extern "C" int hqc_pair_pack_fft(const double* phi, const double* psi, const int* table, int mp,
+                                 int nao, int nao_pad, int nmo, int g, int ld, double* z, int sign,
+                                 void* stream) {
+  return pair_pack_fft<double2>(phi, psi, table, mp, nao, nao_pad, nmo, g, ld,
+                                reinterpret_cast<double2*>(z), sign, stream);
+}
+
+extern "C" int hqc_pair_pack_fft_c64(const double* phi, const double* psi, const int* table,
+                                     int mp, int nao, int nao_pad, int nmo, int g, int ld, float* z,
+                                     int sign, void* stream) {
+  return pair_pack_fft<float2>(phi, psi, table, mp, nao, nao_pad, nmo, g, ld,
+                               reinterpret_cast<float2*>(z), sign, stream);
 }

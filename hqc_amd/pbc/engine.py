@@ -626,7 +626,8 @@ class Engine:
 
         With A_i = sqrt(f_i) dV FFT(phi * psi_i), K = sum_i sum_G VG Re[A_i A_i^H]
         over G != 0, plus the G=0 term c0 S~ D S~ with the literal 0.22578495.
-        Two occupied orbitals share one complex FFT (hqc_pair_pack); the split kernel
+        Two occupied orbitals share one complex FFT (hqc_pair_pack_fft forms them on the
+        transform's first pass); the split kernel
         returns the [Re | Im] blocks on half the G-set, and the sum over orbitals of
         M diag(2 VG) M^T is the AO x grid MFMA contraction, batched over all pairs
         of all geometries.  low (host list of bools over [0, na), or None): those
@@ -665,8 +666,12 @@ class Engine:
                     mp = t.shape[0]
                     z = torch.empty(mp, c.nao, c.N, device=dev,
                                     dtype=torch.complex64 if f32 else torch.complex128)
-                    K.pair_pack(phi, psi, t, c.nao, c.g, z)
-                    K.fft3d(z.view(mp * c.nao, c.g, c.g, c.g), -1)
+                    if c.g <= K.PAIR_FFT_MAX_G and mp * c.nao <= K.PAIR_FFT_MAX_GRIDS:
+                        # the pair densities formed on the transform's first pass
+                        K.pair_pack(phi, psi, t, c.nao, c.g, z, fft_sign=-1)
+                    else:
+                        K.pair_pack(phi, psi, t, c.nao, c.g, z)
+                        K.fft3d(z.view(mp * c.nao, c.g, c.g, c.g), -1)
                     M = torch.empty(2 * mp, c.nao_pad, c.ld, device=dev,
                                     dtype=torch.float32 if f32 else torch.float64)
                     K.pair_split(z, c.nao_pad, c.ld, M)

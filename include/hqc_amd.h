@@ -244,6 +244,18 @@ int hqc_pair_pack(const double* phi, const double* psi, const int* table, int mp
 /* _c64: z as complex64 (each FP64 product rounded once), for hqc_fft3d_c2c_f32. */
 int hqc_pair_pack_c64(const double* phi, const double* psi, const int* table, int mp, int nao,
                       int nao_pad, int nmo, int g, int ld, float* z, void* stream);
+/* hqc_pair_pack_fft: hqc_pair_pack followed by the 3-D transform of sign `sign`
+ * (hqc_fft3d_c2c_f64) in one pass less: the pair densities are formed while the
+ * transform's y-z planes are loaded, so z is written once, by the transform (the
+ * same products, the same bits).  HQC_E_SIZE where the y-z plane of the grid does
+ * not fit the LDS (g > 98) or mp * nao > 65535: use the two calls there.
+ * _c64: the same with hqc_pair_pack_c64 and hqc_fft3d_c2c_f32.  (The same
+ * replacement as hqc_pair_pack: hqc/pbc/solver.py:412-431.) */
+int hqc_pair_pack_fft(const double* phi, const double* psi, const int* table, int mp, int nao,
+                      int nao_pad, int nmo, int g, int ld, double* z, int sign, void* stream);
+int hqc_pair_pack_fft_c64(const double* phi, const double* psi, const int* table, int mp,
+                          int nao, int nao_pad, int nmo, int g, int ld, float* z, int sign,
+                          void* stream);
 
 /* Split each transformed pair grid Z = FFT(a + i b) into the real row blocks
  * M[2m] = [Re A_a(H) | Im A_a(H)] and M[2m+1] = [Re A_b(H) | Im A_b(H)] over the

@@ -858,3 +858,41 @@ def test_pair_split(K, g, dt):
     torch.cuda.synchronize()
     assert not torch.isnan(M).any()
     assert torch.equal(M, ref.to(rt))
+
+
+@pytest.mark.parametrize("g", [1, 3, 13, 25, 49, 61])
+@pytest.mark.parametrize("dt", [torch.complex128, torch.complex64])
+def test_pair_pack_fft_equals_pack_then_fft(K, g, dt):
+    """The pair densities formed on the transform's first pass (hqc_pair_pack_fft)
+    give the grids of hqc_pair_pack followed by the 3-D FFT bit for bit, for rows of
+    two geometries, lone orbitals (j = -1) and both transform signs; the product with
+    the pair table is also checked against torch's FFT of the pair densities."""
+    print(f"[pair pack fft] g={g} {dt}", flush=True)
+    torch.manual_seed(100 + g)
+    B, nao, nao_pad, nmo = 2, 5, 16, 6
+    N = g ** 3
+    ld = (N + 63) // 64 * 64
+    phi = torch.zeros(B, nao_pad, ld, dtype=torch.float64, device=DEV)
+    phi[:, :nao, :N] = torch.randn(B, nao, N, dtype=torch.float64, device=DEV)
+    psi = torch.zeros(B, nmo, ld, dtype=torch.float64, device=DEV)
+    psi[:, :, :N] = torch.randn(B, nmo, N, dtype=torch.float64, device=DEV)
+    table = torch.tensor([[0, 0, 1], [1, 2, 5], [0, 3, -1], [1, 4, 0]], dtype=torch.int32)
+    mp = table.shape[0]
+    for sign in (-1, 1):
+        z1 = torch.empty(mp, nao, N, dtype=dt, device=DEV)
+        K.pair_pack(phi, psi, table, nao, g, z1)
+        K.fft3d(z1.view(mp * nao, g, g, g), sign)
+        z2 = torch.full((mp, nao, N), float("nan"), dtype=dt, device=DEV)
+        K.pair_pack(phi, psi, table, nao, g, z2, fft_sign=sign)
+        torch.cuda.synchronize()
+        assert torch.equal(z1, z2)
+    # the forward transform of phi_p (psi_i + i psi_j) by torch (rocFFT)
+    ref = torch.empty(mp, nao, N, dtype=torch.complex128, device=DEV)
+    for m, (b, i, j) in enumerate(table.tolist()):
+        pj = psi[b, j, :N] if j >= 0 else torch.zeros_like(psi[b, i, :N])
+        dens = phi[b, :nao, :N] * torch.complex(psi[b, i, :N], pj)
+        ref[m] = torch.fft.fftn(dens.view(nao, g, g, g), dim=(1, 2, 3)).reshape(nao, N)
+    z3 = torch.empty(mp, nao, N, dtype=dt, device=DEV)
+    K.pair_pack(phi, psi, table, nao, g, z3, fft_sign=-1)
+    tol = (1e-12 if dt == torch.complex128 else 2e-5) * max(g, 2) ** 1.5
+    assert torch.allclose(z3.to(torch.complex128), ref, rtol=0, atol=tol * ref.abs().max().item())
