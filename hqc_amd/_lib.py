@@ -48,8 +48,8 @@ _SIGS = {
     "hqc_pair_pack_c64": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_split_c64": [_P, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_pack": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
-    "hqc_pair_pack_fft": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _I, _P],
-    "hqc_pair_pack_fft_c64": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _I, _P],
+    "hqc_pair_pack_fft": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _I, _I, _I, _P],
+    "hqc_pair_pack_fft_c64": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _I, _I, _I, _P],
     "hqc_pair_split_f64": [_P, _I, _I, _I, _I, _I, _P, _P],
     "hqc_pair_split_f32": [_P, _I, _I, _I, _I, _I, _P, _P],
     "hqc_vloc_G": [_P, _I, _I, _I, _D, _P, _P, _P],
@@ -369,13 +369,18 @@ def pair_pack(phi, psi, table, nao, g, z, fft_sign=None):
     """z (mp, nao, g^3) complex128 <- phi[b,p] (psi[b,i] + i psi[b,j]) per row (b, i, j)
     of the host int32 table (validated here, then copied to the device).  fft_sign
     (+1 or -1): z is also 3-D transformed with that sign, the pair densities formed
-    on the transform's first pass (hqc_pair_pack_fft)."""
+    on the transform's first pass (hqc_pair_pack_fft); z may then be any
+    (mp, rows >= nao, stride >= g^3) complex tensor, grid (m, p) in z[m, p, :g^3]."""
     B, nao_pad, ld = phi.shape
     nmo = psi.shape[1]
     mp = table.shape[0]
     assert not table.is_cuda and table.dtype == torch.int32 and table.shape == (mp, 3)
     assert psi.shape == (B, nmo, ld)
-    assert z.dtype in (torch.complex128, torch.complex64) and z.shape == (mp, nao, g ** 3)
+    assert z.dtype in (torch.complex128, torch.complex64) and z.is_contiguous()
+    if fft_sign is None:
+        assert z.shape == (mp, nao, g ** 3)
+    else:
+        assert z.dim() == 3 and z.shape[0] == mp and z.shape[1] >= nao and z.shape[2] >= g ** 3
     if mp == 0:
         return
     if not (0 <= int(table[:, 0].min()) and int(table[:, 0].max()) < B
@@ -392,7 +397,7 @@ def pair_pack(phi, psi, table, nao, g, z, fft_sign=None):
         assert fft_sign in (1, -1)
         _call(name.replace("_pack", "_pack_fft"), _ptr(phi, F64, "phi"), _ptr(psi, F64, "psi"),
               _ptr(td, torch.int32, "table"), mp, nao, nao_pad, nmo, g, ld, _ptr(z, None, "z"),
-              int(fft_sign), _stream())
+              z.shape[1], z.shape[2], int(fft_sign), _stream())
 
 
 def pair_split(z, nao_pad, ld, M):

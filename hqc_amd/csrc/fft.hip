@@ -38,6 +38,21 @@ struct FftPlan {
   int rad[FFT_MAX_STAGES];
 };
 
+// Where grid b of a batch starts: rows of `rows` grids each begin a group of
+// `rows_pad` rows, `stride` elements apart, so the exchange's transformed pair
+// grids can land in the padded row layout its contraction reads.  {1, 1, g^3}:
+// grids back to back.
+struct GridMap {
+  int rows, rows_pad;
+  long stride;
+};
+
+__device__ __forceinline__ long grid_off(const GridMap& m, int b) {
+  const int grp = b / m.rows;
+  return ((long)grp * m.rows_pad + (b - grp * m.rows)) * m.stride;
+}
+
+
 __device__ __forceinline__ double sfma(double a, double b, double c) { return fma(a, b, c); }
 __device__ __forceinline__ float sfma(float a, float b, float c) { return fmaf(a, b, c); }
 
@@ -453,13 +468,14 @@ static bool fft_inplace_fits(const FftPlan& plan, int N, int lpb, int nt, int MI
 
 template <typename C, int MI>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(C* __restrict__ data, int g, int axis,
-                                                          int lpb, FftPlan plan, int sign) {
+                                                          int lpb, FftPlan plan, int sign,
+                                                          GridMap map) {
   extern __shared__ unsigned char sm_raw[];
   C* bufA = reinterpret_cast<C*>(sm_raw);
   C* bufB = bufA + g * lpb;
   C* tw = MI ? bufB : bufB + g * lpb;
   const long gg = (long)g * g;
-  C* base = data + (long)blockIdx.y * gg * g;
+  C* base = data + grid_off(map, blockIdx.y);
   const int line0 = blockIdx.x * lpb;
   for (int m = threadIdx.x; m < g; m += blockDim.x) tw[m] = twiddle<C>(m, g, sign);
   const long stride = axis == 2 ? 1 : (axis == 1 ? g : gg);
@@ -533,13 +549,13 @@ __device__ __forceinline__ float2 pair_c(double a, double b, float2*) {
 
 template <int NT, typename C, int MI, bool PACK>
 __global__ __launch_bounds__(NT) void k_fft_plane(C* __restrict__ data, int g, FftPlan plan,
-                                                  int sign, PairSrc src) {
+                                                  int sign, PairSrc src, GridMap map) {
   extern __shared__ unsigned char sm_raw[];
   const int gg = g * g;
   C* bufA = reinterpret_cast<C*>(sm_raw);
   C* bufB = bufA + gg;
   C* tw = MI ? bufB : bufB + gg;
-  C* plane = data + ((long)blockIdx.y * g + blockIdx.x) * gg;
+  C* plane = data + grid_off(map, blockIdx.y) + (long)blockIdx.x * gg;
   for (int m = threadIdx.x; m < g; m += blockDim.x) tw[m] = twiddle<C>(m, g, sign);
   if (PACK) {
     const int m = blockIdx.y / src.nao, p = blockIdx.y - m * src.nao;
@@ -599,13 +615,16 @@ static const void* plane_kernel(int mi) {
 
 template <int NT, typename C, bool PACK>
 static void launch_plane(int mi, dim3 grid, size_t lds, hipStream_t st, C* data, int g,
-                         const FftPlan& plan, int sign, const PairSrc& src) {
+                         const FftPlan& plan, int sign, const PairSrc& src, const GridMap& map) {
   if (mi == 0)
-    hipLaunchKernelGGL((k_fft_plane<NT, C, 0, PACK>), grid, dim3(NT), lds, st, data, g, plan, sign, src);
+    hipLaunchKernelGGL((k_fft_plane<NT, C, 0, PACK>), grid, dim3(NT), lds, st, data, g, plan, sign,
+                       src, map);
   else if (mi == 1)
-    hipLaunchKernelGGL((k_fft_plane<NT, C, 1, PACK>), grid, dim3(NT), lds, st, data, g, plan, sign, src);
+    hipLaunchKernelGGL((k_fft_plane<NT, C, 1, PACK>), grid, dim3(NT), lds, st, data, g, plan, sign,
+                       src, map);
   else
-    hipLaunchKernelGGL((k_fft_plane<NT, C, 2, PACK>), grid, dim3(NT), lds, st, data, g, plan, sign, src);
+    hipLaunchKernelGGL((k_fft_plane<NT, C, 2, PACK>), grid, dim3(NT), lds, st, data, g, plan, sign,
+                       src, map);
 }
 
 // Whether the in-place passes are wanted by default: in FP64 only, and only where
@@ -635,7 +654,9 @@ static int inplace_items(const FftPlan& plan, int g, int lpb, int nt, size_t csi
 // pack (non-null): the batch's grids are formed from pack's pair source on the plane
 // pass (which must then be taken: HQC_E_SIZE where the plane does not fit LDS)
 template <typename C>
-static int fft3d(C* data, int B, int g, int sign, void* stream, const PairSrc* pack = nullptr) {
+static int fft3d(C* data, int B, int g, int sign, void* stream, const PairSrc* pack = nullptr,
+                 const GridMap* gmap = nullptr) {
+  const GridMap map = gmap ? *gmap : GridMap{1, 1, (long)g * g * g};
   if (B < 0 || g <= 0 || (sign != 1 && sign != -1)) return HQC_E_ARG;
   if (g > FFT_MAX_G) return HQC_E_SIZE;
   if (B == 0) return HQC_OK;
@@ -683,18 +704,18 @@ static int fft3d(C* data, int B, int g, int sign, void* stream, const PairSrc* p
     const dim3 grid(g, B);
     if (pack) {
       if (nt == 256)
-        launch_plane<256, C, true>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+        launch_plane<256, C, true>(mi, grid, plane_lds, st, data, g, plan, sign, src, map);
       else if (nt == 512)
-        launch_plane<512, C, true>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+        launch_plane<512, C, true>(mi, grid, plane_lds, st, data, g, plan, sign, src, map);
       else
-        launch_plane<1024, C, true>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+        launch_plane<1024, C, true>(mi, grid, plane_lds, st, data, g, plan, sign, src, map);
     } else {
       if (nt == 256)
-        launch_plane<256, C, false>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+        launch_plane<256, C, false>(mi, grid, plane_lds, st, data, g, plan, sign, src, map);
       else if (nt == 512)
-        launch_plane<512, C, false>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+        launch_plane<512, C, false>(mi, grid, plane_lds, st, data, g, plan, sign, src, map);
       else
-        launch_plane<1024, C, false>(mi, grid, plane_lds, st, data, g, plan, sign, src);
+        launch_plane<1024, C, false>(mi, grid, plane_lds, st, data, g, plan, sign, src, map);
     }
     HQC_CHECK_LAUNCH();
     first_axis = 0;
@@ -711,13 +732,13 @@ static int fft3d(C* data, int B, int g, int sign, void* stream, const PairSrc* p
     const dim3 grid(hqc_div_up(lines, lpb), B);
     if (ami == 0)
       hipLaunchKernelGGL((k_fft_axis<C, 0>), grid, dim3(FFT_THREADS), lds, st, data, g, axis,
-                         lpb, plan, sign);
+                         lpb, plan, sign, map);
     else if (ami == 1)
       hipLaunchKernelGGL((k_fft_axis<C, 1>), grid, dim3(FFT_THREADS), lds, st, data, g, axis,
-                         lpb, plan, sign);
+                         lpb, plan, sign, map);
     else
       hipLaunchKernelGGL((k_fft_axis<C, 2>), grid, dim3(FFT_THREADS), lds, st, data, g, axis,
-                         lpb, plan, sign);
+                         lpb, plan, sign, map);
     HQC_CHECK_LAUNCH();
   }
   return HQC_OK;
@@ -733,25 +754,28 @@ extern "C" int hqc_fft3d_c2c_f32(float* data, int B, int g, int sign, void* stre
 
 template <typename C>
 static int pair_pack_fft(const double* phi, const double* psi, const int* table, int mp, int nao,
-                         int nao_pad, int nmo, int g, int ld, C* z, int sign, void* stream) {
-  if (mp < 0 || nao <= 0 || nao > nao_pad || nmo <= 0 || g <= 0) return HQC_E_ARG;
-  if ((long)ld < (long)g * g * g || (long)mp * nao > 65535) return HQC_E_SIZE;
+                         int nao_pad, int nmo, int g, int ld, C* z, int z_rows, int z_stride,
+                         int sign, void* stream) {
+  if (mp < 0 || nao <= 0 || nao > nao_pad || nmo <= 0 || g <= 0 || z_rows < nao) return HQC_E_ARG;
+  const long N = (long)g * g * g;
+  if ((long)ld < N || (long)z_stride < N || (long)mp * nao > 65535) return HQC_E_SIZE;
   if (mp == 0) return HQC_OK;
   if (!phi || !psi || !table || !z) return HQC_E_ARG;
   const PairSrc src{phi, psi, table, nao, nao_pad, nmo, (long)ld};
-  return fft3d<C>(z, mp * nao, g, sign, stream, &src);
+  const GridMap map{nao, z_rows, (long)z_stride};
+  return fft3d<C>(z, mp * nao, g, sign, stream, &src, &map);
 }
 
 extern "C" int hqc_pair_pack_fft(const double* phi, const double* psi, const int* table, int mp,
-                                 int nao, int nao_pad, int nmo, int g, int ld, double* z, int sign,
-                                 void* stream) {
+                                 int nao, int nao_pad, int nmo, int g, int ld, double* z, int z_rows,
+                                 int z_stride, int sign, void* stream) {
   return pair_pack_fft<double2>(phi, psi, table, mp, nao, nao_pad, nmo, g, ld,
-                                reinterpret_cast<double2*>(z), sign, stream);
+                                reinterpret_cast<double2*>(z), z_rows, z_stride, sign, stream);
 }
 
 extern "C" int hqc_pair_pack_fft_c64(const double* phi, const double* psi, const int* table,
                                      int mp, int nao, int nao_pad, int nmo, int g, int ld, float* z,
-                                     int sign, void* stream) {
+                                     int z_rows, int z_stride, int sign, void* stream) {
   return pair_pack_fft<float2>(phi, psi, table, mp, nao, nao_pad, nmo, g, ld,
-                               reinterpret_cast<float2*>(z), sign, stream);
+                               reinterpret_cast<float2*>(z), z_rows, z_stride, sign, stream);
 }

@@ -73,6 +73,13 @@ class CellConsts:
         wh[:nh] = 2 * VG.ravel()[kk]
         wh[nh:2 * nh] = wh[:nh]
         self.VGh = torch.tensor(wh, dtype=f64, device=device)
+        # the same weights for the transformed pair grids read whole as real rows
+        # [Re Z, Im Z interleaved] (hqc_pair_pack_fft): VG on every G (0 at G = 0)
+        self.ldz = _round_up(2 * N, 64)
+        wz = np.zeros(self.ldz)
+        wz[0:2 * N:2] = VG.ravel()
+        wz[1:2 * N:2] = VG.ravel()
+        self.VGz = torch.tensor(wz, dtype=f64, device=device)
         self.c0 = 4 * math.pi / L * EXCHANGE_G0_CONST   # eris0 factor (solver.py:431)
 
 
@@ -248,10 +255,12 @@ class Engine:
         self._work = {}
 
     # ------------------------------------------------------------------ utils
-    def _workspace(self, B, prec):
-        # one grow-only buffer per precision; calls of any batch size share it
-        nbytes = (K.vmat_h2_workspace(B, self.c.nao_pad, self.c.ld) if prec == 16 else
-                  K.vmat_workspace(B, self.c.nao_pad, self.c.ld, prec))
+    def _workspace(self, B, prec, ld=None):
+        # one grow-only buffer per precision; calls of any batch size share it (ld: the
+        # operand's row length, the grid's by default)
+        ld = self.c.ld if ld is None else ld
+        nbytes = (K.vmat_h2_workspace(B, self.c.nao_pad, ld) if prec == 16 else
+                  K.vmat_workspace(B, self.c.nao_pad, ld, prec))
         buf = self._work.get(prec)
         if buf is None or buf.numel() * 8 < nbytes:
             buf = torch.empty(max((nbytes + 7) // 8, 1), dtype=torch.float64,
@@ -297,7 +306,7 @@ class Engine:
                          bidx, None if vmax is None else vmax.contiguous())
         else:
             prec = 64 if phi.dtype == torch.float64 else 32
-            K.ao_vmat(phi, v.contiguous(), dV, V, self._workspace(B, prec), bidx)
+            K.ao_vmat(phi, v.contiguous(), dV, V, self._workspace(B, prec, phi.shape[-1]), bidx)
         return V[:, :c.nao, :c.nao]
 
     def density(self, phi, D, X=None):
@@ -580,11 +589,17 @@ class Engine:
             st["Sgrid"] = self.vmat(phi, ones, dVs)      # dV phi phi^T (exchange G=0 term)
             st["VGh"] = torch.empty(B, c.ld, dtype=f64, device=dev)
             st["c0b"] = torch.empty(B, dtype=f64, device=dev)
+            fused = c.g <= K.PAIR_FFT_MAX_G
+            if fused:
+                ldz = groups[0][0].ldz
+                st["VGz"] = torch.empty(B, ldz, dtype=f64, device=dev)
             for cst, idx in groups:
                 nb = B if idx is None else len(idx)
                 self._put(st["VGh"], idx, cst.VGh.expand(nb, c.ld))
+                if fused:
+                    self._put(st["VGz"], idx, cst.VGz.expand(nb, ldz))
                 self._put(st["c0b"], idx, torch.full((nb,), cst.c0, dtype=f64, device=dev))
-            per += ["Sgrid", "VGh", "c0b"]
+            per += ["Sgrid", "VGh", "c0b"] + (["VGz"] if fused else [])
         if not inplace:
             per += phi_keys
         st["per"] = per
@@ -626,11 +641,15 @@ class Engine:
 
         With A_i = sqrt(f_i) dV FFT(phi * psi_i), K = sum_i sum_G VG Re[A_i A_i^H]
         over G != 0, plus the G=0 term c0 S~ D S~ with the literal 0.22578495.
-        Two occupied orbitals share one complex FFT (hqc_pair_pack_fft forms them on the
-        transform's first pass); the split kernel
-        returns the [Re | Im] blocks on half the G-set, and the sum over orbitals of
-        M diag(2 VG) M^T is the AO x grid MFMA contraction, batched over all pairs
-        of all geometries.  low (host list of bools over [0, na), or None): those
+        Two occupied orbitals a, b share one complex FFT Z = A_a + i A_b.  On grids whose
+        y-z plane fits the LDS (the solver's) the pair densities are formed on the
+        transform's first pass (hqc_pair_pack_fft) into real rows [Re Z | Im Z], and
+        K = sum over pairs of Z diag(VG) Z^T on every G: the terms odd in G cancel over
+        the symmetric G-set, leaving VG (Re[A_a A_a^H] + Re[A_b A_b^H]).  Elsewhere
+        pack, transform and the split kernel give the [Re | Im] blocks of A_a and A_b on
+        half the G-set, contracted with 2 VG.  Either contraction is the AO x grid MFMA
+        kernel, batched over all pairs of all geometries.  low (host list of bools
+        over [0, na), or None): those
         geometries take the pair densities, their transforms and the contraction in
         FP32 (complex64 grids, FP32 MFMA), the others in FP64."""
         c = self.c
@@ -659,30 +678,49 @@ class Engine:
                     continue
                 table = torch.tensor(sel, dtype=torch.int32)
                 zb, mb = (8, 4) if f32 else (16, 8)
-                per_pair = c.nao * c.N * zb + 2 * c.nao_pad * (c.ld + c.nao_pad + 1) * mb
-                chunk = max(1, min(len(sel), int(self.EXCHANGE_CHUNK_BYTES // per_pair)))
-                for s0 in range(0, len(sel), chunk):
-                    t = table[s0:s0 + chunk]
-                    mp = t.shape[0]
-                    z = torch.empty(mp, c.nao, c.N, device=dev,
-                                    dtype=torch.complex64 if f32 else torch.complex128)
-                    if c.g <= K.PAIR_FFT_MAX_G and mp * c.nao <= K.PAIR_FFT_MAX_GRIDS:
-                        # the pair densities formed on the transform's first pass
+                cplx = torch.complex64 if f32 else torch.complex128
+                if "VGz" in st:
+                    # pair densities formed on the transform's first pass, written as
+                    # real rows [Re Z | Im Z] of the padded operand layout: K = sum over
+                    # pairs of Z diag(VG) Z^T on every G (no split; see hqc_pair_pack_fft)
+                    ldz = st["VGz"].shape[1]
+                    per_pair = c.nao_pad * ldz * zb // 2 + c.nao_pad * (c.nao_pad + 1) * mb
+                    chunk = max(1, min(len(sel), int(self.EXCHANGE_CHUNK_BYTES // per_pair),
+                                       K.PAIR_FFT_MAX_GRIDS // c.nao))
+                    zbuf = torch.zeros(chunk, c.nao_pad, ldz // 2, device=dev, dtype=cplx)
+                    for s0 in range(0, len(sel), chunk):
+                        t = table[s0:s0 + chunk]
+                        mp = t.shape[0]
+                        z = zbuf[:mp]                    # padding rows and columns stay zero
                         K.pair_pack(phi, psi, t, c.nao, c.g, z, fft_sign=-1)
-                    else:
+                        zr = torch.view_as_real(z).view(mp, c.nao_pad, ldz)
+                        part = self.vmat_dv(zr, st["VGz"][t[:, 0].long().to(dev)], 1.0)
+                        geo, cnt = torch.unique_consecutive(t[:, 0], return_counts=True)
+                        sums = torch.stack([q.sum(0) for q in torch.split(part, cnt.tolist())])
+                        Kmat[geo.long().to(dev)] += sums
+                    del zbuf
+                else:
+                    per_pair = c.nao * c.N * zb + 2 * c.nao_pad * (c.ld + c.nao_pad + 1) * mb
+                    chunk = max(1, min(len(sel), int(self.EXCHANGE_CHUNK_BYTES // per_pair)))
+                    for s0 in range(0, len(sel), chunk):
+                        t = table[s0:s0 + chunk]
+                        mp = t.shape[0]
+                        z = torch.empty(mp, c.nao, c.N, device=dev, dtype=cplx)
                         K.pair_pack(phi, psi, t, c.nao, c.g, z)
                         K.fft3d(z.view(mp * c.nao, c.g, c.g, c.g), -1)
-                    M = torch.empty(2 * mp, c.nao_pad, c.ld, device=dev,
-                                    dtype=torch.float32 if f32 else torch.float64)
-                    K.pair_split(z, c.nao_pad, c.ld, M)
-                    del z
-                    gid = t[:, 0].long().repeat_interleave(2).to(dev)
-                    part = self.vmat_dv(M, VGh[gid], 1.0)
-                    # rows of one geometry are contiguous: a fixed-order sum per geometry
-                    # (index_add_ would accumulate with atomics, in no fixed order)
-                    geo, cnt = torch.unique_consecutive(t[:, 0], return_counts=True)
-                    sums = torch.stack([q.sum(0) for q in torch.split(part, (2 * cnt).tolist())])
-                    Kmat[geo.long().to(dev)] += sums
+                        M = torch.empty(2 * mp, c.nao_pad, c.ld, device=dev,
+                                        dtype=torch.float32 if f32 else torch.float64)
+                        K.pair_split(z, c.nao_pad, c.ld, M)
+                        del z
+                        gid = t[:, 0].long().repeat_interleave(2).to(dev)
+                        part = self.vmat_dv(M, VGh[gid], 1.0)
+                        # rows of one geometry are contiguous: a fixed-order sum per
+                        # geometry (index_add_ would accumulate with atomics, in no
+                        # fixed order)
+                        geo, cnt = torch.unique_consecutive(t[:, 0], return_counts=True)
+                        sums = torch.stack([q.sum(0) for q in
+                                            torch.split(part, (2 * cnt).tolist())])
+                        Kmat[geo.long().to(dev)] += sums
                 self.stats["x32" if f32 else "x64"] += 1
         D = (mo * occ.unsqueeze(-2)) @ mo.transpose(-1, -2).contiguous()
         Sg = st["Sgrid"][:na]

@@ -247,15 +247,22 @@ int hqc_pair_pack_c64(const double* phi, const double* psi, const int* table, in
 /* hqc_pair_pack_fft: hqc_pair_pack followed by the 3-D transform of sign `sign`
  * (hqc_fft3d_c2c_f64) in one pass less: the pair densities are formed while the
  * transform's y-z planes are loaded, so z is written once, by the transform (the
- * same products, the same bits).  HQC_E_SIZE where the y-z plane of the grid does
- * not fit the LDS (g > 98) or mp * nao > 65535: use the two calls there.
- * _c64: the same with hqc_pair_pack_c64 and hqc_fft3d_c2c_f32.  (The same
- * replacement as hqc_pair_pack: hqc/pbc/solver.py:412-431.) */
+ * same products, the same bits).  Grid (m, p) lands at z + (m z_rows + p) z_stride
+ * (complex elements; z_rows >= nao, z_stride >= g^3; other elements untouched):
+ * z_rows = nao, z_stride = g^3 is hqc_pair_pack's layout, and z_rows = nao_pad with
+ * zeroed padding makes z, read as real rows [Re Z | Im Z ...] of 2 z_stride values,
+ * an operand of hqc_ao_vmat_* (then K_pq = sum_G VG Re[Z_p(G) conj Z_q(G)] over all
+ * G, which equals the two orbitals' sum_G VG (Re[A_a A_a*] + Re[A_b A_b*]): the
+ * terms odd in G cancel over the symmetric G-set, so no split is needed).
+ * HQC_E_SIZE where the y-z plane does not fit the LDS (g > 98) or mp * nao > 65535:
+ * use hqc_pair_pack and the transform there.  _c64: the same with hqc_pair_pack_c64
+ * and hqc_fft3d_c2c_f32.  (The replacement of hqc_pair_pack, hqc/pbc/solver.py:412-431.) */
 int hqc_pair_pack_fft(const double* phi, const double* psi, const int* table, int mp, int nao,
-                      int nao_pad, int nmo, int g, int ld, double* z, int sign, void* stream);
+                      int nao_pad, int nmo, int g, int ld, double* z, int z_rows, int z_stride,
+                      int sign, void* stream);
 int hqc_pair_pack_fft_c64(const double* phi, const double* psi, const int* table, int mp,
-                          int nao, int nao_pad, int nmo, int g, int ld, float* z, int sign,
-                          void* stream);
+                          int nao, int nao_pad, int nmo, int g, int ld, float* z, int z_rows,
+                          int z_stride, int sign, void* stream);
 
 /* Split each transformed pair grid Z = FFT(a + i b) into the real row blocks
  * M[2m] = [Re A_a(H) | Im A_a(H)] and M[2m+1] = [Re A_b(H) | Im A_b(H)] over the

@@ -896,3 +896,56 @@ def test_pair_pack_fft_equals_pack_then_fft(K, g, dt):
     K.pair_pack(phi, psi, table, nao, g, z3, fft_sign=-1)
     tol = (1e-12 if dt == torch.complex128 else 2e-5) * max(g, 2) ** 1.5
     assert torch.allclose(z3.to(torch.complex128), ref, rtol=0, atol=tol * ref.abs().max().item())
+
+
+def test_pair_pack_fft_padded_rows(K):
+    """The fused pack + transform into the padded row layout of the exchange
+    contraction: grids at z[m, p, :g^3] equal pack + FFT, the padding rows and
+    columns are left as they were (zero)."""
+    print("[pair pack fft padded]", flush=True)
+    torch.manual_seed(5)
+    g, B, nao, nao_pad, nmo = 9, 2, 5, 16, 4
+    N = g ** 3
+    ld = (N + 63) // 64 * 64
+    ldz = (2 * N + 63) // 64 * 64
+    phi = torch.zeros(B, nao_pad, ld, dtype=torch.float64, device=DEV)
+    phi[:, :nao, :N] = torch.randn(B, nao, N, dtype=torch.float64, device=DEV)
+    psi = torch.zeros(B, nmo, ld, dtype=torch.float64, device=DEV)
+    psi[:, :, :N] = torch.randn(B, nmo, N, dtype=torch.float64, device=DEV)
+    table = torch.tensor([[1, 0, 3], [0, 2, -1], [1, 1, 2]], dtype=torch.int32)
+    mp = table.shape[0]
+    for dt in (torch.complex128, torch.complex64):
+        z1 = torch.empty(mp, nao, N, dtype=dt, device=DEV)
+        K.pair_pack(phi, psi, table, nao, g, z1)
+        K.fft3d(z1.view(mp * nao, g, g, g), -1)
+        zp = torch.zeros(mp, nao_pad, ldz // 2, dtype=dt, device=DEV)
+        K.pair_pack(phi, psi, table, nao, g, zp, fft_sign=-1)
+        torch.cuda.synchronize()
+        assert torch.equal(zp[:, :nao, :N], z1)
+        assert (zp[:, nao:] == 0).all() and (zp[:, :, N:] == 0).all()
+
+
+@pytest.mark.parametrize("n,basis", [(4, "gth-dzv"), (6, "gth-szv")])
+def test_exchange_without_split_equals_split(K, n, basis):
+    """K from the transformed pair grids read whole (weights VG on every G, no split)
+    equals K from the half-spectrum split of the same grids (weights 2 VG on half the
+    G-set): the terms odd in G cancel over the symmetric G-set.  FP64 to 1e-12 of
+    max|K|, and the FP32 (complex64) forms to 2e-6."""
+    from hqc_amd.pbc.engine import Cell, Engine
+    print(f"[exchange without split] {n} H {basis}", flush=True)
+    rs, gl = 1.3, 0.3
+    L = (4 / 3 * math.pi * n) ** (1 / 3)
+    eng = Engine(Cell(n, L, rs, basis, grid_length=gl, device=DEV), dft=False)
+    xp = torch.tensor(np.random.default_rng(n).uniform(0, L, (2, n, 3)) * rs, device=DEV)
+    st = eng.setup(xp)
+    assert "VGz" in st
+    w, mo, _ = eng._eigh_orth(st["Hcore"], st["v"])
+    D, occ = eng.density_matrix(mo, w)
+    st_split = dict(st)
+    del st_split["VGz"]
+    for low in (None, [True, True]):
+        Kf = eng.exchange(st, 2, mo, occ, low)
+        Ks = eng.exchange(st_split, 2, mo, occ, low)
+        torch.cuda.synchronize()
+        tol = 1e-12 if low is None else 2e-6
+        assert (Kf - Ks).abs().max().item() <= tol * Ks.abs().max().item()
