@@ -687,11 +687,15 @@ class Engine:
                     per_pair = c.nao_pad * ldz * zb // 2 + c.nao_pad * (c.nao_pad + 1) * mb
                     chunk = max(1, min(len(sel), int(self.EXCHANGE_CHUNK_BYTES // per_pair),
                                        K.PAIR_FFT_MAX_GRIDS // c.nao))
-                    zbuf = torch.zeros(chunk, c.nao_pad, ldz // 2, device=dev, dtype=cplx)
+                    # the transform writes every grid whole; only the padding rows and
+                    # the columns past 2N are zeroed, once
+                    zbuf = torch.empty(chunk, c.nao_pad, ldz // 2, device=dev, dtype=cplx)
+                    zbuf[:, c.nao:].zero_()
+                    zbuf[:, :c.nao, c.N:].zero_()
                     for s0 in range(0, len(sel), chunk):
                         t = table[s0:s0 + chunk]
                         mp = t.shape[0]
-                        z = zbuf[:mp]                    # padding rows and columns stay zero
+                        z = zbuf[:mp]
                         K.pair_pack(phi, psi, t, c.nao, c.g, z, fft_sign=-1)
                         zr = torch.view_as_real(z).view(mp, c.nao_pad, ldz)
                         part = self.vmat_dv(zr, st["VGz"][t[:, 0].long().to(dev)], 1.0)
