@@ -593,13 +593,16 @@ def main():
                "calls_per_step": n_vmat / nsteps, "traffic": measured_traffic(vname, B),
                "mfma_util_pct": measured_mfma_util(vname, B)})
     if q8:
-        # int8 digits (4 bytes per phi value) streamed once per launch: HBM roof; the
-        # int8 MFMA work of the kept digit pairs is reported beside it
+        # bound by the int8 MFMA pipe (16 busy cycles per 16x16x64 instruction, busy
+        # ~65% of the time at the clock the load allows), not by its stream of int8
+        # digits (4 bytes per phi value), which is reported beside it; units of the
+        # MFMA roof: tera int8 operations per second
         dk = {"kernel": "k_orb_density_q8 (sum_i f_i (C^T phi)_i^2 from int8 digits, "
-                        "exact int32 products)", "bound": "hbm", "unit": "GB/s",
-              "bytes_per_launch": by_d, "peak": MI355X_HBM_GBS,
-              "int8_mfma_tops": ops_d / (ms_d * 1e-3) / 1e12,
-              "int8_mfma_frac": ops_d / (ms_d * 1e-3) / 1e12 / MI355X_PEAK["i8"],
+                        "exact int32 products; int8 MFMA ops)", "bound": "mfma",
+              "unit": "TFLOP/s", "ops_per_launch": ops_d, "peak": MI355X_PEAK["i8"],
+              "bytes_per_launch": by_d,
+              "hbm_gbs": by_d / (ms_d * 1e-3) / 1e9,
+              "hbm_frac": by_d / (ms_d * 1e-3) / 1e9 / MI355X_HBM_GBS,
               "traffic": measured_traffic("k_orb_density_q8", B),
               "mfma_util_pct": measured_mfma_util("k_orb_density_q8", B)}
     else:
@@ -613,7 +616,8 @@ def main():
         if k["unit"] == "GB/s":
             k["achieved"] = k["bytes_per_launch"] / (k["ms_per_launch"] * 1e-3) / 1e9
         else:
-            k["achieved"] = k["flops_per_launch"] / (k["ms_per_launch"] * 1e-3) / 1e12
+            work = k.get("ops_per_launch", k["flops_per_launch"])
+            k["achieved"] = work / (k["ms_per_launch"] * 1e-3) / 1e12
         k["frac"] = k["achieved"] / k["peak"]
     dom = max(cand, key=lambda k: k["ms_per_launch"] * k["calls_per_step"])
     w_main = st["w1"].clone()
@@ -632,8 +636,9 @@ def main():
                    "path": a.path, "batch": B, "n_iter_mean": nit, "all_converged": conv,
                    "contractions": stats, "jacobi_sweeps_per_eigh": sweeps,
                    "diis_loop": tail},
-        # compute-bound roof: MFMA for the potential matrix, FP64 vector FMAs (same
-        # 78.6 TFLOP/s peak) for the orbital-form density
+        # the dominant kernel's own roof: HBM for the split-FP16 V, the int8 MFMA pipe
+        # for the digit density, MFMA / FP64 vector FMAs (same 78.6 TFLOP/s peak) for
+        # the FP64 forms
         "roofline": {"bound": dom["bound"], "kernel": dom["kernel"], "achieved": dom["achieved"],
                      "peak": dom["peak"], "unit": dom["unit"], "frac": dom["frac"],
                      "traffic": dom["traffic"],
@@ -646,7 +651,9 @@ def main():
                      "profiles/" + os.path.basename(MFMA_FILE) + " (rocprofv3 --pmc MfmaUtil "
                      "of the kernel probe; not measured in this run)",
                      "ms_per_launch": dom["ms_per_launch"],
-                     "flops_per_launch": dom["flops_per_launch"]},
+                     "flops_per_launch": dom["flops_per_launch"],
+                     **{k: dom[k] for k in ("ops_per_launch", "bytes_per_launch", "hbm_gbs",
+                                            "hbm_frac") if k in dom}},
         "kernels": cand,
         "tuning": tuned if tuned is not None else "library defaults (no --autotune)",
     }
