@@ -108,7 +108,7 @@ def time_vmat(pes, B, reps=20):
     return ms, flops
 
 
-def time_orb_density(pes, B, q8=False, reps=20):
+def time_orb_density(pes, B, q8=False, reps=20, top=5):
     """Average duration of the accurate orbital-form density kernel (rho = sum_i f_i
     psi_i^2) at the workload shape with the occupations of the last run: FP64 vector
     FMAs, or (q8) int8 digits of phi with exact integer products.  Returns (ms,
@@ -131,7 +131,7 @@ def time_orb_density(pes, B, q8=False, reps=20):
         pe = torch.empty(B, c.ld, dtype=torch.int16, device="cuda")
         K.phi_slice_i8(phi, ns, q, pe)
         del phi
-        run = lambda: K.orb_density_q8(q, pe, At, rho, ns)
+        run = lambda: K.orb_density_q8(q, pe, At, rho, ns, top=top)
     else:
         run = lambda: K.orb_density(phi, At, rho)
     for _ in range(3):
@@ -147,9 +147,9 @@ def time_orb_density(pes, B, q8=False, reps=20):
     flops = 2.0 * (c.nao + 1) * keep * c.N * B       # psi = C^T phi, then sum f psi^2
     # int8 digits: ns bytes per phi value + a 2-byte exponent per point in, rho out
     nbytes = B * c.ld * (ns * c.nao_pad + 2.0 + 8.0)
-    # digit pairs (k, l) with l < ns, k <= 5 - l, each a 16x16x64 product over the AO
-    # depth padded to 64 and the orbitals padded to 16
-    pairs = sum(6 - l for l in range(ns))
+    # digit pairs (k, l) with l < ns, k <= top - l, each a 16x16x64 product over the
+    # AO depth padded to 64 and the orbitals padded to 16
+    pairs = sum(top + 1 - l for l in range(ns))
     kdepth = (c.nao_pad + 63) // 64 * 64
     ops = 2.0 * m_pad * kdepth * c.ld * B * pairs
     return ms, flops, keep, nbytes, ops
@@ -571,7 +571,7 @@ def main():
     # dominant kernels: potential matrix (FP64 or FP32) and FP64 orbital density
     kind = vmat_kind(pes)
     ms_v, fl_v = time_vmat(pes, B)
-    q8 = a.path == "mixed" and stats.get("rhoq8", 0) > 0
+    q8 = a.path == "mixed" and stats.get("rhoq8", 0) + stats.get("rhoq8_top4", 0) > 0
     ms_d, fl_d, keep, by_d, ops_d = time_orb_density(pes, B, q8)
     n_vmat = stats["v64"] if a.path == "fp64" else stats["v32"]
     n_dens64 = stats["rhoq8"] if q8 else stats["rho64"]
@@ -598,7 +598,7 @@ def main():
         # digits (4 bytes per phi value), which is reported beside it; units of the
         # MFMA roof: tera int8 operations per second
         dk = {"kernel": "k_orb_density_q8 (sum_i f_i (C^T phi)_i^2 from int8 digits, "
-                        "exact int32 products; int8 MFMA ops)", "bound": "mfma",
+                        "exact int32 products to level 5; int8 MFMA ops)", "bound": "mfma",
               "unit": "TFLOP/s", "ops_per_launch": ops_d, "peak": MI355X_PEAK["i8"],
               "bytes_per_launch": by_d,
               "hbm_gbs": by_d / (ms_d * 1e-3) / 1e9,
@@ -612,6 +612,16 @@ def main():
     dk.update({"ms_per_launch": ms_d, "flops_per_launch": fl_d, "orbitals": keep,
                "calls_per_step": n_dens64 / nsteps})
     cand = [vk, dk]
+    if q8 and stats.get("rhoq8_top4", 0):
+        # the same kernel with digit products to level 4 (the early DIIS iterations)
+        ms4, fl4, _, by4, ops4 = time_orb_density(pes, B, True, top=4)
+        d4 = dict(dk, kernel="k_orb_density_q8 (the same, digit products to level 4; int8 "
+                             "MFMA ops)", ops_per_launch=ops4, ms_per_launch=ms4,
+                  flops_per_launch=fl4, hbm_gbs=by4 / (ms4 * 1e-3) / 1e9,
+                  hbm_frac=by4 / (ms4 * 1e-3) / 1e9 / MI355X_HBM_GBS,
+                  calls_per_step=stats["rhoq8_top4"] / nsteps, mfma_util_pct=None,
+                  traffic=None)
+        cand.append(d4)
     for k in cand:
         if k["unit"] == "GB/s":
             k["achieved"] = k["bytes_per_launch"] / (k["ms_per_launch"] * 1e-3) / 1e9

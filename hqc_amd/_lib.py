@@ -42,7 +42,7 @@ _SIGS = {
     "hqc_ao_vmat_h2": [_P, _P, _P, _P, _I, _I, _I, _D, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_phi_slice_i8": [_P, _I, _I, _I, _I, _P, _P, _P, _P],
     "hqc_mixed_v_prep": [_P, _P, _P, _I, _I, _P, _P, _P, _P],
-    "hqc_orb_density_q8": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
+    "hqc_orb_density_q8": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
     "hqc_fft3d_c2c_f32": [_P, _I, _I, _I, _P],
     "hqc_pair_pack_c64": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P],
@@ -305,10 +305,11 @@ def phi_slice_i8(phi, nslice, q, pe, amax=None):
           _stream())
 
 
-def orb_density_q8(q, pe, At, rho, nslice, bidx=None, nuse=None):
+def orb_density_q8(q, pe, At, rho, nslice, bidx=None, nuse=None, top=5):
     """rho (B, ld) = sum_i (At[:, :, i]^T phi)^2 from the int8 digits of phi_slice_i8
     (exact integer accumulation; bidx selects the geometries of q and pe; nuse of the
-    nslice stored digits are used, all by default)."""
+    nslice stored digits are used, all by default; digit products up to level top,
+    4 or 5)."""
     B, nao_pad, m_pad = At.shape
     ld = rho.shape[1]
     assert rho.shape == (B, ld) and q.shape[1] == ld * nslice * nao_pad and pe.shape[1] == ld
@@ -316,7 +317,7 @@ def orb_density_q8(q, pe, At, rho, nslice, bidx=None, nuse=None):
     bi = _bidx(q, bidx, B)
     _call("hqc_orb_density_q8", _ptr(q, torch.int8, "q"), _ptr(pe, torch.int16, "pe"),
           _ptr(bi, I32, "bidx"), _ptr(At, F64, "At"), B, nao_pad, m_pad, ld, nslice,
-          nslice if nuse is None else int(nuse), _ptr(rho, F64, "rho"), _stream())
+          nslice if nuse is None else int(nuse), int(top), _ptr(rho, F64, "rho"), _stream())
 
 
 def vmat_h2_workspace(B, nao_pad, ld):

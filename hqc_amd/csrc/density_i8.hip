@@ -5,18 +5,22 @@
 // phi, phi, D) with D = C f C^T (hqc/pbc/solver.py:690); here At = C sqrt(f).
 //
 // Slicing.  A value x of a vector whose largest magnitude is below 2^e is cut
-// into int8 digits by
-//     y_0 = x 2^(6-e),  x_s = rint(y_s),  y_(s+1) = 128 (y_s - x_s),
-// every step exact in FP64, |x_s| <= 64, and
-//     x = 2^(e-6) sum_s 2^(-7 s) x_s + r,   |r| <= 2^(e-7 S)  for S digits.
+// into int8 digits of base 256 by
+//     y_0 = x 2^(6-e),  x_s = rint(y_s),  y_(s+1) = 256 (y_s - x_s),
+// every step exact in FP64, |x_0| <= 64 and |x_s| <= 128; a digit +128 (only
+// when y_s - x_s was exactly 1/2, so x_(s-1) is even and below 127) becomes -128
+// with a carry of 1 into x_(s-1), which keeps every digit in [-128, 127] and the
+// value unchanged:
+//     x = 2^(e-6) sum_s 2^(-8 s) x_s + r,   |r| <= 2^(e-7-8(S-1))  for S digits.
 // phi is sliced once per geometry with a per-grid-point exponent (S_phi digits,
 // a fixed perturbation of the basis: smooth in the SCF state); the orbital
 // coefficients are sliced per call with a per-orbital exponent into T+1 digits,
 // and every digit product with k + l <= T is accumulated exactly in int32 per
-// level d = k + l (|level| <= 4 * 128 * 64 * 64 < 2^31).  What changes with the
+// level d = k + l (|level| <= 4 * 128 * 128 * 128 < 2^31).  What changes with the
 // SCF state from call to call (coefficient rounding, dropped levels) is then
-// ~2^(-7(T+1)) = 2^-42 of max|At_i| max|phi(r)| per AO.  psi_i(r) =
-// 2^(e_r + e_i - 12) sum_d 2^(-7 d) P_d, combined in FP64 by Horner's rule.
+// ~2^(-6-8T) of max|At_i| max|phi(r)| per AO: 2^-46 with T = 5, 2^-38 with T = 4
+// (the SCF's early iterations).  psi_i(r) = 2^(e_r + e_i - 12) sum_d 2^(-8 d) P_d,
+// combined in FP64 by Horner's rule.
 //
 // Layout of the phi slices (per geometry, ld/16 tiles of 16 grid points):
 //     q[tile][s][mb][16 points][16 AOs]   (NB = nao_pad/16 AO blocks mb),
@@ -27,8 +31,7 @@
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4q __attribute__((ext_vector_type(4)));
 
-#define Q8_T 5            // highest digit level kept: T + 1 coefficient digits
-#define Q8_SC (Q8_T + 1)
+#define Q8_TMAX 5         // highest digit level kept at most: T + 1 coefficient digits
 #define Q8_MAXKS 2        // nao_pad <= 128 (two K steps of 64 AOs)
 #define Q8_NW 4           // waves per workgroup
 // point tiles per wave at least: the coefficient digits are cut once per
@@ -40,6 +43,23 @@ typedef unsigned u32x4q __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ int q8_exponent(double m) {
   // smallest e with m < 2^e (0 for m = 0)
   return m > 0.0 ? ilogb(m) + 1 : 0;
+}
+
+// the N base-256 digits of y = x 2^(6-e) (|y| < 64), each in [-128, 127]
+template <int N>
+__device__ __forceinline__ void q8_digits(double y, int (&dg)[N]) {
+#pragma unroll
+  for (int s = 0; s < N; ++s) {
+    const double d = rint(y);
+    dg[s] = (int)d;
+    y = (y - d) * 256.0;
+  }
+#pragma unroll
+  for (int s = N - 1; s > 0; --s) {
+    const int c = dg[s] == 128;
+    dg[s] -= 256 * c;
+    dg[s - 1] += c;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -96,13 +116,10 @@ __global__ __launch_bounds__(256) void k_phi_slice_i8(const double* __restrict__
       for (int c = 0; c < 4; ++c) word[s][c] = 0u;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-      double y = ldexp(v[t][i], 6 - e);
+      int dg[SP];
+      q8_digits(ldexp(v[t][i], 6 - e), dg);
 #pragma unroll
-      for (int s = 0; s < SP; ++s) {
-        const double d = rint(y);
-        word[s][i >> 2] |= ((unsigned)(int)d & 0xffu) << (8 * (i & 3));
-        y = (y - d) * 128.0;
-      }
+      for (int s = 0; s < SP; ++s) word[s][i >> 2] |= ((unsigned)dg[s] & 0xffu) << (8 * (i & 3));
     }
 #pragma unroll
     for (int s = 0; s < SP; ++s)
@@ -122,7 +139,7 @@ __global__ __launch_bounds__(256) void k_phi_slice_i8(const double* __restrict__
 // KS = 2: two waves per SIMD, which is what the LDS allows (two workgroups per
 // CU); left to itself the compiler gave SP = 3 260 registers and one wave
 // (3.89 -> 3.33 ms for 256 geometries, 62 orbitals, 112 AOs)
-template <int SP, int KS>
+template <int SP, int KS, int T>
 __global__ __launch_bounds__(64 * Q8_NW) __attribute__((amdgpu_waves_per_eu(KS))) void
 k_orb_density_q8(
     const signed char* __restrict__ q, const short* __restrict__ pe, const int* __restrict__ bidx,
@@ -130,9 +147,10 @@ k_orb_density_q8(
     double* __restrict__ rho) {
   constexpr int STRA = 64 * KS + 16;
   constexpr int PLANE = 64 * STRA;              // one digit of 64 orbitals
+  constexpr int SC = T + 1;                     // coefficient digits
   extern __shared__ unsigned char smem_q8[];
-  signed char* As = reinterpret_cast<signed char*>(smem_q8);          // [Q8_SC][64][STRA]
-  double* wo = reinterpret_cast<double*>(smem_q8 + Q8_SC * PLANE);   // [64] 2^(2 e_i)
+  signed char* As = reinterpret_cast<signed char*>(smem_q8);          // [SC][64][STRA]
+  double* wo = reinterpret_cast<double*>(smem_q8 + SC * PLANE);      // [64] 2^(2 e_i)
   double* red = wo + 64;                                              // [4][64]
   const int NB = nao_pad >> 4;
   const int NPT = ld >> 4;
@@ -179,22 +197,20 @@ k_orb_density_q8(
       if (w == 0) wo[lane] = ldexp(1.0, 2 * e);
       // digits of 4 consecutive AOs per task: orbital i = lane, AO quad from w
       for (int quad = w; quad < 16 * KS; quad += 4) {
-        unsigned word[Q8_SC];
+        unsigned word[SC];
 #pragma unroll
-        for (int s = 0; s < Q8_SC; ++s) word[s] = 0u;
+        for (int s = 0; s < SC; ++s) word[s] = 0u;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int mu = 4 * quad + c;
-          double y = (lane < MW && mu < nao_pad) ? ldexp(Ab[(long)mu * m_pad + og + lane], 6 - e) : 0.0;
+          int dg[SC];
+          q8_digits((lane < MW && mu < nao_pad) ? ldexp(Ab[(long)mu * m_pad + og + lane], 6 - e)
+                                                 : 0.0, dg);
 #pragma unroll
-          for (int s = 0; s < Q8_SC; ++s) {
-            const double d = rint(y);
-            word[s] |= ((unsigned)(int)d & 0xffu) << (8 * c);
-            y = (y - d) * 128.0;
-          }
+          for (int s = 0; s < SC; ++s) word[s] |= ((unsigned)dg[s] & 0xffu) << (8 * c);
         }
 #pragma unroll
-        for (int s = 0; s < Q8_SC; ++s)
+        for (int s = 0; s < SC; ++s)
           *reinterpret_cast<unsigned*>(As + s * PLANE + lane * STRA + 4 * quad) = word[s];
       }
     }
@@ -207,17 +223,17 @@ k_orb_density_q8(
                      int pta, int ptb) {
       double ra = 0.0, rb = 0.0;
       for (int t = 0; t < MT; ++t) {
-        i32x4 acca[Q8_T + 1], accb[Q8_T + 1];
+        i32x4 acca[T + 1], accb[T + 1];
 #pragma unroll
-        for (int d = 0; d <= Q8_T; ++d) {
+        for (int d = 0; d <= T; ++d) {
           acca[d] = i32x4{0, 0, 0, 0};
           accb[d] = i32x4{0, 0, 0, 0};
         }
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-          i32x4 a[Q8_SC];
+          i32x4 a[SC];
 #pragma unroll
-          for (int k = 0; k < Q8_SC; ++k)
+          for (int k = 0; k < SC; ++k)
             a[k] = *reinterpret_cast<const i32x4*>(As + k * PLANE + (16 * t + j) * STRA + 64 * ks +
                                                    16 * kq);
 #pragma unroll
@@ -225,7 +241,7 @@ k_orb_density_q8(
             const i32x4 bla = __builtin_bit_cast(i32x4, Ba[l][ks]);
             const i32x4 blb = __builtin_bit_cast(i32x4, Bb[l][ks]);
 #pragma unroll
-            for (int k = 0; k + l <= Q8_T; ++k) {
+            for (int k = 0; k + l <= T; ++k) {
               acca[k + l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[k], bla, acca[k + l], 0, 0, 0);
               accb[k + l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[k], blb, accb[k + l], 0, 0, 0);
             }
@@ -233,11 +249,11 @@ k_orb_density_q8(
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          double Qa = (double)acca[Q8_T][e], Qb = (double)accb[Q8_T][e];
+          double Qa = (double)acca[T][e], Qb = (double)accb[T][e];
 #pragma unroll
-          for (int d = Q8_T - 1; d >= 0; --d) {
-            Qa = fma(Qa, 0.0078125, (double)acca[d][e]);
-            Qb = fma(Qb, 0.0078125, (double)accb[d][e]);
+          for (int d = T - 1; d >= 0; --d) {
+            Qa = fma(Qa, 0x1p-8, (double)acca[d][e]);
+            Qb = fma(Qb, 0x1p-8, (double)accb[d][e]);
           }
           const double wv = wo[16 * t + 4 * kq + e];
           ra = fma(wv * Qa, Qa, ra);
@@ -273,8 +289,8 @@ k_orb_density_q8(
   }
 }
 
-static size_t q8_lds_bytes(int KS) {
-  return (size_t)Q8_SC * 64 * (64 * KS + 16) + 64 * sizeof(double) + 256 * sizeof(double);
+static size_t q8_lds_bytes(int KS, int T) {
+  return (size_t)(T + 1) * 64 * (64 * KS + 16) + 64 * sizeof(double) + 256 * sizeof(double);
 }
 
 extern "C" int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, int nslice,
@@ -298,15 +314,16 @@ extern "C" int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, i
 
 extern "C" int hqc_orb_density_q8(const signed char* q, const short* pe, const int* bidx,
                                   const double* At, int B, int nao_pad, int m_pad, int ld,
-                                  int nslice, int nuse, double* rho, void* stream) {
+                                  int nslice, int nuse, int top, double* rho, void* stream) {
   if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || m_pad <= 0 || (m_pad & 15) || ld <= 0 ||
-      (ld & 63) || nslice < 3 || nslice > 5 || nuse < 3 || nuse > nslice)
+      (ld & 63) || nslice < 3 || nslice > 5 || nuse < 3 || nuse > nslice || top < 4 ||
+      top > Q8_TMAX)
     return HQC_E_ARG;
   if (nao_pad > 64 * Q8_MAXKS) return HQC_E_SIZE;
   if (B == 0) return HQC_OK;
   if (!q || !pe || !At || !rho) return HQC_E_ARG;
   const int KS = nao_pad > 64 ? 2 : 1;
-  const size_t lds = q8_lds_bytes(KS);
+  const size_t lds = q8_lds_bytes(KS, top);
   const int npt = ld >> 4;
   long nwg = hqc_tuned(HQC_TUNE_Q8_BLOCKS, Q8_TARGET_BLOCKS) / B;
   const long min_tiles = hqc_tuned(HQC_TUNE_Q8_MIN_TILES, Q8_MIN_TILES);
@@ -315,16 +332,18 @@ extern "C" int hqc_orb_density_q8(const signed char* q, const short* pe, const i
   if (nwg < 1) nwg = 1;
   const dim3 grid((unsigned)nwg, B);
   hipStream_t s = (hipStream_t)stream;
-#define Q8_CASE(SP_, KS_)                                                                      \
-  if (nuse == SP_ && KS == KS_) {                                                              \
+#define Q8_CASE(SP_, KS_, T_)                                                                  \
+  if (nuse == SP_ && KS == KS_ && top == T_) {                                                 \
     if (lds > 64 * 1024 &&                                                                     \
-        hipFuncSetAttribute((const void*)k_orb_density_q8<SP_, KS_>,                           \
+        hipFuncSetAttribute((const void*)k_orb_density_q8<SP_, KS_, T_>,                       \
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) \
       (void)hipGetLastError();                                                                 \
-    hipLaunchKernelGGL((k_orb_density_q8<SP_, KS_>), grid, dim3(64 * Q8_NW), lds, s, q, pe,    \
+    hipLaunchKernelGGL((k_orb_density_q8<SP_, KS_, T_>), grid, dim3(64 * Q8_NW), lds, s, q, pe, \
                        bidx, At, nao_pad, m_pad, ld, nslice, rho);                             \
   }
-  Q8_CASE(3, 1) Q8_CASE(3, 2) Q8_CASE(4, 1) Q8_CASE(4, 2) Q8_CASE(5, 1) Q8_CASE(5, 2)
+  Q8_CASE(3, 1, 4) Q8_CASE(3, 2, 4) Q8_CASE(4, 1, 4) Q8_CASE(4, 2, 4) Q8_CASE(5, 1, 4)
+  Q8_CASE(5, 2, 4) Q8_CASE(3, 1, 5) Q8_CASE(3, 2, 5) Q8_CASE(4, 1, 5) Q8_CASE(4, 2, 5)
+  Q8_CASE(5, 1, 5) Q8_CASE(5, 2, 5)
 #undef Q8_CASE
   HQC_CHECK_LAUNCH();
   return HQC_OK;

@@ -176,14 +176,18 @@ class Engine:
     # throughput, 1e-6, below the real error, stalls geometries and loses 18%)
     MIXED_SPLIT16 = True
     MIXED_EPS_SPLIT16 = 4e-6
-    # past the warm-up the density comes from int8 digits of phi with exact integer
-    # products (n_ao <= 128; FP64 vector FMAs beyond).  4 digits: phi is fixed to
-    # 2^-28 of its per-point maximum (a basis perturbation that is the same on every
-    # call); the coefficient digits add ~2^-42 per AO, so no residual floor appears
+    # past the warm-up the density comes from base-256 int8 digits of phi with exact
+    # integer products (n_ao <= 128; FP64 vector FMAs beyond).  4 digits: phi is fixed
+    # to 2^-30 of its per-point maximum (a basis perturbation that is the same on
+    # every call); the coefficient digits add ~2^-46 per AO with digit products up to
+    # level 5, so no residual floor appears.  While the DIIS residual is still >=
+    # MIXED_Q8_TOP4_ERR, products up to level 4 (14 instead of 18, ~2^-38 per AO,
+    # four orders of magnitude below that residual's effect) serve
     MIXED_Q8 = True
     Q8_SLICES = 4
-    # the warm-up density from the first 3 digits (2^-21 per value, a fixed basis
-    # with no residual floor; 0: the split-FP16 form)
+    MIXED_Q8_TOP4_ERR = 1e-6
+    # the warm-up density from the first 3 digits and level 4 (2^-22 per value, a
+    # fixed basis with no residual floor; 0: the split-FP16 form)
     Q8_WARMUP_DIGITS = 3
     mat_dtype = torch.float64        # n_ao x n_ao matrices (complex128 at a k-point)
     # Hartree-Fock in the mixed mode: the exchange pair densities, their transforms
@@ -247,7 +251,7 @@ class Engine:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
         # kernel launches: FP64 / FP32 densities and potential matrices
-        self.stats = {"rho64": 0, "rhoq8": 0, "rho32": 0, "v64": 0, "v32": 0, "x64": 0,
+        self.stats = {"rho64": 0, "rhoq8": 0, "rhoq8_top4": 0, "rho32": 0, "v64": 0, "v32": 0, "x64": 0,
                       "x32": 0}
         self.sweeps = torch.zeros((), dtype=torch.long, device=cell.device)
         self.eigh_calls = 0
@@ -340,13 +344,13 @@ class Engine:
                       self._phi_scale if phi.dtype == torch.int32 else None)
         return rho
 
-    def orb_density_q8(self, st, mo, occ, bidx=None, keep=None, nuse=None):
+    def orb_density_q8(self, st, mo, occ, bidx=None, keep=None, nuse=None, top=5):
         """orb_density from the int8 digits of phi (st['phi_q8'], st['phi_e8']); nuse
-        of the stored digits (all by default)."""
+        of the stored digits (all by default), digit products up to level top."""
         c = self.c
         At = self._at(mo, occ, keep)
         rho = torch.empty(mo.shape[0], c.ld, dtype=torch.float64, device=c.device)
-        K.orb_density_q8(st["phi_q8"], st["phi_e8"], At, rho, self.Q8_SLICES, bidx, nuse)
+        K.orb_density_q8(st["phi_q8"], st["phi_e8"], At, rho, self.Q8_SLICES, bidx, nuse, top)
         return rho
 
     def poisson(self, rho, save=False, geo=None, paired=False):
@@ -756,6 +760,22 @@ class Engine:
         return (up(yes), len(yes)), (up(no), len(no)), h[n:]
 
     @staticmethod
+    def _classes(cls, ncls, extra=()):
+        """([(index, count) of the entries of class c] for c < ncls, host values of the
+        integer tensors in extra) from one host round trip; an index is int32 on the
+        device, or None when it covers all of cls."""
+        n = cls.numel()
+        h = torch.cat([cls.reshape(-1).to(torch.int64)] +
+                      [t.reshape(-1).to(torch.int64) for t in extra]).tolist()
+        out = []
+        for k in range(ncls):
+            lst = [i for i in range(n) if h[i] == k]
+            out.append((None if len(lst) == n else
+                        torch.tensor(lst, dtype=torch.int32).pin_memory().to(
+                            cls.device, non_blocking=True), len(lst)))
+        return out, h[n:]
+
+    @staticmethod
     def _subset(mask):
         """(index tensor int32 or None when the mask covers the batch, count); one
         host round trip (the size of nonzero's result)."""
@@ -785,13 +805,18 @@ class Engine:
             inc = None
         rho = torch.empty(na, c.ld, dtype=torch.float64, device=c.device)
         if inc is None:
-            # both classes and the occupied-orbital count in one round trip
+            # the classes (low precision; accurate with digit products to level 4, to
+            # level 5) and the occupied-orbital count in one round trip
             kc = (occ > self.OCC_CUT).sum(-1).amax()
-            (idx, n), full_sub, (keep,) = self._split(direct, (kc,))
+            top4 = err >= self.MIXED_Q8_TOP4_ERR if "phi_q8" in st else torch.zeros_like(direct)
+            cls = torch.where(direct, 0, torch.where(top4, 1, 2))
+            ((idx, n), sub4, sub5), (keep,) = self._classes(cls, 3, (kc,))
+            accurate = ((sub4, 4), (sub5, 5))
         else:
             full = ~(direct | inc)
             idx, n = self._subset(direct)
             keep = None
+            accurate = ((self._subset(full), 5),)
         if n:
             # orbital form sum_i f_i psi_i^2: non-negative in any precision, unlike the
             # D form, whose FP32 rounding can turn a tiny density negative
@@ -799,7 +824,7 @@ class Engine:
             occ_d = occ if idx is None else occ[idx.long()]
             if "phi_q8" in st and self.Q8_WARMUP_DIGITS:
                 r = self.orb_density_q8(st, mo_d, occ_d, self._gix(st, na, idx), keep,
-                                        self.Q8_WARMUP_DIGITS)
+                                        self.Q8_WARMUP_DIGITS, top=4)
             else:
                 lp = "phi_h2" if "phi_h2" in st else "phi_lp"
                 r = self.orb_density(st[lp], mo_d, occ_d, self._gix(st, na, idx), keep)
@@ -814,13 +839,14 @@ class Engine:
             r += sel("rhoa") if il is None else st["rhoa"][il]
             self._scatter(rho, idx, r)
             self.stats["rho32"] += 1
-        idx, n = full_sub if inc is None else self._subset(full)
-        if n:
+        for (idx, n), top in accurate:
+            if not n:
+                continue
             il = torch.arange(na, device=c.device) if idx is None else idx.long()
             gi = self._gix(st, na, idx)
             if "phi_q8" in st:
-                r64 = self.orb_density_q8(st, mo[il], occ[il], gi, keep)
-                self.stats["rhoq8"] += 1
+                r64 = self.orb_density_q8(st, mo[il], occ[il], gi, keep, top=top)
+                self.stats["rhoq8" if top == 5 else "rhoq8_top4"] += 1
             else:
                 r64 = self.orb_density(st["phi"], mo[il], occ[il], gi, keep)
                 self.stats["rho64"] += 1

@@ -154,25 +154,26 @@ int hqc_orb_density_h2(const unsigned* ph, const int* bidx, const double* At, in
                        void* stream);
 /* q8: from int8 digits with exact integer accumulation (Ozaki-style slicing).
  * hqc_phi_slice_i8 cuts phi (G,nao_pad,ld) FP64, once per geometry, into
- * nslice (3..5) int8 digits per value under a per-grid-point power-of-two
- * exponent: q (G, ld*nslice*nao_pad) bytes in tiles of 16 points
+ * nslice (3..5) base-256 int8 digits per value under a per-grid-point
+ * power-of-two exponent: q (G, ld*nslice*nao_pad) bytes in tiles of 16 points
  * [tile][digit][AO block of 16][point][AO], pe (G,ld) the exponents.
- * hqc_orb_density_q8 cuts At per call into 6 digits per orbital (per-orbital
- * exponent), forms every digit product of level <= 5 on
- * v_mfma_i32_16x16x64_i8 with int32 accumulation (exact), combines the levels
- * in FP64 and sums psi^2 in FP64.  Error: the fixed phi rounding,
- * 2^-(7 nslice) of max_k |phi_k(r)| per value (a perturbation of the basis
- * that does not change between calls), plus ~2^-42 of max|At_i| max|phi(r)|
- * per AO from the coefficient digits.  nao_pad <= 128 (HQC_E_SIZE beyond).
- * nuse (3..nslice) of the nslice stored digits enter the density (fewer: a
- * coarser fixed basis, 2^-(7 nuse) per value, and fewer bytes read).
+ * hqc_orb_density_q8 cuts At per call into top + 1 digits per orbital
+ * (per-orbital exponent), forms every digit product of level <= top (4 or 5)
+ * on v_mfma_i32_16x16x64_i8 with int32 accumulation (exact), combines the
+ * levels in FP64 and sums psi^2 in FP64.  Error: the fixed phi rounding,
+ * 2^(1-6-8(nuse-1)) of max_k |phi_k(r)| per value (a perturbation of the basis
+ * that does not change between calls), plus ~2^(-6-8 top) of max|At_i|
+ * max|phi(r)| per AO from the coefficient digits (2^-46 at top 5; top 4 takes
+ * 14 instead of 18 digit products with 4 phi digits).  nao_pad <= 128
+ * (HQC_E_SIZE beyond).  nuse (3..nslice) of the nslice stored digits enter the
+ * density (fewer: a coarser fixed basis and fewer bytes read).
  * amax (G) or NULL: if given (zero-filled by the caller), receives max |phi|
  * over each geometry's nao_pad x ld values, exactly, from the same pass. */
 int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, int nslice,
                      signed char* q, short* pe, double* amax, void* stream);
 int hqc_orb_density_q8(const signed char* q, const short* pe, const int* bidx,
                        const double* At, int B, int nao_pad, int m_pad, int ld, int nslice,
-                       int nuse, double* rho, void* stream);
+                       int nuse, int top, double* rho, void* stream);
 
 /* Grid potential matrix V[b] = dV * phi diag(v[b]) phi^T (symmetric, full).
  * Replaces the 'pr,qr,r->pq' contraction (hqc/pbc/solver.py:693) and, through

@@ -7,7 +7,7 @@ import pytest
 import torch
 
 from oracle import cpu_ref as R
-from q8_digits import q8_density_digits
+from q8_digits import q8_density_digits, q8_digits, q8_exponent
 
 pytestmark = pytest.mark.gpu
 
@@ -211,11 +211,12 @@ def test_orb_density_split16(K, nao_pad, m):
 @pytest.mark.parametrize("nao_pad,m,ns", [(16, 8, 4), (48, 20, 3), (112, 60, 4), (112, 100, 5),
                                           (128, 140, 4)])
 def test_orb_density_q8(K, nao_pad, m, ns):
-    """Orbital density from int8 digits (exact integer products): equal to the host
-    restatement of the digit algorithm to FP64 rounding, and within the digit
-    error bound of the FP64 density,
+    """Orbital density from base-256 int8 digits (exact integer products), with
+    digit products to level 5 and to level 4: equal to the host restatement of the
+    digit algorithm to FP64 rounding, and within the digit error bound of the FP64
+    density,
         |rho~ - rho| <= sum_i (2|psi_i| + d_i) d_i,
-        d_i(r) = nao_pad max_k|At_ki| max_k|phi_k(r)| (2^(1-7 ns) + 2^-36)."""
+        d_i(r) = nao_pad max_k|At_ki| max_k|phi_k(r)| (2^(-6-8(ns-1)) + ns 2^(-3-8 top))."""
     print(f"[orb density q8] nao_pad={nao_pad} m={m} ns={ns}", flush=True)
     torch.manual_seed(7)
     g = 11
@@ -227,6 +228,13 @@ def test_orb_density_q8(K, nao_pad, m, ns):
     # per-point magnitudes over 30 binary orders, some points all zero
     phi[:, :, :N] *= torch.exp2(-torch.randint(0, 30, (B, 1, N), device=DEV).double())
     phi[:, :, 5:9] = 0.0
+    # values whose digit expansion carries (a remainder of exactly 1/2 after an even
+    # digit): at point 20 of geometry 0 the maximum is 1 (y_0 = 32 phi), AOs 1-7 hold
+    # (n + 1/2) / 32 with n even (the tie at digit 0), AO 8 one at digit 1
+    phi[0, :, 20] = 0.25
+    phi[0, 0, 20] = 1.0
+    phi[0, 1:8, 20] = (16.5 + 2.0 * torch.arange(7, device=DEV)) / 32
+    phi[0, 8, 20] = (10 + 64.5 / 256) / 32
     m_pad = (m + 15) // 16 * 16
     At = torch.zeros(B, nao_pad, m_pad, dtype=torch.float64, device=DEV)
     At[:, :, :m] = torch.randn(B, nao_pad, m, dtype=torch.float64, device=DEV)
@@ -237,33 +245,41 @@ def test_orb_density_q8(K, nao_pad, m, ns):
     pe = torch.empty(B, ld, dtype=torch.int16, device=DEV)
     amax = torch.full((B,), float("nan"), dtype=torch.float64, device=DEV)
     K.phi_slice_i8(phi, ns, q, pe, amax)
-    rho = torch.full((B, ld), float("nan"), dtype=torch.float64, device=DEV)
-    K.orb_density_q8(q, pe, At, rho, ns)
     torch.cuda.synchronize()
     assert torch.equal(amax, phi.abs().amax(dim=(1, 2)))       # max |phi| per geometry
-    assert torch.isfinite(rho).all()
     ph, A = phi.cpu().numpy(), At.cpu().numpy()
-    emul = q8_density_digits(ph, A, ns)
-    r = rho.cpu().numpy()
-    assert np.allclose(r, emul, rtol=1e-13, atol=1e-300)
+    # the stored digits themselves: [tile][digit][AO block][point][AO]
+    er = q8_exponent(np.abs(ph).max(1))
+    xd = q8_digits(ph, er[:, None, :], ns)
+    qq = q.cpu().numpy().reshape(B, ld // 16, ns, nao_pad // 16, 16, 16)
+    want = np.stack(xd, 0).reshape(ns, B, nao_pad // 16, 16, ld // 16, 16)
+    assert np.array_equal(qq, want.transpose(1, 4, 0, 2, 5, 3).astype(np.int8))
+    assert np.array_equal(pe.cpu().numpy(), er)
     psi = np.matmul(A.transpose(0, 2, 1), ph)                    # (B, m, ld)
     rref = (psi ** 2).sum(1)
-    d = (nao_pad * np.abs(A).max(1)[:, :, None] * np.abs(ph).max(1)[:, None, :]
-         * (2.0 ** (1 - 7 * ns) + 2.0 ** -36))
-    bound = ((2 * np.abs(psi) + d) * d).sum(1)
-    assert (np.abs(r - rref) <= bound).all()
-    # through a geometry index list: slot 0 reads geometry 1 and slot 1 geometry 0
-    bidx = torch.tensor([1, 0], dtype=torch.int32, device=DEV)
-    rho2 = torch.empty_like(rho)
-    K.orb_density_q8(q, pe, At, rho2, ns, bidx)
-    emul2 = q8_density_digits(ph[::-1].copy(), A, ns)
-    assert np.allclose(rho2.cpu().numpy(), emul2, rtol=1e-13, atol=1e-300)
-    # the first 3 of the stored digits are the 3-digit expansion
-    if ns > 3:
-        rho3 = torch.empty_like(rho)
-        K.orb_density_q8(q, pe, At, rho3, ns, nuse=3)
-        assert np.allclose(rho3.cpu().numpy(), q8_density_digits(ph, A, 3), rtol=1e-13,
-                           atol=1e-300)
+    for top in (5, 4):
+        rho = torch.full((B, ld), float("nan"), dtype=torch.float64, device=DEV)
+        K.orb_density_q8(q, pe, At, rho, ns, top=top)
+        assert torch.isfinite(rho).all()
+        emul = q8_density_digits(ph, A, ns, top)
+        r = rho.cpu().numpy()
+        assert np.allclose(r, emul, rtol=1e-13, atol=1e-300), top
+        d = (nao_pad * np.abs(A).max(1)[:, :, None] * np.abs(ph).max(1)[:, None, :]
+             * (2.0 ** (-6 - 8 * (ns - 1)) + ns * 2.0 ** (-3 - 8 * top)))
+        bound = ((2 * np.abs(psi) + d) * d).sum(1)
+        assert (np.abs(r - rref) <= bound).all(), top
+        # through a geometry index list: slot 0 reads geometry 1 and slot 1 geometry 0
+        bidx = torch.tensor([1, 0], dtype=torch.int32, device=DEV)
+        rho2 = torch.empty_like(rho)
+        K.orb_density_q8(q, pe, At, rho2, ns, bidx, top=top)
+        emul2 = q8_density_digits(ph[::-1].copy(), A, ns, top)
+        assert np.allclose(rho2.cpu().numpy(), emul2, rtol=1e-13, atol=1e-300), top
+        # the first 3 of the stored digits (the prefix of the ns-digit expansion)
+        if ns > 3:
+            rho3 = torch.empty_like(rho)
+            K.orb_density_q8(q, pe, At, rho3, ns, nuse=3, top=top)
+            assert np.allclose(rho3.cpu().numpy(), q8_density_digits(ph, A, 3, top, ns),
+                               rtol=1e-13, atol=1e-300), top
 
 
 @pytest.mark.parametrize("g", [1, 2, 5, 7, 13, 17, 24, 25, 27, 31, 33, 48, 49, 51, 55, 57, 61,
