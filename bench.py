@@ -326,8 +326,9 @@ def kernel_probe(name, B, reps=5):
         rho = torch.zeros(B, c.ld, dtype=torch.float64, device="cuda")
         for _ in range(reps):
             K.orb_density(phi, At, rho)
-    elif name == "orb_density_q8":
+    elif name in ("orb_density_q8", "orb_density_q8_top4"):
         keep, ns = 64, pes.solver.engine.Q8_SLICES
+        top = 4 if name.endswith("top4") else 5
         At = torch.rand(B, c.nao_pad, keep, dtype=torch.float64, device="cuda")
         phi = torch.rand(B, c.nao_pad, c.ld, dtype=torch.float64, device="cuda")
         q = torch.empty(B, c.ld * ns * c.nao_pad, dtype=torch.int8, device="cuda")
@@ -336,7 +337,7 @@ def kernel_probe(name, B, reps=5):
         del phi
         rho = torch.zeros(B, c.ld, dtype=torch.float64, device="cuda")
         for _ in range(reps):
-            K.orb_density_q8(q, pe, At, rho, ns)
+            K.orb_density_q8(q, pe, At, rho, ns, top=top)
     elif name in ("exchange", "exchange_c64", "exchange_fused", "exchange_fused_c64"):
         # the Hartree-Fock exchange pass of one geometry: 16 orbital pairs packed
         # against the AO values, 3-D transformed (complex128, or complex64 for the
@@ -388,19 +389,33 @@ def summarize_trace(d, top=25):
     return "\n".join(lines)
 
 
-def summarize_pmc(d):
+def summarize_pmc(d, n_xcd=8, n_simd=1024):
     """Mean value per dispatch of each counter, per kernel, from rocprofv3 --pmc
-    csv output."""
+    csv output; then per kernel the mean dispatch duration (the csv's timestamps)
+    and, where the counters are there, the engine clock (GRBM_GUI_ACTIVE, which sums
+    the XCDs, per XCD over the duration) and the MFMA pipe's busy share
+    (SQ_VALU_MFMA_BUSY_CYCLES over GRBM_GUI_ACTIVE per XCD x the SIMDs)."""
     import csv
     import glob
-    acc = {}
+    acc, dur = {}, {}
     for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
         for row in csv.DictReader(open(f)):
-            key = (row["Kernel_Name"][:60], row["Counter_Name"])
-            acc.setdefault(key, []).append(float(row["Counter_Value"]))
+            k = row["Kernel_Name"][:60]
+            acc.setdefault((k, row["Counter_Name"]), []).append(float(row["Counter_Value"]))
+            dur.setdefault(k, {})[row["Dispatch_Id"]] = (
+                int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) * 1e-9
     lines = [f"{'kernel':62s} {'counter':14s} {'dispatches':>10s} {'mean':>14s}"]
     for (k, cn), vals in sorted(acc.items()):
         lines.append(f"{k:62s} {cn:14s} {len(vals):10d} {sum(vals) / len(vals):14.1f}")
+    lines.append(f"{'kernel':62s} {'duration_ms':>11s} {'clock_GHz':>9s} {'mfma_busy_%':>11s}")
+    for k in sorted(dur):
+        t = sum(dur[k].values()) / len(dur[k])
+        mean = lambda cn: (sum(acc[(k, cn)]) / len(acc[(k, cn)])) if (k, cn) in acc else None
+        gui, busy = mean("GRBM_GUI_ACTIVE"), mean("SQ_VALU_MFMA_BUSY_CYCLES")
+        clk = f"{gui / n_xcd / t / 1e9:9.2f}" if gui and t > 0 else f"{'-':>9s}"
+        util = f"{100 * busy / (gui / n_xcd * n_simd):11.1f}" if gui and busy is not None \
+            else f"{'-':>11s}"
+        lines.append(f"{k:62s} {t * 1e3:11.3f} {clk} {util}")
     return "\n".join(lines)
 
 
@@ -510,7 +525,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP64 batched and plain runs (accuracy and speedups)")
     ap.add_argument("--kernel-probe", default=None,
-                    help="vmat_f64|vmat_f32|vmat_h2|orb_density|orb_density_q8|exchange|"
+                    help="vmat_f64|vmat_f32|vmat_h2|orb_density|orb_density_q8[_top4]|exchange|"
                          "exchange_c64|exchange_fused|exchange_fused_c64|copy: launch that "
                          "kernel (or the exchange pass) only")
     ap.add_argument("--workload", default="metric",
@@ -603,8 +618,8 @@ def main():
               "bytes_per_launch": by_d,
               "hbm_gbs": by_d / (ms_d * 1e-3) / 1e9,
               "hbm_frac": by_d / (ms_d * 1e-3) / 1e9 / MI355X_HBM_GBS,
-              "traffic": measured_traffic("k_orb_density_q8", B),
-              "mfma_util_pct": measured_mfma_util("k_orb_density_q8", B)}
+              "traffic": measured_traffic("k_orb_density_q8<4, 2, 5>", B),
+              "mfma_util_pct": measured_mfma_util("k_orb_density_q8<4, 2, 5>", B)}
     else:
         dk = {"kernel": "k_orb_density_valu (sum_i f_i (C^T phi)_i^2, FP64 vector FMA)",
               "bound": "mfma", "unit": "TFLOP/s", "peak": MI355X_PEAK["f64"],
@@ -619,8 +634,9 @@ def main():
                              "MFMA ops)", ops_per_launch=ops4, ms_per_launch=ms4,
                   flops_per_launch=fl4, hbm_gbs=by4 / (ms4 * 1e-3) / 1e9,
                   hbm_frac=by4 / (ms4 * 1e-3) / 1e9 / MI355X_HBM_GBS,
-                  calls_per_step=stats["rhoq8_top4"] / nsteps, mfma_util_pct=None,
-                  traffic=None)
+                  calls_per_step=stats["rhoq8_top4"] / nsteps,
+                  mfma_util_pct=measured_mfma_util("k_orb_density_q8<4, 2, 4>", B),
+                  traffic=measured_traffic("k_orb_density_q8<4, 2, 4>", B))
         cand.append(d4)
     for k in cand:
         if k["unit"] == "GB/s":
