@@ -539,9 +539,10 @@ def main():
                     help="hf-batch: (n, grid) buckets solved concurrently on their own streams")
     ap.add_argument("--summarize-trace", default=None, metavar="DIR")
     ap.add_argument("--summarize-pmc", default=None, metavar="DIR")
+    ap.add_argument("--top", type=int, default=25, help="rows of --summarize-trace")
     a = ap.parse_args()
     if a.summarize_trace:
-        print(summarize_trace(a.summarize_trace))
+        print(summarize_trace(a.summarize_trace, a.top))
         return
     if a.summarize_pmc:
         print(summarize_pmc(a.summarize_pmc))

@@ -181,8 +181,9 @@ class Engine:
     # to 2^-30 of its per-point maximum (a basis perturbation that is the same on
     # every call); the coefficient digits add ~2^-46 per AO with digit products up to
     # level 5, so no residual floor appears.  While the DIIS residual is still >=
-    # MIXED_Q8_TOP4_ERR, products up to level 4 (14 instead of 18, ~2^-38 per AO,
-    # four orders of magnitude below that residual's effect) serve
+    # MIXED_Q8_TOP4_ERR, products up to level 4 serve (14 instead of 18; ~2^-38 per
+    # AO, ~1.5e-11 of rho, which moves the energy by ~5e-10 Ha: far below what such a
+    # residual means)
     MIXED_Q8 = True
     Q8_SLICES = 4
     MIXED_Q8_TOP4_ERR = 1e-6
@@ -251,8 +252,8 @@ class Engine:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
         # kernel launches: FP64 / FP32 densities and potential matrices
-        self.stats = {"rho64": 0, "rhoq8": 0, "rhoq8_top4": 0, "rho32": 0, "v64": 0, "v32": 0, "x64": 0,
-                      "x32": 0}
+        self.stats = {"rho64": 0, "rhoq8": 0, "rhoq8_top4": 0, "rho32": 0, "v64": 0, "v32": 0,
+                      "x64": 0, "x32": 0}
         self.sweeps = torch.zeros((), dtype=torch.long, device=cell.device)
         self.eigh_calls = 0
         self.active = []
