@@ -10,7 +10,11 @@ defaults otherwise); energy (E + V_pp) plus analytic forces for every
 geometry.  Inputs are already on the GPU when the timed region starts.
 
 `value` is the throughput of --path (default: the mixed-precision batched
-path, batch 256).  Beside it the run measures, untimed for `value`:
+path, batch 512: the batch's state, ~110 GB of AO values and digits plus the
+force pass's 54 GB, fits the 288 GB of HBM, and the SCF's last iterations, where a
+few slow geometries run alone, cost about the same for any batch size, so they
+weigh half as much per geometry as at 256).  Beside it the run measures,
+untimed for `value`:
   * the FP64 path on the same batch, whose results also give the accuracy of
     the mixed path (max |dE| per atom, |dF|, |d bands| against FP64);
   * the plain FP64 path, one geometry per call (seeds 0..3);
@@ -155,7 +159,7 @@ def time_orb_density(pes, B, q8=False, reps=20, top=5):
     return ms, flops, keep, nbytes, ops
 
 
-TRAFFIC_BATCH = 256
+TRAFFIC_BATCH = 512
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", f"pmc_traffic_54H_dzv_b{TRAFFIC_BATCH}.txt")
 
 
@@ -517,7 +521,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--path", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-literal", action="store_true",
@@ -573,7 +577,9 @@ def main():
             TUNING[path] = rec["choice"]
             tuned[path] = rec["choice"]
         torch.cuda.empty_cache()
+    torch.cuda.reset_peak_memory_stats()
     pes, dt, E, F = run_path(a.path, x, a.steps, a.warmup)
+    hbm_peak = torch.cuda.max_memory_allocated() / 1e9
     st = pes.solver.last
     eng = pes.solver.engine
     nit = st["nit"].double().mean().item()
@@ -662,7 +668,7 @@ def main():
                                f"(grid_length 0.16), tol 1e-8, energy+forces, batch {B}",
                    "path": a.path, "batch": B, "n_iter_mean": nit, "all_converged": conv,
                    "contractions": stats, "jacobi_sweeps_per_eigh": sweeps,
-                   "diis_loop": tail},
+                   "diis_loop": tail, "hbm_peak_gb": round(hbm_peak, 1)},
         # the dominant kernel's own roof: HBM for the split-FP16 V, the int8 MFMA pipe
         # for the digit density, MFMA / FP64 vector FMAs (same 78.6 TFLOP/s peak) for
         # the FP64 forms
