@@ -18,7 +18,7 @@ recorded as JSON ({"shape", "device", "choice", "timings"}) and applied with
 apply(choice) on the calling host thread, or passed to an engine as
 Engine(..., tuning=choice).
 
-    python -m hqc_amd.tune --n 54 --rs 1.25 --basis gth-dzv --grid-length 0.16 --batch 256
+    python -m hqc_amd.tune --n 54 --rs 1.25 --basis gth-dzv --grid-length 0.16 --batch 512
 """
 import argparse
 import itertools
@@ -140,7 +140,7 @@ def main(argv=None):
     ap.add_argument("--rs", type=float, default=1.25)
     ap.add_argument("--basis", default="gth-dzv")
     ap.add_argument("--grid-length", type=float, default=0.16)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--out", default=None, help="write the record here (JSON)")
