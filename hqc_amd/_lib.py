@@ -56,7 +56,7 @@ _SIGS = {
     "hqc_real_to_complex": [_P, _I, _I, _I, _D, _P, _P],
     "hqc_complex_real": [_P, _I, _I, _I, _P, _P],
     "hqc_poisson_mul": [_P, _P, _I, _I, _I, _P, _P],
-    "hqc_xc_blocks": [_I],
+    "hqc_xc_blocks": [_I, _I],
     "hqc_xc_eval": [_P, _P, _I, _I, _I, _I, _I, _P, _P, _P],
     "hqc_real_pair_to_complex": [_P, _I, _I, _I, _D, _P, _P],
     "hqc_grad_G": [_P, _I, _I, _D, _P, _D, _P, _P],
@@ -141,7 +141,7 @@ I32 = torch.int32
 
 TUNE_KEYS = {"vmat_blocks": 0, "vh_blocks": 1, "vmat_parts": 2, "vmat_tpw": 3, "q8_blocks": 4,
              "q8_min_tiles": 5, "fft_plane_threads": 6, "vbig_blocks": 7,
-             "fft_two_buffer": 8, "fft_ip_items": 9, "eigh_recompute": 10}
+             "fft_two_buffer": 8, "fft_ip_items": 9, "eigh_recompute": 10, "xc_blocks": 11}
 
 
 def set_tuning(name, value):
@@ -498,8 +498,10 @@ def poisson_mul(c, VG, saved=None):
           _ptr(saved, None, "saved"), _stream())
 
 
-def xc_blocks(g):
-    return lib().hqc_xc_blocks(g)
+def xc_blocks(B, g):
+    """Per-geometry partial sums of hqc_xc_eval for a batch of B (the part array's
+    middle axis), on this host thread."""
+    return lib().hqc_xc_blocks(int(B), int(g))
 
 
 def xc_eval(rho, vh, g, xc_id, vout, part, vh_paired=False):
@@ -507,6 +509,7 @@ def xc_eval(rho, vh, g, xc_id, vout, part, vh_paired=False):
     vh_paired (the output of a transform of real_pair_to_complex)."""
     B, ld = rho.shape
     assert vh.dtype == torch.complex128 and vh.shape[0] == ((B + 1) // 2 if vh_paired else B)
+    assert part.shape == (B, xc_blocks(B, g), 2) and vout.shape == (B, ld)
     _call("hqc_xc_eval", _ptr(rho, F64, "rho"), _ptr(vh, None, "vh"), int(bool(vh_paired)), B,
           g, ld, int(xc_id), _ptr(vout, F64, "vout"), _ptr(part, F64, "part"), _stream())
 

@@ -138,7 +138,12 @@ extern "C" int hqc_poisson_mul(double* c, const double* VG, int B, int g, int vg
 //   v_xc = eps + rho d eps/d rho   (the reference takes this with jax.grad,
 //   hqc/pbc/solver.py:673).
 // ---------------------------------------------------------------------------
-#define XC_BLOCKS 256
+// workgroups per geometry: enough points per thread to hide the FP64
+// transcendentals' latency at full batches (8192 workgroups in all), at most 256
+// for a few geometries (B = 512: 0.83 against 1.17 ms with 256 per geometry)
+#define XC_TARGET_WGS 8192
+#define XC_MIN_BLOCKS 16
+#define XC_MAX_BLOCKS 256
 
 __device__ __forceinline__ void xc_point(double rho, int xc_id, double& eps, double& v) {
   const double r13 = cbrt(rho);
@@ -212,16 +217,23 @@ __global__ __launch_bounds__(256) void k_xc_eval(const double* __restrict__ rho,
   }
 }
 
-extern "C" int hqc_xc_blocks(int g) {
+static int xc_blocks(int B) {
+  int d = XC_TARGET_WGS / (B > 0 ? B : 1);
+  d = d < XC_MIN_BLOCKS ? XC_MIN_BLOCKS : (d > XC_MAX_BLOCKS ? XC_MAX_BLOCKS : d);
+  const int v = hqc_tuned(HQC_TUNE_XC_BLOCKS, d);
+  return v < 65535 ? v : 65535;
+}
+
+extern "C" int hqc_xc_blocks(int B, int g) {
   (void)g;
-  return XC_BLOCKS;
+  return xc_blocks(B);
 }
 
 extern "C" int hqc_xc_eval(const double* rho, const double* vh, int vh_paired, int B, int g,
                            int ld, int xc_id, double* vout, double* part, void* stream) {
   if (B < 0 || g <= 0 || ld < g * g * g || xc_id < 0 || xc_id > 5) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
-  hipLaunchKernelGGL(k_xc_eval, dim3(XC_BLOCKS, B), dim3(256), 0, (hipStream_t)stream, rho,
+  hipLaunchKernelGGL(k_xc_eval, dim3(xc_blocks(B), B), dim3(256), 0, (hipStream_t)stream, rho,
                      (const double2*)vh, g * g * g, ld, xc_id, vh_paired ? 1 : 0, vout, part);
   HQC_CHECK_LAUNCH();
   return HQC_OK;

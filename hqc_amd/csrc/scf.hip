@@ -573,7 +573,8 @@ struct Run {
         (r = hqc_fft3d_c2c_f64(cb, ng, c.g, +1, st)) ||
         (r = hqc_xc_eval(rho, cb, paired ? 1 : 0, na, c.g, ld, c.xc_id, v, part, st)))
       return r;
-    hipLaunchKernelGGL(k_scf_e2, dim3((na + 255) / 256), dim3(256), 0, st, part, nblk, x->dV, na,
+    hipLaunchKernelGGL(k_scf_e2, dim3((na + 255) / 256), dim3(256), 0, st, part,
+                       hqc_xc_blocks(na, c.g), x->dV, na,
                        E2);
     HQC_CHECK_LAUNCH();
     if (!wantv) return HQC_OK;
@@ -743,7 +744,7 @@ static int scf_core(hqc_scf_ctx* x, const double* xp, int B, double* mo_out, dou
   R.ld = x->ld;
   R.m = c.diis_space;
   R.nn = (long)x->nao * x->nao;
-  R.nblk = hqc_xc_blocks(c.g);
+  R.nblk = hqc_xc_blocks(1, c.g);          // the most partial sums per geometry (any na)
   R.xp = xp;
   if (rocblas_set_stream(x->blas, R.st) != rocblas_status_success) return HQC_E_HIP;
   const long nn = R.nn, ld = R.ld, np = R.np, m = R.m;

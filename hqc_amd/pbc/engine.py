@@ -398,7 +398,7 @@ class Engine:
         vh, rg = self.poisson(rho, save=self.gga or save_rg, geo=geo, paired=paired)
         v = torch.zeros(B, c.ld, dtype=torch.float64, device=c.device)
         if not self.gga:
-            part = torch.empty(B, K.xc_blocks(g), 2, dtype=torch.float64, device=c.device)
+            part = torch.empty(B, K.xc_blocks(B, g), 2, dtype=torch.float64, device=c.device)
             K.xc_eval(rho, vh, g, self.xc_id, v, part, vh_paired=paired)
             return v, part.sum(1), rg
         grad = torch.empty(B, 3, g, g, g, dtype=torch.complex128, device=c.device)

@@ -64,7 +64,9 @@ int hqc_version(void);
 #define HQC_TUNE_EIGH_RECOMPUTE 10   /* 1: Jacobi recomputes pair positions and packed offsets
                                         per use instead of reading the round's table (the
                                         same results; for comparison) */
-#define HQC_TUNE_COUNT 11
+#define HQC_TUNE_XC_BLOCKS 11        /* workgroups per geometry of the LDA xc pass (8192 / B
+                                        within 16..256; the partial sums follow it) */
+#define HQC_TUNE_COUNT 12
 int hqc_set_tuning(int key, int value);
 int hqc_get_tuning(int key);
 
@@ -312,8 +314,9 @@ int hqc_poisson_mul(double* c, const double* VG, int B, int g, int vg_batched, d
  * geometry b = Re (b even) or Im (b odd) of grid b/2);
  * vout (B,ld) = v_H + v_xc (padding left untouched);
  * part (B, nblk, 2) per-block sums of rho*eps_xc and rho*v_H; nblk is
- * returned by hqc_xc_blocks. */
-int hqc_xc_blocks(int g);
+ * returned by hqc_xc_blocks for the same B on the same host thread (it follows
+ * the batch size: 8192 workgroups in all, 16..256 per geometry). */
+int hqc_xc_blocks(int B, int g);
 int hqc_xc_eval(const double* rho, const double* vh, int vh_paired, int B, int g, int ld,
                 int xc_id, double* vout, double* part, void* stream);
 

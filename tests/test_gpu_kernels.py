@@ -513,7 +513,7 @@ def test_xc_eval(K, xc):
     vh.view(1, -1).real[0] = torch.tensor(vh_np, device=DEV)
     xc_id = {"lda,vwn": 1, "lda,": 2, "lda,hl": 3, "lda,bh": 4, "lda,gl": 5}[xc]
     v = torch.zeros(1, ld, dtype=torch.float64, device=DEV)
-    part = torch.empty(1, K.xc_blocks(g), 2, dtype=torch.float64, device=DEV)
+    part = torch.empty(1, K.xc_blocks(1, g), 2, dtype=torch.float64, device=DEV)
     K.xc_eval(rho, vh, g, xc_id, v, part)
     eps, vxc = R.xc_eps_v(rho_np, xc)
     assert np.allclose(v[0, :N].cpu().numpy(), vh_np + vxc, rtol=1e-13, atol=1e-13)

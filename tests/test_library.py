@@ -39,7 +39,7 @@ def test_version_and_size_queries(libpath):
     from hqc_amd import _lib
     assert _lib.lib().hqc_version() >= 100
     # pure host-side queries (no device work)
-    assert _lib.xc_blocks(49) > 0 and _lib.force_blocks(49) > 0
+    assert _lib.xc_blocks(1, 49) == 256 and _lib.xc_blocks(512, 49) == 16 and _lib.force_blocks(49) > 0
     n = _lib.vmat_workspace(2, 112, 117696, 64)
     assert n > 0 and n % (256 * 8) == 0
     with pytest.raises(RuntimeError):
