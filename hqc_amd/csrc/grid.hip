@@ -11,36 +11,91 @@ __device__ __forceinline__ double gcomp(int k, int g, double L) {
 }
 
 // ---------------------------------------------------------------------------
-// v_loc(G) = -VG sum_A exp(-i G.R_A)     (hqc/pbc/solver.py:406-407)
+// Structure factors exp(-i G.R_A) on the odd grid factor into 1-D phases,
+// exp(-i Gx x) exp(-i Gy y) exp(-i Gz z): a workgroup builds the x, y and z
+// phase tables of a chunk of atoms in LDS (3 g sincos per atom instead of one
+// per grid point and atom) and forms each point's phase from two complex
+// products.  Atoms in chunks of at most SF_ATOMS, fewer where the tables (chunk x
+// 3 g x 16 bytes) would pass 64 KB.
 // ---------------------------------------------------------------------------
-__global__ void k_vloc_G(const double* __restrict__ xp, int n, int g, double L,
-                         const double* __restrict__ VG, double2* __restrict__ out) {
-  const int N = g * g * g;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
-  if (r >= N) return;
-  const int kz = r % g, ky = (r / g) % g, kx = r / (g * g);
-  const double Gx = gcomp(kx, g, L), Gy = gcomp(ky, g, L), Gz = gcomp(kz, g, L);
-  const double* x = xp + (long)b * n * 3;
-  double sr = 0.0, si = 0.0;
-  for (int A = 0; A < n; ++A) {
-    const double th = Gx * x[3 * A] + Gy * x[3 * A + 1] + Gz * x[3 * A + 2];
+#define SF_ATOMS 16
+#define SF_PTS 8            // grid points per thread in k_vloc_G
+
+// tab[a][d][k] = exp(i sgn G_d(k) x_{a,d}) for the atoms [a0, a0 + na) of x (n x 3)
+__device__ __forceinline__ void sf_tables(const double* __restrict__ x, int a0, int na, int g,
+                                          double L, double sgn, double2* tab) {
+  for (int t = threadIdx.x; t < na * 3 * g; t += blockDim.x) {
+    const int k = t % g, d = (t / g) % 3, a = t / (3 * g);
     double s, c;
-    sincos(th, &s, &c);
-    sr += c;
-    si -= s;
+    sincos(sgn * gcomp(k, g, L) * x[(a0 + a) * 3 + d], &s, &c);
+    tab[t] = make_double2(c, s);
   }
-  const double vg = VG[r];
-  out[(long)b * N + r] = make_double2(-sr * vg, -si * vg);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// v_loc(G) = -VG sum_A exp(-i G.R_A)     (hqc/pbc/solver.py:406-407)
+// grid (ceil(N / (256 SF_PTS)), B): SF_PTS points per thread, strided by 256
+__global__ __launch_bounds__(256) void k_vloc_G(const double* __restrict__ xp, int n, int g,
+                                                double L, int at, const double* __restrict__ VG,
+                                                double2* __restrict__ out) {
+  extern __shared__ double2 tab[];                // [at][3][g]
+  const int N = g * g * g;
+  const int b = blockIdx.y;
+  const int r0 = blockIdx.x * 256 * SF_PTS + threadIdx.x;
+  const double* x = xp + (long)b * n * 3;
+  double2 acc[SF_PTS];
+  int kx[SF_PTS], ky[SF_PTS], kz[SF_PTS];
+#pragma unroll
+  for (int j = 0; j < SF_PTS; ++j) {
+    acc[j] = make_double2(0.0, 0.0);
+    const int r = min(r0 + 256 * j, N - 1);
+    kz[j] = r % g;
+    ky[j] = (r / g) % g;
+    kx[j] = r / (g * g);
+  }
+  for (int a0 = 0; a0 < n; a0 += at) {
+    const int na = min(at, n - a0);
+    __syncthreads();                              // the previous chunk's tables are read
+    sf_tables(x, a0, na, g, L, -1.0, tab);
+    __syncthreads();
+    for (int a = 0; a < na; ++a) {
+      const double2* ta = tab + a * 3 * g;
+#pragma unroll
+      for (int j = 0; j < SF_PTS; ++j) {
+        const double2 p = cmul(cmul(ta[kx[j]], ta[g + ky[j]]), ta[2 * g + kz[j]]);
+        acc[j].x += p.x;
+        acc[j].y += p.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < SF_PTS; ++j) {
+    const int r = r0 + 256 * j;
+    if (r < N) {
+      const double vg = VG[r];
+      out[(long)b * N + r] = make_double2(-acc[j].x * vg, -acc[j].y * vg);
+    }
+  }
+}
+
+// atoms per table chunk: SF_ATOMS, or fewer so the chunk's tables fit 64 KB
+static int sf_chunk(int g) {
+  const int at = (int)(65536 / (3 * (long)g * sizeof(double2)));
+  return at < SF_ATOMS ? at : SF_ATOMS;
 }
 
 extern "C" int hqc_vloc_G(const double* xp, int B, int n, int g, double L, const double* VG,
                           double* out, void* stream) {
-  if (B < 0 || n <= 0 || g <= 0) return HQC_E_ARG;
+  if (B < 0 || n <= 0 || g <= 0 || sf_chunk(g) < 1) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
   const int N = g * g * g;
-  hipLaunchKernelGGL(k_vloc_G, dim3(hqc_div_up(N, 256), B), dim3(256), 0, (hipStream_t)stream,
-                     xp, n, g, L, VG, (double2*)out);
+  const int at = sf_chunk(g);
+  const size_t lds = (size_t)at * 3 * g * sizeof(double2);
+  hipLaunchKernelGGL(k_vloc_G, dim3(hqc_div_up(N, 256 * SF_PTS), B), dim3(256), lds,
+                     (hipStream_t)stream, xp, n, g, L, at, VG, (double2*)out);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }
@@ -244,44 +299,63 @@ extern "C" int hqc_xc_eval(const double* rho, const double* vh, int vh_paired, i
 //   dE[b,A,c] = Re sum_G i G_c VG e^{-iG.R_A} conj(rho~(G))
 //             = sum_G G_c VG (a sin th + b cos th),  rho~ = a + i b, th = G.R_A
 // ---------------------------------------------------------------------------
+// grid (ceil(n / at), B): a workgroup takes a chunk of atoms over every grid
+// point; with rho~ = a + i b and E_A = exp(+i G.R_A), a sin th + b cos th = Im(rho~ E_A)
 __global__ __launch_bounds__(256) void k_vloc_force(const double* __restrict__ xp, int n, int g,
-                                                    double L, const double* __restrict__ VG,
+                                                    double L, int at,
+                                                    const double* __restrict__ VG,
                                                     const double2* __restrict__ rg,
                                                     double* __restrict__ dE) {
+  extern __shared__ double2 tab[];                // [at][3][g]
   __shared__ double red[4];
-  const int A = blockIdx.x, b = blockIdx.y;
+  const int a0 = blockIdx.x * at, b = blockIdx.y;
+  const int na = min(at, n - a0);
   const int N = g * g * g;
-  const double* x = xp + ((long)b * n + A) * 3;
-  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  sf_tables(xp + (long)b * n * 3, a0, na, g, L, 1.0, tab);
+  __syncthreads();
+  double f[SF_ATOMS][3];
+#pragma unroll
+  for (int a = 0; a < SF_ATOMS; ++a) f[a][0] = f[a][1] = f[a][2] = 0.0;
   for (int r = threadIdx.x; r < N; r += blockDim.x) {
     const int kz = r % g, ky = (r / g) % g, kx = r / (g * g);
-    const double Gx = gcomp(kx, g, L), Gy = gcomp(ky, g, L), Gz = gcomp(kz, g, L);
-    const double th = Gx * x[0] + Gy * x[1] + Gz * x[2];
-    double s, c;
-    sincos(th, &s, &c);
     const double2 z = rg[(long)b * N + r];
-    const double t = VG[r] * (z.x * s + z.y * c);
-    f0 += Gx * t;
-    f1 += Gy * t;
-    f2 += Gz * t;
+    const double vg = VG[r];
+    const double wx = vg * z.x, wy = vg * z.y;
+    const double Gx = gcomp(kx, g, L), Gy = gcomp(ky, g, L), Gz = gcomp(kz, g, L);
+#pragma unroll
+    for (int a = 0; a < SF_ATOMS; ++a) {
+      if (a < na) {
+        const double2* ta = tab + a * 3 * g;
+        const double2 p = cmul(cmul(ta[kx], ta[g + ky]), ta[2 * g + kz]);
+        const double t = wx * p.y + wy * p.x;      // Im(w E_A)
+        f[a][0] += Gx * t;
+        f[a][1] += Gy * t;
+        f[a][2] += Gz * t;
+      }
+    }
   }
-  f0 = block_sum(f0, red);
-  f1 = block_sum(f1, red);
-  f2 = block_sum(f2, red);
-  if (threadIdx.x == 0) {
-    double* o = dE + ((long)b * n + A) * 3;
-    o[0] = f0;
-    o[1] = f1;
-    o[2] = f2;
+  // unrolled with a uniform guard: an index not known at compile time would move f
+  // to scratch memory
+#pragma unroll
+  for (int a = 0; a < SF_ATOMS; ++a) {
+    if (a < na) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double v = block_sum(f[a][c], red);
+        if (threadIdx.x == 0) dE[((long)b * n + a0 + a) * 3 + c] = v;
+      }
+    }
   }
 }
 
 extern "C" int hqc_vloc_force(const double* xp, int B, int n, int g, double L, const double* VG,
                               const double* rhoG, double* dE, void* stream) {
-  if (B < 0 || n <= 0 || g <= 0) return HQC_E_ARG;
+  if (B < 0 || n <= 0 || g <= 0 || sf_chunk(g) < 1) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
-  hipLaunchKernelGGL(k_vloc_force, dim3(n, B), dim3(256), 0, (hipStream_t)stream, xp, n, g, L,
-                     VG, (const double2*)rhoG, dE);
+  const int at = sf_chunk(g);
+  const size_t lds = (size_t)at * 3 * g * sizeof(double2);
+  hipLaunchKernelGGL(k_vloc_force, dim3(hqc_div_up(n, at), B), dim3(256), lds,
+                     (hipStream_t)stream, xp, n, g, L, at, VG, (const double2*)rhoG, dE);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }

@@ -593,6 +593,38 @@ def test_vloc_and_poisson(K):
     assert np.allclose(vl[0, :s.N].cpu().numpy(), R.vloc_real(s, xpb), rtol=1e-11, atol=1e-11)
 
 
+@pytest.mark.parametrize("g,n", [(27, 40), (49, 54), (65, 17)])
+def test_structure_factor_kernels(K, g, n):
+    """v_loc(G) and the local-potential force sums from per-atom 1-D phase tables
+    (several atom chunks, several geometries) against the direct sums
+        v_loc(G) = -VG sum_A exp(-i G.R_A),
+        dE[A, c] = sum_G G_c VG Im(rho~(G) exp(i G.R_A))."""
+    print(f"[structure factors] g={g} n={n}", flush=True)
+    rng = np.random.default_rng(g + n)
+    B, L = 3, 7.3
+    xp = rng.uniform(0, L, (B, n, 3))
+    k = np.fft.fftfreq(g, 1.0 / g)                    # 0..g/2, then -g/2..-1
+    G1 = 2 * np.pi * k / L
+    Gx, Gy, Gz = np.meshgrid(G1, G1, G1, indexing="ij")
+    G2 = Gx ** 2 + Gy ** 2 + Gz ** 2
+    VG = np.where(G2 > 0, 4 * np.pi / np.where(G2 > 0, G2, 1.0) / L ** 3, 0.0)
+    out = torch.empty(B, g, g, g, dtype=torch.complex128, device=DEV)
+    xt = torch.tensor(xp, device=DEV)
+    K.vloc_G(xt, g, L, torch.tensor(VG.ravel(), device=DEV), out)
+    rg = rng.standard_normal((B, g, g, g)) + 1j * rng.standard_normal((B, g, g, g))
+    dE = torch.empty(B, n, 3, dtype=torch.float64, device=DEV)
+    K.vloc_force(xt, g, L, torch.tensor(VG.ravel(), device=DEV),
+                 torch.tensor(rg.reshape(B, -1), device=DEV), dE)
+    for b in range(B):
+        ph = np.exp(-1j * (np.multiply.outer(Gx, xp[b, :, 0]) + np.multiply.outer(Gy, xp[b, :, 1])
+                           + np.multiply.outer(Gz, xp[b, :, 2])))      # (g, g, g, n)
+        ref = -VG * ph.sum(-1)
+        assert np.allclose(out[b].cpu().numpy(), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+        t = (VG[..., None] * rg[b][..., None] * np.conj(ph)).imag        # (g, g, g, n)
+        refF = np.stack([(Gc[..., None] * t).sum((0, 1, 2)) for Gc in (Gx, Gy, Gz)], -1)
+        assert np.allclose(dE[b].cpu().numpy(), refF, rtol=0, atol=1e-11 * np.abs(refF).max())
+
+
 def test_ewald(K):
     from hqc_amd.pbc.potential import ewald_pp
     n, rs = 14, 1.31
