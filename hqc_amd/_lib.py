@@ -22,6 +22,7 @@ _D = ctypes.c_double
 
 _SIGS = {
     "hqc_version": [],
+    "hqc_swap_rows": [_P, _P, _I, _P, _P, _I, _P],
     "hqc_set_tuning": [_I, _I],
     "hqc_get_tuning": [_I],
     "hqc_gto_tables": [_P, _I, _I, _I, _D, _I, _P, _I, _I, _P, _P],
@@ -349,6 +350,32 @@ def ao_vmat_h2(ph, v, scale, dV, V, work, bidx=None, vmax=None):
     _call("hqc_ao_vmat_h2", _ptr(ph, I32, "ph"), _ptr(bi, I32, "bidx"), _ptr(v, F64, "v"),
           _ptr(vmax, F64, "vmax"), B, nao_pad, ld, float(scale), float(dV), _ptr(V, F64, "V"),
           _ptr(work, None, "work"), work.numel() * work.element_size(), _stream())
+
+
+SWAP_MAX_TENSORS = 64
+
+
+def swap_rows(tensors, pairs):
+    """Swap rows pairs[0, k] and pairs[1, k] of every tensor along its leading axis in
+    one launch per SWAP_MAX_TENSORS tensors (hqc_swap_rows).  pairs: a (2, npairs)
+    int64 GPU tensor, the pairs disjoint; tensors: contiguous GPU tensors whose
+    leading axis holds every index named in pairs."""
+    assert pairs.is_cuda and pairs.dtype == torch.int64 and pairs.dim() == 2 \
+        and pairs.shape[0] == 2 and pairs.is_contiguous()
+    npairs = pairs.shape[1]
+    if npairs == 0 or not tensors:
+        return
+    pa = pairs.data_ptr()
+    for i in range(0, len(tensors), SWAP_MAX_TENSORS):
+        part = tensors[i:i + SWAP_MAX_TENSORS]
+        bases = (ctypes.c_void_p * len(part))()
+        rows = (ctypes.c_long * len(part))()
+        for j, t in enumerate(part):
+            bases[j] = _ptr(t, None, "swap_rows tensor").value
+            assert t.dim() and t.numel(), "swap_rows tensor must have rows"
+            rows[j] = t.numel() // t.shape[0] * t.element_size()
+        _call("hqc_swap_rows", bases, rows, len(part), ctypes.c_void_p(pa),
+              ctypes.c_void_p(pa + 8 * npairs), npairs, _stream())
 
 
 def fft3d(c, sign):

@@ -601,3 +601,65 @@ extern "C" int hqc_ewald_recip(const double* frac, int B, int n, const double* G
 }
 
 extern "C" int hqc_version(void) { return 100; }
+
+// ---------------------------------------------------------------------------
+// Row swaps over many tensors in one launch: rows a[k] and b[k] (k < npairs) of
+// every tensor exchange places, a tensor being ntensors entries of (base, bytes
+// per leading-axis row).  The batched SCF moves the geometries still iterating
+// to a prefix of every per-geometry tensor this way: one launch instead of a
+// gather and a scatter per tensor.  The pairs must be disjoint (each row in at
+// most one pair).  grid (npairs, ntensors), 256 threads per row.
+// ---------------------------------------------------------------------------
+struct SwapTable {
+  char* base[HQC_SWAP_MAX_TENSORS];
+  long row_bytes[HQC_SWAP_MAX_TENSORS];
+};
+
+__global__ __launch_bounds__(256) void k_swap_rows(SwapTable tb, const long* __restrict__ a,
+                                                   const long* __restrict__ b) {
+  const int t = blockIdx.y;
+  const long rb = tb.row_bytes[t];
+  char* ra = tb.base[t] + a[blockIdx.x] * rb;
+  char* rbp = tb.base[t] + b[blockIdx.x] * rb;
+  if ((((uintptr_t)tb.base[t] | (uintptr_t)rb) & 15) == 0) {
+    uint4* x = reinterpret_cast<uint4*>(ra);
+    uint4* y = reinterpret_cast<uint4*>(rbp);
+    for (long i = threadIdx.x; i < rb / 16; i += blockDim.x) {
+      const uint4 u = x[i];
+      x[i] = y[i];
+      y[i] = u;
+    }
+  } else if ((((uintptr_t)tb.base[t] | (uintptr_t)rb) & 3) == 0) {
+    unsigned* x = reinterpret_cast<unsigned*>(ra);
+    unsigned* y = reinterpret_cast<unsigned*>(rbp);
+    for (long i = threadIdx.x; i < rb / 4; i += blockDim.x) {
+      const unsigned u = x[i];
+      x[i] = y[i];
+      y[i] = u;
+    }
+  } else {
+    for (long i = threadIdx.x; i < rb; i += blockDim.x) {
+      const char u = ra[i];
+      ra[i] = rbp[i];
+      rbp[i] = u;
+    }
+  }
+}
+
+extern "C" int hqc_swap_rows(void* const* bases, const long* row_bytes, int ntensors,
+                             const long* a, const long* b, int npairs, void* stream) {
+  if (ntensors < 0 || npairs < 0 || (ntensors && (!bases || !row_bytes))) return HQC_E_ARG;
+  if (ntensors > HQC_SWAP_MAX_TENSORS) return HQC_E_SIZE;
+  if (ntensors == 0 || npairs == 0) return HQC_OK;
+  if (!a || !b || npairs > 65535) return HQC_E_ARG;
+  SwapTable tb;
+  for (int t = 0; t < ntensors; ++t) {
+    if (!bases[t] || row_bytes[t] < 0) return HQC_E_ARG;
+    tb.base[t] = static_cast<char*>(bases[t]);
+    tb.row_bytes[t] = row_bytes[t];
+  }
+  hipLaunchKernelGGL(k_swap_rows, dim3(npairs, ntensors), dim3(256), 0, (hipStream_t)stream, tb,
+                     a, b);
+  HQC_CHECK_LAUNCH();
+  return HQC_OK;
+}

@@ -1002,13 +1002,18 @@ class Engine:
         movers = [i for i in range(new_na, na) if keep[i]]
         if holes:
             dev = order.device
-            ia = torch.tensor(holes + movers, dtype=torch.long).pin_memory().to(dev, non_blocking=True)
-            ib = torch.tensor(movers + holes, dtype=torch.long).pin_memory().to(dev, non_blocking=True)
-            for k in keys:
-                st[k][ia] = st[k][ib]
-            for k in loc:
-                loc[k][ia] = loc[k][ib]
-            order[ia] = order[ib]
+            pairs = torch.tensor([holes, movers], dtype=torch.long).pin_memory().to(
+                dev, non_blocking=True)
+            ts = [st[k] for k in keys] + list(loc.values()) + [order]
+            ts = [t for t in ts if t.numel()]
+            flat = [t for t in ts if t.is_contiguous()]
+            assert len({t.data_ptr() for t in flat}) == len(flat), "aliased per-geometry state"
+            K.swap_rows(flat, pairs)                 # one launch for every contiguous tensor
+            rest = [t for t in ts if not t.is_contiguous()]
+            if rest:
+                ia, ib = pairs.reshape(-1), pairs.flip(0).reshape(-1)
+                for t in rest:
+                    t[ia] = t[ib]
         return new_na
 
     def scf(self, st):

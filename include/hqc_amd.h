@@ -41,6 +41,17 @@ extern "C" {
 /* Library version (major*100 + minor). */
 int hqc_version(void);
 
+/* Row swaps over many tensors in one launch (no counterpart in the reference,
+ * whose SCF keeps its whole batch): rows a[k] and b[k] (device int64, k < npairs
+ * <= 65535, pairs disjoint) of each of ntensors tensors trade places; tensor t is
+ * bases[t] (device) with row_bytes[t] bytes per leading-axis row; bases and
+ * row_bytes are host arrays.  The batched SCF compacts the geometries still
+ * iterating to a prefix of its per-geometry state with it.  HQC_E_SIZE beyond
+ * HQC_SWAP_MAX_TENSORS tensors (the caller splits the list). */
+#define HQC_SWAP_MAX_TENSORS 64
+int hqc_swap_rows(void* const* bases, const long* row_bytes, int ntensors, const long* a,
+                  const long* b, int npairs, void* stream);
+
 /* Launch parameters of the calling host thread (no counterpart in the
  * reference, whose XLA programs are tuned by its compiler): set by the project's
  * autotuner (hqc_amd/tune.py) for a workload shape.  0 = built-in default.

@@ -997,3 +997,41 @@ def test_exchange_without_split_equals_split(K, n, basis):
         torch.cuda.synchronize()
         tol = 1e-12 if low is None else 2e-6
         assert (Kf - Ks).abs().max().item() <= tol * Ks.abs().max().item()
+
+
+def test_swap_rows_matches_gather_scatter(K):
+    """hqc_swap_rows against torch's t[ia] = t[ib] with ia = holes + movers and
+    ib = movers + holes, the compaction step of the batched SCF, over tensors of
+    16-, 4- and 1-byte aligned rows and more tensors than one launch takes."""
+    gen = torch.Generator().manual_seed(3)
+    B = 37
+    shapes = [((B, 5, 7), torch.float64), ((B,), torch.float64), ((B, 3), torch.int32),
+              ((B, 5), torch.int8), ((B, 2, 9), torch.complex128), ((B,), torch.bool),
+              ((B, 11), torch.int16), ((B,), torch.long)]
+    ts = []
+    for i in range(70):
+        sh, dt = shapes[i % len(shapes)]
+        x = torch.randint(-100, 100, sh, generator=gen)
+        ts.append(x.to(dt).to(DEV) if dt != torch.bool else (x > 0).to(DEV))
+    keep = (torch.rand(B, generator=gen) > 0.45).tolist()
+    na = sum(keep)
+    holes = [i for i in range(na) if not keep[i]]
+    movers = [i for i in range(na, B) if keep[i]]
+    assert holes and len(holes) == len(movers)
+    ref = [t.clone() for t in ts]
+    ia = torch.tensor(holes + movers, device=DEV)
+    ib = torch.tensor(movers + holes, device=DEV)
+    for t in ref:
+        t[ia] = t[ib]
+    pairs = torch.tensor([holes, movers], dtype=torch.long, device=DEV)
+    K.swap_rows(ts, pairs)
+    torch.cuda.synchronize()
+    for t, r in zip(ts, ref):
+        assert torch.equal(t, r)
+    K.swap_rows(ts, pairs)                         # a swap undone by the same swap
+    K.swap_rows(ts, pairs[:, :0])                  # no pairs: nothing launched
+    torch.cuda.synchronize()
+    for t, r in zip(ts, ref):
+        t2 = r.clone()
+        t2[ia] = t2[ib]
+        assert torch.equal(t, t2)
