@@ -142,7 +142,8 @@ I32 = torch.int32
 
 TUNE_KEYS = {"vmat_blocks": 0, "vh_blocks": 1, "vmat_parts": 2, "vmat_tpw": 3, "q8_blocks": 4,
              "q8_min_tiles": 5, "fft_plane_threads": 6, "vbig_blocks": 7,
-             "fft_two_buffer": 8, "fft_ip_items": 9, "eigh_recompute": 10, "xc_blocks": 11}
+             "fft_two_buffer": 8, "fft_ip_items": 9, "eigh_recompute": 10, "xc_blocks": 11,
+             "force_pointwise": 12, "ao_pointwise": 13}
 
 
 def set_tuning(name, value):

@@ -99,6 +99,71 @@ __global__ __launch_bounds__(256) void k_ao_values(const double* __restrict__ P,
   }
 }
 
+// AO values along grid lines: lane = iz, so each store is one grid line of an AO
+// (a coalesced row segment), each lane keeps its z factors t_z(iz)[a][0..1] in
+// registers for the whole launch, and a line's x and y factors are the same for
+// every lane (scalar loads).  The pointwise form above reads its z factors per
+// point with lanes 8*nl bytes apart, many cache lines per wave load; this form
+// reads them once.  Workgroup = AOV_LNW waves on the lines [AOV_LINES*x,
+// +AOV_LINES) of one atom; ZC = ceil(g / 64) lane chunks; na <= AOV_LMAX_NA.
+#define AOV_LNW 4
+#define AOV_LINES 256
+#define AOV_LMAX_NA 8
+
+template <typename T, int ZC>
+__global__ __launch_bounds__(64 * AOV_LNW) void k_ao_values_lines(
+    const double* __restrict__ P, int n, int g, int na, int nl,
+    const double* __restrict__ coef, const int* __restrict__ comp, int nloc, int nao_pad,
+    int ld, T* __restrict__ phi) {
+  const int A = blockIdx.y, b = blockIdx.z;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long tstride = (long)g * na * nl;
+  const double* Tx = P + (long)(b * n + A) * 3 * tstride;
+  const double* Ty = Tx + tstride;
+  const double* Tz = Ty + tstride;
+  double tz[ZC][AOV_LMAX_NA][2];
+#pragma unroll
+  for (int zc = 0; zc < ZC; ++zc) {
+    const int iz = min(lane + 64 * zc, g - 1);    // lanes past g compute unused values
+#pragma unroll
+    for (int a = 0; a < AOV_LMAX_NA; ++a) {
+      const double* t = Tz + ((long)iz * na + min(a, na - 1)) * nl;
+      tz[zc][a][0] = t[0];
+      tz[zc][a][1] = nl > 1 ? t[1] : 0.0;         // power 1: p functions only
+    }
+  }
+  T* out = phi + ((long)b * nao_pad + (long)A * nloc) * ld;
+  const int l1 = min(g * g, (int)(blockIdx.x + 1) * AOV_LINES);
+  for (int line = blockIdx.x * AOV_LINES + w; line < l1; line += AOV_LNW) {
+    const int ix = line / g, iy = line - ix * g;
+    const double* px_t = Tx + (long)ix * na * nl;
+    const double* py_t = Ty + (long)iy * na * nl;
+    const long r0 = (long)line * g;
+    for (int o = 0; o < nloc; ++o) {
+      const int c = comp[o];
+      const int px = (c == 0), py = (c == 1), pz = (c == 2);
+      double acc[ZC];
+#pragma unroll
+      for (int zc = 0; zc < ZC; ++zc) acc[zc] = 0.0;
+#pragma unroll
+      for (int a = 0; a < AOV_LMAX_NA; ++a) {
+        if (a >= na) break;                          // wave-uniform
+        const double wa = coef[a * nloc + o];
+        if (wa == 0.0) continue;
+        const double cxy = wa * px_t[a * nl + px] * py_t[a * nl + py];
+#pragma unroll
+        for (int zc = 0; zc < ZC; ++zc) acc[zc] += cxy * (pz ? tz[zc][a][1] : tz[zc][a][0]);
+      }
+#pragma unroll
+      for (int zc = 0; zc < ZC; ++zc) {
+        const int iz = lane + 64 * zc;
+        if (iz < g) out[(long)o * ld + r0 + iz] = (T)acc[zc];
+      }
+    }
+  }
+}
+
 template <typename T>
 static int launch_ao_values(const double* P, int B, int n, int g, int na, int nl,
                             const double* coef, const int* comp, int nloc, int nao_pad, int ld,
@@ -108,6 +173,17 @@ static int launch_ao_values(const double* P, int B, int n, int g, int na, int nl
   if (nao_pad < n * nloc || ld < g * g * g) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
   const int N = g * g * g;
+  if (g <= 128 && na <= AOV_LMAX_NA && !hqc_tuned(HQC_TUNE_AO_POINTWISE, 0)) {
+    const dim3 grid(hqc_div_up(g * g, AOV_LINES), n, B);
+    if (g <= 64)
+      hipLaunchKernelGGL((k_ao_values_lines<T, 1>), grid, dim3(64 * AOV_LNW), 0,
+                         (hipStream_t)stream, P, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi);
+    else
+      hipLaunchKernelGGL((k_ao_values_lines<T, 2>), grid, dim3(64 * AOV_LNW), 0,
+                         (hipStream_t)stream, P, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi);
+    HQC_CHECK_LAUNCH();
+    return HQC_OK;
+  }
   hipLaunchKernelGGL(k_ao_values<T>, dim3(hqc_div_up(N, 256 * AOV_PPT), n, B), dim3(256), 0,
                      (hipStream_t)stream, P, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi);
   HQC_CHECK_LAUNCH();
@@ -409,6 +485,183 @@ __global__ __launch_bounds__(256) void k_grid_forces(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Grid adjoint, line form.  Each primitive a of an AO is a product of 1-D
+// factors, so with Z = 2 v X (+ Zx) of the AO's row
+//   sum_r Z(r) d/dx phi(r) = sum_a w_a sum_iz t_z(iz) sum_{ix,iy} t'_x(ix) t_y(iy) Z(ix,iy,iz)
+// (t' the derivative factor; y and z alike).  Lane = iz, so a grid line (ix, iy)
+// is one coalesced wave load; the line's x and y factors are the same for every
+// lane (LDS broadcasts), and each lane keeps three running sums per primitive.
+// The z factors and the reduction over lanes and waves come once at the end.
+// Per point and primitive: three FMAs against the pointwise form's table reads
+// and products in all three directions.
+// Work split: the HQC_FORCE_BLOCKS slots of an atom are C = HQC_FORCE_BLOCKS /
+// nloc contiguous line ranges of each AO row (slot = o C + chunk; slots past
+// nloc C get zeros), and a workgroup's waves take U consecutive lines each in
+// turn: workgroups that run together read neighbouring parts of the same rows.
+// (One workgroup per whole row ran at a third of the copy rate, its thousands of
+// concurrent streams each at another place in HBM.)  ZC = ceil(g / 64) lane
+// chunks; GFL_PCH primitives per pass (one pass for these bases).
+// ---------------------------------------------------------------------------
+#define GFL_NW 4
+#define GFL_PCH 4
+#define GFL_MAXG 128
+
+// MODE bit 0: v (and X) given; bit 1: Zx given.  Every load of a batch of U lines
+// is issued before the first is used (no branch between them: lines past the
+// range and lanes past the line are clamped and masked to zero).
+template <int ZC, int MODE, int U>
+__global__ __launch_bounds__(64 * GFL_NW) void k_grid_forces_lines(
+    const double* __restrict__ P, int n, int g, int na, int nl,
+    const double* __restrict__ alpha, const double* __restrict__ coef,
+    const int* __restrict__ comp, int nloc, int nao_pad, int ld,
+    const double* __restrict__ X, const double* __restrict__ v,
+    const double* __restrict__ Zx, double dV, double* __restrict__ part) {
+  __shared__ double red[4];
+  __shared__ double2 sx[GFL_MAXG][GFL_PCH];       // (value, derivative) factors per ix
+  __shared__ double2 sy[GFL_MAXG][GFL_PCH];       // the same per iy
+  const int slot = blockIdx.x, A = blockIdx.y, b = blockIdx.z;
+  double* out = part + (((long)b * n + A) * gridDim.x + slot) * 3;
+  const int C = gridDim.x / nloc;                 // line ranges per AO
+  const int o = slot / C, chunk = slot - o * C;
+  if (o >= nloc) {                               // unused slots: zero partials
+    if (threadIdx.x < 3) out[threadIdx.x] = 0.0;
+    return;
+  }
+  const int nline = g * g;
+  const int per = (nline + C - 1) / C;
+  const int lb = chunk * per, le = min(nline, lb + per);
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = comp[o];
+  const int px = (c == 0), py = (c == 1), pz = (c == 2);
+  const long tstride = (long)g * na * nl;         // one direction's tables
+  const double* Tx = P + (long)(b * n + A) * 3 * tstride;
+  const double* Ty = Tx + tstride;
+  const double* Tz = Ty + tstride;
+  const long row = (long)b * nao_pad + (long)A * nloc + o;
+  const double* Xr = X + row * ld;
+  const double* Zr = Zx + row * ld;
+  const double* vb = v + (long)b * ld;
+  double f[3] = {0.0, 0.0, 0.0};
+  for (int a0 = 0; a0 < na && lb < le;) {
+    // the next GFL_PCH primitives with a nonzero coefficient in this AO
+    int act[GFL_PCH];
+    double wt[GFL_PCH], m2a[GFL_PCH];
+    int np = 0, a = a0;
+#pragma unroll
+    for (int j = 0; j < GFL_PCH; ++j) {
+      while (a < na && coef[a * nloc + o] == 0.0) ++a;
+      act[j] = a < na ? a : 0;
+      wt[j] = a < na ? coef[a * nloc + o] : 0.0;
+      m2a[j] = -2.0 * alpha[act[j]];
+      if (a < na) {
+        ++np;
+        ++a;
+      }
+    }
+    a0 = a;
+    if (np == 0) break;
+    __syncthreads();                              // the previous chunk's factors are read
+    for (int i = threadIdx.x; i < 2 * g * GFL_PCH; i += blockDim.x) {
+      const int d = i / (g * GFL_PCH), q = i - d * g * GFL_PCH;
+      const int ii = q / GFL_PCH, j = q - ii * GFL_PCH;
+      const int p = d ? py : px;
+      int aj = act[0];
+      double mj = m2a[0];
+#pragma unroll
+      for (int jj = 1; jj < GFL_PCH; ++jj)
+        if (j == jj) {
+          aj = act[jj];
+          mj = m2a[jj];
+        }
+      const double* t = (d ? Ty : Tx) + ((long)ii * na + aj) * nl;
+      const double2 fv = make_double2(t[p], mj * t[p + 1] + (p ? t[0] : 0.0));
+      if (d) sy[ii][j] = fv; else sx[ii][j] = fv;
+    }
+    __syncthreads();
+    double acc[ZC][3][GFL_PCH];
+#pragma unroll
+    for (int zc = 0; zc < ZC; ++zc)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int j = 0; j < GFL_PCH; ++j) acc[zc][k][j] = 0.0;
+    for (int l0 = lb + w * U; l0 < le; l0 += GFL_NW * U) {
+      double z[U][ZC];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long r0 = (long)min(l0 + u, le - 1) * g;
+#pragma unroll
+        for (int zc = 0; zc < ZC; ++zc) {
+          const long r = r0 + min(lane + 64 * zc, g - 1);
+          double zz = 0.0;
+          if (MODE & 1) zz = 2.0 * vb[r] * Xr[r];
+          if (MODE & 2) zz += Zr[r];
+          z[u][zc] = zz;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int l = min(l0 + u, le - 1);
+        const int ix = l / g, iy = l - ix * g;
+        const bool line_ok = l0 + u < le;         // wave-uniform
+#pragma unroll
+        for (int zc = 0; zc < ZC; ++zc)
+          z[u][zc] = (line_ok && lane + 64 * zc < g) ? z[u][zc] : 0.0;
+#pragma unroll
+        for (int j = 0; j < GFL_PCH; ++j) {
+          if (j >= np) continue;                   // wave-uniform
+          const double2 xf = sx[ix][j], yf = sy[iy][j];
+          const double c1 = xf.y * yf.x, c2 = xf.x * yf.y, c3 = xf.x * yf.x;
+#pragma unroll
+          for (int zc = 0; zc < ZC; ++zc) {
+            acc[zc][0][j] = fma(c1, z[u][zc], acc[zc][0][j]);
+            acc[zc][1][j] = fma(c2, z[u][zc], acc[zc][1][j]);
+            acc[zc][2][j] = fma(c3, z[u][zc], acc[zc][2][j]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int zc = 0; zc < ZC; ++zc) {
+      const int iz = lane + 64 * zc;
+      if (iz >= g) continue;
+#pragma unroll
+      for (int j = 0; j < GFL_PCH; ++j) {
+        if (j >= np) continue;
+        const double* t = Tz + ((long)iz * na + act[j]) * nl;
+        const double vz = t[pz];
+        const double dz = m2a[j] * t[pz + 1] + (pz ? t[0] : 0.0);
+        f[0] = fma(wt[j] * vz, acc[zc][0][j], f[0]);
+        f[1] = fma(wt[j] * vz, acc[zc][1][j], f[1]);
+        f[2] = fma(wt[j] * dz, acc[zc][2][j], f[2]);
+      }
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    const double t = block_sum(f[k], red);
+    if (threadIdx.x == 0) out[k] = t * dV;
+  }
+}
+
+template <int ZC, int U>
+static void launch_grid_forces_lines(dim3 grid, hipStream_t s, const double* P, int n, int g,
+                                     int na, int nl, const double* alpha, const double* coef,
+                                     const int* comp, int nloc, int nao_pad, int ld,
+                                     const double* X, const double* v, const double* Zx,
+                                     double dV, double* part) {
+  const int mode = (v ? 1 : 0) | (Zx ? 2 : 0);
+#define GFL_LAUNCH(M)                                                                           \
+  hipLaunchKernelGGL((k_grid_forces_lines<ZC, M, U>), grid, dim3(64 * GFL_NW), 0, s, P, n, g, \
+                     na, nl, alpha, coef, comp, nloc, nao_pad, ld, X, v, Zx, dV, part)
+  if (mode == 1) GFL_LAUNCH(1);
+  else if (mode == 2) GFL_LAUNCH(2);
+  else if (mode == 3) GFL_LAUNCH(3);
+  else GFL_LAUNCH(0);
+#undef GFL_LAUNCH
+}
+
 extern "C" int hqc_force_blocks(int g) {
   (void)g;
   return HQC_FORCE_BLOCKS;
@@ -423,6 +676,19 @@ extern "C" int hqc_grid_forces(const double* P, int B, int n, int g, int na, int
   if (nl < 3) return HQC_E_ARG;   // gradients need l_max + 2 powers (>= 2 for s)
   if (v && !X) return HQC_E_ARG;
   if (B == 0) return HQC_OK;
+  if (nloc > HQC_FORCE_BLOCKS) return HQC_E_SIZE;
+  if (g <= GFL_MAXG && !hqc_tuned(HQC_TUNE_FORCE_POINTWISE, 0)) {
+    // line form (one workgroup per AO; slots o >= nloc get zeros)
+    const dim3 grid(HQC_FORCE_BLOCKS, n, B);
+    if (g <= 64)
+      launch_grid_forces_lines<1, 8>(grid, (hipStream_t)stream, P, n, g, na, nl, alpha, coef,
+                                     comp, nloc, nao_pad, ld, X, v, Zx, dV, part);
+    else
+      launch_grid_forces_lines<2, 4>(grid, (hipStream_t)stream, P, n, g, na, nl, alpha, coef,
+                                     comp, nloc, nao_pad, ld, X, v, Zx, dV, part);
+    HQC_CHECK_LAUNCH();
+    return HQC_OK;
+  }
   const size_t tab_bytes = (size_t)3 * g * na * nl * sizeof(double);
   const int tab_lds = tab_bytes <= 48 * 1024;
   hipLaunchKernelGGL(k_grid_forces, dim3(HQC_FORCE_BLOCKS, n, B), dim3(256),

@@ -18,7 +18,9 @@ extern "C" int hqc_set_tuning(int key, int value) {
       value != 1024)
     return HQC_E_ARG;
   if ((key == HQC_TUNE_FFT_TWO_BUFFER && value > 1) || (key == HQC_TUNE_FFT_IP_ITEMS && value > 2) ||
-      (key == HQC_TUNE_EIGH_RECOMPUTE && value > 1))
+      (key == HQC_TUNE_EIGH_RECOMPUTE && value > 1) ||
+      (key == HQC_TUNE_FORCE_POINTWISE && value > 1) ||
+      (key == HQC_TUNE_AO_POINTWISE && value > 1))
     return HQC_E_ARG;
   g_tune[key] = value;
   return HQC_OK;

@@ -77,7 +77,13 @@ int hqc_swap_rows(void* const* bases, const long* row_bytes, int ntensors, const
                                         same results; for comparison) */
 #define HQC_TUNE_XC_BLOCKS 11        /* workgroups per geometry of the LDA xc pass (8192 / B
                                         within 16..256; the partial sums follow it) */
-#define HQC_TUNE_COUNT 12
+#define HQC_TUNE_FORCE_POINTWISE 12  /* 1: the grid adjoint of the forces point by point
+                                        (table reads per point) instead of along grid
+                                        lines (the same sums in another order; for
+                                        comparison) */
+#define HQC_TUNE_AO_POINTWISE 13     /* 1: AO values point by point instead of along grid
+                                        lines (the same values; for comparison) */
+#define HQC_TUNE_COUNT 14
 int hqc_set_tuning(int key, int value);
 int hqc_get_tuning(int key);
 
@@ -422,7 +428,11 @@ int hqc_ewald_recip(const double* frac, int B, int n, const double* Gv, const do
  * grad phi is recomputed from tables P built with nl = l_max + 2 powers and at
  * least 3 (nl = 3 for s and p bases; HQC_E_ARG below 3); alpha (na) are the
  * primitive exponents.
- * part (B, n, nblk, 3), nblk = hqc_force_blocks(g). */
+ * part (B, n, nblk, 3), nblk = hqc_force_blocks(g): partial sums over disjoint
+ * parts of the work, F = their sum over the block axis (for g <= 128: block
+ * o*C + k, C = nblk / nloc, holds AO o's sum over the k-th of C ranges of grid
+ * lines, zero past nloc*C; HQC_TUNE_FORCE_POINTWISE or g > 128: blocks of grid
+ * points).  HQC_E_SIZE for nloc > nblk. */
 int hqc_force_blocks(int g);
 int hqc_grid_forces(const double* P, int B, int n, int g, int na, int nl, const double* alpha,
                     const double* coef, const int* comp, int nloc, int nao_pad, int ld,

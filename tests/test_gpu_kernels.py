@@ -384,6 +384,40 @@ def test_gto_tables_and_values(K):
     assert np.allclose(phi[0, :s.n_ao, :s.N].cpu().numpy(), ref, rtol=1e-13, atol=1e-14)
 
 
+@pytest.mark.parametrize("basis,grid_length", [("gth-szv", 0.3), ("gth-dzv", 0.2),
+                                               ("gth-dzvp", 0.3), ("gth-dzvp", 0.052)])
+def test_ao_values_line_form(K, basis, grid_length):
+    """AO values along grid lines (lane = z index) against the pointwise form, FP64
+    and FP32, for s-only tables (one power) and p functions, on grids of one and
+    two lane chunks (25..65+ points per line), at three geometries."""
+    s, xpb, L = _sys(n=3, basis=basis, grid_length=grid_length)
+    print(f"[ao values lines] {basis} g={s.g}", flush=True)
+    b = s.basis
+    nl = b.l_max + 1
+    rng = np.random.default_rng(s.g)
+    xp = torch.tensor(np.stack([xpb, rng.uniform(0, L, xpb.shape), rng.uniform(0, L, xpb.shape)]),
+                      device=DEV)
+    P = torch.empty(3, s.n, 3, s.g, len(b.alpha), nl, dtype=torch.float64, device=DEV)
+    K.gto_tables(xp, s.g, s.L, int(24 // s.L), torch.tensor(b.alpha, device=DEV), nl, P)
+    nao_pad = (s.n_ao + 15) // 16 * 16
+    ld = (s.N + 255) // 256 * 256
+    coef = torch.tensor(b.coef, device=DEV).contiguous()
+    comp = torch.tensor(b.comp, dtype=torch.int32, device=DEV)
+    for dt in (torch.float64, torch.float32):
+        out = []
+        for pointwise in (0, 1):
+            K.set_tuning("ao_pointwise", pointwise)
+            phi = torch.full((3, nao_pad, ld), 7.0, dtype=dt, device=DEV)
+            K.ao_values(P, coef, comp, nao_pad, ld, phi)
+            out.append(phi)
+        K.set_tuning("ao_pointwise", 0)
+        lines, pw = out[0][:, :s.n_ao, :s.N], out[1][:, :s.n_ao, :s.N]
+        tol = 1e-14 if dt == torch.float64 else 1e-6
+        assert (lines - pw).abs().max().item() <= tol * pw.abs().max().item()
+        # rows and columns past the AOs and the grid are left as they were
+        assert bool((out[0][:, s.n_ao:] == 7).all()) and bool((out[0][:, :, s.N:] == 7).all())
+
+
 def test_overlap_kinetic(K):
     for basis in ("gth-szv", "gth-dzv", "gth-dzvp"):
         s, xpb, L = _sys(n=3, basis=basis)
@@ -1035,3 +1069,63 @@ def test_swap_rows_matches_gather_scatter(K):
         t2 = r.clone()
         t2[ia] = t2[ib]
         assert torch.equal(t, t2)
+
+
+@pytest.mark.parametrize("g,n,basis,zx", [(25, 3, "gth-dzv", False), (49, 2, "gth-dzvp", True),
+                                          (65, 2, "gth-dzvp", False), (33, 2, "gth-szv", True)])
+def test_grid_forces_line_form(K, g, n, basis, zx):
+    """The grid adjoint along grid lines (one workgroup per AO, lane = z index) and
+    the pointwise form against the direct sum
+        F[b,A,c] = dV sum_r sum_{p in A} Z_p(r) d phi_p/d r_c,  Z = 2 v X + Zx,
+    grad phi from the kernels' own table rule, on random tables and operands
+    (65 points per line: two lane chunks; p functions in gth-dzvp)."""
+    from hqc_amd.basis import HBasis
+    print(f"[grid forces] g={g} n={n} {basis} zx={zx}", flush=True)
+    hb = HBasis(basis)
+    alpha, coef, comp = hb.alpha, np.asarray(hb.coef), np.asarray(hb.comp)
+    na, nloc = len(alpha), len(comp)
+    nl, B, dV = 3, 2, 0.37
+    N = g ** 3
+    ld = (N + 255) // 256 * 256
+    nao_pad = (n * nloc + 15) // 16 * 16
+    rng = np.random.default_rng(g + n)
+    P = rng.standard_normal((B, n, 3, g, na, nl))
+    X = rng.standard_normal((B, nao_pad, ld))
+    v = rng.standard_normal((B, ld))
+    Zx = rng.standard_normal((B, nao_pad, ld)) if zx else None
+    t = lambda a: None if a is None else torch.tensor(a, device=DEV)
+    args = (t(P), t(alpha), t(coef), torch.tensor(comp, dtype=torch.int32, device=DEV),
+            nao_pad, ld, t(X), t(v), t(Zx), dV)
+    nblk = K.force_blocks(g)
+    out = {}
+    for pointwise in (0, 1):
+        K.set_tuning("force_pointwise", pointwise)
+        part = torch.full((B, n, nblk, 3), float("nan"), dtype=torch.float64, device=DEV)
+        K.grid_forces(*args, part)
+        out[pointwise] = part.sum(2).cpu().numpy()
+    K.set_tuning("force_pointwise", 0)
+    ref = np.zeros((B, n, 3))
+    for b in range(B):
+        for A in range(n):
+            for o in range(nloc):
+                Z = 2 * v[b, :N] * X[b, A * nloc + o, :N]
+                if zx:
+                    Z = Z + Zx[b, A * nloc + o, :N]
+                Z = Z.reshape(g, g, g)
+                for a in range(na):
+                    w = coef[a, o]
+                    if w == 0:
+                        continue
+                    val, der = [], []
+                    for d in range(3):
+                        pw = int(comp[o] == d)
+                        tt = P[b, A, d, :, a]
+                        val.append(tt[:, pw])
+                        der.append(-2 * alpha[a] * tt[:, pw + 1] + (tt[:, 0] if pw else 0))
+                    for c in range(3):
+                        f = [der[d] if d == c else val[d] for d in range(3)]
+                        ref[b, A, c] += w * np.einsum("i,j,k,ijk->", f[0], f[1], f[2], Z) * dV
+    scale = np.abs(ref).max()
+    for pointwise in (0, 1):
+        assert np.isfinite(out[pointwise]).all()
+        assert np.abs(out[pointwise] - ref).max() <= 1e-12 * scale, pointwise
