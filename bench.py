@@ -594,9 +594,7 @@ def main():
     kind = vmat_kind(pes)
     ms_v, fl_v = time_vmat(pes, B)
     q8 = a.path == "mixed" and stats.get("rhoq8", 0) + stats.get("rhoq8_top4", 0) > 0
-    ms_d, fl_d, keep, by_d, ops_d = time_orb_density(pes, B, q8)
     n_vmat = stats["v64"] if a.path == "fp64" else stats["v32"]
-    n_dens64 = stats["rhoq8"] if q8 else stats["rho64"]
     nsteps = max(a.steps + a.warmup, 1)
     c = eng.c
     if kind == "h2":
@@ -614,37 +612,36 @@ def main():
     vk.update({"ms_per_launch": ms_v, "flops_per_launch": fl_v,
                "calls_per_step": n_vmat / nsteps, "traffic": measured_traffic(vname, B),
                "mfma_util_pct": measured_mfma_util(vname, B)})
+    cand = [vk]
     if q8:
         # bound by the int8 MFMA pipe (16 busy cycles per 16x16x64 instruction, busy
         # ~65% of the time at the clock the load allows), not by its stream of int8
         # digits (4 bytes per phi value), which is reported beside it; units of the
-        # MFMA roof: tera int8 operations per second
-        dk = {"kernel": "k_orb_density_q8 (sum_i f_i (C^T phi)_i^2 from int8 digits, "
-                        "exact int32 products to level 5; int8 MFMA ops)", "bound": "mfma",
-              "unit": "TFLOP/s", "ops_per_launch": ops_d, "peak": MI355X_PEAK["i8"],
-              "bytes_per_launch": by_d,
-              "hbm_gbs": by_d / (ms_d * 1e-3) / 1e9,
-              "hbm_frac": by_d / (ms_d * 1e-3) / 1e9 / MI355X_HBM_GBS,
-              "traffic": measured_traffic("k_orb_density_q8<4, 2, 5>", B),
-              "mfma_util_pct": measured_mfma_util("k_orb_density_q8<4, 2, 5>", B)}
+        # MFMA roof: tera int8 operations per second.  One entry per digit-product
+        # level the SCF ran (level 4 by default; level 5 below the engine's switch).
+        for top, key in ((4, "rhoq8_top4"), (5, "rhoq8")):
+            if not stats.get(key, 0):
+                continue
+            ms_d, fl_d, keep, by_d, ops_d = time_orb_density(pes, B, True, top=top)
+            kname = f"k_orb_density_q8<4, 2, {top}>"
+            cand.append({
+                "kernel": "k_orb_density_q8 (sum_i f_i (C^T phi)_i^2 from int8 digits, exact "
+                          f"int32 products to level {top}; int8 MFMA ops)",
+                "bound": "mfma", "unit": "TFLOP/s", "ops_per_launch": ops_d,
+                "peak": MI355X_PEAK["i8"], "bytes_per_launch": by_d,
+                "hbm_gbs": by_d / (ms_d * 1e-3) / 1e9,
+                "hbm_frac": by_d / (ms_d * 1e-3) / 1e9 / MI355X_HBM_GBS,
+                "traffic": measured_traffic(kname, B),
+                "mfma_util_pct": measured_mfma_util(kname, B),
+                "ms_per_launch": ms_d, "flops_per_launch": fl_d, "orbitals": keep,
+                "calls_per_step": stats[key] / nsteps})
     else:
-        dk = {"kernel": "k_orb_density_valu (sum_i f_i (C^T phi)_i^2, FP64 vector FMA)",
-              "bound": "mfma", "unit": "TFLOP/s", "peak": MI355X_PEAK["f64"],
-              "traffic": measured_traffic("k_orb_density_valu", B)}
-    dk.update({"ms_per_launch": ms_d, "flops_per_launch": fl_d, "orbitals": keep,
-               "calls_per_step": n_dens64 / nsteps})
-    cand = [vk, dk]
-    if q8 and stats.get("rhoq8_top4", 0):
-        # the same kernel with digit products to level 4 (the early DIIS iterations)
-        ms4, fl4, _, by4, ops4 = time_orb_density(pes, B, True, top=4)
-        d4 = dict(dk, kernel="k_orb_density_q8 (the same, digit products to level 4; int8 "
-                             "MFMA ops)", ops_per_launch=ops4, ms_per_launch=ms4,
-                  flops_per_launch=fl4, hbm_gbs=by4 / (ms4 * 1e-3) / 1e9,
-                  hbm_frac=by4 / (ms4 * 1e-3) / 1e9 / MI355X_HBM_GBS,
-                  calls_per_step=stats["rhoq8_top4"] / nsteps,
-                  mfma_util_pct=measured_mfma_util("k_orb_density_q8<4, 2, 4>", B),
-                  traffic=measured_traffic("k_orb_density_q8<4, 2, 4>", B))
-        cand.append(d4)
+        ms_d, fl_d, keep, by_d, ops_d = time_orb_density(pes, B, False)
+        cand.append({"kernel": "k_orb_density_valu (sum_i f_i (C^T phi)_i^2, FP64 vector FMA)",
+                     "bound": "mfma", "unit": "TFLOP/s", "peak": MI355X_PEAK["f64"],
+                     "traffic": measured_traffic("k_orb_density_valu", B),
+                     "ms_per_launch": ms_d, "flops_per_launch": fl_d, "orbitals": keep,
+                     "calls_per_step": stats["rho64"] / nsteps})
     for k in cand:
         if k["unit"] == "GB/s":
             k["achieved"] = k["bytes_per_launch"] / (k["ms_per_launch"] * 1e-3) / 1e9

@@ -119,7 +119,7 @@ def autotune(engine, B, reps=5, seed=0):
                                    dtype=torch.float64).to(dev) - 0.5
         rho = torch.empty(B, c.ld, dtype=torch.float64, device=dev)
         t = _sweep({"q8_blocks": [1024, 2048, 4096], "q8_min_tiles": [4, 8, 16]},
-                   lambda: K.orb_density_q8(q, pe, At, rho, ns), reps)
+                   lambda: K.orb_density_q8(q, pe, At, rho, ns, top=4), reps)
         rec["timings"]["orb_density_q8"] = t
         rec["choice"].update(json.loads(min(t, key=t.get)))
         del q, pe
