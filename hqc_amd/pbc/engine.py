@@ -911,6 +911,22 @@ class Engine:
             self.stats["v64"] += 1
         return V2
 
+    def _final_grid_pass(self, st, B, D):
+        """E_H + E_xc of the final density for a DFT solve followed by forces: one FP64
+        D-form pass gives rho and the force adjoint's X = D phi together, and the
+        potential, rho~(G) and X are kept in st['force_grid'] for the force pass (the
+        same density as the orbital form, summed in another order)."""
+        c = self.c
+        X = torch.empty(B, c.nao_pad, c.ld, dtype=torch.float64, device=c.device)
+        rho = torch.empty(B, c.ld, dtype=torch.float64, device=c.device)
+        K.ao_density(st["phi"], self._pad(D), rho, X, bidx=st["gidx"])
+        self.stats["rho64"] += 1
+        v, E, rg = self.xc_potential(rho, save_rg=True, geo=self._geo(st, B))
+        st["force_grid"] = (X, v, rg)
+        dVs = self._dv(st, B)
+        Exc, EH = E[:, 0] * dVs, 0.5 * E[:, 1] * dVs
+        return EH + Exc
+
     def fock2e(self, st, na, D, mo, occ, full=False, need_v=True):
         """(J + V_xc or J - K/2, E_H + E_xc or E_H + E_x) for geometries [0, na).
         need_v=False (FP64 DFT only): the energies alone, V is None."""
@@ -1173,7 +1189,10 @@ class Engine:
         # the Fock matrix of the final density is needed for HF (exchange enters the
         # residual checks) and for the forces of a cut basis; DFT energies need rho only
         need_v = not self.dft or "lindep" in st
-        V2, E2 = self.fock2e(st, B, D, mo, occ, full=True, need_v=need_v)
+        if forces and not need_v and st.get("kpt") is None:
+            V2, E2 = None, self._final_grid_pass(st, B, D)
+        else:
+            V2, E2 = self.fock2e(st, B, D, mo, occ, full=True, need_v=need_v)
         Ki = _tr(st["T"], D).real
         Vep = _tr(st["V"], D).real
         Se = self.entropy(occ)

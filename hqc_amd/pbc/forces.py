@@ -198,12 +198,16 @@ def compute_forces(engine, st, kappa=10, Gmax=15):
     t = 2.0 * ((W * other).unsqueeze(-1) * dS - (D * other).unsqueeze(-1) * dT).sum(2)
     # AOs are atom-major: a fixed-order sum per atom (index_add_ would sum with atomics)
     F = t.reshape(B, c.n, c.nloc, 3).sum(2)
-    # grid adjoint (FP64 phi regardless of the SCF precision)
-    # the D-form kernel writes every row and grid column of X and rho
-    X = torch.empty(B, c.nao_pad, c.ld, dtype=f64, device=dev)
-    rho = torch.empty(B, c.ld, dtype=f64, device=dev)
-    K.ao_density(st["phi"], engine._pad(D), rho, X, bidx=st["gidx"])
-    v, _, rg = engine.xc_potential(rho, save_rg=True, geo=engine._geo(st, B))
+    # grid adjoint (FP64 phi regardless of the SCF precision): X = D phi, v and rho~(G)
+    # of the final density, from the solve's last pass when it made them (DFT)
+    if "force_grid" in st:
+        X, v, rg = st.pop("force_grid")
+    else:
+        # the D-form kernel writes every row and grid column of X and rho
+        X = torch.empty(B, c.nao_pad, c.ld, dtype=f64, device=dev)
+        rho = torch.empty(B, c.ld, dtype=f64, device=dev)
+        K.ao_density(st["phi"], engine._pad(D), rho, X, bidx=st["gidx"])
+        v, _, rg = engine.xc_potential(rho, save_rg=True, geo=engine._geo(st, B))
     veff = (v + st["vloc"]).contiguous()
     Zx = None if engine.dft else _exchange_adjoint(engine, st)
     Epp = torch.empty(B, dtype=f64, device=dev)
