@@ -28,6 +28,7 @@ _SIGS = {
     "hqc_gto_tables": [_P, _I, _I, _I, _D, _I, _P, _I, _I, _P, _P],
     "hqc_ao_values_f64": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
     "hqc_ao_values_f32": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P],
+    "hqc_ao_values_f64_amax": [_P, _I, _I, _I, _I, _I, _P, _P, _I, _I, _I, _P, _P, _P],
     "hqc_ao_density_f64": [_P, _P, _P, _I, _I, _I, _P, _P, _P],
     "hqc_ao_density_f32": [_P, _P, _P, _I, _I, _I, _P, _P],
     "hqc_orb_density_f64": [_P, _P, _P, _I, _I, _I, _I, _P, _P],
@@ -42,6 +43,7 @@ _SIGS = {
     "hqc_ao_vmat_h2_workspace": [_I, _I, _I, ctypes.POINTER(ctypes.c_size_t)],
     "hqc_ao_vmat_h2": [_P, _P, _P, _P, _I, _I, _I, _D, _D, _P, _P, ctypes.c_size_t, _P],
     "hqc_phi_slice_i8": [_P, _I, _I, _I, _I, _P, _P, _P, _P],
+    "hqc_phi_slice_i8_h2": [_P, _I, _I, _I, _I, _D, _P, _P, _P, _P],
     "hqc_mixed_v_prep": [_P, _P, _P, _I, _I, _P, _P, _P, _P],
     "hqc_orb_density_q8": [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P],
     "hqc_fft3d_c2c_f64": [_P, _I, _I, _I, _P],
@@ -163,10 +165,19 @@ def gto_tables(xp, g, L, tmax, alpha, nl, out):
           _ptr(alpha, F64), na, nl, _ptr(out, F64, "P"), _stream())
 
 
-def ao_values(P, coef, comp, nao_pad, ld, out):
+def ao_values(P, coef, comp, nao_pad, ld, out, amax=None):
+    """AO values into out (B, nao_pad, ld), FP64 or FP32; amax (B,) FP64 (FP64 out
+    only), if given, is zero-filled here and receives max |phi| per geometry."""
     B, n, _, g, na, nl = P.shape
     nloc = coef.shape[1]
     assert out.shape == (B, nao_pad, ld) and coef.shape[0] == na
+    if amax is not None:
+        assert out.dtype == F64 and amax.dtype == F64 and amax.shape == (B,)
+        amax.zero_()
+        _call("hqc_ao_values_f64_amax", _ptr(P, F64, "P"), B, n, g, na, nl, _ptr(coef, F64),
+              _ptr(comp, I32), nloc, nao_pad, ld, _ptr(out, F64, "phi"),
+              _ptr(amax, F64, "amax"), _stream())
+        return
     name = "hqc_ao_values_f64" if out.dtype == F64 else "hqc_ao_values_f32"
     _call(name, _ptr(P, F64, "P"), B, n, g, na, nl, _ptr(coef, F64), _ptr(comp, I32), nloc,
           nao_pad, ld, _ptr(out, None, "phi"), _stream())
@@ -289,6 +300,18 @@ def phi_split_f16(phi, scale, out):
 
 
 Q8_MAX_NAO_PAD = 128
+
+
+def phi_slice_i8_h2(phi, nslice, scale, q, pe, h2):
+    """phi_slice_i8's digits and exponents and, from the same read of phi, h2 (an
+    int32 view of the FP16 pairs phi_split_f16(phi, scale) gives)."""
+    G, nao_pad, ld = phi.shape
+    assert q.dtype == torch.int8 and q.shape == (G, ld * nslice * nao_pad)
+    assert pe.dtype == torch.int16 and pe.shape == (G, ld)
+    assert h2.dtype == I32 and h2.shape == phi.shape
+    _call("hqc_phi_slice_i8_h2", _ptr(phi, F64, "phi"), G, nao_pad, ld, nslice, float(scale),
+          _ptr(q, torch.int8, "q"), _ptr(pe, torch.int16, "pe"), _ptr(h2, I32, "h2"),
+          _stream())
 
 
 def phi_slice_i8(phi, nslice, q, pe, amax=None):

@@ -36,6 +36,22 @@ __device__ __forceinline__ float wave_sum_f(float v) {
   return v;
 }
 
+// Block-wide maximum, as block_sum (result valid in thread 0).
+__device__ __forceinline__ double block_max(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    t = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) t = fmax(t, red[i]);
+  }
+  __syncthreads();
+  return t;
+}
+
 // Block-wide sum (blockDim.x a multiple of 64, at most 1024).  Result valid in
 // thread 0.  `red` must hold blockDim.x/64 doubles of shared memory.
 __device__ __forceinline__ double block_sum(double v, double* red) {

@@ -69,12 +69,22 @@ __device__ __forceinline__ void q8_digits(double y, int (&dg)[N]) {
 // caller) receives max |phi| of each geometry: the workgroup's largest point
 // maximum enters by one 64-bit atomic max on its bit pattern (non-negative
 // doubles order as their bits), which saves the caller a separate pass over phi.
+// H2: the same pass also writes the FP16 (hi, lo) pairs of scale*phi in phi's
+// layout, the values hqc_phi_split_f16 gives (one read of phi for both forms).
 // ---------------------------------------------------------------------------
-template <int SP>
+__device__ __forceinline__ unsigned f16_pair_bits(double x) {
+  const _Float16 hi = (_Float16)x;
+  const _Float16 lo = (_Float16)(x - (double)hi);
+  return (unsigned)__builtin_bit_cast(unsigned short, hi) |
+         ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
+}
+
+template <int SP, bool H2>
 __global__ __launch_bounds__(256) void k_phi_slice_i8(const double* __restrict__ phi, int nao_pad,
                                                       int ld, signed char* __restrict__ q,
                                                       short* __restrict__ pe,
-                                                      unsigned long long* __restrict__ amax) {
+                                                      unsigned long long* __restrict__ amax,
+                                                      double scale, unsigned* __restrict__ h2) {
   __shared__ double red[4][64];
   const int NB = nao_pad >> 4;
   const int g = blockIdx.y;
@@ -90,6 +100,16 @@ __global__ __launch_bounds__(256) void k_phi_slice_i8(const double* __restrict__
     for (int i = 0; i < 16; ++i) {
       v[t][i] = mb < NB ? P[(long)(16 * mb + i) * ld] : 0.0;
       m = fmax(m, fabs(v[t][i]));
+    }
+  }
+  if (H2) {
+    unsigned* Hp = h2 + (long)g * nao_pad * ld + p;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int mb = w + 4 * t;
+      if (mb >= NB) continue;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) Hp[(long)(16 * mb + i) * ld] = f16_pair_bits(v[t][i] * scale);
     }
   }
   red[w][lane] = m;
@@ -293,23 +313,47 @@ static size_t q8_lds_bytes(int KS, int T) {
   return (size_t)(T + 1) * 64 * (64 * KS + 16) + 64 * sizeof(double) + 256 * sizeof(double);
 }
 
-extern "C" int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, int nslice,
-                                signed char* q, short* pe, double* amax, void* stream) {
+static int launch_phi_slice(const double* phi, int G, int nao_pad, int ld, int nslice,
+                            signed char* q, short* pe, double* amax, double scale, unsigned* h2,
+                            void* stream) {
   if (G < 0 || nao_pad <= 0 || (nao_pad & 15) || ld <= 0 || (ld & 63) || !phi || !q || !pe)
     return HQC_E_ARG;
+  if (h2 && !(scale > 0.0)) return HQC_E_ARG;
   if (nao_pad > 64 * Q8_MAXKS) return HQC_E_SIZE;
   if (G == 0) return HQC_OK;
   const dim3 grid(ld / 64, G);
   hipStream_t s = (hipStream_t)stream;
   unsigned long long* am = reinterpret_cast<unsigned long long*>(amax);
+#define SLICE_CASE(SP)                                                                        \
+  case SP:                                                                                    \
+    if (h2)                                                                                   \
+      hipLaunchKernelGGL((k_phi_slice_i8<SP, true>), grid, dim3(256), 0, s, phi, nao_pad, ld, \
+                         q, pe, am, scale, h2);                                               \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_phi_slice_i8<SP, false>), grid, dim3(256), 0, s, phi, nao_pad,    \
+                         ld, q, pe, am, scale, h2);                                           \
+    break;
   switch (nslice) {
-    case 3: hipLaunchKernelGGL(k_phi_slice_i8<3>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe, am); break;
-    case 4: hipLaunchKernelGGL(k_phi_slice_i8<4>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe, am); break;
-    case 5: hipLaunchKernelGGL(k_phi_slice_i8<5>, grid, dim3(256), 0, s, phi, nao_pad, ld, q, pe, am); break;
+    SLICE_CASE(3)
+    SLICE_CASE(4)
+    SLICE_CASE(5)
     default: return HQC_E_ARG;
   }
+#undef SLICE_CASE
   HQC_CHECK_LAUNCH();
   return HQC_OK;
+}
+
+extern "C" int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, int nslice,
+                                signed char* q, short* pe, double* amax, void* stream) {
+  return launch_phi_slice(phi, G, nao_pad, ld, nslice, q, pe, amax, 1.0, nullptr, stream);
+}
+
+extern "C" int hqc_phi_slice_i8_h2(const double* phi, int G, int nao_pad, int ld, int nslice,
+                                   double scale, signed char* q, short* pe, unsigned* h2,
+                                   void* stream) {
+  if (!h2) return HQC_E_ARG;
+  return launch_phi_slice(phi, G, nao_pad, ld, nslice, q, pe, nullptr, scale, h2, stream);
 }
 
 extern "C" int hqc_orb_density_q8(const signed char* q, const short* pe, const int* bidx,

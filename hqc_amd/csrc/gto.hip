@@ -66,9 +66,12 @@ __global__ __launch_bounds__(256) void k_ao_values(const double* __restrict__ P,
                                                    int na, int nl,
                                                    const double* __restrict__ coef,
                                                    const int* __restrict__ comp, int nloc,
-                                                   int nao_pad, int ld, T* __restrict__ phi) {
+                                                   int nao_pad, int ld, T* __restrict__ phi,
+                                                   unsigned long long* __restrict__ amax) {
   __shared__ double s_coef[HQC_MAX_NA * HQC_MAX_NLOC];
   __shared__ int s_comp[HQC_MAX_NLOC];
+  __shared__ double red[4];
+  double mx = 0.0;
   for (int i = threadIdx.x; i < na * nloc; i += blockDim.x) s_coef[i] = coef[i];
   for (int i = threadIdx.x; i < nloc; i += blockDim.x) s_comp[i] = comp[i];
   __syncthreads();
@@ -95,7 +98,12 @@ __global__ __launch_bounds__(256) void k_ao_values(const double* __restrict__ P,
         if (w != 0.0) acc += w * Px[a * nl + px] * Py[a * nl + py] * Pz[a * nl + pz];
       }
       out[(long)o * ld] = (T)acc;
+      mx = fmax(mx, fabs(acc));
     }
+  }
+  if (amax) {
+    const double m = block_max(mx, red);
+    if (threadIdx.x == 0) atomicMax(amax + b, (unsigned long long)__double_as_longlong(m));
   }
 }
 
@@ -114,9 +122,11 @@ template <typename T, int ZC>
 __global__ __launch_bounds__(64 * AOV_LNW) void k_ao_values_lines(
     const double* __restrict__ P, int n, int g, int na, int nl,
     const double* __restrict__ coef, const int* __restrict__ comp, int nloc, int nao_pad,
-    int ld, T* __restrict__ phi) {
+    int ld, T* __restrict__ phi, unsigned long long* __restrict__ amax) {
+  __shared__ double red[AOV_LNW];
   const int A = blockIdx.y, b = blockIdx.z;
   const int lane = threadIdx.x & 63;
+  double mx = 0.0;                                   // max |phi| of this thread's values
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long tstride = (long)g * na * nl;
   const double* Tx = P + (long)(b * n + A) * 3 * tstride;
@@ -158,16 +168,26 @@ __global__ __launch_bounds__(64 * AOV_LNW) void k_ao_values_lines(
 #pragma unroll
       for (int zc = 0; zc < ZC; ++zc) {
         const int iz = lane + 64 * zc;
-        if (iz < g) out[(long)o * ld + r0 + iz] = (T)acc[zc];
+        if (iz < g) {
+          out[(long)o * ld + r0 + iz] = (T)acc[zc];
+          mx = fmax(mx, fabs(acc[zc]));
+        }
       }
     }
+  }
+  if (amax) {
+    // the workgroup's maximum by one 64-bit atomic max on its bit pattern
+    // (non-negative doubles order as their bits)
+    const double m = block_max(mx, red);
+    if (threadIdx.x == 0) atomicMax(amax + b, (unsigned long long)__double_as_longlong(m));
   }
 }
 
 template <typename T>
 static int launch_ao_values(const double* P, int B, int n, int g, int na, int nl,
                             const double* coef, const int* comp, int nloc, int nao_pad, int ld,
-                            T* phi, void* stream) {
+                            T* phi, void* stream, double* amax_d = nullptr) {
+  unsigned long long* amax = reinterpret_cast<unsigned long long*>(amax_d);
   if (B < 0 || n <= 0 || g <= 0 || na <= 0 || nl <= 0 || nloc <= 0) return HQC_E_ARG;
   if (na > HQC_MAX_NA || nloc > HQC_MAX_NLOC) return HQC_E_SIZE;
   if (nao_pad < n * nloc || ld < g * g * g) return HQC_E_ARG;
@@ -177,15 +197,18 @@ static int launch_ao_values(const double* P, int B, int n, int g, int na, int nl
     const dim3 grid(hqc_div_up(g * g, AOV_LINES), n, B);
     if (g <= 64)
       hipLaunchKernelGGL((k_ao_values_lines<T, 1>), grid, dim3(64 * AOV_LNW), 0,
-                         (hipStream_t)stream, P, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi);
+                         (hipStream_t)stream, P, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi,
+                         amax);
     else
       hipLaunchKernelGGL((k_ao_values_lines<T, 2>), grid, dim3(64 * AOV_LNW), 0,
-                         (hipStream_t)stream, P, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi);
+                         (hipStream_t)stream, P, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi,
+                         amax);
     HQC_CHECK_LAUNCH();
     return HQC_OK;
   }
   hipLaunchKernelGGL(k_ao_values<T>, dim3(hqc_div_up(N, 256 * AOV_PPT), n, B), dim3(256), 0,
-                     (hipStream_t)stream, P, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi);
+                     (hipStream_t)stream, P, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi,
+                     amax);
   HQC_CHECK_LAUNCH();
   return HQC_OK;
 }
@@ -195,6 +218,14 @@ extern "C" int hqc_ao_values_f64(const double* P, int B, int n, int g, int na, i
                                  int ld, double* phi, void* stream) {
   return launch_ao_values<double>(P, B, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi,
                                   stream);
+}
+
+extern "C" int hqc_ao_values_f64_amax(const double* P, int B, int n, int g, int na, int nl,
+                                      const double* coef, const int* comp, int nloc, int nao_pad,
+                                      int ld, double* phi, double* amax, void* stream) {
+  if (!amax) return HQC_E_ARG;
+  return launch_ao_values<double>(P, B, n, g, na, nl, coef, comp, nloc, nao_pad, ld, phi,
+                                  stream, amax);
 }
 
 extern "C" int hqc_ao_values_f32(const double* P, int B, int n, int g, int na, int nl,

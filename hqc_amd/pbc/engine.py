@@ -510,6 +510,10 @@ class Engine:
         # FP32 AO values: the FP32 kernels' operand, unless FP16 pairs serve them all
         need32 = mixed and (not use_h2 or self.MIXED_RHO_INCREMENTS)
         phi32 = ao_buffer(B, torch.float32) if need32 else None
+        # int8 digits and FP16 pairs cut in one read of phi: the AO-value pass gives
+        # max |phi| per geometry first (the pairs' scale)
+        q8 = mixed and self.MIXED_Q8 and c.nao_pad <= K.Q8_MAX_NAO_PAD
+        amax_g = torch.empty(B, dtype=f64, device=dev) if q8 and use_h2 else None
         vl = torch.zeros(B, c.ld, dtype=f64, device=dev)
         for cst, idx in groups:
             x = self._sub(st["xpb"], idx)
@@ -531,8 +535,13 @@ class Engine:
             P = torch.empty(nb, c.n, 3, c.g, c.na, c.nl, dtype=f64, device=dev)
             K.gto_tables(x, c.g, cst.L, cst.tmax_img, c.alpha, c.nl, P)
             pg = phi if idx is None else ao_buffer(nb, f64)
-            K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, pg)
+            ag = None
+            if amax_g is not None:
+                ag = amax_g if idx is None else torch.empty(nb, dtype=f64, device=dev)
+            K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, pg, amax=ag)
             self._put(phi, idx, pg)
+            if ag is not None:
+                self._put(amax_g, idx, ag)
             if need32:
                 pg32 = phi32 if idx is None else ao_buffer(nb, torch.float32)
                 K.ao_values(P, c.coef, c.comp, c.nao_pad, c.ld, pg32)
@@ -556,16 +565,6 @@ class Engine:
             if need32:
                 st["phi_lp"] = phi32
                 phi_keys.append("phi_lp")
-            amax_g = None
-            if self.MIXED_Q8 and c.nao_pad <= K.Q8_MAX_NAO_PAD:
-                # int8 digits of phi for the accurate density (exact integer products);
-                # the same pass returns max |phi| per geometry for the FP16 scale
-                ns = self.Q8_SLICES
-                st["phi_q8"] = torch.empty(B, c.ld * ns * c.nao_pad, dtype=torch.int8, device=dev)
-                st["phi_e8"] = torch.empty(B, c.ld, dtype=torch.int16, device=dev)
-                amax_g = torch.empty(B, dtype=f64, device=dev)
-                K.phi_slice_i8(phi, ns, st["phi_q8"], st["phi_e8"], amax_g)
-                phi_keys += ["phi_q8", "phi_e8"]
             if use_h2:
                 # FP16 (hi, lo) pairs of s*phi, s = 2^k with |s phi| <= 2^14
                 am = amax_g.max() if amax_g is not None else \
@@ -573,8 +572,20 @@ class Engine:
                 amax = float(am.item())
                 self._phi_scale = 2.0 ** math.floor(math.log2(2.0 ** 14 / max(amax, 1e-300)))
                 st["phi_h2"] = torch.empty(B, c.nao_pad, c.ld, dtype=torch.int32, device=dev)
-                K.phi_split_f16(phi, self._phi_scale, st["phi_h2"])
                 phi_keys.append("phi_h2")
+            if q8:
+                # int8 digits of phi for the accurate density (exact integer products)
+                ns = self.Q8_SLICES
+                st["phi_q8"] = torch.empty(B, c.ld * ns * c.nao_pad, dtype=torch.int8, device=dev)
+                st["phi_e8"] = torch.empty(B, c.ld, dtype=torch.int16, device=dev)
+                if use_h2:
+                    K.phi_slice_i8_h2(phi, ns, self._phi_scale, st["phi_q8"], st["phi_e8"],
+                                      st["phi_h2"])
+                else:
+                    K.phi_slice_i8(phi, ns, st["phi_q8"], st["phi_e8"])
+                phi_keys += ["phi_q8", "phi_e8"]
+            elif use_h2:
+                K.phi_split_f16(phi, self._phi_scale, st["phi_h2"])
             if self.MIXED_RHO_INCREMENTS:
                 st["phisq"] = (phi[:, :c.nao, :c.N] ** 2).sum(1)         # sum_p phi_p(r)^2
                 st["Da"] = torch.zeros(B, c.nao, c.nao, dtype=f64, device=dev)

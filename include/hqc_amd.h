@@ -107,6 +107,13 @@ int hqc_ao_values_f64(const double* P, int B, int n, int g, int na, int nl,
 int hqc_ao_values_f32(const double* P, int B, int n, int g, int na, int nl,
                       const double* coef, const int* comp, int nloc, int nao_pad,
                       int ld, float* phi, void* stream);
+/* hqc_ao_values_f64 that also gives amax[b] = max |phi[b]| over the values it
+ * writes (amax (B) zero-filled by the caller; the padding it leaves is zero in a
+ * zero-filled phi).  The mixed set-up takes the FP16 pair scale from it before
+ * its one pass over phi (hqc_phi_slice_i8_h2). */
+int hqc_ao_values_f64_amax(const double* P, int B, int n, int g, int na, int nl,
+                           const double* coef, const int* comp, int nloc, int nao_pad,
+                           int ld, double* phi, double* amax, void* stream);
 
 /* k-point tables and AO values (hqc/pbc/gto.py:371-420,
  * eval_pbc_gaussian_power_x_kpt / eval_pbc_ao_kpt):
@@ -190,6 +197,11 @@ int hqc_orb_density_h2(const unsigned* ph, const int* bidx, const double* At, in
  * over each geometry's nao_pad x ld values, exactly, from the same pass. */
 int hqc_phi_slice_i8(const double* phi, int G, int nao_pad, int ld, int nslice,
                      signed char* q, short* pe, double* amax, void* stream);
+/* hqc_phi_slice_i8 (without amax) that in the same read of phi also writes h2
+ * (G,nao_pad,ld): the FP16 pairs of scale*phi, the values hqc_phi_split_f16 gives
+ * (scale = 2^k, |scale phi| <= 2^14). */
+int hqc_phi_slice_i8_h2(const double* phi, int G, int nao_pad, int ld, int nslice,
+                        double scale, signed char* q, short* pe, unsigned* h2, void* stream);
 int hqc_orb_density_q8(const signed char* q, const short* pe, const int* bidx,
                        const double* At, int B, int nao_pad, int m_pad, int ld, int nslice,
                        int nuse, int top, double* rho, void* stream);

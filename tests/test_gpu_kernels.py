@@ -416,6 +416,40 @@ def test_ao_values_line_form(K, basis, grid_length):
         assert (lines - pw).abs().max().item() <= tol * pw.abs().max().item()
         # rows and columns past the AOs and the grid are left as they were
         assert bool((out[0][:, s.n_ao:] == 7).all()) and bool((out[0][:, :, s.N:] == 7).all())
+    # max |phi| per geometry from the AO-value pass, exactly, both forms
+    for pointwise in (0, 1):
+        K.set_tuning("ao_pointwise", pointwise)
+        phi = torch.zeros(3, nao_pad, ld, dtype=torch.float64, device=DEV)
+        amax = torch.full((3,), 5.0, dtype=torch.float64, device=DEV)
+        K.ao_values(P, coef, comp, nao_pad, ld, phi, amax=amax)
+        assert torch.equal(amax, phi.abs().amax((1, 2))), pointwise
+    K.set_tuning("ao_pointwise", 0)
+
+
+@pytest.mark.parametrize("ns", [3, 4, 5])
+def test_phi_slice_with_pairs(K, ns):
+    """hqc_phi_slice_i8_h2 writes the digits and exponents of hqc_phi_slice_i8 and
+    the FP16 pairs of hqc_phi_split_f16 from one read of phi, bit for bit (values
+    of both signs over eight binades, a zero row, a column of one large value)."""
+    print(f"[slice + pairs] ns={ns}", flush=True)
+    gen = torch.Generator().manual_seed(ns)
+    G, nao_pad, ld = 3, 112, 128 * 5
+    phi = (torch.rand(G, nao_pad, ld, generator=gen, dtype=torch.float64) - 0.5) * \
+        torch.exp2(-8 * torch.rand(G, nao_pad, ld, generator=gen, dtype=torch.float64))
+    phi[:, 7] = 0.0
+    phi[1, :, 33] = 0.0
+    phi[1, 5, 33] = 3.75
+    phi = phi.to(DEV)
+    scale = 2.0 ** 12
+    q = torch.empty(G, ld * ns * nao_pad, dtype=torch.int8, device=DEV)
+    pe = torch.empty(G, ld, dtype=torch.int16, device=DEV)
+    K.phi_slice_i8(phi, ns, q, pe)
+    h2 = torch.empty(G, nao_pad, ld, dtype=torch.int32, device=DEV)
+    K.phi_split_f16(phi, scale, h2)
+    q2, pe2 = torch.full_like(q, 9), torch.full_like(pe, 9)
+    h22 = torch.full_like(h2, 9)
+    K.phi_slice_i8_h2(phi, ns, scale, q2, pe2, h22)
+    assert torch.equal(q, q2) and torch.equal(pe, pe2) and torch.equal(h2, h22)
 
 
 def test_overlap_kinetic(K):
