@@ -189,9 +189,12 @@ class Engine:
     MIXED_Q8 = True
     Q8_SLICES = 4
     MIXED_Q8_TOP4_ERR = 0.0
-    # the warm-up density from the first 3 digits and level 4 (2^-22 per value, a
-    # fixed basis with no residual floor; 0: the split-FP16 form)
+    # the warm-up density (DIIS residual >= MIXED_RHO_DIRECT_ERR) from the first 3
+    # digits (2^-22 per value, a fixed basis with no residual floor; 0: the
+    # split-FP16 form) and digit products to level 3 (9 products; ~2^-30 per AO of
+    # coefficient noise, five orders below such a residual)
     Q8_WARMUP_DIGITS = 3
+    Q8_WARMUP_TOP = 3
     mat_dtype = torch.float64        # n_ao x n_ao matrices (complex128 at a k-point)
     # Hartree-Fock in the mixed mode: the exchange pair densities, their transforms
     # (complex64) and the contraction in FP32 while the DIIS residual is >= this, in
@@ -838,7 +841,7 @@ class Engine:
             occ_d = occ if idx is None else occ[idx.long()]
             if "phi_q8" in st and self.Q8_WARMUP_DIGITS:
                 r = self.orb_density_q8(st, mo_d, occ_d, self._gix(st, na, idx), keep,
-                                        self.Q8_WARMUP_DIGITS, top=4)
+                                        self.Q8_WARMUP_DIGITS, top=self.Q8_WARMUP_TOP)
             else:
                 lp = "phi_h2" if "phi_h2" in st else "phi_lp"
                 r = self.orb_density(st[lp], mo_d, occ_d, self._gix(st, na, idx), keep)

@@ -280,6 +280,18 @@ def test_orb_density_q8(K, nao_pad, m, ns):
             K.orb_density_q8(q, pe, At, rho3, ns, nuse=3, top=top)
             assert np.allclose(rho3.cpu().numpy(), q8_density_digits(ph, A, 3, top, ns),
                                rtol=1e-13, atol=1e-300), top
+    # level 3 (9 digit products) from the first 3 stored digits, the early SCF
+    # iterations' form: the host restatement and the error bound with 3 digits
+    rho3 = torch.full((B, ld), float("nan"), dtype=torch.float64, device=DEV)
+    K.orb_density_q8(q, pe, At, rho3, ns, nuse=3, top=3)
+    r3 = rho3.cpu().numpy()
+    assert np.allclose(r3, q8_density_digits(ph, A, 3, 3, ns), rtol=1e-13, atol=1e-300)
+    d = (nao_pad * np.abs(A).max(1)[:, :, None] * np.abs(ph).max(1)[:, None, :]
+         * (2.0 ** (-6 - 8 * 2) + 3 * 2.0 ** (-3 - 8 * 3)))
+    assert (np.abs(r3 - rref) <= ((2 * np.abs(psi) + d) * d).sum(1)).all()
+    if ns > 3:
+        with pytest.raises(RuntimeError):               # level 3 takes 3 phi digits only
+            K.orb_density_q8(q, pe, At, rho3, ns, nuse=4, top=3)
 
 
 @pytest.mark.parametrize("g", [1, 2, 5, 7, 13, 17, 24, 25, 27, 31, 33, 48, 49, 51, 55, 57, 61,
