@@ -195,6 +195,16 @@ def measured_mfma_util(kernel_prefix, B):
     return None
 
 
+def _converged_diffs(E, F, w, E2, F2, w2, both):
+    """Largest differences between two runs over the geometries both converged."""
+    from hqc_amd.pbc.engine import RY
+    if not bool(both.any()):
+        return {}
+    return {"max_dE_per_atom_Ha_converged": (E - E2)[both].abs().max().item() / RY / N_ATOMS,
+            "max_dF_Ha_per_Bohr_converged": (F - F2)[both].abs().max().item(),
+            "max_dbands_Ha_converged": (w - w2)[both].abs().max().item()}
+
+
 def run_path(path, x, steps, warmup):
     """(pes, seconds per call of energy_and_forces(x), E, F)."""
     import torch
@@ -593,7 +603,8 @@ def main():
     # dominant kernels: potential matrix (FP64 or FP32) and FP64 orbital density
     kind = vmat_kind(pes)
     ms_v, fl_v = time_vmat(pes, B)
-    q8 = a.path == "mixed" and stats.get("rhoq8", 0) + stats.get("rhoq8_top4", 0) > 0
+    q8 = a.path == "mixed" and sum(stats.get(k, 0) for k in ("rhoq8", "rhoq8_top4",
+                                                             "rhoq8_top3")) > 0
     n_vmat = stats["v64"] if a.path == "fp64" else stats["v32"]
     nsteps = max(a.steps + a.warmup, 1)
     c = eng.c
@@ -618,8 +629,9 @@ def main():
         # ~65% of the time at the clock the load allows), not by its stream of int8
         # digits (4 bytes per phi value), which is reported beside it; units of the
         # MFMA roof: tera int8 operations per second.  One entry per digit-product
-        # level the SCF ran (level 4 by default; level 5 below the engine's switch).
-        for top, key in ((4, "rhoq8_top4"), (5, "rhoq8")):
+        # level the SCF ran (level 3 while the DIIS residual is >= 1e-6, level 4 after,
+        # by default).
+        for top, key in ((3, "rhoq8_top3"), (4, "rhoq8_top4"), (5, "rhoq8")):
             if not stats.get(key, 0):
                 continue
             ms_d, fl_d, keep, by_d, ops_d = time_orb_density(pes, B, True, top=top)
@@ -651,6 +663,7 @@ def main():
         k["frac"] = k["achieved"] / k["peak"]
     dom = max(cand, key=lambda k: k["ms_per_launch"] * k["calls_per_step"])
     w_main = st["w1"].clone()
+    conv_main = st["conv"].bool().clone()
     del pes, st, eng
     torch.cuda.empty_cache()
     out = {
@@ -698,6 +711,11 @@ def main():
             "max_dE_per_atom_Ha": (E - E2).abs().max().item() / RY / N_ATOMS,
             "max_dF_Ha_per_Bohr": (F - F2).abs().max().item(),
             "max_dbands_Ha": (w_a - w_b).abs().max().item(),
+            # the same over the geometries both runs converged (a geometry left at
+            # max_cycle stops wherever its last iteration was, in either run)
+            "converged_both": int((conv_main & pes2.solver.last["conv"].bool()).sum().item()),
+            **_converged_diffs(E, F, w_a, E2, F2, w_b,
+                               conv_main & pes2.solver.last["conv"].bool()),
             "n_iter_mean_other": pes2.solver.last["nit"].double().mean().item(),
             "all_converged_other": bool(pes2.solver.last["conv"].all().item())}
         del pes2

@@ -334,7 +334,7 @@ def orb_density_q8(q, pe, At, rho, nslice, bidx=None, nuse=None, top=5):
     """rho (B, ld) = sum_i (At[:, :, i]^T phi)^2 from the int8 digits of phi_slice_i8
     (exact integer accumulation; bidx selects the geometries of q and pe; nuse of the
     nslice stored digits are used, all by default; digit products up to level top,
-    4 or 5, or 3 with nuse = 3)."""
+    4 or 5, or 3 with nuse 3 or 4)."""
     B, nao_pad, m_pad = At.shape
     ld = rho.shape[1]
     assert rho.shape == (B, ld) and q.shape[1] == ld * nslice * nao_pad and pe.shape[1] == ld

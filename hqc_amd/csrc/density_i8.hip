@@ -361,7 +361,7 @@ extern "C" int hqc_orb_density_q8(const signed char* q, const short* pe, const i
                                   int nslice, int nuse, int top, double* rho, void* stream) {
   if (B < 0 || nao_pad <= 0 || (nao_pad & 15) || m_pad <= 0 || (m_pad & 15) || ld <= 0 ||
       (ld & 63) || nslice < 3 || nslice > 5 || nuse < 3 || nuse > nslice || top < 3 ||
-      top > Q8_TMAX || (top == 3 && nuse != 3))
+      top > Q8_TMAX || (top == 3 && nuse > 4))
     return HQC_E_ARG;
   if (nao_pad > 64 * Q8_MAXKS) return HQC_E_SIZE;
   if (B == 0) return HQC_OK;
@@ -385,7 +385,7 @@ extern "C" int hqc_orb_density_q8(const signed char* q, const short* pe, const i
     hipLaunchKernelGGL((k_orb_density_q8<SP_, KS_, T_>), grid, dim3(64 * Q8_NW), lds, s, q, pe, \
                        bidx, At, nao_pad, m_pad, ld, nslice, rho);                             \
   }
-  Q8_CASE(3, 1, 3) Q8_CASE(3, 2, 3)
+  Q8_CASE(3, 1, 3) Q8_CASE(3, 2, 3) Q8_CASE(4, 1, 3) Q8_CASE(4, 2, 3)
   Q8_CASE(3, 1, 4) Q8_CASE(3, 2, 4) Q8_CASE(4, 1, 4) Q8_CASE(4, 2, 4) Q8_CASE(5, 1, 4)
   Q8_CASE(5, 2, 4) Q8_CASE(3, 1, 5) Q8_CASE(3, 2, 5) Q8_CASE(4, 1, 5) Q8_CASE(4, 2, 5)
   Q8_CASE(5, 1, 5) Q8_CASE(5, 2, 5)

@@ -179,16 +179,19 @@ class Engine:
     # past the warm-up the density comes from base-256 int8 digits of phi with exact
     # integer products (n_ao <= 128; FP64 vector FMAs beyond).  4 digits: phi is fixed
     # to 2^-30 of its per-point maximum (a basis perturbation that is the same on
-    # every call).  Digit products up to level 4 (14 of the 16 pairs) add ~2^-38 per
-    # AO of coefficient noise, ~1.5e-11 of rho: a Fock-matrix noise ~1e-11 Ha, three
-    # orders below the SCF tolerance, and ~5e-10 Ha in the energy, so no residual
-    # floor appears and level 4 serves to convergence (measured at 512 geometries of
-    # 54 H: the same iteration counts and the same differences to FP64 and to the
-    # oracle as level 5, 18 pairs, ~2^-46, at 14/18 of its MFMA time).  Level 5 is
-    # taken only below a DIIS residual of MIXED_Q8_TOP4_ERR (0: never).
+    # every call).  Digit products up to level Q8_TOP = 4 (14 of the 16 pairs) add
+    # ~2^-38 per AO of coefficient noise, ~1.5e-11 of rho: a Fock-matrix noise ~1e-11
+    # Ha, three orders below the SCF tolerance, and ~5e-10 Ha in the energy, so no
+    # residual floor appears and level 4 serves to convergence (measured at 512
+    # geometries of 54 H: the same iteration counts and the same differences to FP64
+    # and to the oracle as level 5, 18 pairs, ~2^-46, at 14/18 of its MFMA time).
+    # While the DIIS residual is >= MIXED_Q8_COARSE_ERR the products stop at level
+    # Q8_TOP_COARSE = 3 (10 pairs, ~2^-30 per AO: three orders below such a residual)
     MIXED_Q8 = True
     Q8_SLICES = 4
-    MIXED_Q8_TOP4_ERR = 0.0
+    Q8_TOP = 4
+    Q8_TOP_COARSE = 3
+    MIXED_Q8_COARSE_ERR = 1e-6
     # the warm-up density (DIIS residual >= MIXED_RHO_DIRECT_ERR) from the first 3
     # digits (2^-22 per value, a fixed basis with no residual floor; 0: the
     # split-FP16 form) and digit products to level 3 (9 products; ~2^-30 per AO of
@@ -257,7 +260,8 @@ class Engine:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
         # kernel launches: FP64 / FP32 densities and potential matrices
-        self.stats = {"rho64": 0, "rhoq8": 0, "rhoq8_top4": 0, "rho32": 0, "v64": 0, "v32": 0,
+        self.stats = {"rho64": 0, "rhoq8": 0, "rhoq8_top4": 0, "rhoq8_top3": 0, "rho32": 0,
+                      "v64": 0, "v32": 0,
                       "x64": 0, "x32": 0}
         self.sweeps = torch.zeros((), dtype=torch.long, device=cell.device)
         self.eigh_calls = 0
@@ -825,10 +829,11 @@ class Engine:
             # the classes (low precision; accurate with digit products to level 4, to
             # level 5) and the occupied-orbital count in one round trip
             kc = (occ > self.OCC_CUT).sum(-1).amax()
-            top4 = err >= self.MIXED_Q8_TOP4_ERR if "phi_q8" in st else torch.zeros_like(direct)
-            cls = torch.where(direct, 0, torch.where(top4, 1, 2))
-            ((idx, n), sub4, sub5), (keep,) = self._classes(cls, 3, (kc,))
-            accurate = ((sub4, 4), (sub5, 5))
+            coarse = err >= self.MIXED_Q8_COARSE_ERR if "phi_q8" in st else \
+                torch.zeros_like(direct)
+            cls = torch.where(direct, 0, torch.where(coarse, 1, 2))
+            ((idx, n), sub_c, sub_f), (keep,) = self._classes(cls, 3, (kc,))
+            accurate = ((sub_c, self.Q8_TOP_COARSE), (sub_f, self.Q8_TOP))
         else:
             full = ~(direct | inc)
             idx, n = self._subset(direct)
@@ -863,7 +868,7 @@ class Engine:
             gi = self._gix(st, na, idx)
             if "phi_q8" in st:
                 r64 = self.orb_density_q8(st, mo[il], occ[il], gi, keep, top=top)
-                self.stats["rhoq8" if top == 5 else "rhoq8_top4"] += 1
+                self.stats[{3: "rhoq8_top3", 4: "rhoq8_top4"}.get(top, "rhoq8")] += 1
             else:
                 r64 = self.orb_density(st["phi"], mo[il], occ[il], gi, keep)
                 self.stats["rho64"] += 1

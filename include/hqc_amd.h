@@ -185,7 +185,7 @@ int hqc_orb_density_h2(const unsigned* ph, const int* bidx, const double* At, in
  * [tile][digit][AO block of 16][point][AO], pe (G,ld) the exponents.
  * hqc_orb_density_q8 cuts At per call into top + 1 digits per orbital
  * (per-orbital exponent), forms every digit product of level <= top (4 or 5;
- * 3 with nuse = 3 only, 9 products, for coarse early iterations)
+ * 3 with nuse 3 or 4, 9 or 10 products, for the coarser SCF iterations)
  * on v_mfma_i32_16x16x64_i8 with int32 accumulation (exact), combines the
  * levels in FP64 and sums psi^2 in FP64.  Error: the fixed phi rounding,
  * 2^(1-6-8(nuse-1)) of max_k |phi_k(r)| per value (a perturbation of the basis

@@ -290,8 +290,16 @@ def test_orb_density_q8(K, nao_pad, m, ns):
          * (2.0 ** (-6 - 8 * 2) + 3 * 2.0 ** (-3 - 8 * 3)))
     assert (np.abs(r3 - rref) <= ((2 * np.abs(psi) + d) * d).sum(1)).all()
     if ns > 3:
-        with pytest.raises(RuntimeError):               # level 3 takes 3 phi digits only
-            K.orb_density_q8(q, pe, At, rho3, ns, nuse=4, top=3)
+        # level 3 from 4 digits (10 products): the coarse accurate form
+        K.orb_density_q8(q, pe, At, rho3, ns, nuse=4, top=3)
+        r4 = rho3.cpu().numpy()
+        assert np.allclose(r4, q8_density_digits(ph, A, 4, 3, ns), rtol=1e-13, atol=1e-300)
+        d = (nao_pad * np.abs(A).max(1)[:, :, None] * np.abs(ph).max(1)[:, None, :]
+             * (2.0 ** (-6 - 8 * 3) + 4 * 2.0 ** (-3 - 8 * 3)))
+        assert (np.abs(r4 - rref) <= ((2 * np.abs(psi) + d) * d).sum(1)).all()
+    if ns > 4:
+        with pytest.raises(RuntimeError):               # level 3 takes 3 or 4 phi digits
+            K.orb_density_q8(q, pe, At, rho3, ns, nuse=5, top=3)
 
 
 @pytest.mark.parametrize("g", [1, 2, 5, 7, 13, 17, 24, 25, 27, 31, 33, 48, 49, 51, 55, 57, 61,

@@ -111,18 +111,18 @@ def test_batched_equals_single():
         assert abs(pes(xps[b]).item() - Eb[b]) < 1e-10
 
 
-@pytest.mark.parametrize("rho_inc,split16,q8,level5_err",
-                         [(False, False, False, 0.0), (True, False, False, 0.0),
-                          (False, True, False, 0.0), (False, True, True, 0.0),
-                          (False, True, True, 1e-6)],
+@pytest.mark.parametrize("rho_inc,split16,q8,fine",
+                         [(False, False, False, 4), (True, False, False, 4),
+                          (False, True, False, 4), (False, True, True, 4),
+                          (False, True, True, 5)],
                          ids=["v_fp32", "rho_increments32", "v_split16", "v_split16_rho_q8",
                               "v_split16_rho_q8_level5"])
-def test_mixed_precision_matches_fp64(rho_inc, split16, q8, level5_err):
+def test_mixed_precision_matches_fp64(rho_inc, split16, q8, fine):
     """Defect-corrected low-precision increments converge to the FP64 answer within
     the accuracy bar (dE <= 1 uHa/atom, dF <= 1e-5 Ha/Bohr, dbands <= 1e-6 Ha), with
     the density past the warm-up in FP64 orbital form, from int8 digits (q8: digit
-    products to level 4 throughout, or to level 5 below a DIIS residual of 1e-6), or
-    as FP32 D-form increments."""
+    products to level 3 while the DIIS residual is >= 1e-6, to level `fine` (4 or 5)
+    after), or as FP32 D-form increments."""
     from hqc_amd.pbc import make_pes
     n, rs = 16, 1.25
     L = (4 / 3 * np.pi * n) ** (1 / 3)
@@ -133,7 +133,7 @@ def test_mixed_precision_matches_fp64(rho_inc, split16, q8, level5_err):
     pmx.solver.engine.MIXED_RHO_INCREMENTS = rho_inc
     pmx.solver.engine.MIXED_SPLIT16 = split16
     pmx.solver.engine.MIXED_Q8 = q8
-    pmx.solver.engine.MIXED_Q8_TOP4_ERR = level5_err
+    pmx.solver.engine.Q8_TOP = fine
     E64, F64 = p64.energy_and_forces(xps)
     Emx, Fmx = pmx.energy_and_forces(xps)
     assert bool(pmx.solver.last["conv"].all())
@@ -142,7 +142,9 @@ def test_mixed_precision_matches_fp64(rho_inc, split16, q8, level5_err):
     assert (p64.solver.last["w1"] - pmx.solver.last["w1"]).abs().max().item() <= 1e-6
     assert pmx.solver.engine.stats["v32"] > 0 and pmx.solver.engine.stats["rho32"] > 0
     st = pmx.solver.engine.stats
-    assert (st["rhoq8_top4"] > 0) == q8 and (st["rhoq8"] > 0) == (q8 and level5_err > 0)
+    assert (st["rhoq8_top3"] > 0) == q8
+    assert (st["rhoq8_top4"] > 0) == (q8 and fine == 4)
+    assert (st["rhoq8"] > 0) == (q8 and fine == 5)
     assert pmx.solver.last is not None
 
 
