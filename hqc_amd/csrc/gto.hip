@@ -571,9 +571,9 @@ __global__ __launch_bounds__(64 * GFL_NW) void k_grid_forces_lines(
   const double* Ty = Tx + tstride;
   const double* Tz = Ty + tstride;
   const long row = (long)b * nao_pad + (long)A * nloc + o;
-  const double* Xr = X + row * ld;
-  const double* Zr = Zx + row * ld;
-  const double* vb = v + (long)b * ld;
+  const double* Xr = (MODE & 1) ? X + row * ld : nullptr;    // read only as MODE says
+  const double* Zr = (MODE & 2) ? Zx + row * ld : nullptr;
+  const double* vb = (MODE & 1) ? v + (long)b * ld : nullptr;
   double f[3] = {0.0, 0.0, 0.0};
   for (int a0 = 0; a0 < na && lb < le;) {
     // the next GFL_PCH primitives with a nonzero coefficient in this AO

@@ -1195,7 +1195,10 @@ class Engine:
 
     # ---------------------------------------------------------------- results
     def solve(self, xpb, kpt=None, Lb=None, forces=False):
-        """Full solver output for a batch (B, n, 3) Bohr.
+        """Full solver output for a batch (B, n, 3) Bohr.  forces=True also keeps what
+        the force pass reuses: the S/T centre derivatives and, for a DFT solve at
+        Gamma, st['force_grid'] = (X = D phi, v, rho~(G)) of the final density, which
+        compute_forces takes (and so releases).
 
         Per-geometry device state in st stays in slot order (st['order'][i] is
         the geometry held by slot i); the user-facing results mo, D, w1, E, conv,
